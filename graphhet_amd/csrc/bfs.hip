@@ -1,0 +1,545 @@
+// Breadth-first search on gfx950: direction-optimising, three frontier modes.
+//
+// Reference: csr_bfs / Cuda_BFS (GPU_BFS_csr.cuh:258-335) expands a queue into
+// an edge queue (gather) and filters it back into a vertex queue (contract),
+// two launches and four blocking copies per level, top-down only.  Semantics
+// are those of CPU_BFS (CPU_BFS.hpp:9-41): depth 0 at the source, INT_MAX when
+// unreached.  BFS depths are unique, so every mode below gives bit-identical
+// output; the mode only changes speed.
+//
+// Modes per level:
+//   PUSH    queue frontier; edge-balanced expansion (global degree scan, 2048-edge
+//           tiles, LDS-staged owner search and LDS-staged next queue).
+//   PULL    bitmap frontier; each unvisited vertex scans its in-list and stops at
+//           the first frontier parent; one wave owns 64 vertices = one 64-bit
+//           word of the bitmaps, so the next frontier is written without atomics.
+//   BITMAP  bitmap frontier, top-down: lanes expand light vertices, waves expand
+//           medium ones, the whole grid shares hub (deg > 4096) vertices.
+#include <cmath>
+
+#include "graph_internal.hpp"
+#include "sched.hpp"
+
+namespace ghet {
+namespace {
+
+struct BfsCtr {
+    int32_t count;
+    int32_t hubs;
+    unsigned long long out_edges;
+    unsigned long long in_edges;
+    unsigned long long pad;
+};
+
+constexpr int kBT = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBT * kItems;  // edges per push tile
+constexpr int kHubDeg = 4096;
+
+__device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
+    int lo = 0, hi = len;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_seed(int32_t* depth, uint32_t* visited, int32_t* queue, int32_t src) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        depth[src] = 0;
+        visited[src >> 5] |= 1u << (src & 31);
+        queue[0] = src;
+    }
+}
+
+__global__ void k_zero_ctr(BfsCtr* c) {
+    if (threadIdx.x == 0) {
+        c->count = 0;
+        c->hubs = 0;
+        c->out_edges = 0;
+        c->in_edges = 0;
+    }
+}
+
+__global__ void k_frontier_deg(const int32_t* __restrict__ q, int32_t F, const int32_t* __restrict__ rowptr,
+                               int32_t* __restrict__ deg) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < F) {
+        int32_t u = q[i];
+        deg[i] = rowptr[u + 1] - rowptr[u];
+    }
+}
+
+// Block-wide sum of two 64-bit values, added atomically to the counters.
+__device__ __forceinline__ void block_add_edges(unsigned long long od, unsigned long long id, BfsCtr* ctr) {
+    __shared__ unsigned long long s_red[2][kBT / 64];
+    od = wave_sum(od);
+    id = wave_sum(id);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_red[0][w] = od;
+        s_red[1][w] = id;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int i = 0; i < kBT / 64; ++i) {
+            a += s_red[0][i];
+            b += s_red[1][i];
+        }
+        if (a) atomicAdd(&ctr->out_edges, a);
+        if (b) atomicAdd(&ctr->in_edges, b);
+    }
+    __syncthreads();
+}
+
+// PUSH: one tile of kTile consecutive frontier edges per block.
+__global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__ q, int32_t F,
+                                                     const int32_t* __restrict__ scan, int32_t E,
+                                                     const int32_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ col, uint32_t* visited,
+                                                     int32_t* __restrict__ depth, int32_t next_level,
+                                                     int32_t* __restrict__ next_q, BfsCtr* ctr,
+                                                     const int32_t* __restrict__ in_rowptr) {
+    __shared__ int32_t s_scan[kTile];
+    __shared__ int32_t s_start[kTile];
+    __shared__ int32_t s_out[kTile];
+    __shared__ int32_t s_i0, s_span, s_cnt, s_base;
+    const int tid = threadIdx.x;
+    for (int64_t tile = blockIdx.x; tile * kTile < E; tile += gridDim.x) {
+        const int32_t t0 = static_cast<int32_t>(tile * kTile);
+        const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
+        if (tid == 0) {
+            int i0 = upper_bound_i32(scan, F + 1, t0) - 1;
+            int i1 = upper_bound_i32(scan, F + 1, t1 - 1) - 1;
+            s_i0 = i0;
+            s_span = i1 - i0 + 1;
+            s_cnt = 0;
+        }
+        __syncthreads();
+        const int i0 = s_i0, span = s_span;
+        const bool use_lds = span <= kTile;
+        if (use_lds) {
+            for (int j = tid; j < span; j += kBT) {
+                s_scan[j] = scan[i0 + j];
+                s_start[j] = rowptr[q[i0 + j]];
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < kItems; ++k) {
+            const int32_t e = t0 + k * kBT + tid;
+            bool disc = false;
+            int32_t v = 0;
+            if (e < t1) {
+                int32_t pos;
+                if (use_lds) {
+                    int j = upper_bound_i32(s_scan, span, e) - 1;
+                    pos = s_start[j] + (e - s_scan[j]);
+                } else {
+                    int j = upper_bound_i32(scan + i0, span, e) - 1;
+                    pos = rowptr[q[i0 + j]] + (e - scan[i0 + j]);
+                }
+                v = col[pos];
+                const uint32_t bit = 1u << (v & 31);
+                if (!(visited[v >> 5] & bit)) {
+                    uint32_t old = atomicOr(&visited[v >> 5], bit);
+                    if (!(old & bit)) {
+                        depth[v] = next_level;
+                        disc = true;
+                    }
+                }
+            }
+            uint64_t mask = __ballot(disc);
+            if (mask) {
+                int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+                int base = 0;
+                if (lane_id() == leader) base = atomicAdd(&s_cnt, __popcll(mask));
+                base = __shfl(base, leader);
+                if (disc) s_out[base + lanes_below(mask)] = v;
+            }
+        }
+        __syncthreads();
+        if (tid == 0 && s_cnt) s_base = atomicAdd(&ctr->count, s_cnt);
+        __syncthreads();
+        unsigned long long od = 0, id = 0;
+        const int cnt = s_cnt, base = s_base;
+        for (int j = tid; j < cnt; j += kBT) {
+            int32_t v = s_out[j];
+            next_q[base + j] = v;
+            od += rowptr[v + 1] - rowptr[v];
+            if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
+        }
+        block_add_edges(od, id, ctr);
+    }
+}
+
+// PULL: bottom-up over the in-CSR.  One wave = 64 vertices = words 2g, 2g+1.
+__global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restrict__ in_rowptr,
+                                              const int32_t* __restrict__ in_col, const uint32_t* __restrict__ cur_bm,
+                                              uint32_t* __restrict__ next_bm, uint32_t* __restrict__ visited,
+                                              int32_t* __restrict__ depth, int32_t next_level, BfsCtr* ctr,
+                                              const int32_t* __restrict__ out_rowptr) {
+    const int lane = lane_id();
+    const int64_t groups = ceil_div(n, 64);
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    unsigned long long od = 0, id = 0;
+    int cnt = 0;
+    for (int64_t g = wave_global; g < groups; g += wave_stride) {
+        const int64_t v = g * 64 + lane;
+        const uint32_t vis = visited[2 * g + (lane >> 5)];
+        bool found = false;
+        if (v < n && !((vis >> (lane & 31)) & 1u)) {
+            const int32_t b = in_rowptr[v], e = in_rowptr[v + 1];
+            for (int32_t i = b; i < e; ++i) {
+                int32_t u = in_col[i];
+                if ((cur_bm[u >> 5] >> (u & 31)) & 1u) {
+                    found = true;
+                    break;
+                }
+            }
+        }
+        const uint64_t mask = __ballot(found);
+        if (lane == 0) {
+            next_bm[2 * g] = static_cast<uint32_t>(mask);
+            visited[2 * g] = vis | static_cast<uint32_t>(mask);
+        } else if (lane == 32) {
+            next_bm[2 * g + 1] = static_cast<uint32_t>(mask >> 32);
+            visited[2 * g + 1] = vis | static_cast<uint32_t>(mask >> 32);
+        }
+        if (found) {
+            depth[v] = next_level;
+            od += out_rowptr[v + 1] - out_rowptr[v];
+            id += in_rowptr[v + 1] - in_rowptr[v];
+        }
+        cnt += __popcll(mask);
+    }
+    od = wave_sum(od);
+    id = wave_sum(id);
+    if (lane == 0 && cnt) {
+        atomicAdd(&ctr->count, cnt);
+        atomicAdd(&ctr->out_edges, od);
+        atomicAdd(&ctr->in_edges, id);
+    }
+}
+
+__device__ __forceinline__ bool visit(int32_t v, uint32_t* visited, uint32_t* next_bm, int32_t* depth,
+                                      int32_t next_level) {
+    const uint32_t bit = 1u << (v & 31);
+    if (visited[v >> 5] & bit) return false;
+    uint32_t old = atomicOr(&visited[v >> 5], bit);
+    if (old & bit) return false;
+    depth[v] = next_level;
+    atomicOr(&next_bm[v >> 5], bit);
+    return true;
+}
+
+// BITMAP: top-down from a frontier bitmap.  Light vertices (deg <= 32) are
+// expanded by their own lane, medium ones by the wave, hubs are queued.
+__global__ void __launch_bounds__(kBT) k_bitmap_topdown(int32_t n, const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const uint32_t* __restrict__ cur_bm, uint32_t* next_bm,
+                                                        uint32_t* visited, int32_t* __restrict__ depth,
+                                                        int32_t next_level, BfsCtr* ctr, int32_t* hub_q,
+                                                        const int32_t* __restrict__ in_rowptr) {
+    const int lane = lane_id();
+    const int64_t groups = ceil_div(n, 64);
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    unsigned long long od = 0, id = 0;
+    int cnt = 0;
+    for (int64_t g = wave_global; g < groups; g += wave_stride) {
+        const uint32_t word = cur_bm[2 * g + (lane >> 5)];
+        const int64_t u = g * 64 + lane;
+        const bool in_f = (word >> (lane & 31)) & 1u;
+        if (__ballot(in_f) == 0) continue;
+        int32_t b = 0, e = 0;
+        if (in_f) {
+            b = rowptr[u];
+            e = rowptr[u + 1];
+        }
+        const int32_t deg = e - b;
+        const bool light = in_f && deg <= 32;
+        const bool hub = in_f && deg > kHubDeg;
+        if (light) {
+            for (int32_t i = b; i < e; ++i) {
+                int32_t v = col[i];
+                if (visit(v, visited, next_bm, depth, next_level)) {
+                    ++cnt;
+                    od += rowptr[v + 1] - rowptr[v];
+                    if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
+                }
+            }
+        }
+        if (hub) {
+            int pos = atomicAdd(&ctr->hubs, 1);
+            hub_q[pos] = static_cast<int32_t>(u);
+        }
+        uint64_t med = __ballot(in_f && !light && !hub);
+        while (med) {
+            const int src_lane = __ffsll(static_cast<unsigned long long>(med)) - 1;
+            med &= med - 1;
+            const int32_t mb = __shfl(b, src_lane), me = __shfl(e, src_lane);
+            for (int32_t i = mb + lane; i < me; i += 64) {
+                int32_t v = col[i];
+                if (visit(v, visited, next_bm, depth, next_level)) {
+                    ++cnt;
+                    od += rowptr[v + 1] - rowptr[v];
+                    if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
+                }
+            }
+        }
+    }
+    od = wave_sum(od);
+    id = wave_sum(id);
+    cnt = wave_sum(cnt);
+    if (lane == 0 && cnt) {
+        atomicAdd(&ctr->count, cnt);
+        atomicAdd(&ctr->out_edges, od);
+        atomicAdd(&ctr->in_edges, id);
+    }
+}
+
+// Hub vertices queued by k_bitmap_topdown: every block takes a slice of every hub.
+__global__ void __launch_bounds__(kBT) k_hub_expand(const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ col, uint32_t* next_bm,
+                                                    uint32_t* visited, int32_t* __restrict__ depth,
+                                                    int32_t next_level, BfsCtr* ctr, const int32_t* hub_q,
+                                                    const int32_t* __restrict__ in_rowptr) {
+    const int32_t H = ctr->hubs;
+    unsigned long long od = 0, id = 0;
+    int cnt = 0;
+    for (int32_t h = 0; h < H; ++h) {
+        const int32_t u = hub_q[h];
+        const int32_t b = rowptr[u], e = rowptr[u + 1];
+        for (int64_t i = b + static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; i < e;
+             i += static_cast<int64_t>(gridDim.x) * kBT) {
+            int32_t v = col[i];
+            if (visit(v, visited, next_bm, depth, next_level)) {
+                ++cnt;
+                od += rowptr[v + 1] - rowptr[v];
+                if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
+            }
+        }
+    }
+    od = wave_sum(od);
+    id = wave_sum(id);
+    cnt = wave_sum(cnt);
+    if (lane_id() == 0 && cnt) {
+        atomicAdd(&ctr->count, cnt);
+        atomicAdd(&ctr->out_edges, od);
+        atomicAdd(&ctr->in_edges, id);
+    }
+}
+
+__global__ void k_queue_to_bitmap(const int32_t* __restrict__ q, int32_t F, uint32_t* bm) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < F) {
+        int32_t v = q[i];
+        atomicOr(&bm[v >> 5], 1u << (v & 31));
+    }
+}
+
+__global__ void __launch_bounds__(kBT) k_bitmap_to_queue(int32_t n, const uint32_t* __restrict__ bm,
+                                                         int32_t* __restrict__ q, int32_t* counter) {
+    const int lane = lane_id();
+    const int64_t groups = ceil_div(n, 64);
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    for (int64_t g = wave_global; g < groups; g += wave_stride) {
+        const uint32_t word = bm[2 * g + (lane >> 5)];
+        const bool set = (word >> (lane & 31)) & 1u;
+        int pos = wave_append(counter, set);
+        if (set) q[pos] = static_cast<int32_t>(g * 64 + lane);
+    }
+}
+
+// Parent = smallest in-neighbour one level up (in-lists are ascending, so the
+// first hit is the smallest).  -1 for the source and unreached vertices.
+__global__ void k_bfs_parent(int32_t n, const int32_t* __restrict__ in_rowptr, const int32_t* __restrict__ in_col,
+                             const int32_t* __restrict__ depth, int32_t* __restrict__ pre) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t d = depth[v];
+        int32_t p = -1;
+        if (d != kIntInf && d > 0) {
+            for (int32_t i = in_rowptr[v]; i < in_rowptr[v + 1]; ++i) {
+                int32_t u = in_col[i];
+                if (depth[u] == d - 1) {
+                    p = u;
+                    break;
+                }
+            }
+        }
+        pre[v] = p;
+    }
+}
+
+unsigned grid_for_waves(int64_t groups) {
+    // one wave per 64-vertex group, capped: 256 CUs x 8 blocks of 4 waves
+    int64_t blocks = ceil_div(groups, kBT / 64);
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 16)));
+}
+
+}  // namespace
+
+int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, const ghet_opts* o,
+            ghet_run_stats* st) {
+    const int32_t n = g->n;
+    if (src < 0 || src >= n) return set_error(GHET_ERR_INVALID, "source %d out of range [0,%d)", src, n);
+    if (!g->out.present()) return set_error(GHET_ERR_INVALID, "graph has no out-CSR");
+    hipStream_t s = pick_stream(g, o);
+    int schedule = (o && o->size >= sizeof(ghet_opts)) ? o->schedule : GHET_SCHED_HEURISTIC;
+    const bool have_in = g->in.present();
+    if (!have_in && (schedule == GHET_SCHED_PULL || schedule == GHET_SCHED_REFERENCE)) schedule = GHET_SCHED_PUSH;
+    if (pre_out && !have_in) return set_error(GHET_ERR_INVALID, "BFS parents need the in-CSR");
+
+    const int64_t words = 2 * ceil_div(std::max(n, 1), 64);
+    int32_t *depth, *qa, *qb, *scan, *aux, *hubq;
+    uint32_t *visited, *bma, *bmb;
+    BfsCtr* ctr;
+    void* p;
+    GHET_CHECK(g->ws.get(WS_DEPTH, 4 * size_t(n), &p)); depth = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_VISITED, 4 * size_t(words), &p)); visited = static_cast<uint32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_FRONT_A, 4 * size_t(words), &p)); bma = static_cast<uint32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_FRONT_B, 4 * size_t(words), &p)); bmb = static_cast<uint32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_QUEUE_A, 4 * size_t(n), &p)); qa = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_QUEUE_B, 4 * size_t(n), &p)); qb = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_MISC, 4 * size_t(n) + 256, &p)); hubq = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_COUNTERS, 256, &p)); ctr = static_cast<BfsCtr*>(p);
+
+    SchedState sched;
+    sched_begin(&sched, schedule, GHET_NET_BFS, n, g->out.m, have_in ? g->in.m : g->out.m);
+    ghet_level_trace* trace = (o && o->size >= sizeof(ghet_opts)) ? o->trace : nullptr;
+    const int trace_cap = trace ? o->trace_cap : 0;
+    int trace_len = 0;
+
+    GHET_HIP(hipEventRecord(g->ev_begin, s));
+    GHET_CHECK(fill<int32_t>(depth, kIntInf, n, s));
+    GHET_CHECK(fill<uint32_t>(visited, 0u, words, s));
+    hipLaunchKernelGGL(k_seed, dim3(1), dim3(64), 0, s, depth, visited, qa, src);
+
+    int32_t F = 1;
+    int64_t m_f = g->h_out_rowptr[src + 1] - g->h_out_rowptr[src];
+    int64_t visited_in = have_in ? (g->h_in_rowptr[src + 1] - g->h_in_rowptr[src]) : 0;
+    int64_t n_visited = 1;
+    bool have_q = true, have_bm = false;
+    int32_t level = 0;
+    int prev_mode = GHET_MODE_PUSH;
+    int64_t prev_F = 0;
+    const int32_t* in_rp = have_in ? g->in.rowptr : nullptr;
+    while (F > 0) {
+        LevelInfo li;
+        li.n = n;
+        li.m = g->out.m;
+        li.m_in = have_in ? g->in.m : g->out.m;
+        li.level = level;
+        li.frontier = F;
+        li.frontier_edges = m_f;
+        li.unvisited = n - n_visited;
+        li.unvisited_in_edges = li.m_in - visited_in;
+        li.prev_frontier = prev_F;
+        li.prev_mode = prev_mode;
+        li.have_in = have_in;
+        int mode = sched_pick_bfs(&sched, li);
+        if (trace && trace_len < trace_cap) sched_features(li, trace[trace_len].features);
+        if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k0, s));
+
+        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, ctr);
+        const int32_t next_level = level + 1;
+        if (mode == GHET_MODE_PUSH) {
+            if (!have_q) {
+                GHET_CHECK(fill<int32_t>(&ctr->count, 0, 1, s));
+                hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid_for_waves(words / 2)), dim3(kBT), 0, s, n, bma,
+                                   qa, &ctr->count);
+                hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, ctr);
+                have_q = true;
+            }
+            hipLaunchKernelGGL(k_frontier_deg, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s, qa, F,
+                               g->out.rowptr, scan);
+            GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
+            const int32_t E = static_cast<int32_t>(m_f);
+            if (E > 0) {
+                unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), 65535));
+                hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, qa, F, scan, E, g->out.rowptr,
+                                   g->out.col, visited, depth, next_level, qb, ctr, in_rp);
+            }
+            std::swap(qa, qb);
+            have_q = true;
+            have_bm = false;
+        } else {
+            if (!have_bm) {
+                GHET_CHECK(fill<uint32_t>(bma, 0u, words, s));
+                hipLaunchKernelGGL(k_queue_to_bitmap, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s,
+                                   qa, F, bma);
+                have_bm = true;
+            }
+            if (mode == GHET_MODE_PULL) {
+                hipLaunchKernelGGL(k_pull, dim3(grid_for_waves(words / 2)), dim3(kBT), 0, s, n, g->in.rowptr,
+                                   g->in.col, bma, bmb, visited, depth, next_level, ctr, g->out.rowptr);
+            } else {
+                GHET_CHECK(fill<uint32_t>(bmb, 0u, words, s));
+                hipLaunchKernelGGL(k_bitmap_topdown, dim3(grid_for_waves(words / 2)), dim3(kBT), 0, s, n,
+                                   g->out.rowptr, g->out.col, bma, bmb, visited, depth, next_level, ctr, hubq, in_rp);
+                hipLaunchKernelGGL(k_hub_expand, dim3(1024), dim3(kBT), 0, s, g->out.rowptr, g->out.col, bmb, visited,
+                                   depth, next_level, ctr, hubq, in_rp);
+            }
+            std::swap(bma, bmb);
+            have_bm = true;
+            have_q = false;
+        }
+        GHET_HIP(hipGetLastError());
+        GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
+        if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k1, s));
+        GHET_HIP(hipStreamSynchronize(s));
+        const BfsCtr* hc = reinterpret_cast<const BfsCtr*>(g->pinned);
+        if (trace && trace_len < trace_cap) {
+            float ms = 0;
+            GHET_HIP(hipEventElapsedTime(&ms, g->ev_k0, g->ev_k1));
+            trace[trace_len].mode = mode;
+            trace[trace_len].ms = ms;
+            ++trace_len;
+        }
+        prev_F = F;
+        prev_mode = mode;
+        F = hc->count;
+        m_f = static_cast<int64_t>(hc->out_edges);
+        visited_in += static_cast<int64_t>(hc->in_edges);
+        n_visited += F;
+        ++level;
+    }
+    if (pre_out) {
+        int32_t* pre;
+        GHET_CHECK(g->ws.get(WS_PRE, 4 * size_t(n), &p));
+        pre = static_cast<int32_t*>(p);
+        hipLaunchKernelGGL(k_bfs_parent, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 8192))),
+                           dim3(256), 0, s, n, g->in.rowptr, g->in.col, depth, pre);
+    }
+    GHET_HIP(hipEventRecord(g->ev_end, s));
+    GHET_HIP(hipGetLastError());
+    if (depth_out) GHET_HIP(hipMemcpyAsync(depth_out, depth, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    if (pre_out) GHET_HIP(hipMemcpyAsync(pre_out, g->ws.ptr[WS_PRE], 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    if (st) {
+        GHET_CHECK(reached_stats_i32(depth, g->out.rowptr, n, reinterpret_cast<int64_t*>(scan), s));
+        GHET_HIP(hipMemcpyAsync(g->pinned, scan, 16, hipMemcpyDeviceToHost, s));
+    }
+    GHET_HIP(hipStreamSynchronize(s));
+    if (st) {
+        float ms = 0;
+        GHET_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
+        st->device_ms = ms;
+        st->levels = level;
+        const int64_t* rc = reinterpret_cast<const int64_t*>(g->pinned);
+        st->vertices_reached = rc[0];
+        st->edges_traversed = rc[1];
+        st->trace_len = trace_len;
+        st->status_flags = sched.fallback ? 1 : 0;
+    }
+    return sched.fallback ? GHET_WARN_FALLBACK : GHET_OK;
+}
+
+}  // namespace ghet

@@ -1,0 +1,325 @@
+// Community detection by label propagation (CDLP) on gfx950.
+//
+// Reference: CDLP_GPU (GPU_CDLP_csr.cuh:75-187) copies every neighbour label
+// into a 2m-entry array, segment-sorts it with cub::DeviceSegmentedSort and
+// takes the longest run; CDLP (CPU_Community_Detection.hpp:12-64) counts
+// labels of in- and out-neighbours (with multiplicity) in a hash map and takes
+// the most frequent, ties to the smallest label; a vertex without neighbours
+// keeps its label.  (The reference GPU kernel reads the next vertex's segment
+// for an isolated vertex, GPU_CDLP_csr.cuh:56; this path follows the CPU and
+// LDBC semantics.)
+//
+// No sort: vertices are bucketed by all-degree d once per graph and each class
+// counts labels where it fits best:
+//   d <= 32      one lane, labels staged in LDS, pairwise counting;
+//   d <= 256     one wave, labels in LDS, each lane counts its labels;
+//   d <= 2048    one workgroup, open-addressing hash table in LDS (4096 slots);
+//   d > 2048     several workgroups per vertex insert into a global hash table
+//                (2^ceil(log2(2d)) slots), then one workgroup takes the argmax
+//                and clears the table for the next iteration.
+// Winner = max count, ties to the smallest label: the maximum of the key
+// (count << 32) | (0xFFFFFFFF - label).
+#include "graph_internal.hpp"
+
+namespace ghet {
+namespace {
+
+constexpr int kBT = 256;
+constexpr int kSmall = 32, kWave = 256, kLds = 2048, kLdsSlots = 4096;
+constexpr int kHeavyChunk = 8192;
+constexpr int32_t kEmpty = -1;
+
+__device__ __forceinline__ uint64_t pack(uint32_t count, int32_t label) {
+    return (static_cast<uint64_t>(count) << 32) | (0xFFFFFFFFu - static_cast<uint32_t>(label));
+}
+__device__ __forceinline__ int32_t unpack_label(uint64_t k) {
+    return static_cast<int32_t>(0xFFFFFFFFu - static_cast<uint32_t>(k & 0xFFFFFFFFu));
+}
+__device__ __forceinline__ uint32_t hash_slot(int32_t label, uint32_t mask) {
+    return (static_cast<uint32_t>(label) * 0x9E3779B1u) & mask;
+}
+
+__global__ void k_label_init(int32_t* a, int32_t* b, int32_t n) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        a[v] = static_cast<int32_t>(v);
+        b[v] = static_cast<int32_t>(v);
+    }
+}
+
+__global__ void __launch_bounds__(kBT) k_cdlp_small(const int32_t* __restrict__ list, int32_t count,
+                                                    const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ col, const int32_t* __restrict__ lab,
+                                                    int32_t* __restrict__ out) {
+    __shared__ int32_t s[kBT * kSmall];
+    int32_t* mine = s + threadIdx.x * kSmall;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; i < count;
+         i += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int32_t v = list[i];
+        const int32_t b = rowptr[v], d = rowptr[v + 1] - b;
+        for (int j = 0; j < d; ++j) mine[j] = lab[col[b + j]];
+        uint64_t best = 0;
+        for (int j = 0; j < d; ++j) {
+            const int32_t x = mine[j];
+            uint32_t c = 0;
+            for (int k = 0; k < d; ++k) c += (mine[k] == x);
+            const uint64_t key = pack(c, x);
+            best = key > best ? key : best;
+        }
+        out[v] = unpack_label(best);
+    }
+}
+
+__global__ void __launch_bounds__(kBT) k_cdlp_wave(const int32_t* __restrict__ list, int32_t count,
+                                                   const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                   const int32_t* __restrict__ lab, int32_t* __restrict__ out) {
+    __shared__ int32_t s[(kBT / 64) * kWave];
+    const int lane = lane_id();
+    int32_t* mine = s + (threadIdx.x >> 6) * kWave;
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    for (int64_t i = wave_global; i < count; i += wave_stride) {
+        const int32_t v = list[i];
+        const int32_t b = rowptr[v], d = rowptr[v + 1] - b;
+        for (int j = lane; j < d; j += 64) mine[j] = lab[col[b + j]];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        uint64_t best = 0;
+        for (int j = lane; j < d; j += 64) {
+            const int32_t x = mine[j];
+            uint32_t c = 0;
+            for (int k = 0; k < d; ++k) c += (mine[k] == x);
+            const uint64_t key = pack(c, x);
+            best = key > best ? key : best;
+        }
+        best = wave_max(best);
+        if (lane == 0) out[v] = unpack_label(best);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ void __launch_bounds__(kBT) k_cdlp_lds(const int32_t* __restrict__ list, int32_t count,
+                                                  const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ lab, int32_t* __restrict__ out) {
+    __shared__ int32_t keys[kLdsSlots];
+    __shared__ uint32_t cnts[kLdsSlots];
+    __shared__ uint64_t s_red[kBT / 64];
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const int32_t v = list[i];
+        const int32_t b = rowptr[v], d = rowptr[v + 1] - b;
+        for (int j = threadIdx.x; j < kLdsSlots; j += kBT) {
+            keys[j] = kEmpty;
+            cnts[j] = 0;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < d; j += kBT) {
+            const int32_t x = lab[col[b + j]];
+            uint32_t h = hash_slot(x, kLdsSlots - 1);
+            for (;;) {
+                int32_t prev = atomicCAS(&keys[h], kEmpty, x);
+                if (prev == kEmpty || prev == x) {
+                    atomicAdd(&cnts[h], 1u);
+                    break;
+                }
+                h = (h + 1) & (kLdsSlots - 1);
+            }
+        }
+        __syncthreads();
+        uint64_t best = 0;
+        for (int j = threadIdx.x; j < kLdsSlots; j += kBT) {
+            if (keys[j] != kEmpty) {
+                const uint64_t key = pack(cnts[j], keys[j]);
+                best = key > best ? key : best;
+            }
+        }
+        best = wave_max(best);
+        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = best;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t t = 0;
+            for (int w = 0; w < kBT / 64; ++w) t = s_red[w] > t ? s_red[w] : t;
+            out[v] = unpack_label(t);
+        }
+        __syncthreads();
+    }
+}
+
+// Heavy vertices: chunk c inserts the labels of edges [beg, beg+kHeavyChunk)
+// of heavy vertex hv into its global table.
+__global__ void __launch_bounds__(kBT) k_cdlp_heavy_count(
+    const int32_t* __restrict__ list, const int64_t* __restrict__ base, const int32_t* __restrict__ chunk_vtx,
+    const int32_t* __restrict__ chunk_beg, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+    const int32_t* __restrict__ lab, int32_t* tkeys, uint32_t* tcnts) {
+    const int32_t hv = chunk_vtx[blockIdx.x];
+    const int32_t v = list[hv];
+    const int32_t beg = chunk_beg[blockIdx.x], end = min(beg + kHeavyChunk, rowptr[v + 1]);
+    const int64_t b0 = base[hv];
+    const uint32_t mask = static_cast<uint32_t>(base[hv + 1] - b0 - 1);
+    int32_t* keys = tkeys + b0;
+    uint32_t* cnts = tcnts + b0;
+    for (int32_t j = beg + threadIdx.x; j < end; j += kBT) {
+        const int32_t x = lab[col[j]];
+        uint32_t h = hash_slot(x, mask);
+        for (;;) {
+            int32_t prev = atomicCAS(&keys[h], kEmpty, x);
+            if (prev == kEmpty || prev == x) {
+                atomicAdd(&cnts[h], 1u);
+                break;
+            }
+            h = (h + 1) & mask;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBT) k_cdlp_heavy_pick(const int32_t* __restrict__ list,
+                                                         const int64_t* __restrict__ base, int32_t* tkeys,
+                                                         uint32_t* tcnts, int32_t* __restrict__ out) {
+    __shared__ uint64_t s_red[kBT / 64];
+    const int32_t hv = blockIdx.x;
+    const int64_t b0 = base[hv], size = base[hv + 1] - b0;
+    uint64_t best = 0;
+    for (int64_t j = threadIdx.x; j < size; j += kBT) {
+        const int32_t k = tkeys[b0 + j];
+        if (k != kEmpty) {
+            const uint64_t key = pack(tcnts[b0 + j], k);
+            best = key > best ? key : best;
+            tkeys[b0 + j] = kEmpty;
+            tcnts[b0 + j] = 0;
+        }
+    }
+    best = wave_max(best);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kBT / 64; ++w) t = s_red[w] > t ? s_red[w] : t;
+        out[list[hv]] = unpack_label(t);
+    }
+}
+
+int build_cdlp_layout(ghet_graph* g) {
+    CDLPLayout& L = g->cdlp;
+    if (L.built) return GHET_OK;
+    const int32_t n = g->n;
+    const std::vector<int32_t>& rp = g->h_all_rowptr;
+    std::vector<int32_t> lists[4];
+    for (int32_t v = 0; v < n; ++v) {
+        const int32_t d = rp[v + 1] - rp[v];
+        if (d == 0) continue;
+        lists[d <= kSmall ? 0 : d <= kWave ? 1 : d <= kLds ? 2 : 3].push_back(v);
+    }
+    std::vector<int64_t> hbase(lists[3].size() + 1, 0);
+    std::vector<int32_t> cv, cb;
+    for (size_t h = 0; h < lists[3].size(); ++h) {
+        const int32_t v = lists[3][h], d = rp[v + 1] - rp[v];
+        int64_t cap = 1;
+        while (cap < 2LL * d) cap <<= 1;
+        hbase[h + 1] = hbase[h] + cap;
+        for (int32_t e = rp[v]; e < rp[v + 1]; e += kHeavyChunk) {
+            cv.push_back(static_cast<int32_t>(h));
+            cb.push_back(e);
+        }
+    }
+    size_t bytes = 0;
+    for (auto& l : lists) bytes += ((4 * l.size() + 255) & ~size_t(255));
+    bytes += ((8 * hbase.size() + 255) & ~size_t(255)) + 2 * ((4 * cv.size() + 255) & ~size_t(255));
+    const int64_t slots = hbase.back();
+    bytes += 2 * ((4 * size_t(slots) + 255) & ~size_t(255));
+    GHET_HIP(hipMalloc(&L.mem, std::max<size_t>(bytes, 256)));
+    char* p = static_cast<char*>(L.mem);
+    auto take = [&](size_t b) { char* r = p; p += (b + 255) & ~size_t(255); return r; };
+    for (int c = 0; c < 4; ++c) {
+        L.count[c] = static_cast<int32_t>(lists[c].size());
+        L.list[c] = reinterpret_cast<int32_t*>(take(4 * lists[c].size()));
+        if (!lists[c].empty()) GHET_HIP(hipMemcpy(L.list[c], lists[c].data(), 4 * lists[c].size(), hipMemcpyHostToDevice));
+    }
+    L.heavy_base = reinterpret_cast<int64_t*>(take(8 * hbase.size()));
+    GHET_HIP(hipMemcpy(L.heavy_base, hbase.data(), 8 * hbase.size(), hipMemcpyHostToDevice));
+    L.heavy_chunks = static_cast<int32_t>(cv.size());
+    L.heavy_chunk_vtx = reinterpret_cast<int32_t*>(take(4 * cv.size()));
+    L.heavy_chunk_beg = reinterpret_cast<int32_t*>(take(4 * cb.size()));
+    if (!cv.empty()) {
+        GHET_HIP(hipMemcpy(L.heavy_chunk_vtx, cv.data(), 4 * cv.size(), hipMemcpyHostToDevice));
+        GHET_HIP(hipMemcpy(L.heavy_chunk_beg, cb.data(), 4 * cb.size(), hipMemcpyHostToDevice));
+    }
+    L.heavy_slots = slots;
+    L.heavy_keys = reinterpret_cast<int32_t*>(take(4 * size_t(slots)));
+    L.heavy_cnts = reinterpret_cast<uint32_t*>(take(4 * size_t(slots)));
+    if (slots) {
+        GHET_HIP(hipMemset(L.heavy_keys, 0xFF, 4 * size_t(slots)));
+        GHET_HIP(hipMemset(L.heavy_cnts, 0, 4 * size_t(slots)));
+    }
+    L.built = true;
+    return GHET_OK;
+}
+
+unsigned grid_cap(int64_t blocks) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 65535)));
+}
+
+}  // namespace
+
+int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* o, ghet_run_stats* st) {
+    const int32_t n = g->n;
+    if (iters < 0) return set_error(GHET_ERR_INVALID, "iterations %d < 0", iters);
+    if (!g->all.present() || g->h_all_rowptr.empty())
+        return set_error(GHET_ERR_INVALID, "CDLP needs the all-CSR (in-list ++ out-list)");
+    if (n == 0) return GHET_OK;
+    hipStream_t s = pick_stream(g, o);
+    GHET_CHECK(build_cdlp_layout(g));
+    CDLPLayout& L = g->cdlp;
+    void* p;
+    int32_t *la, *lb;
+    GHET_CHECK(g->ws.get(WS_LABEL_A, 4 * size_t(n), &p)); la = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_LABEL_B, 4 * size_t(n), &p)); lb = static_cast<int32_t*>(p);
+    const bool timing = st != nullptr;
+    hipEvent_t* kev = nullptr;
+    if (timing) GHET_CHECK(kernel_events(g, 2 * std::max(iters, 1), &kev));
+
+    GHET_HIP(hipEventRecord(g->ev_begin, s));
+    hipLaunchKernelGGL(k_label_init, dim3(grid_cap(ceil_div(n, 256))), dim3(256), 0, s, la, lb, n);
+    for (int32_t it = 0; it < iters; ++it) {
+        if (timing) GHET_HIP(hipEventRecord(kev[2 * it], s));
+        const int32_t* A = g->all.rowptr;
+        const int32_t* C = g->all.col;
+        if (L.count[0])
+            hipLaunchKernelGGL(k_cdlp_small, dim3(grid_cap(ceil_div(L.count[0], kBT))), dim3(kBT), 0, s, L.list[0],
+                               L.count[0], A, C, la, lb);
+        if (L.count[1])
+            hipLaunchKernelGGL(k_cdlp_wave, dim3(grid_cap(ceil_div(L.count[1], kBT / 64))), dim3(kBT), 0, s,
+                               L.list[1], L.count[1], A, C, la, lb);
+        if (L.count[2])
+            hipLaunchKernelGGL(k_cdlp_lds, dim3(grid_cap(L.count[2])), dim3(kBT), 0, s, L.list[2], L.count[2], A, C,
+                               la, lb);
+        if (L.count[3]) {
+            hipLaunchKernelGGL(k_cdlp_heavy_count, dim3(L.heavy_chunks), dim3(kBT), 0, s, L.list[3], L.heavy_base,
+                               L.heavy_chunk_vtx, L.heavy_chunk_beg, A, C, la, L.heavy_keys, L.heavy_cnts);
+            hipLaunchKernelGGL(k_cdlp_heavy_pick, dim3(L.count[3]), dim3(kBT), 0, s, L.list[3], L.heavy_base,
+                               L.heavy_keys, L.heavy_cnts, lb);
+        }
+        if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
+        std::swap(la, lb);
+    }
+    GHET_HIP(hipEventRecord(g->ev_end, s));
+    GHET_HIP(hipGetLastError());
+    if (label_out) GHET_HIP(hipMemcpyAsync(label_out, la, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    GHET_HIP(hipStreamSynchronize(s));
+    if (st) {
+        double hot = 0;
+        for (int32_t it = 0; it < iters; ++it) {
+            float kms = 0;
+            GHET_HIP(hipEventElapsedTime(&kms, kev[2 * it], kev[2 * it + 1]));
+            hot += kms;
+        }
+        float ms = 0;
+        GHET_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
+        st->device_ms = ms;
+        st->hot_kernel_ms = hot;
+        st->hot_kernel_launches = iters;
+        st->levels = iters;
+        for (int c = 0; c < 4; ++c) st->extra[c] = L.count[c];
+    }
+    return GHET_OK;
+}
+
+}  // namespace ghet

@@ -1,0 +1,106 @@
+// Device-side building blocks for gfx950 (CDNA4, wave64): error macros,
+// wave-level ballot/prefix helpers and relaxed agent-scope atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+
+#include "host_common.hpp"
+
+#define GHET_HIP(call)                                                                              \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return ::ghet::set_error(GHET_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                     __FILE__, __LINE__);                                          \
+    } while (0)
+
+#define GHET_CHECK(expr)            \
+    do {                            \
+        int st_ = (expr);           \
+        if (st_ != GHET_OK) return st_; \
+    } while (0)
+
+namespace ghet {
+
+constexpr int kWave = 64;
+constexpr int32_t kIntInf = 0x7FFFFFFF;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Number of set lanes in `mask` below this lane.
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0));
+}
+
+// Wave-aggregated append: every active lane calls it; lanes with pred=true get
+// a unique slot in [*counter, *counter + count).  One atomic per wave.
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    return pred ? base + lanes_below(mask) : -1;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int32_t load_relaxed(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t load_relaxed(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// FP64 atomic min on non-negative doubles through the signed 64-bit bit
+// pattern: for x, y >= 0 (finite or DBL_MAX), x < y  <=>  bits(x) < bits(y).
+__device__ __forceinline__ double atomic_min_nonneg(double* addr, double v) {
+    long long old = atomicMin(reinterpret_cast<long long*>(addr), __double_as_longlong(v));
+    return __longlong_as_double(old);
+}
+
+// Compensated (Kahan-Babuska/Neumaier) FP32 accumulator.
+struct Kahan32 {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x) {
+        float t = s + x;
+        if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ void merge(const Kahan32& o) {
+        add(o.s);
+        c += o.c;
+    }
+    __device__ __forceinline__ float value() const { return s + c; }
+};
+
+__host__ __device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace ghet

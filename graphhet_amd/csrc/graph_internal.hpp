@@ -1,0 +1,140 @@
+// Internal definition of the device graph handle and the per-algorithm
+// launchers that the C ABI (abi.hip) dispatches to.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "dev_common.hpp"
+
+namespace ghet {
+
+// A CSR resident in device memory.  rowptr has n+1 entries.
+struct DevCSR {
+    int32_t n = 0;
+    int64_t m = 0;
+    int32_t* rowptr = nullptr;
+    int32_t* col = nullptr;
+    double* w = nullptr;
+    bool present() const { return rowptr != nullptr; }
+};
+
+// Grow-only device scratch, one buffer per slot, reused across calls on the
+// same graph (the reference instead allocates and frees per call).
+enum WsSlot {
+    WS_DEPTH, WS_VISITED, WS_FRONT_A, WS_FRONT_B, WS_QUEUE_A, WS_QUEUE_B, WS_SCAN, WS_SCAN_AUX,
+    WS_COUNTERS, WS_DIST, WS_STAMP_NEAR, WS_STAMP_FAR, WS_FAR_A, WS_FAR_B, WS_PARENT,
+    WS_PR_CONTRIB, WS_PR_RANK, WS_PR_PART, WS_PR_LONGPART, WS_LABEL_A, WS_LABEL_B, WS_HASH,
+    WS_TC_COUNT, WS_PRE, WS_MULTI_A, WS_MULTI_B, WS_MULTI_C, WS_MULTI_D, WS_MISC,
+    WS_NUM_SLOTS
+};
+
+struct Workspace {
+    void* ptr[WS_NUM_SLOTS] = {};
+    size_t bytes[WS_NUM_SLOTS] = {};
+    // Returns a buffer of at least `want` bytes for `slot` (contents undefined).
+    int get(int slot, size_t want, void** out);
+    void release();
+    int64_t total() const {
+        int64_t t = 0;
+        for (size_t b : bytes) t += static_cast<int64_t>(b);
+        return t;
+    }
+};
+
+// PageRank tile layout over the in-CSR (built once per graph).
+struct PRLayout {
+    bool built = false;
+    int32_t num_tiles = 0;        // tiles of consecutive short rows
+    int32_t num_long_chunks = 0;  // chunks of long rows
+    int32_t num_long_rows = 0;
+    int32_t* tile_row = nullptr;     // [num_tiles] first row of each tile
+    int32_t* tile_end_ptr = nullptr; // [num_tiles] one past the last row (long rows are never inside)
+    int32_t* long_row = nullptr;     // [num_long_rows] row ids
+    int32_t* long_chunk0 = nullptr;  // [num_long_rows+1] first chunk of each long row
+    int32_t* long_done = nullptr;    // [num_long_rows] arrival counters (self-resetting)
+    int32_t* outdeg = nullptr;       // [n] out-degree (sinks have 0)
+    void* mem = nullptr;
+};
+
+// Degree buckets over the all-CSR for CDLP (vertex lists per class).
+struct CDLPLayout {
+    bool built = false;
+    int32_t count[5] = {};  // per class: 0 small, 1 wave, 2 lds, 3 heavy
+    int32_t* list[5] = {};
+    int64_t heavy_slots = 0;       // total hash slots for heavy vertices
+    int64_t* heavy_base = nullptr; // [count[3]+1] slot offset per heavy vertex
+    int32_t heavy_chunks = 0;
+    int32_t* heavy_chunk_vtx = nullptr;  // [heavy_chunks] index into heavy list
+    int32_t* heavy_chunk_beg = nullptr;  // [heavy_chunks] first edge of the chunk
+    int32_t* heavy_keys = nullptr;       // [heavy_slots] labels (-1 = empty)
+    uint32_t* heavy_cnts = nullptr;      // [heavy_slots] counts
+    void* mem = nullptr;
+};
+
+// Degree-ordered DAG of a symmetric graph for TC.
+struct TCLayout {
+    bool built = false;
+    bool symmetric = false;   // out-CSR symmetric and loop-free
+    DevCSR dag;               // oriented by (degree, id)
+    int32_t* active = nullptr;  // vertices with >= 2 out-neighbours in the CSR used
+    int32_t num_active = 0;
+    void* mem = nullptr;
+};
+
+}  // namespace ghet
+
+struct ghet_graph {
+    int32_t n = 0;
+    bool directed = true;
+    bool owns_csr = true;
+    int device = 0;
+    ghet::DevCSR out, in, all;
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    std::vector<int32_t> h_out_rowptr, h_in_rowptr, h_all_rowptr;  // host copies of offsets
+    ghet::Workspace ws;
+    ghet::PRLayout pr;
+    ghet::CDLPLayout cdlp;
+    ghet::TCLayout tc;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    int32_t* pinned = nullptr;  // small pinned host buffer for per-level counters (64 ints)
+    std::vector<hipEvent_t> kevents;  // per-launch timing events, created on demand
+};
+
+namespace ghet {
+
+// Resolves the stream to launch on (opts->stream or the graph's own stream).
+hipStream_t pick_stream(ghet_graph* g, const ghet_opts* opts);
+// At least `count` timing events owned by the graph.
+int kernel_events(ghet_graph* g, int count, hipEvent_t** out);
+
+// Launchers (return a ghet_status).  Output pointers are host memory or NULL.
+int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, const ghet_opts* o,
+            ghet_run_stats* st);
+int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* pre_out, const ghet_opts* o,
+             ghet_run_stats* st);
+int run_wcc(ghet_graph* g, int32_t* label_out, const ghet_opts* o, ghet_run_stats* st);
+int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const ghet_opts* o, ghet_run_stats* st);
+int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* o, ghet_run_stats* st);
+int run_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* o, ghet_run_stats* st);
+
+// Device-wide exclusive scan of int32 -> int32 (out may alias in); n up to 2^31-1.
+// `aux` must hold scan_aux_ints(n) ints.  Writes the total to out[n].
+int64_t scan_aux_ints(int64_t n);
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* aux, hipStream_t s);
+
+// Reached-vertex statistics: n_r and m_r (sum of out-degrees) of vertices with
+// depth != INT_MAX (or dist != inf).  Results in counters[0..1] (int64).
+int reached_stats_i32(const int32_t* depth, const int32_t* rowptr, int32_t n, int64_t* counters, hipStream_t s);
+int reached_stats_f64(const double* dist, double inf, const int32_t* rowptr, int32_t n, int64_t* counters,
+                      hipStream_t s);
+
+template <typename T>
+int fill(T* p, T v, int64_t n, hipStream_t s);
+
+}  // namespace ghet

@@ -1,0 +1,177 @@
+// Device-wide primitives shared by the algorithms: fill, exclusive scan
+// (reduce / scan-of-partials / apply, all deterministic) and reached-vertex
+// statistics.
+#include "graph_internal.hpp"
+
+namespace ghet {
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, T v, int64_t n) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+int fill(T* p, T v, int64_t n, hipStream_t s) {
+    if (n <= 0) return GHET_OK;
+    int64_t blocks = std::min<int64_t>(ceil_div(n, 256), 8192);
+    hipLaunchKernelGGL(k_fill<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p, v, n);
+    GHET_HIP(hipGetLastError());
+    return GHET_OK;
+}
+template int fill<int32_t>(int32_t*, int32_t, int64_t, hipStream_t);
+template int fill<uint32_t>(uint32_t*, uint32_t, int64_t, hipStream_t);
+template int fill<int64_t>(int64_t*, int64_t, int64_t, hipStream_t);
+template int fill<uint64_t>(uint64_t*, uint64_t, int64_t, hipStream_t);
+template int fill<double>(double*, double, int64_t, hipStream_t);
+template int fill<float>(float*, float, int64_t, hipStream_t);
+
+// ---------------------------------------------------------------- scan ----
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 4096 elements per block
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+template <typename T>
+__device__ T block_exclusive_scan(T x, T* lds_waves, T& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    T incl = x;
+    for (int off = 1; off < 64; off <<= 1) {
+        T y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) lds_waves[wave] = incl;
+    __syncthreads();
+    T wave_off = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        T s = lds_waves[w];
+        if (w < wave) wave_off += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return wave_off + incl - x;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
+                                                               int32_t* __restrict__ aux) {
+    __shared__ int32_t lds[kScanThreads / 64];
+    int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    int32_t s = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        int64_t i = base + j * kScanThreads + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t t = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) t += lds[w];
+        aux[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_aux(int32_t* __restrict__ aux, int64_t nb) {
+    __shared__ int32_t lds[16];
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t b = 0; b < nb; b += 1024) {
+        int64_t i = b + threadIdx.x;
+        int32_t x = i < nb ? aux[i] : 0;
+        int32_t tot;
+        int32_t ex = block_exclusive_scan(x, lds, tot);
+        if (i < nb) aux[i] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) aux[nb] = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int32_t* in, int32_t* out, int64_t n,
+                                                              const int32_t* __restrict__ aux, int64_t nb) {
+    __shared__ int32_t lds[kScanThreads / 64];
+    int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kScanItems;
+    int32_t v[kScanItems];
+    int32_t s = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        int64_t i = base + j;
+        v[j] = i < n ? in[i] : 0;
+        s += v[j];
+    }
+    int32_t tot;
+    int32_t run = block_exclusive_scan(s, lds, tot) + aux[blockIdx.x];
+    for (int j = 0; j < kScanItems; ++j) {
+        int64_t i = base + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = aux[nb];
+}
+
+int64_t scan_aux_ints(int64_t n) { return ceil_div(std::max<int64_t>(n, 1), kScanTile) + 1; }
+
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* aux, hipStream_t s) {
+    if (n <= 0) {
+        GHET_CHECK(fill<int32_t>(out, 0, 1, s));
+        return GHET_OK;
+    }
+    int64_t nb = ceil_div(n, kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in, n, aux);
+    hipLaunchKernelGGL(k_scan_aux, dim3(1), dim3(1024), 0, s, aux, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in, out, n, aux, nb);
+    GHET_HIP(hipGetLastError());
+    return GHET_OK;
+}
+
+// ------------------------------------------------------- reached stats ----
+template <typename Pred>
+__device__ void reached_body(Pred reached, const int32_t* __restrict__ rowptr, int32_t n,
+                             unsigned long long* counters) {
+    int64_t cnt = 0, edges = 0;
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (reached(v)) {
+            ++cnt;
+            edges += rowptr[v + 1] - rowptr[v];
+        }
+    }
+    cnt = wave_sum(cnt);
+    edges = wave_sum(edges);
+    if ((threadIdx.x & 63) == 0 && (cnt | edges)) {
+        atomicAdd(&counters[0], static_cast<unsigned long long>(cnt));
+        atomicAdd(&counters[1], static_cast<unsigned long long>(edges));
+    }
+}
+
+__global__ void k_reached_i32(const int32_t* __restrict__ depth, const int32_t* __restrict__ rowptr, int32_t n,
+                              unsigned long long* counters) {
+    reached_body([&](int64_t v) { return depth[v] != kIntInf; }, rowptr, n, counters);
+}
+__global__ void k_reached_f64(const double* __restrict__ dist, double inf, const int32_t* __restrict__ rowptr,
+                              int32_t n, unsigned long long* counters) {
+    reached_body([&](int64_t v) { return dist[v] != inf; }, rowptr, n, counters);
+}
+
+int reached_stats_i32(const int32_t* depth, const int32_t* rowptr, int32_t n, int64_t* counters, hipStream_t s) {
+    GHET_CHECK(fill<int64_t>(counters, 0, 2, s));
+    unsigned blocks = static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 4096));
+    hipLaunchKernelGGL(k_reached_i32, dim3(std::max(blocks, 1u)), dim3(256), 0, s, depth, rowptr, n,
+                       reinterpret_cast<unsigned long long*>(counters));
+    GHET_HIP(hipGetLastError());
+    return GHET_OK;
+}
+int reached_stats_f64(const double* dist, double inf, const int32_t* rowptr, int32_t n, int64_t* counters,
+                      hipStream_t s) {
+    GHET_CHECK(fill<int64_t>(counters, 0, 2, s));
+    unsigned blocks = static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 4096));
+    hipLaunchKernelGGL(k_reached_f64, dim3(std::max(blocks, 1u)), dim3(256), 0, s, dist, inf, rowptr, n,
+                       reinterpret_cast<unsigned long long*>(counters));
+    GHET_HIP(hipGetLastError());
+    return GHET_OK;
+}
+
+}  // namespace ghet

@@ -1,0 +1,200 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and with
+GraphHET's own CPU outputs (golden fixtures).
+
+Bars: BFS depths, WCC labels, CDLP labels and TC counts bit-exact; SSSP
+within 1e-12 relative (the fixpoint argument makes it exact, and the tests
+also check bit equality); PageRank L1 <= 1e-6 and max-abs <= 1e-7 against the
+FP64 oracle."""
+import numpy as np
+import pytest
+
+import graphhet_amd as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+INT_INF = 2**31 - 1
+DBL_MAX = np.finfo(np.float64).max
+PR_L1, PR_MAXABS = 1e-6, 1e-7
+SCHEDULES = [G.SCHED_HEURISTIC, G.SCHED_PUSH, G.SCHED_PULL, G.SCHED_BITMAP, G.SCHED_REFERENCE]
+
+
+def dev_from_golden(g):
+    out = G.Csr(int(g["n"]), g["out_rowptr"], g["out_col"], g["out_w"])
+    inn = G.Csr(int(g["n"]), g["in_rowptr"], g["in_col"], g["in_w"])
+    return G.DeviceGraph(out, inn)
+
+
+def sym_from_golden(g):
+    return G.DeviceGraph(G.Csr(int(g["n"]), g["sym_rowptr"], g["sym_col"]), undirected=True, weights=False)
+
+
+def test_golden_bfs_all_schedules(golden):
+    dg = dev_from_golden(golden)
+    for sch in SCHEDULES:
+        d, st = dg.bfs(int(golden["src"]), schedule=sch)
+        np.testing.assert_array_equal(d, golden["bfs"], err_msg=f"schedule {sch}")
+
+
+def test_golden_sssp(golden):
+    dg = dev_from_golden(golden)
+    for delta in (0.0, 0.01, 0.3, 5.0):
+        d, st = dg.sssp(int(golden["src"]), delta=delta)
+        np.testing.assert_allclose(d, golden["sssp"], rtol=1e-12, atol=0)
+        np.testing.assert_array_equal(d, golden["sssp"])  # fixpoint: bit-identical
+
+
+def test_golden_wcc(golden):
+    dg = dev_from_golden(golden)
+    lab, st = dg.wcc()
+    np.testing.assert_array_equal(lab, golden["wcc"])
+
+
+def test_golden_pr(golden):
+    dg = dev_from_golden(golden)
+    r, st = dg.pr(10, 0.85)
+    err = np.abs(r - golden["pr"])
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (err.sum(), err.max())
+    r2, _ = dg.pr(10, 0.85, normalise=False)
+    err = np.abs(r2 - golden["pr"])
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (err.sum(), err.max())
+
+
+def test_golden_cdlp(golden):
+    dg = dev_from_golden(golden)
+    lab, st = dg.cdlp(10)
+    np.testing.assert_array_equal(lab, golden["cdlp"])
+
+
+def test_golden_tc(golden):
+    sg = sym_from_golden(golden)
+    c, st = sg.tc()
+    np.testing.assert_array_equal(c, golden["tc_sym"])
+    dg = dev_from_golden(golden)  # directed input: the reference loop itself
+    c, st = dg.tc()
+    np.testing.assert_array_equal(c, golden["tc_dir"])
+
+
+def test_golden_parents(golden):
+    g = golden
+    dg = dev_from_golden(g)
+    src = int(g["src"])
+    d, pre, _ = dg.bfs_pre(src)
+    np.testing.assert_array_equal(d, g["bfs"])
+    np.testing.assert_array_equal(pre, O.bfs_parent(g["in_rowptr"], g["in_col"], g["bfs"]))
+    dist, pre, _ = dg.sssp_pre(src)
+    np.testing.assert_array_equal(dist, g["sssp"])
+    np.testing.assert_array_equal(pre, O.sssp_parent(g["in_rowptr"], g["in_col"], g["in_w"], g["sssp"], src))
+
+
+@pytest.fixture(scope="module", params=[("rmat", 16, 1), ("rmat", 14, 9), ("er", 14, 4)])
+def medium(request):
+    kind, scale, seed = request.param
+    if kind == "rmat":
+        hg = G.HostGraph.rmat(scale, 16, seed=seed)
+    else:
+        hg = G.HostGraph.er(1 << scale, 16 << scale, seed=seed)
+    o, i, a = hg.view(G.VIEW_OUT), hg.view(G.VIEW_IN), hg.view(G.VIEW_ALL)
+    dg = G.DeviceGraph(o, i, a)
+    src = int(np.argmax(o.degrees()))
+    return dict(hg=hg, o=o, i=i, dg=dg, src=src, name=f"{kind}{scale}")
+
+
+def test_medium_bfs(medium):
+    o, dg, src = medium["o"], medium["dg"], medium["src"]
+    ref = O.bfs(o.rowptr, o.col, src)
+    for sch in SCHEDULES:
+        d, st = dg.bfs(src, schedule=sch)
+        np.testing.assert_array_equal(d, ref, err_msg=f"schedule {sch}")
+        reached = ref != INT_INF
+        assert st.vertices_reached == reached.sum()
+        assert st.edges_traversed == np.diff(o.rowptr)[reached].sum()
+    # a few other sources, including an isolated / sink vertex
+    deg = o.degrees()
+    for s in [0, int(np.argmin(deg)), o.n - 1]:
+        d, _ = dg.bfs(s)
+        np.testing.assert_array_equal(d, O.bfs(o.rowptr, o.col, s))
+
+
+def test_medium_sssp(medium):
+    o, dg, src = medium["o"], medium["dg"], medium["src"]
+    ref = O.sssp(o.rowptr, o.col, o.w, src)
+    for delta in (0.0, 0.05, 1.0):
+        d, st = dg.sssp(src, delta=delta)
+        np.testing.assert_array_equal(d, ref)
+    # a finite `inf` like the reference's default leaves far vertices at inf
+    d, _ = dg.sssp(src, inf=0.5)
+    ref_cap = np.where(ref < 0.5, ref, 0.5)
+    np.testing.assert_array_equal(d, ref_cap)
+
+
+def test_medium_wcc_pr_cdlp(medium):
+    o, i, dg = medium["o"], medium["i"], medium["dg"]
+    np.testing.assert_array_equal(dg.wcc()[0], O.wcc(o.rowptr, o.col))
+    ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 20)
+    r, st = dg.pr(20, 0.85)
+    err = np.abs(r - ref)
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (err.sum(), err.max())
+    np.testing.assert_array_equal(dg.cdlp(10)[0], O.cdlp(i.rowptr, i.col, o.rowptr, o.col, 10))
+
+
+def test_medium_tc(medium):
+    s = medium["hg"].view(G.VIEW_SYM)
+    sg = G.DeviceGraph(s, undirected=True, weights=False)
+    c, st = sg.tc()
+    assert st.extra[0] == 1.0  # symmetric -> DAG form
+    np.testing.assert_array_equal(c, O.tc(s.rowptr, s.col))
+
+
+def test_results_are_deterministic(medium):
+    dg = medium["dg"]
+    a, _ = dg.pr(10)
+    b, _ = dg.pr(10)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(dg.cdlp(5)[0], dg.cdlp(5)[0])
+
+
+def test_edge_cases():
+    # single vertex, no edges
+    dg = G.DeviceGraph(G.Csr(1, np.array([0, 0], np.int32), np.zeros(0, np.int32), np.zeros(0)))
+    assert dg.bfs(0)[0].tolist() == [0]
+    assert dg.sssp(0)[0].tolist() == [0.0]
+    assert dg.wcc()[0].tolist() == [0]
+    np.testing.assert_allclose(dg.pr(5)[0], [1.0])
+    assert dg.cdlp(3)[0].tolist() == [0]
+    assert dg.tc()[0].tolist() == [0]
+    # all isolated
+    n = 100
+    dg = G.DeviceGraph(G.Csr(n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0)))
+    d, _ = dg.bfs(7)
+    assert d[7] == 0 and (d == INT_INF).sum() == n - 1
+    np.testing.assert_array_equal(dg.wcc()[0], np.arange(n))
+    np.testing.assert_array_equal(dg.cdlp(2)[0], np.arange(n))
+    # invalid source
+    with pytest.raises(G.GhetError) as e:
+        dg.bfs(n)
+    assert e.value.code == G.ERR_INVALID
+    with pytest.raises(G.GhetError):
+        dg.sssp(-1)
+
+
+def test_hub_heavy_graph():
+    """Stars with hubs above every tile/LDS threshold (PR long rows, CDLP
+    global hash, BFS hub path)."""
+    n = 20000
+    src = list(range(1, n)) + [0] * (n - 1) + [1] * 3000 + list(range(2, 3002))
+    dst = [0] * (n - 1) + list(range(1, n)) + list(range(2, 3002)) + [1] * 3000
+    w = np.linspace(0.01, 1.0, len(src))
+    hg = G.HostGraph.from_edges(n, src, dst, w)
+    o, i, a = hg.view(G.VIEW_OUT), hg.view(G.VIEW_IN), hg.view(G.VIEW_ALL)
+    dg = G.DeviceGraph(o, i, a)
+    for sch in SCHEDULES:
+        np.testing.assert_array_equal(dg.bfs(5, schedule=sch)[0], O.bfs(o.rowptr, o.col, 5))
+    np.testing.assert_array_equal(dg.sssp(5)[0], O.sssp(o.rowptr, o.col, o.w, 5))
+    ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 10)
+    err = np.abs(dg.pr(10)[0] - ref)
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS
+    np.testing.assert_array_equal(dg.cdlp(4)[0], O.cdlp(i.rowptr, i.col, o.rowptr, o.col, 4))
+    np.testing.assert_array_equal(dg.wcc()[0], O.wcc(o.rowptr, o.col))
+    s = hg.view(G.VIEW_SYM)
+    np.testing.assert_array_equal(G.DeviceGraph(s, undirected=True, weights=False).tc()[0], O.tc(s.rowptr, s.col))
