@@ -356,25 +356,46 @@ __global__ void __launch_bounds__(kBT) k_bitmap_to_queue(int32_t n, const uint32
     }
 }
 
-// Parent = smallest in-neighbour one level up (in-lists are ascending, so the
-// first hit is the smallest).  -1 for the source and unreached vertices.
-__global__ void k_bfs_parent(int32_t n, const int32_t* __restrict__ in_rowptr, const int32_t* __restrict__ in_col,
-                             const int32_t* __restrict__ depth, int32_t* __restrict__ pre) {
-    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
-         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int32_t d = depth[v];
-        int32_t p = -1;
-        if (d != kIntInf && d > 0) {
-            for (int32_t i = in_rowptr[v]; i < in_rowptr[v + 1]; ++i) {
-                int32_t u = in_col[i];
-                if (depth[u] == d - 1) {
-                    p = u;
-                    break;
-                }
+// Parent = smallest u with an edge u->v and depth[u] == depth[v] - 1
+// (atomicMin over out-edges: deterministic).  Lanes expand light rows,
+// the wave expands heavy ones.  pre must hold INT_MAX beforehand.
+__global__ void __launch_bounds__(kBT) k_bfs_parent(int32_t n, const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ depth, int32_t* pre) {
+    const int lane = lane_id();
+    const int64_t groups = ceil_div(n, 64);
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    for (int64_t g = wave_global; g < groups; g += wave_stride) {
+        const int64_t u = g * 64 + lane;
+        int32_t b = 0, e = 0, du = kIntInf;
+        if (u < n) {
+            du = depth[u];
+            if (du != kIntInf) {
+                b = rowptr[u];
+                e = rowptr[u + 1];
             }
         }
-        pre[v] = p;
+        const bool light = (e - b) <= 32;
+        if (light)
+            for (int32_t i = b; i < e; ++i)
+                if (depth[col[i]] == du + 1) atomicMin(&pre[col[i]], static_cast<int32_t>(u));
+        uint64_t heavy = __ballot(!light);
+        while (heavy) {
+            const int src = __ffsll(static_cast<unsigned long long>(heavy)) - 1;
+            heavy &= heavy - 1;
+            const int32_t hb = __shfl(b, src), he = __shfl(e, src), hd = __shfl(du, src);
+            const int32_t hu = static_cast<int32_t>(g * 64 + src);
+            for (int32_t i = hb + lane; i < he; i += 64)
+                if (depth[col[i]] == hd + 1) atomicMin(&pre[col[i]], hu);
+        }
     }
+}
+
+__global__ void k_pre_finish(int32_t* pre, int32_t n) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        if (pre[v] == kIntInf) pre[v] = -1;
 }
 
 unsigned grid_for_waves(int64_t groups) {
@@ -394,7 +415,6 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     int schedule = (o && o->size >= sizeof(ghet_opts)) ? o->schedule : GHET_SCHED_HEURISTIC;
     const bool have_in = g->in.present();
     if (!have_in && (schedule == GHET_SCHED_PULL || schedule == GHET_SCHED_REFERENCE)) schedule = GHET_SCHED_PUSH;
-    if (pre_out && !have_in) return set_error(GHET_ERR_INVALID, "BFS parents need the in-CSR");
 
     const int64_t words = 2 * ceil_div(std::max(n, 1), 64);
     int32_t *depth, *qa, *qb, *scan, *aux, *hubq;
@@ -513,11 +533,13 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         ++level;
     }
     if (pre_out) {
-        int32_t* pre;
         GHET_CHECK(g->ws.get(WS_PRE, 4 * size_t(n), &p));
-        pre = static_cast<int32_t*>(p);
-        hipLaunchKernelGGL(k_bfs_parent, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 8192))),
-                           dim3(256), 0, s, n, g->in.rowptr, g->in.col, depth, pre);
+        int32_t* pre = static_cast<int32_t*>(p);
+        GHET_CHECK(fill<int32_t>(pre, kIntInf, n, s));
+        hipLaunchKernelGGL(k_bfs_parent, dim3(grid_for_waves(words / 2)), dim3(kBT), 0, s, n, g->out.rowptr,
+                           g->out.col, depth, pre);
+        hipLaunchKernelGGL(k_pre_finish, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 8192))),
+                           dim3(256), 0, s, pre, n);
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());

@@ -295,6 +295,15 @@ int ghet_graph_wrap_device(int32_t n, int64_t m_out, const int32_t* d_out_rowptr
     st = pull(g->h_out_rowptr, g->out.rowptr);
     if (st == GHET_OK) st = pull(g->h_in_rowptr, g->in.rowptr);
     if (st == GHET_OK) st = pull(g->h_all_rowptr, g->all.rowptr);
+    // A negative edge count means "take it from the offsets".
+    if (st == GHET_OK && m_out < 0) g->out.m = m_out = g->h_out_rowptr[n];
+    if (st == GHET_OK && g->in.present() && g->directed && m_in < 0) g->in.m = g->h_in_rowptr[n];
+    if (st == GHET_OK && g->all.present() && g->directed && m_all < 0) g->all.m = g->h_all_rowptr[n];
+    if (st == GHET_OK && !g->directed) {
+        g->in = g->out;
+        g->all = g->out;
+        g->all.w = nullptr;
+    }
     if (st == GHET_OK && (g->h_out_rowptr[0] != 0 || g->h_out_rowptr[n] != m_out))
         st = set_error(GHET_ERR_INVALID, "wrap: out offsets do not span %lld edges", (long long)m_out);
     if (st != GHET_OK) { destroy_graph(g); return st; }

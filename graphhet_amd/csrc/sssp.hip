@@ -216,27 +216,53 @@ __global__ void k_far_split(const int32_t* __restrict__ far_in, int32_t count, c
     if (lane_id() == 0 && od) atomicAdd(&ctr->near_edges, od);
 }
 
-// Parent = smallest in-neighbour u != v with d[u] + w(u,v) == d[v].
-__global__ void k_sssp_parent(int32_t n, const int32_t* __restrict__ in_rowptr, const int32_t* __restrict__ in_col,
-                              const double* __restrict__ in_w, const double* __restrict__ dist, double inf,
-                              int32_t src, int32_t* __restrict__ pre) {
-    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
-         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const double dv = dist[v];
-        int32_t p = -1;
-        if (dv != inf && v != src) {
-            for (int32_t i = in_rowptr[v]; i < in_rowptr[v + 1]; ++i) {
-                int32_t u = in_col[i];
-                if (u == v) continue;
-                const double du = dist[u];
-                if (du != inf && du + in_w[i] == dv) {
-                    p = u;
-                    break;
-                }
+// Parent = smallest u != v with an edge u->v and d[u] + w(u,v) == d[v]
+// (atomicMin over out-edges: deterministic).  pre must hold INT_MAX
+// beforehand; the source keeps INT_MAX (mapped to -1 afterwards).
+__global__ void __launch_bounds__(256) k_sssp_parent(int32_t n, const int32_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ col, const double* __restrict__ w,
+                                                     const double* __restrict__ dist, double inf, int32_t src,
+                                                     int32_t* pre) {
+    const int lane = lane_id();
+    const int64_t groups = ceil_div(n, 64);
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * 256) >> 6;
+    for (int64_t g = wave_global; g < groups; g += wave_stride) {
+        const int64_t u = g * 64 + lane;
+        int32_t b = 0, e = 0;
+        double du = inf;
+        if (u < n) {
+            du = dist[u];
+            if (du != inf) {
+                b = rowptr[u];
+                e = rowptr[u + 1];
             }
         }
-        pre[v] = p;
+        const bool light = (e - b) <= 32;
+        if (light)
+            for (int32_t i = b; i < e; ++i) {
+                const int32_t v = col[i];
+                if (v != u && v != src && du + w[i] == dist[v]) atomicMin(&pre[v], static_cast<int32_t>(u));
+            }
+        uint64_t heavy = __ballot(!light);
+        while (heavy) {
+            const int sl = __ffsll(static_cast<unsigned long long>(heavy)) - 1;
+            heavy &= heavy - 1;
+            const int32_t hb = __shfl(b, sl), he = __shfl(e, sl);
+            const double hd = __shfl(du, sl);
+            const int32_t hu = static_cast<int32_t>(g * 64 + sl);
+            for (int32_t i = hb + lane; i < he; i += 64) {
+                const int32_t v = col[i];
+                if (v != hu && v != src && hd + w[i] == dist[v]) atomicMin(&pre[v], hu);
+            }
+        }
     }
+}
+
+__global__ void k_pre_finish(int32_t* pre, int32_t n) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        if (pre[v] == kIntInf) pre[v] = -1;
 }
 
 }  // namespace
@@ -254,7 +280,6 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     if (src < 0 || src >= n) return set_error(GHET_ERR_INVALID, "source %d out of range [0,%d)", src, n);
     if (!g->out.present() || !g->out.w) return set_error(GHET_ERR_INVALID, "SSSP needs the weighted out-CSR");
     if (!(inf > 0) || std::isnan(inf)) return set_error(GHET_ERR_INVALID, "inf must be > 0");
-    if (pre_out && (!g->in.present() || !g->in.w)) return set_error(GHET_ERR_INVALID, "SSSP parents need the weighted in-CSR");
     hipStream_t s = pick_stream(g, o);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
     const int schedule = have_opts ? o->schedule : GHET_SCHED_HEURISTIC;
@@ -391,9 +416,13 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     }
     if (pre_out) {
         GHET_CHECK(g->ws.get(WS_PRE, 4 * size_t(n), &p));
-        hipLaunchKernelGGL(k_sssp_parent, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 8192))),
-                           dim3(256), 0, s, n, g->in.rowptr, g->in.col, g->in.w, dist, inf, src,
-                           static_cast<int32_t*>(p));
+        int32_t* pre = static_cast<int32_t*>(p);
+        GHET_CHECK(fill<int32_t>(pre, kIntInf, n, s));
+        const unsigned gw = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096)));
+        hipLaunchKernelGGL(k_sssp_parent, dim3(gw), dim3(256), 0, s, n, g->out.rowptr, g->out.col, g->out.w, dist, inf,
+                           src, pre);
+        hipLaunchKernelGGL(k_pre_finish, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 8192))),
+                           dim3(256), 0, s, pre, n);
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());

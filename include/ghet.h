@@ -109,8 +109,9 @@ int ghet_graph_create(int32_t n,
                       uint32_t flags, ghet_graph** out);
 /* Borrow existing device CSR arrays (the raw-core signatures of the reference
  * take device pointers: GPU_BFS_csr.cuh:258, GPU_SSSP_csr.cuh:253,
- * GPU_TC_csr.cuh:46).  Any pointer group may be NULL.  Nothing is freed on
- * destroy except the library's own derived layouts and workspace. */
+ * GPU_TC_csr.cuh:46).  Any pointer group except out may be NULL; a negative
+ * edge count is read from the group's offsets.  Nothing is freed on destroy
+ * except the library's own derived layouts and workspace. */
 int ghet_graph_wrap_device(int32_t n,
                            int64_t m_out, const int32_t* d_out_rowptr, const int32_t* d_out_col,
                            const double* d_out_w,
