@@ -28,7 +28,7 @@ enum WsSlot {
     WS_DEPTH, WS_VISITED, WS_FRONT_A, WS_FRONT_B, WS_QUEUE_A, WS_QUEUE_B, WS_SCAN, WS_SCAN_AUX,
     WS_COUNTERS, WS_DIST, WS_STAMP_NEAR, WS_STAMP_FAR, WS_FAR_A, WS_FAR_B, WS_PARENT,
     WS_PR_CONTRIB, WS_PR_RANK, WS_PR_PART, WS_PR_LONGPART, WS_LABEL_A, WS_LABEL_B, WS_HASH,
-    WS_TC_COUNT, WS_PRE, WS_MULTI_A, WS_MULTI_B, WS_MULTI_C, WS_MULTI_D, WS_MISC,
+    WS_TC_COUNT, WS_PRE, WS_PR_OUT, WS_MULTI_A, WS_MULTI_B, WS_MULTI_C, WS_MULTI_D, WS_MISC,
     WS_NUM_SLOTS
 };
 
@@ -56,7 +56,10 @@ struct PRLayout {
     int32_t* long_row = nullptr;     // [num_long_rows] row ids
     int32_t* long_chunk0 = nullptr;  // [num_long_rows+1] first chunk of each long row
     int32_t* long_done = nullptr;    // [num_long_rows] arrival counters (self-resetting)
-    int32_t* outdeg = nullptr;       // [n] out-degree (sinks have 0)
+    int32_t* outdeg = nullptr;       // [n] out-degree of relabeled vertex r (sinks have 0)
+    int32_t* perm = nullptr;         // [n] relabeled id -> caller's id (out-degree descending)
+    int32_t* rowptr = nullptr;       // [n+1] relabeled in-CSR offsets
+    int32_t* col = nullptr;          // [m] relabeled in-CSR sources (relabeled ids)
     void* mem = nullptr;
 };
 
