@@ -57,7 +57,11 @@ def main():
     ap.add_argument("--algos", default="bfs,sssp,pr,wcc,cdlp,tc")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-s", type=float, default=20.0, help="budget for the CPU baseline")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one plain device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mb:
+        copy_probe(args.copy_probe_mb)
 
     if args.gpus != 1:
         log("bench.py measures one GPU: this build has no multi-GPU path "
@@ -160,6 +164,25 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_probe(mb: int) -> None:
+    """One hipMemcpy device-to-device of `mb` MiB (the runtime's copy kernel):
+    a plain copy of known size, to scale FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(2):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: 2 x {mb} MiB device-to-device")
 
 
 def cpu_baseline(o, i, src, args, gpu_stats):

@@ -23,18 +23,23 @@
 namespace ghet {
 namespace {
 
+// Per-level counters.  qcount/hubs hand out queue positions (one atomic per
+// LDS flush / per hub); the sums are spread over kSlots slots:
+// v[0] vertices discovered, v[1] their out-edges, v[2] their in-edges.
 struct BfsCtr {
-    int32_t count;
+    int32_t qcount;
     int32_t hubs;
-    unsigned long long out_edges;
-    unsigned long long in_edges;
-    unsigned long long pad;
+    int32_t pad[14];
+    CtrSlot slots[kSlots];
 };
+constexpr int kCtrWords = sizeof(BfsCtr) / 8;
 
 constexpr int kBT = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBT * kItems;  // edges per push tile
+constexpr int kOut = 2 * kTile;      // LDS output queue capacity
 constexpr int kHubDeg = 4096;
+constexpr int kMaxBlocks = 2048;     // persistent grid cap (8 blocks per CU)
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
     int lo = 0, hi = len;
@@ -55,12 +60,8 @@ __global__ void k_seed(int32_t* depth, uint32_t* visited, int32_t* queue, int32_
 }
 
 __global__ void k_zero_ctr(BfsCtr* c) {
-    if (threadIdx.x == 0) {
-        c->count = 0;
-        c->hubs = 0;
-        c->out_edges = 0;
-        c->in_edges = 0;
-    }
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(c);
+    for (int i = threadIdx.x; i < kCtrWords; i += blockDim.x) w[i] = 0;
 }
 
 __global__ void k_frontier_deg(const int32_t* __restrict__ q, int32_t F, const int32_t* __restrict__ rowptr,
@@ -72,30 +73,21 @@ __global__ void k_frontier_deg(const int32_t* __restrict__ q, int32_t F, const i
     }
 }
 
-// Block-wide sum of two 64-bit values, added atomically to the counters.
-__device__ __forceinline__ void block_add_edges(unsigned long long od, unsigned long long id, BfsCtr* ctr) {
-    __shared__ unsigned long long s_red[2][kBT / 64];
-    od = wave_sum(od);
-    id = wave_sum(id);
-    int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        s_red[0][w] = od;
-        s_red[1][w] = id;
-    }
-    __syncthreads();
+// Block totals of (count, out-edges, in-edges) into the block's slot.
+__device__ __forceinline__ void block_report(unsigned long long cnt, unsigned long long od, unsigned long long id,
+                                             BfsCtr* ctr, unsigned long long* s_red) {
+    cnt = block_sum_u64<kBT>(cnt, s_red);
+    od = block_sum_u64<kBT>(od, s_red);
+    id = block_sum_u64<kBT>(id, s_red);
     if (threadIdx.x == 0) {
-        unsigned long long a = 0, b = 0;
-        for (int i = 0; i < kBT / 64; ++i) {
-            a += s_red[0][i];
-            b += s_red[1][i];
-        }
-        if (a) atomicAdd(&ctr->out_edges, a);
-        if (b) atomicAdd(&ctr->in_edges, b);
+        slot_add(ctr->slots, 0, cnt);
+        slot_add(ctr->slots, 1, od);
+        slot_add(ctr->slots, 2, id);
     }
-    __syncthreads();
 }
 
-// PUSH: one tile of kTile consecutive frontier edges per block.
+// PUSH: persistent blocks over tiles of kTile consecutive frontier edges.
+// Discoveries are staged in LDS and flushed to the next queue in bulk.
 __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__ q, int32_t F,
                                                      const int32_t* __restrict__ scan, int32_t E,
                                                      const int32_t* __restrict__ rowptr,
@@ -105,9 +97,17 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
                                                      const int32_t* __restrict__ in_rowptr) {
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
-    __shared__ int32_t s_out[kTile];
+    __shared__ int32_t s_out[kOut];
     __shared__ int32_t s_i0, s_span, s_cnt, s_base;
+    __shared__ unsigned long long s_red[kBT / 64];
     const int tid = threadIdx.x;
+    if (tid == 0) s_cnt = 0;
+    unsigned long long cnt = 0, od = 0, id = 0;
+    auto on_item = [&](int32_t v) {
+        ++cnt;
+        od += rowptr[v + 1] - rowptr[v];
+        if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
+    };
     for (int64_t tile = blockIdx.x; tile * kTile < E; tile += gridDim.x) {
         const int32_t t0 = static_cast<int32_t>(tile * kTile);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
@@ -116,7 +116,6 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
             int i1 = upper_bound_i32(scan, F + 1, t1 - 1) - 1;
             s_i0 = i0;
             s_span = i1 - i0 + 1;
-            s_cnt = 0;
         }
         __syncthreads();
         const int i0 = s_i0, span = s_span;
@@ -151,44 +150,28 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
                     }
                 }
             }
-            uint64_t mask = __ballot(disc);
-            if (mask) {
-                int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-                int base = 0;
-                if (lane_id() == leader) base = atomicAdd(&s_cnt, __popcll(mask));
-                base = __shfl(base, leader);
-                if (disc) s_out[base + lanes_below(mask)] = v;
-            }
+            lq_push(s_out, &s_cnt, disc, v);
         }
-        __syncthreads();
-        if (tid == 0 && s_cnt) s_base = atomicAdd(&ctr->count, s_cnt);
-        __syncthreads();
-        unsigned long long od = 0, id = 0;
-        const int cnt = s_cnt, base = s_base;
-        for (int j = tid; j < cnt; j += kBT) {
-            int32_t v = s_out[j];
-            next_q[base + j] = v;
-            od += rowptr[v + 1] - rowptr[v];
-            if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
-        }
-        block_add_edges(od, id, ctr);
+        lq_flush<kBT>(s_out, &s_cnt, &s_base, &ctr->qcount, next_q, kOut - kTile, on_item);
     }
+    lq_flush<kBT>(s_out, &s_cnt, &s_base, &ctr->qcount, next_q, 0, on_item);
+    block_report(cnt, od, id, ctr, s_red);
 }
 
-// PULL: bottom-up over the in-CSR.  One wave = 64 vertices = words 2g, 2g+1.
+// PULL: bottom-up over the in-CSR.  Each block iteration covers 256
+// consecutive vertices; wave w owns 64 of them = bitmap words 2g, 2g+1, so the
+// next frontier and the visited bits are written without atomics.
 __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restrict__ in_rowptr,
                                               const int32_t* __restrict__ in_col, const uint32_t* __restrict__ cur_bm,
                                               uint32_t* __restrict__ next_bm, uint32_t* __restrict__ visited,
                                               int32_t* __restrict__ depth, int32_t next_level, BfsCtr* ctr,
                                               const int32_t* __restrict__ out_rowptr) {
+    __shared__ unsigned long long s_red[kBT / 64];
     const int lane = lane_id();
-    const int64_t groups = ceil_div(n, 64);
-    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
-    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
-    unsigned long long od = 0, id = 0;
-    int cnt = 0;
-    for (int64_t g = wave_global; g < groups; g += wave_stride) {
-        const int64_t v = g * 64 + lane;
+    unsigned long long od = 0, id = 0, cnt = 0;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < n; base += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int64_t v = base + threadIdx.x;
+        const int64_t g = v >> 6;
         const uint32_t vis = visited[2 * g + (lane >> 5)];
         bool found = false;
         if (v < n && !((vis >> (lane & 31)) & 1u)) {
@@ -211,18 +194,12 @@ __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restri
         }
         if (found) {
             depth[v] = next_level;
+            ++cnt;
             od += out_rowptr[v + 1] - out_rowptr[v];
             id += in_rowptr[v + 1] - in_rowptr[v];
         }
-        cnt += __popcll(mask);
     }
-    od = wave_sum(od);
-    id = wave_sum(id);
-    if (lane == 0 && cnt) {
-        atomicAdd(&ctr->count, cnt);
-        atomicAdd(&ctr->out_edges, od);
-        atomicAdd(&ctr->in_edges, id);
-    }
+    block_report(cnt, od, id, ctr, s_red);
 }
 
 __device__ __forceinline__ bool visit(int32_t v, uint32_t* visited, uint32_t* next_bm, int32_t* depth,
@@ -237,22 +214,25 @@ __device__ __forceinline__ bool visit(int32_t v, uint32_t* visited, uint32_t* ne
 }
 
 // BITMAP: top-down from a frontier bitmap.  Light vertices (deg <= 32) are
-// expanded by their own lane, medium ones by the wave, hubs are queued.
+// expanded by their own lane, medium ones by the wave, hubs are queued for
+// k_hub_expand.
 __global__ void __launch_bounds__(kBT) k_bitmap_topdown(int32_t n, const int32_t* __restrict__ rowptr,
                                                         const int32_t* __restrict__ col,
                                                         const uint32_t* __restrict__ cur_bm, uint32_t* next_bm,
                                                         uint32_t* visited, int32_t* __restrict__ depth,
                                                         int32_t next_level, BfsCtr* ctr, int32_t* hub_q,
                                                         const int32_t* __restrict__ in_rowptr) {
+    __shared__ unsigned long long s_red[kBT / 64];
     const int lane = lane_id();
-    const int64_t groups = ceil_div(n, 64);
-    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
-    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
-    unsigned long long od = 0, id = 0;
-    int cnt = 0;
-    for (int64_t g = wave_global; g < groups; g += wave_stride) {
-        const uint32_t word = cur_bm[2 * g + (lane >> 5)];
-        const int64_t u = g * 64 + lane;
+    unsigned long long od = 0, id = 0, cnt = 0;
+    auto count = [&](int32_t v) {
+        ++cnt;
+        od += rowptr[v + 1] - rowptr[v];
+        if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
+    };
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < n; base += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int64_t u = base + threadIdx.x;
+        const uint32_t word = cur_bm[2 * (u >> 6) + (lane >> 5)];
         const bool in_f = (word >> (lane & 31)) & 1u;
         if (__ballot(in_f) == 0) continue;
         int32_t b = 0, e = 0;
@@ -263,20 +243,12 @@ __global__ void __launch_bounds__(kBT) k_bitmap_topdown(int32_t n, const int32_t
         const int32_t deg = e - b;
         const bool light = in_f && deg <= 32;
         const bool hub = in_f && deg > kHubDeg;
-        if (light) {
+        if (light)
             for (int32_t i = b; i < e; ++i) {
                 int32_t v = col[i];
-                if (visit(v, visited, next_bm, depth, next_level)) {
-                    ++cnt;
-                    od += rowptr[v + 1] - rowptr[v];
-                    if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
-                }
+                if (visit(v, visited, next_bm, depth, next_level)) count(v);
             }
-        }
-        if (hub) {
-            int pos = atomicAdd(&ctr->hubs, 1);
-            hub_q[pos] = static_cast<int32_t>(u);
-        }
+        if (hub) hub_q[atomicAdd(&ctr->hubs, 1)] = static_cast<int32_t>(u);
         uint64_t med = __ballot(in_f && !light && !hub);
         while (med) {
             const int src_lane = __ffsll(static_cast<unsigned long long>(med)) - 1;
@@ -284,22 +256,11 @@ __global__ void __launch_bounds__(kBT) k_bitmap_topdown(int32_t n, const int32_t
             const int32_t mb = __shfl(b, src_lane), me = __shfl(e, src_lane);
             for (int32_t i = mb + lane; i < me; i += 64) {
                 int32_t v = col[i];
-                if (visit(v, visited, next_bm, depth, next_level)) {
-                    ++cnt;
-                    od += rowptr[v + 1] - rowptr[v];
-                    if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
-                }
+                if (visit(v, visited, next_bm, depth, next_level)) count(v);
             }
         }
     }
-    od = wave_sum(od);
-    id = wave_sum(id);
-    cnt = wave_sum(cnt);
-    if (lane == 0 && cnt) {
-        atomicAdd(&ctr->count, cnt);
-        atomicAdd(&ctr->out_edges, od);
-        atomicAdd(&ctr->in_edges, id);
-    }
+    block_report(cnt, od, id, ctr, s_red);
 }
 
 // Hub vertices queued by k_bitmap_topdown: every block takes a slice of every hub.
@@ -308,9 +269,9 @@ __global__ void __launch_bounds__(kBT) k_hub_expand(const int32_t* __restrict__ 
                                                     uint32_t* visited, int32_t* __restrict__ depth,
                                                     int32_t next_level, BfsCtr* ctr, const int32_t* hub_q,
                                                     const int32_t* __restrict__ in_rowptr) {
+    __shared__ unsigned long long s_red[kBT / 64];
     const int32_t H = ctr->hubs;
-    unsigned long long od = 0, id = 0;
-    int cnt = 0;
+    unsigned long long od = 0, id = 0, cnt = 0;
     for (int32_t h = 0; h < H; ++h) {
         const int32_t u = hub_q[h];
         const int32_t b = rowptr[u], e = rowptr[u + 1];
@@ -324,14 +285,7 @@ __global__ void __launch_bounds__(kBT) k_hub_expand(const int32_t* __restrict__ 
             }
         }
     }
-    od = wave_sum(od);
-    id = wave_sum(id);
-    cnt = wave_sum(cnt);
-    if (lane_id() == 0 && cnt) {
-        atomicAdd(&ctr->count, cnt);
-        atomicAdd(&ctr->out_edges, od);
-        atomicAdd(&ctr->in_edges, id);
-    }
+    block_report(cnt, od, id, ctr, s_red);
 }
 
 __global__ void k_queue_to_bitmap(const int32_t* __restrict__ q, int32_t F, uint32_t* bm) {
@@ -342,18 +296,21 @@ __global__ void k_queue_to_bitmap(const int32_t* __restrict__ q, int32_t F, uint
     }
 }
 
+// Frontier bitmap -> queue, staged in LDS (one global atomic per flush).
 __global__ void __launch_bounds__(kBT) k_bitmap_to_queue(int32_t n, const uint32_t* __restrict__ bm,
                                                          int32_t* __restrict__ q, int32_t* counter) {
-    const int lane = lane_id();
-    const int64_t groups = ceil_div(n, 64);
-    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
-    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
-    for (int64_t g = wave_global; g < groups; g += wave_stride) {
-        const uint32_t word = bm[2 * g + (lane >> 5)];
-        const bool set = (word >> (lane & 31)) & 1u;
-        int pos = wave_append(counter, set);
-        if (set) q[pos] = static_cast<int32_t>(g * 64 + lane);
+    __shared__ int32_t s_out[kOut];
+    __shared__ int32_t s_cnt, s_base;
+    if (threadIdx.x == 0) s_cnt = 0;
+    auto nop = [](int32_t) {};
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < n; base += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int64_t v = base + threadIdx.x;
+        const uint32_t word = bm[2 * (v >> 6) + (lane_id() >> 5)];
+        const bool set = v < n && ((word >> (lane_id() & 31)) & 1u);
+        lq_push(s_out, &s_cnt, set, static_cast<int32_t>(v));
+        lq_flush<kBT>(s_out, &s_cnt, &s_base, counter, q, kOut - kBT, nop);
     }
+    lq_flush<kBT>(s_out, &s_cnt, &s_base, counter, q, 0, nop);
 }
 
 // Parent = smallest u with an edge u->v and depth[u] == depth[v] - 1
@@ -398,10 +355,8 @@ __global__ void k_pre_finish(int32_t* pre, int32_t n) {
         if (pre[v] == kIntInf) pre[v] = -1;
 }
 
-unsigned grid_for_waves(int64_t groups) {
-    // one wave per 64-vertex group, capped: 256 CUs x 8 blocks of 4 waves
-    int64_t blocks = ceil_div(groups, kBT / 64);
-    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 256 * 16)));
+unsigned grid_for_vertices(int64_t n) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBT), 4096)));
 }
 
 }  // namespace
@@ -416,7 +371,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     const bool have_in = g->in.present();
     if (!have_in && (schedule == GHET_SCHED_PULL || schedule == GHET_SCHED_REFERENCE)) schedule = GHET_SCHED_PUSH;
 
-    const int64_t words = 2 * ceil_div(std::max(n, 1), 64);
+    // 8 words per 256 vertices: block iterations of 256 never index past the end
+    const int64_t words = 8 * ceil_div(std::max(n, 1), 256);
     int32_t *depth, *qa, *qb, *scan, *aux, *hubq;
     uint32_t *visited, *bma, *bmb;
     BfsCtr* ctr;
@@ -430,13 +386,14 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_MISC, 4 * size_t(n) + 256, &p)); hubq = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_COUNTERS, 256, &p)); ctr = static_cast<BfsCtr*>(p);
+    GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(BfsCtr), &p)); ctr = static_cast<BfsCtr*>(p);
 
     SchedState sched;
     sched_begin(&sched, schedule, GHET_NET_BFS, n, g->out.m, have_in ? g->in.m : g->out.m);
     ghet_level_trace* trace = (o && o->size >= sizeof(ghet_opts)) ? o->trace : nullptr;
     const int trace_cap = trace ? o->trace_cap : 0;
     int trace_len = 0;
+    const BfsCtr* hc = reinterpret_cast<const BfsCtr*>(g->pinned);
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     GHET_CHECK(fill<int32_t>(depth, kIntInf, n, s));
@@ -466,25 +423,25 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         li.prev_mode = prev_mode;
         li.have_in = have_in;
         int mode = sched_pick_bfs(&sched, li);
-        if (trace && trace_len < trace_cap) sched_features(li, trace[trace_len].features);
-        if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k0, s));
-
-        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, ctr);
+        const bool tr = trace && trace_len < trace_cap;
+        if (tr) {
+            sched_features(li, trace[trace_len].features);
+            GHET_HIP(hipEventRecord(g->ev_k0, s));
+        }
         const int32_t next_level = level + 1;
+        if (mode == GHET_MODE_PUSH && !have_q) {
+            hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(512), 0, s, ctr);
+            hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, bma, qa, &ctr->qcount);
+            have_q = true;
+        }
+        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(512), 0, s, ctr);
         if (mode == GHET_MODE_PUSH) {
-            if (!have_q) {
-                GHET_CHECK(fill<int32_t>(&ctr->count, 0, 1, s));
-                hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid_for_waves(words / 2)), dim3(kBT), 0, s, n, bma,
-                                   qa, &ctr->count);
-                hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, s, ctr);
-                have_q = true;
-            }
             hipLaunchKernelGGL(k_frontier_deg, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s, qa, F,
                                g->out.rowptr, scan);
             GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
             const int32_t E = static_cast<int32_t>(m_f);
             if (E > 0) {
-                unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), 65535));
+                unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), kMaxBlocks));
                 hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, qa, F, scan, E, g->out.rowptr,
                                    g->out.col, visited, depth, next_level, qb, ctr, in_rp);
             }
@@ -499,12 +456,12 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
                 have_bm = true;
             }
             if (mode == GHET_MODE_PULL) {
-                hipLaunchKernelGGL(k_pull, dim3(grid_for_waves(words / 2)), dim3(kBT), 0, s, n, g->in.rowptr,
-                                   g->in.col, bma, bmb, visited, depth, next_level, ctr, g->out.rowptr);
+                hipLaunchKernelGGL(k_pull, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->in.rowptr, g->in.col,
+                                   bma, bmb, visited, depth, next_level, ctr, g->out.rowptr);
             } else {
                 GHET_CHECK(fill<uint32_t>(bmb, 0u, words, s));
-                hipLaunchKernelGGL(k_bitmap_topdown, dim3(grid_for_waves(words / 2)), dim3(kBT), 0, s, n,
-                                   g->out.rowptr, g->out.col, bma, bmb, visited, depth, next_level, ctr, hubq, in_rp);
+                hipLaunchKernelGGL(k_bitmap_topdown, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->out.rowptr,
+                                   g->out.col, bma, bmb, visited, depth, next_level, ctr, hubq, in_rp);
                 hipLaunchKernelGGL(k_hub_expand, dim3(1024), dim3(kBT), 0, s, g->out.rowptr, g->out.col, bmb, visited,
                                    depth, next_level, ctr, hubq, in_rp);
             }
@@ -514,10 +471,9 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         }
         GHET_HIP(hipGetLastError());
         GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
-        if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k1, s));
+        if (tr) GHET_HIP(hipEventRecord(g->ev_k1, s));
         GHET_HIP(hipStreamSynchronize(s));
-        const BfsCtr* hc = reinterpret_cast<const BfsCtr*>(g->pinned);
-        if (trace && trace_len < trace_cap) {
+        if (tr) {
             float ms = 0;
             GHET_HIP(hipEventElapsedTime(&ms, g->ev_k0, g->ev_k1));
             trace[trace_len].mode = mode;
@@ -526,9 +482,9 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         }
         prev_F = F;
         prev_mode = mode;
-        F = hc->count;
-        m_f = static_cast<int64_t>(hc->out_edges);
-        visited_in += static_cast<int64_t>(hc->in_edges);
+        F = static_cast<int32_t>(slot_sum(hc->slots, 0));
+        m_f = static_cast<int64_t>(slot_sum(hc->slots, 1));
+        visited_in += static_cast<int64_t>(slot_sum(hc->slots, 2));
         n_visited += F;
         ++level;
     }
@@ -536,18 +492,18 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         GHET_CHECK(g->ws.get(WS_PRE, 4 * size_t(n), &p));
         int32_t* pre = static_cast<int32_t*>(p);
         GHET_CHECK(fill<int32_t>(pre, kIntInf, n, s));
-        hipLaunchKernelGGL(k_bfs_parent, dim3(grid_for_waves(words / 2)), dim3(kBT), 0, s, n, g->out.rowptr,
-                           g->out.col, depth, pre);
-        hipLaunchKernelGGL(k_pre_finish, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 8192))),
-                           dim3(256), 0, s, pre, n);
+        hipLaunchKernelGGL(k_bfs_parent, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->out.rowptr, g->out.col,
+                           depth, pre);
+        hipLaunchKernelGGL(k_pre_finish, dim3(grid_for_vertices(n)), dim3(256), 0, s, pre, n);
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
     if (depth_out) GHET_HIP(hipMemcpyAsync(depth_out, depth, 4 * size_t(n), hipMemcpyDeviceToHost, s));
     if (pre_out) GHET_HIP(hipMemcpyAsync(pre_out, g->ws.ptr[WS_PRE], 4 * size_t(n), hipMemcpyDeviceToHost, s));
     if (st) {
-        GHET_CHECK(reached_stats_i32(depth, g->out.rowptr, n, reinterpret_cast<int64_t*>(scan), s));
-        GHET_HIP(hipMemcpyAsync(g->pinned, scan, 16, hipMemcpyDeviceToHost, s));
+        CtrSlot* slots = reinterpret_cast<CtrSlot*>(ctr);
+        GHET_CHECK(reached_stats_i32(depth, g->out.rowptr, n, slots, s));
+        GHET_HIP(hipMemcpyAsync(g->pinned, slots, sizeof(CtrSlot) * kSlots, hipMemcpyDeviceToHost, s));
     }
     GHET_HIP(hipStreamSynchronize(s));
     if (st) {
@@ -555,9 +511,9 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         GHET_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
         st->device_ms = ms;
         st->levels = level;
-        const int64_t* rc = reinterpret_cast<const int64_t*>(g->pinned);
-        st->vertices_reached = rc[0];
-        st->edges_traversed = rc[1];
+        const CtrSlot* hs = reinterpret_cast<const CtrSlot*>(g->pinned);
+        st->vertices_reached = static_cast<int64_t>(slot_sum(hs, 0));
+        st->edges_traversed = static_cast<int64_t>(slot_sum(hs, 1));
         st->trace_len = trace_len;
         st->status_flags = sched.fallback ? 1 : 0;
     }

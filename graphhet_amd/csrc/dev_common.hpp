@@ -103,4 +103,71 @@ struct Kahan32 {
 
 __host__ __device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// ---------------------------------------------------------------------------
+// Device-scope atomics on one address serialise across the 8 XCDs (measured
+// ~10 ns each on MI355X), so counters that many blocks bump are spread over
+// kSlots cache-line-sized slots (block b uses slot b % kSlots) and summed on
+// the host.
+constexpr int kSlots = 64;
+struct alignas(64) CtrSlot {
+    unsigned long long v[8];
+};
+
+__device__ __forceinline__ void slot_add(CtrSlot* slots, int field, unsigned long long x) {
+    if (x) atomicAdd(&slots[blockIdx.x & (kSlots - 1)].v[field], x);
+}
+
+inline unsigned long long slot_sum(const CtrSlot* host_slots, int field) {
+    unsigned long long s = 0;
+    for (int i = 0; i < kSlots; ++i) s += host_slots[i].v[field];
+    return s;
+}
+
+// Block-wide sum of a 64-bit value (all threads call; result valid everywhere).
+template <int BT>
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long x, unsigned long long* s_red) {
+    x = wave_sum(x);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    unsigned long long t = 0;
+    for (int w = 0; w < BT / 64; ++w) t += s_red[w];
+    __syncthreads();
+    return t;
+}
+
+// LDS-staged output queue.  lq_push appends the lanes with pred (one LDS
+// atomic per wave); lq_flush (block-uniform) moves the staged items to a
+// global queue with ONE global atomic, calling on_item(v) for each item.
+__device__ __forceinline__ void lq_push(int32_t* buf, int* cnt, bool pred, int32_t v) {
+    const uint64_t mask = __ballot(pred);
+    if (!mask) return;
+    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(cnt, __popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) buf[base + lanes_below(mask)] = v;
+}
+
+template <int BT, typename OnItem>
+__device__ __forceinline__ void lq_flush(int32_t* buf, int* cnt, int* s_base, int* gcount, int32_t* gqueue, int limit,
+                                         OnItem&& on_item) {
+    __syncthreads();
+    const int c = *cnt;
+    if (c == 0 || c <= limit) {
+        __syncthreads();
+        return;
+    }
+    if (threadIdx.x == 0) *s_base = atomicAdd(gcount, c);
+    __syncthreads();
+    const int base = *s_base;
+    for (int j = threadIdx.x; j < c; j += BT) {
+        const int32_t v = buf[j];
+        gqueue[base + j] = v;
+        on_item(v);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *cnt = 0;
+    __syncthreads();
+}
+
 }  // namespace ghet

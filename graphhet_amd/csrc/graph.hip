@@ -99,7 +99,7 @@ int init_runtime(ghet_graph* g) {
     GHET_HIP(hipEventCreate(&g->ev_end));
     GHET_HIP(hipEventCreate(&g->ev_k0));
     GHET_HIP(hipEventCreate(&g->ev_k1));
-    GHET_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->pinned), 4096, hipHostMallocDefault));
+    GHET_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->pinned), kPinnedBytes, hipHostMallocDefault));
     return GHET_OK;
 }
 

@@ -12,6 +12,8 @@
 
 namespace ghet {
 
+constexpr size_t kPinnedBytes = 65536;
+
 // A CSR resident in device memory.  rowptr has n+1 entries.
 struct DevCSR {
     int32_t n = 0;
@@ -105,7 +107,7 @@ struct ghet_graph {
     ghet::TCLayout tc;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
-    int32_t* pinned = nullptr;  // small pinned host buffer for per-level counters (64 ints)
+    int32_t* pinned = nullptr;  // pinned host buffer (kPinnedBytes) for per-level counters
     std::vector<hipEvent_t> kevents;  // per-launch timing events, created on demand
 };
 
@@ -132,9 +134,9 @@ int64_t scan_aux_ints(int64_t n);
 int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* aux, hipStream_t s);
 
 // Reached-vertex statistics: n_r and m_r (sum of out-degrees) of vertices with
-// depth != INT_MAX (or dist != inf).  Results in counters[0..1] (int64).
-int reached_stats_i32(const int32_t* depth, const int32_t* rowptr, int32_t n, int64_t* counters, hipStream_t s);
-int reached_stats_f64(const double* dist, double inf, const int32_t* rowptr, int32_t n, int64_t* counters,
+// depth != INT_MAX (or dist != inf), into slot fields 0 and 1 (host sums them).
+int reached_stats_i32(const int32_t* depth, const int32_t* rowptr, int32_t n, CtrSlot* slots, hipStream_t s);
+int reached_stats_f64(const double* dist, double inf, const int32_t* rowptr, int32_t n, CtrSlot* slots,
                       hipStream_t s);
 
 template <typename T>

@@ -25,17 +25,23 @@
 namespace ghet {
 namespace {
 
+// Queue positions (one atomic per LDS flush) plus spread sums:
+// slot v[0] out-edges of the next near queue, v[1] bits of the smallest kept
+// far distance (atomicMin on non-negative doubles' bit patterns).
 struct SsspCtr {
     int32_t near_count;
     int32_t far_count[2];
-    int32_t pad;
-    unsigned long long near_edges;
-    unsigned long long far_min_bits;
+    int32_t pad[13];
+    CtrSlot slots[kSlots];
 };
+constexpr int kCtrWords = sizeof(SsspCtr) / 8;
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ULL;
 
 constexpr int kBT = 256;
-constexpr int kItems = 8;
-constexpr int kTile = kBT * kItems;
+constexpr int kItems = 4;
+constexpr int kTile = kBT * kItems;  // edges per relax tile
+constexpr int kOut = 2 * kTile;      // LDS capacity of each staged queue
+constexpr int kMaxBlocks = 2048;
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
     int lo = 0, hi = len;
@@ -55,18 +61,14 @@ __global__ void k_sssp_seed(double* dist, int32_t* queue, int32_t* stamp_near, i
     }
 }
 
-__global__ void k_round_reset(SsspCtr* c) {
+// Resets the near queue and the spread sums; keeps the far counts (round) or
+// also clears far list `other` (split).
+__global__ void k_ctr_reset(SsspCtr* c, int other) {
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(c->slots);
+    for (int i = threadIdx.x; i < kSlots * 8; i += blockDim.x) w[i] = (i % 8 == 1) ? kInfBits : 0ULL;
     if (threadIdx.x == 0) {
         c->near_count = 0;
-        c->near_edges = 0;
-    }
-}
-__global__ void k_split_reset(SsspCtr* c, int other) {
-    if (threadIdx.x == 0) {
-        c->near_count = 0;
-        c->near_edges = 0;
-        c->far_count[other] = 0;
-        c->far_min_bits = 0x7FF0000000000000ULL;  // +inf
+        if (other >= 0) c->far_count[other] = 0;
     }
 }
 
@@ -79,6 +81,7 @@ __global__ void k_frontier_deg(const int32_t* __restrict__ q, int32_t F, const i
     }
 }
 
+// One round: persistent blocks over kTile-edge tiles of the near queue.
 __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, int32_t F,
                                                const int32_t* __restrict__ scan, int32_t E,
                                                const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -89,10 +92,17 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
     __shared__ int32_t s_src[kTile];
-    __shared__ int32_t s_out[kTile];
-    __shared__ int32_t s_i0, s_span, s_cnt, s_base;
+    __shared__ int32_t s_near[kOut], s_far[kOut];
+    __shared__ int32_t s_i0, s_span, s_nc, s_fc, s_base;
     __shared__ unsigned long long s_red[kBT / 64];
     const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_nc = 0;
+        s_fc = 0;
+    }
+    unsigned long long od = 0;
+    auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
+    auto nop = [](int32_t) {};
     for (int64_t tile = blockIdx.x; tile * kTile < E; tile += gridDim.x) {
         const int32_t t0 = static_cast<int32_t>(tile * kTile);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
@@ -101,7 +111,6 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
             int i1 = upper_bound_i32(scan, F + 1, t1 - 1) - 1;
             s_i0 = i0;
             s_span = i1 - i0 + 1;
-            s_cnt = 0;
         }
         __syncthreads();
         const int i0 = s_i0, span = s_span;
@@ -117,7 +126,7 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
         __syncthreads();
         for (int k = 0; k < kItems; ++k) {
             const int32_t e = t0 + k * kBT + tid;
-            bool to_near = false;
+            bool to_near = false, to_far = false;
             int32_t v = 0;
             if (e < t1) {
                 int32_t pos, u;
@@ -135,62 +144,49 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
                 if (nd < dist[v]) {
                     const double old = atomic_min_nonneg(&dist[v], nd);
                     if (nd < old) {
-                        if (nd < threshold) {
-                            if (atomicExch(&stamp_near[v], next_round) != next_round) to_near = true;
-                        } else if (atomicExch(&stamp_far[v], far_gen) != far_gen) {
-                            int p = atomicAdd(&ctr->far_count[far_sel], 1);
-                            far_list[p] = v;
-                        }
+                        if (nd < threshold) to_near = atomicExch(&stamp_near[v], next_round) != next_round;
+                        else to_far = atomicExch(&stamp_far[v], far_gen) != far_gen;
                     }
                 }
             }
-            uint64_t mask = __ballot(to_near);
-            if (mask) {
-                int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
-                int base = 0;
-                if (lane_id() == leader) base = atomicAdd(&s_cnt, __popcll(mask));
-                base = __shfl(base, leader);
-                if (to_near) s_out[base + lanes_below(mask)] = v;
-            }
+            lq_push(s_near, &s_nc, to_near, v);
+            lq_push(s_far, &s_fc, to_far, v);
         }
-        __syncthreads();
-        if (tid == 0 && s_cnt) s_base = atomicAdd(&ctr->near_count, s_cnt);
-        __syncthreads();
-        unsigned long long od = 0;
-        const int cnt = s_cnt, base = s_base;
-        for (int j = tid; j < cnt; j += kBT) {
-            int32_t v = s_out[j];
-            next_q[base + j] = v;
-            od += rowptr[v + 1] - rowptr[v];
-        }
-        od = wave_sum(od);
-        if ((tid & 63) == 0) s_red[tid >> 6] = od;
-        __syncthreads();
-        if (tid == 0) {
-            unsigned long long a = 0;
-            for (int w = 0; w < kBT / 64; ++w) a += s_red[w];
-            if (a) atomicAdd(&ctr->near_edges, a);
-        }
-        __syncthreads();
+        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, next_q, kOut - kTile, on_near);
+        lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[far_sel], far_list, kOut - kTile, nop);
     }
+    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, next_q, 0, on_near);
+    lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[far_sel], far_list, 0, nop);
+    od = block_sum_u64<kBT>(od, s_red);
+    if (tid == 0) slot_add(ctr->slots, 0, od);
 }
 
 // Splits far list `sel` at new_thr: entries below old_thr are stale (already
 // settled at their current distance), [old_thr, new_thr) go to the near queue,
 // the rest to far list 1-sel with the new generation stamp.
-__global__ void k_far_split(const int32_t* __restrict__ far_in, int32_t count, const double* __restrict__ dist,
-                            double old_thr, double new_thr, int32_t* stamp_near, int32_t next_round,
-                            int32_t* stamp_far, int32_t new_gen, int32_t* __restrict__ near_q, int32_t* far_out,
-                            int other, const int32_t* __restrict__ rowptr, SsspCtr* ctr) {
-    unsigned long long od = 0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i - threadIdx.x < count;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+__global__ void __launch_bounds__(kBT) k_far_split(const int32_t* __restrict__ far_in, int32_t count,
+                                                   const double* __restrict__ dist, double old_thr, double new_thr,
+                                                   int32_t* stamp_near, int32_t next_round, int32_t* stamp_far,
+                                                   int32_t new_gen, int32_t* __restrict__ near_q, int32_t* far_out,
+                                                   int other, const int32_t* __restrict__ rowptr, SsspCtr* ctr) {
+    __shared__ int32_t s_near[kOut], s_far[kOut];
+    __shared__ int32_t s_nc, s_fc, s_base;
+    __shared__ unsigned long long s_red[kBT / 64];
+    if (threadIdx.x == 0) {
+        s_nc = 0;
+        s_fc = 0;
+    }
+    unsigned long long od = 0, dmin = kInfBits;
+    auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
+    auto nop = [](int32_t) {};
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < count;
+         base += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int64_t i = base + threadIdx.x;
         bool to_near = false, keep = false;
         int32_t v = 0;
-        double d = 0;
         if (i < count) {
             v = far_in[i];
-            d = dist[v];
+            const double d = dist[v];
             if (d >= old_thr) {
                 if (d < new_thr) {
                     stamp_near[v] = next_round;
@@ -198,22 +194,27 @@ __global__ void k_far_split(const int32_t* __restrict__ far_in, int32_t count, c
                 } else {
                     stamp_far[v] = new_gen;
                     keep = true;
+                    const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(d));
+                    dmin = bits < dmin ? bits : dmin;
                 }
             }
         }
-        int p = wave_append(&ctr->near_count, to_near);
-        if (to_near) {
-            near_q[p] = v;
-            od += rowptr[v + 1] - rowptr[v];
-        }
-        int k = wave_append(&ctr->far_count[other], keep);
-        if (keep) {
-            far_out[k] = v;
-            atomicMin(&ctr->far_min_bits, static_cast<unsigned long long>(__double_as_longlong(d)));
-        }
+        lq_push(s_near, &s_nc, to_near, v);
+        lq_push(s_far, &s_fc, keep, v);
+        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, near_q, kOut - kBT, on_near);
+        lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, kOut - kBT, nop);
     }
-    od = wave_sum(od);
-    if (lane_id() == 0 && od) atomicAdd(&ctr->near_edges, od);
+    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, near_q, 0, on_near);
+    lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, 0, nop);
+    od = block_sum_u64<kBT>(od, s_red);
+    dmin = wave_min(dmin);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = dmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kBT / 64; ++w) dmin = s_red[w] < dmin ? s_red[w] : dmin;
+        slot_add(ctr->slots, 0, od);
+        if (dmin != kInfBits) atomicMin(&ctr->slots[blockIdx.x & (kSlots - 1)].v[1], dmin);
+    }
 }
 
 // Parent = smallest u != v with an edge u->v and d[u] + w(u,v) == d[v]
@@ -301,7 +302,8 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     GHET_CHECK(g->ws.get(WS_STAMP_FAR, 4 * size_t(n), &p)); stamp_far = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_COUNTERS, 256, &p)); ctr = static_cast<SsspCtr*>(p);
+    GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(SsspCtr), &p)); ctr = static_cast<SsspCtr*>(p);
+    const SsspCtr* hc = reinterpret_cast<const SsspCtr*>(g->pinned);
 
     SchedState sched;
     sched_begin(&sched, schedule, GHET_NET_SSSP, n, g->out.m, g->out.m);
@@ -310,7 +312,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     GHET_CHECK(fill<double>(dist, inf, n, s));
     GHET_CHECK(fill<int32_t>(stamp_near, -1, n, s));
     GHET_CHECK(fill<int32_t>(stamp_far, -1, n, s));
-    GHET_CHECK(fill<int32_t>(reinterpret_cast<int32_t*>(ctr), 0, 8, s));
+    GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(ctr), 0, kCtrWords, s));
     hipLaunchKernelGGL(k_sssp_seed, dim3(1), dim3(64), 0, s, dist, qa, stamp_near, src);
 
     int32_t F = 1;
@@ -320,7 +322,6 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     int32_t far_count = 0;
     double threshold = 0.0;
     double delta = base_delta;
-    // first phase
     {
         LevelInfo li;
         li.n = n; li.m = g->out.m; li.m_in = g->out.m; li.level = 0; li.frontier = F; li.frontier_edges = E;
@@ -338,13 +339,13 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         // relaxation rounds of the current bucket
         if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k0, s));
         while (F > 0) {
-            hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, s, ctr);
+            hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, -1);
             hipLaunchKernelGGL(k_frontier_deg, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s, qa, F,
                                g->out.rowptr, scan);
             GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
             const int32_t next_round = round + 1;
             if (E > 0) {
-                unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), 65535));
+                unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), kMaxBlocks));
                 hipLaunchKernelGGL(k_relax, dim3(grid), dim3(kBT), 0, s, qa, F, scan, static_cast<int32_t>(E),
                                    g->out.rowptr, g->out.col, g->out.w, dist, threshold, stamp_near, next_round,
                                    stamp_far, far_gen, qb, far[sel], sel, ctr);
@@ -352,9 +353,8 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
             GHET_HIP(hipGetLastError());
             GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(SsspCtr), hipMemcpyDeviceToHost, s));
             GHET_HIP(hipStreamSynchronize(s));
-            const SsspCtr* hc = reinterpret_cast<const SsspCtr*>(g->pinned);
             F = hc->near_count;
-            E = static_cast<int64_t>(hc->near_edges);
+            E = static_cast<int64_t>(slot_sum(hc->slots, 0));
             far_count = hc->far_count[sel];
             std::swap(qa, qb);
             round = next_round;
@@ -387,21 +387,22 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         double new_thr = std::max(threshold + delta, std::nextafter(threshold, HUGE_VAL));
         for (;;) {
             const int other = 1 - sel;
-            hipLaunchKernelGGL(k_split_reset, dim3(1), dim3(64), 0, s, ctr, other);
+            hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, other);
             const int32_t next_round = round + 1;
-            unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, 256), 8192)));
-            hipLaunchKernelGGL(k_far_split, dim3(grid), dim3(256), 0, s, far[sel], far_count, dist, old_thr, new_thr,
+            unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, kBT), 1024)));
+            hipLaunchKernelGGL(k_far_split, dim3(grid), dim3(kBT), 0, s, far[sel], far_count, dist, old_thr, new_thr,
                                stamp_near, next_round, stamp_far, far_gen + 1, qa, far[other], other, g->out.rowptr,
                                ctr);
             GHET_HIP(hipGetLastError());
             GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(SsspCtr), hipMemcpyDeviceToHost, s));
             GHET_HIP(hipStreamSynchronize(s));
-            const SsspCtr* hc = reinterpret_cast<const SsspCtr*>(g->pinned);
             F = hc->near_count;
-            E = static_cast<int64_t>(hc->near_edges);
+            E = static_cast<int64_t>(slot_sum(hc->slots, 0));
             far_count = hc->far_count[other];
+            unsigned long long mb = kInfBits;
+            for (int i = 0; i < kSlots; ++i) mb = std::min(mb, hc->slots[i].v[1]);
             double far_min;
-            std::memcpy(&far_min, &hc->far_min_bits, sizeof(double));
+            std::memcpy(&far_min, &mb, sizeof(double));
             sel = other;
             ++far_gen;
             round = next_round;
@@ -421,16 +422,15 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         const unsigned gw = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096)));
         hipLaunchKernelGGL(k_sssp_parent, dim3(gw), dim3(256), 0, s, n, g->out.rowptr, g->out.col, g->out.w, dist, inf,
                            src, pre);
-        hipLaunchKernelGGL(k_pre_finish, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 8192))),
-                           dim3(256), 0, s, pre, n);
+        hipLaunchKernelGGL(k_pre_finish, dim3(gw), dim3(256), 0, s, pre, n);
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
     if (dist_out) GHET_HIP(hipMemcpyAsync(dist_out, dist, 8 * size_t(n), hipMemcpyDeviceToHost, s));
     if (pre_out) GHET_HIP(hipMemcpyAsync(pre_out, g->ws.ptr[WS_PRE], 4 * size_t(n), hipMemcpyDeviceToHost, s));
     if (st) {
-        GHET_CHECK(reached_stats_f64(dist, inf, g->out.rowptr, n, reinterpret_cast<int64_t*>(scan), s));
-        GHET_HIP(hipMemcpyAsync(g->pinned, scan, 16, hipMemcpyDeviceToHost, s));
+        GHET_CHECK(reached_stats_f64(dist, inf, g->out.rowptr, n, ctr->slots, s));
+        GHET_HIP(hipMemcpyAsync(g->pinned, ctr->slots, sizeof(CtrSlot) * kSlots, hipMemcpyDeviceToHost, s));
     }
     GHET_HIP(hipStreamSynchronize(s));
     if (st) {
@@ -438,9 +438,9 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         GHET_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
         st->device_ms = ms;
         st->levels = static_cast<int32_t>(rounds);
-        const int64_t* rc = reinterpret_cast<const int64_t*>(g->pinned);
-        st->vertices_reached = rc[0];
-        st->edges_traversed = rc[1];
+        const CtrSlot* hs = reinterpret_cast<const CtrSlot*>(g->pinned);
+        st->vertices_reached = static_cast<int64_t>(slot_sum(hs, 0));
+        st->edges_traversed = static_cast<int64_t>(slot_sum(hs, 1));
         st->trace_len = trace_len;
         st->extra[0] = base_delta;
         st->extra[1] = phases;

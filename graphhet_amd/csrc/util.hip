@@ -128,10 +128,11 @@ int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* aux,
 }
 
 // ------------------------------------------------------- reached stats ----
+// Counts n_r and m_r into slots (field 0 and 1); the host sums the slots.
 template <typename Pred>
-__device__ void reached_body(Pred reached, const int32_t* __restrict__ rowptr, int32_t n,
-                             unsigned long long* counters) {
-    int64_t cnt = 0, edges = 0;
+__device__ void reached_body(Pred reached, const int32_t* __restrict__ rowptr, int32_t n, CtrSlot* slots) {
+    __shared__ unsigned long long s_red[4];
+    unsigned long long cnt = 0, edges = 0;
     for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
          v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         if (reached(v)) {
@@ -139,37 +140,35 @@ __device__ void reached_body(Pred reached, const int32_t* __restrict__ rowptr, i
             edges += rowptr[v + 1] - rowptr[v];
         }
     }
-    cnt = wave_sum(cnt);
-    edges = wave_sum(edges);
-    if ((threadIdx.x & 63) == 0 && (cnt | edges)) {
-        atomicAdd(&counters[0], static_cast<unsigned long long>(cnt));
-        atomicAdd(&counters[1], static_cast<unsigned long long>(edges));
+    cnt = block_sum_u64<256>(cnt, s_red);
+    edges = block_sum_u64<256>(edges, s_red);
+    if (threadIdx.x == 0) {
+        slot_add(slots, 0, cnt);
+        slot_add(slots, 1, edges);
     }
 }
 
-__global__ void k_reached_i32(const int32_t* __restrict__ depth, const int32_t* __restrict__ rowptr, int32_t n,
-                              unsigned long long* counters) {
-    reached_body([&](int64_t v) { return depth[v] != kIntInf; }, rowptr, n, counters);
+__global__ void __launch_bounds__(256) k_reached_i32(const int32_t* __restrict__ depth, const int32_t* __restrict__ rowptr,
+                                                     int32_t n, CtrSlot* slots) {
+    reached_body([&](int64_t v) { return depth[v] != kIntInf; }, rowptr, n, slots);
 }
-__global__ void k_reached_f64(const double* __restrict__ dist, double inf, const int32_t* __restrict__ rowptr,
-                              int32_t n, unsigned long long* counters) {
-    reached_body([&](int64_t v) { return dist[v] != inf; }, rowptr, n, counters);
+__global__ void __launch_bounds__(256) k_reached_f64(const double* __restrict__ dist, double inf,
+                                                     const int32_t* __restrict__ rowptr, int32_t n, CtrSlot* slots) {
+    reached_body([&](int64_t v) { return dist[v] != inf; }, rowptr, n, slots);
 }
 
-int reached_stats_i32(const int32_t* depth, const int32_t* rowptr, int32_t n, int64_t* counters, hipStream_t s) {
-    GHET_CHECK(fill<int64_t>(counters, 0, 2, s));
-    unsigned blocks = static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 4096));
-    hipLaunchKernelGGL(k_reached_i32, dim3(std::max(blocks, 1u)), dim3(256), 0, s, depth, rowptr, n,
-                       reinterpret_cast<unsigned long long*>(counters));
+int reached_stats_i32(const int32_t* depth, const int32_t* rowptr, int32_t n, CtrSlot* slots, hipStream_t s) {
+    GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(slots), 0, kSlots * 8, s));
+    unsigned blocks = static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 2048));
+    hipLaunchKernelGGL(k_reached_i32, dim3(std::max(blocks, 1u)), dim3(256), 0, s, depth, rowptr, n, slots);
     GHET_HIP(hipGetLastError());
     return GHET_OK;
 }
-int reached_stats_f64(const double* dist, double inf, const int32_t* rowptr, int32_t n, int64_t* counters,
+int reached_stats_f64(const double* dist, double inf, const int32_t* rowptr, int32_t n, CtrSlot* slots,
                       hipStream_t s) {
-    GHET_CHECK(fill<int64_t>(counters, 0, 2, s));
-    unsigned blocks = static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 4096));
-    hipLaunchKernelGGL(k_reached_f64, dim3(std::max(blocks, 1u)), dim3(256), 0, s, dist, inf, rowptr, n,
-                       reinterpret_cast<unsigned long long*>(counters));
+    GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(slots), 0, kSlots * 8, s));
+    unsigned blocks = static_cast<unsigned>(std::min<int64_t>(ceil_div(n, 256), 2048));
+    hipLaunchKernelGGL(k_reached_f64, dim3(std::max(blocks, 1u)), dim3(256), 0, s, dist, inf, rowptr, n, slots);
     GHET_HIP(hipGetLastError());
     return GHET_OK;
 }
