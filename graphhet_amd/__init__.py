@@ -393,6 +393,11 @@ class DeviceGraph:
                                     C.byref(_opts(schedule)), C.byref(st)))
         return depth, st
 
+    def autotune(self, cache_path: str | None = None) -> int:
+        best = C.c_int32()
+        _check(lib().ghet_autotune(self._h, cache_path.encode() if cache_path else None, C.byref(best)))
+        return best.value
+
     def sssp_multi(self, srcs, inf: float = DBL_MAX, delta: float = 0.0, out=True):
         srcs = np.ascontiguousarray(srcs, dtype=np.int32)
         dist = np.empty((len(srcs), self.n), np.float64) if out else None
@@ -418,19 +423,25 @@ def pack_graphs(csrs: list[Csr]):
     return voff, rowptr.astype(np.int32), col
 
 
-def wcc_many(csrs: list[Csr], opts_schedule=SCHED_HEURISTIC):
+def _many_opts(tune: bool) -> Opts:
+    o = _opts()
+    o.reserved[1] = 0 if tune else 1
+    return o
+
+
+def wcc_many(csrs: list[Csr], tune: bool = True):
     voff, rowptr, col = pack_graphs(csrs)
     lab = np.empty(int(voff[-1]), np.int32)
     st = RunStats()
     _check(lib().ghet_wcc_many(len(csrs), _ptr(voff, C.c_int32), _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32),
-                               _ptr(lab, C.c_int32), C.byref(_opts(opts_schedule)), C.byref(st)))
+                               _ptr(lab, C.c_int32), C.byref(_many_opts(tune)), C.byref(st)))
     return [lab[voff[i]:voff[i + 1]] for i in range(len(csrs))], st
 
 
-def tc_many(csrs: list[Csr]):
+def tc_many(csrs: list[Csr], tune: bool = True):
     voff, rowptr, col = pack_graphs(csrs)
     cnt = np.empty(int(voff[-1]), np.int64)
     st = RunStats()
     _check(lib().ghet_tc_many(len(csrs), _ptr(voff, C.c_int32), _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32),
-                              _ptr(cnt, C.c_int64), C.byref(_opts()), C.byref(st)))
+                              _ptr(cnt, C.c_int64), C.byref(_many_opts(tune)), C.byref(st)))
     return [cnt[voff[i]:voff[i + 1]] for i in range(len(csrs))], st

@@ -138,7 +138,8 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long x
 // LDS-staged output queue.  lq_push appends the lanes with pred (one LDS
 // atomic per wave); lq_flush (block-uniform) moves the staged items to a
 // global queue with ONE global atomic, calling on_item(v) for each item.
-__device__ __forceinline__ void lq_push(int32_t* buf, int* cnt, bool pred, int32_t v) {
+template <typename T>
+__device__ __forceinline__ void lq_push(T* buf, int* cnt, bool pred, T v) {
     const uint64_t mask = __ballot(pred);
     if (!mask) return;
     const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
@@ -148,8 +149,8 @@ __device__ __forceinline__ void lq_push(int32_t* buf, int* cnt, bool pred, int32
     if (pred) buf[base + lanes_below(mask)] = v;
 }
 
-template <int BT, typename OnItem>
-__device__ __forceinline__ void lq_flush(int32_t* buf, int* cnt, int* s_base, int* gcount, int32_t* gqueue, int limit,
+template <int BT, typename T, typename Count, typename OnItem>
+__device__ __forceinline__ void lq_flush(T* buf, int* cnt, Count* s_base, Count* gcount, T* gqueue, int limit,
                                          OnItem&& on_item) {
     __syncthreads();
     const int c = *cnt;
@@ -157,11 +158,11 @@ __device__ __forceinline__ void lq_flush(int32_t* buf, int* cnt, int* s_base, in
         __syncthreads();
         return;
     }
-    if (threadIdx.x == 0) *s_base = atomicAdd(gcount, c);
+    if (threadIdx.x == 0) *s_base = atomicAdd(gcount, static_cast<Count>(c));
     __syncthreads();
-    const int base = *s_base;
+    const Count base = *s_base;
     for (int j = threadIdx.x; j < c; j += BT) {
-        const int32_t v = buf[j];
+        const T v = buf[j];
         gqueue[base + j] = v;
         on_item(v);
     }

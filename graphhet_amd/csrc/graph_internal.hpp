@@ -90,6 +90,16 @@ struct TCLayout {
     void* mem = nullptr;
 };
 
+// Launch shapes the autotuner may change (results never depend on them).
+struct Tuning {
+    bool tuned = false;
+    int tc_ept = 8;               // TC: edges per wave task
+    int tc_waves_per_cu = 32;     // TC: grid size in waves per CU
+    int wcc_light = 16;           // WCC: rows up to this length are linked lane-serially
+    int wcc_waves_per_cu = 64;    // WCC: grid size in waves per CU
+    int sssp_class = 1;           // SSSP: bucket width = base * 2^(class-1)
+};
+
 }  // namespace ghet
 
 struct ghet_graph {
@@ -105,6 +115,7 @@ struct ghet_graph {
     ghet::PRLayout pr;
     ghet::CDLPLayout cdlp;
     ghet::TCLayout tc;
+    ghet::Tuning tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
     int32_t* pinned = nullptr;  // pinned host buffer (kPinnedBytes) for per-level counters
@@ -127,11 +138,20 @@ int run_wcc(ghet_graph* g, int32_t* label_out, const ghet_opts* o, ghet_run_stat
 int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const ghet_opts* o, ghet_run_stats* st);
 int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* o, ghet_run_stats* st);
 int run_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* o, ghet_run_stats* st);
+int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_out, const ghet_opts* o,
+                  ghet_run_stats* st);
+// Capturable pieces (no host synchronisation inside) for hipGraph use.
+int tc_prepare(ghet_graph* g, hipStream_t s);
+int enqueue_tc(ghet_graph* g, unsigned long long* cnt, int ept, int waves_per_cu, hipStream_t s);
+int enqueue_wcc_plain(ghet_graph* g, int32_t* parent, int light, int waves_per_cu, hipStream_t s);
+int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, double* dist_out, const ghet_opts* o,
+                   ghet_run_stats* st);
 
 // Device-wide exclusive scan of int32 -> int32 (out may alias in); n up to 2^31-1.
 // `aux` must hold scan_aux_ints(n) ints.  Writes the total to out[n].
 int64_t scan_aux_ints(int64_t n);
 int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* aux, hipStream_t s);
+int exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, int64_t* aux, hipStream_t s);
 
 // Reached-vertex statistics: n_r and m_r (sum of out-degrees) of vertices with
 // depth != INT_MAX (or dist != inf), into slot fields 0 and 1 (host sums them).

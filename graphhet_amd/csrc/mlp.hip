@@ -21,19 +21,20 @@ int ghet_mlp_forward(int, int32_t, const float*, float*) { return set_error(GHET
 int ghet_mlp_train_step(int, int32_t, const float*, const int32_t*, float, float*, float*) {
     return set_error(GHET_ERR_UNSUPPORTED, "not built yet");
 }
-int ghet_autotune(ghet_graph*, const char*, int32_t*) { return set_error(GHET_ERR_UNSUPPORTED, "not built yet"); }
-int ghet_bfs_multi(ghet_graph*, int32_t, const int32_t*, int32_t*, const ghet_opts*, ghet_run_stats*) {
-    return set_error(GHET_ERR_UNSUPPORTED, "not built yet");
+int ghet_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_out, const ghet_opts* o,
+                   ghet_run_stats* st) {
+    clear_error();
+    if (!g) return set_error(GHET_ERR_INVALID, "graph handle is NULL");
+    if (st) std::memset(st, 0, sizeof(*st));
+    if (hipSetDevice(g->device) != hipSuccess) return set_error(GHET_ERR_HIP, "hipSetDevice failed");
+    return run_bfs_multi(g, k, srcs, depth_out, o, st);
 }
-int ghet_sssp_multi(ghet_graph*, int32_t, const int32_t*, double, double*, const ghet_opts*, ghet_run_stats*) {
-    return set_error(GHET_ERR_UNSUPPORTED, "not built yet");
-}
-int ghet_wcc_many(int32_t, const int32_t*, const int32_t*, const int32_t*, int32_t*, const ghet_opts*,
-                  ghet_run_stats*) {
-    return set_error(GHET_ERR_UNSUPPORTED, "not built yet");
-}
-int ghet_tc_many(int32_t, const int32_t*, const int32_t*, const int32_t*, int64_t*, const ghet_opts*,
-                 ghet_run_stats*) {
-    return set_error(GHET_ERR_UNSUPPORTED, "not built yet");
+int ghet_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, double* dist_out, const ghet_opts* o,
+                    ghet_run_stats* st) {
+    clear_error();
+    if (!g) return set_error(GHET_ERR_INVALID, "graph handle is NULL");
+    if (st) std::memset(st, 0, sizeof(*st));
+    if (hipSetDevice(g->device) != hipSuccess) return set_error(GHET_ERR_HIP, "hipSetDevice failed");
+    return run_sssp_multi(g, k, srcs, inf, dist_out, o, st);
 }
 }

@@ -1,0 +1,371 @@
+// Batched BFS: up to 256 sources per pass in ONE persistent cooperative
+// launch (no host round trip between levels).
+//
+// No reference counterpart: the reference runs one csr_bfs per source
+// (GPU_BFS_csr.cuh:258).  Results are bit-identical to per-source runs.
+//
+// Bit-parallel traversal: every vertex carries 256-bit sets (4 x u64) --
+// seen[v] (sources that reached v) and front[v] (sources whose frontier holds
+// v) -- so one pass over the graph advances all 256 searches.  Per level the
+// grid agrees on a direction from the frontier's out-edge count:
+//   push  frontier vertices OR their set into acc[w] of each out-neighbour
+//         (atomicOr), then every vertex takes next = acc & ~seen;
+//   pull  every vertex not yet seen by all sources ORs front[u] over its
+//         in-neighbours, stopping as soon as nothing unseen is left.
+// Depths go to a vertex-major [n][256] block (a source's new bit at level L
+// writes L); the batch is transposed to source-major rows for the caller.
+#include <hip/hip_cooperative_groups.h>
+
+#include "graph_internal.hpp"
+
+namespace cg = cooperative_groups;
+
+namespace ghet {
+namespace {
+
+constexpr int kBT = 256;
+constexpr int kWords = 4;
+constexpr int kPass = 64 * kWords;  // sources per pass
+
+struct Bits {
+    uint64_t w[kWords];
+};
+
+struct MsArgs {
+    int32_t n;
+    const int32_t *out_rowptr, *out_col, *in_rowptr, *in_col;
+    Bits *seen, *fa, *fb, *acc;
+    int32_t* depth;  // [n][kPass]
+    Bits full;       // active sources of this pass
+    int32_t* flags;  // [3] "something new at level L" (L % 3)
+    CtrSlot* slots;  // [3][kSlots] per level: v[0] next frontier out-edges; then [kSlots] totals: v[1]
+                     // traversed (source, edge) pairs
+    int64_t front_edges0;
+    int64_t push_limit;
+    int32_t max_levels;
+    int32_t* levels_out;
+};
+
+__device__ __forceinline__ bool any_bits(const Bits& b) { return (b.w[0] | b.w[1] | b.w[2] | b.w[3]) != 0; }
+
+__device__ __forceinline__ Bits wave_or(Bits b) {
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < kWords; ++k) b.w[k] |= __shfl_xor(b.w[k], off);
+    return b;
+}
+
+__device__ __forceinline__ bool covers(const Bits& a, const Bits& seen, const Bits& full) {
+    bool c = true;
+    for (int k = 0; k < kWords; ++k) c &= ((a.w[k] | seen.w[k]) & full.w[k]) == full.w[k];
+    return c;
+}
+
+// New bits of v at `level`: update seen, write depths, count stats.
+__device__ __forceinline__ void settle(const MsArgs& a, int64_t v, const Bits& sn, const Bits& nb, int32_t level,
+                                       unsigned long long& ne, unsigned long long& tr) {
+    Bits s2;
+    int pop = 0;
+    for (int k = 0; k < kWords; ++k) {
+        s2.w[k] = sn.w[k] | nb.w[k];
+        pop += __popcll(nb.w[k]);
+    }
+    a.seen[v] = s2;
+    int32_t* row = a.depth + v * kPass;
+    for (int k = 0; k < kWords; ++k) {
+        uint64_t x = nb.w[k];
+        while (x) {
+            const int b = __ffsll(static_cast<unsigned long long>(x)) - 1;
+            x &= x - 1;
+            row[k * 64 + b] = level;
+        }
+    }
+    const unsigned long long d = a.out_rowptr[v + 1] - a.out_rowptr[v];
+    ne += d;
+    tr += d * static_cast<unsigned long long>(pop);
+}
+
+__global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
+    cg::grid_group grid = cg::this_grid();
+    __shared__ unsigned long long s_red[kBT / 64];
+    __shared__ int64_t s_front_edges;
+    __shared__ int s_stop;
+    const int lane = lane_id();
+    const int64_t n = a.n;
+    int64_t front_edges = a.front_edges0;
+    int32_t level = 0;
+    for (; level < a.max_levels; ++level) {
+        const Bits* front = (level & 1) ? a.fb : a.fa;
+        Bits* next = (level & 1) ? a.fa : a.fb;
+        CtrSlot* slots = a.slots + (level % 3) * kSlots;
+        unsigned long long ne = 0, tr = 0;
+        bool any = false;
+        if (front_edges < a.push_limit) {
+            // push phase A: OR each frontier set into acc of its out-neighbours
+            for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kBT; b0 < n; b0 += static_cast<int64_t>(gridDim.x) * kBT) {
+                const int64_t u = b0 + threadIdx.x;
+                Bits f = {};
+                int32_t b = 0, e = 0;
+                if (u < n) {
+                    f = front[u];
+                    if (any_bits(f)) {
+                        b = a.out_rowptr[u];
+                        e = a.out_rowptr[u + 1];
+                    }
+                }
+                const bool light = e - b <= 32;
+                if (light)
+                    for (int32_t i = b; i < e; ++i) {
+                        Bits* t = a.acc + a.out_col[i];
+                        for (int k = 0; k < kWords; ++k)
+                            if (f.w[k]) atomicOr(reinterpret_cast<unsigned long long*>(&t->w[k]), f.w[k]);
+                    }
+                uint64_t heavy = __ballot(!light);
+                while (heavy) {
+                    const int sl = __ffsll(static_cast<unsigned long long>(heavy)) - 1;
+                    heavy &= heavy - 1;
+                    const int32_t hb = __shfl(b, sl), he = __shfl(e, sl);
+                    Bits hf;
+                    for (int k = 0; k < kWords; ++k) hf.w[k] = __shfl(f.w[k], sl);
+                    for (int32_t i = hb + lane; i < he; i += 64) {
+                        Bits* t = a.acc + a.out_col[i];
+                        for (int k = 0; k < kWords; ++k)
+                            if (hf.w[k]) atomicOr(reinterpret_cast<unsigned long long*>(&t->w[k]), hf.w[k]);
+                    }
+                }
+            }
+            grid.sync();
+            // push phase B: next = acc & ~seen, then clear acc
+            for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kBT; b0 < n; b0 += static_cast<int64_t>(gridDim.x) * kBT) {
+                const int64_t v = b0 + threadIdx.x;
+                if (v >= n) continue;
+                const Bits ac = a.acc[v];
+                Bits nb = {};
+                if (any_bits(ac)) {
+                    const Bits sn = a.seen[v];
+                    for (int k = 0; k < kWords; ++k) nb.w[k] = ac.w[k] & ~sn.w[k] & a.full.w[k];
+                    a.acc[v] = Bits{};
+                    if (any_bits(nb)) {
+                        settle(a, v, sn, nb, level + 1, ne, tr);
+                        any = true;
+                    }
+                }
+                next[v] = nb;
+            }
+        } else {
+            // pull: every vertex not yet seen by all sources scans its in-list
+            for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kBT; b0 < n; b0 += static_cast<int64_t>(gridDim.x) * kBT) {
+                const int64_t v = b0 + threadIdx.x;
+                Bits sn = {}, acc = {};
+                int32_t b = 0, e = 0;
+                bool todo = false;
+                if (v < n) {
+                    sn = a.seen[v];
+                    todo = !covers(Bits{}, sn, a.full);
+                    if (todo) {
+                        b = a.in_rowptr[v];
+                        e = a.in_rowptr[v + 1];
+                    }
+                }
+                const bool light = e - b <= 32;
+                if (todo && light)
+                    for (int32_t i = b; i < e; ++i) {
+                        const Bits f = front[a.in_col[i]];
+                        for (int k = 0; k < kWords; ++k) acc.w[k] |= f.w[k];
+                        if (covers(acc, sn, a.full)) break;
+                    }
+                uint64_t heavy = __ballot(!light);
+                while (heavy) {
+                    const int sl = __ffsll(static_cast<unsigned long long>(heavy)) - 1;
+                    heavy &= heavy - 1;
+                    const int32_t hb = __shfl(b, sl), he = __shfl(e, sl);
+                    Bits hs;
+                    for (int k = 0; k < kWords; ++k) hs.w[k] = __shfl(sn.w[k], sl);
+                    Bits hacc = {};
+                    for (int32_t i0 = hb; i0 < he; i0 += 64) {
+                        if (i0 + lane < he) {
+                            const Bits f = front[a.in_col[i0 + lane]];
+                            for (int k = 0; k < kWords; ++k) hacc.w[k] |= f.w[k];
+                        }
+                        hacc = wave_or(hacc);
+                        if (covers(hacc, hs, a.full)) break;
+                    }
+                    if (lane == sl) acc = hacc;
+                }
+                if (v < n) {
+                    Bits nb;
+                    for (int k = 0; k < kWords; ++k) nb.w[k] = acc.w[k] & ~sn.w[k] & a.full.w[k];
+                    if (any_bits(nb)) {
+                        settle(a, v, sn, nb, level + 1, ne, tr);
+                        any = true;
+                    }
+                    next[v] = nb;
+                }
+            }
+        }
+        if (__ballot(any) && lane == 0)
+            __hip_atomic_store(&a.flags[level % 3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ne = block_sum_u64<kBT>(ne, s_red);
+        tr = block_sum_u64<kBT>(tr, s_red);
+        if (threadIdx.x == 0) {
+            slot_add(slots, 0, ne);
+            slot_add(a.slots + 3 * kSlots, 1, tr);
+        }
+        if (blockIdx.x == 0) {
+            // clear the flag and slots of level+1 (last read two levels ago)
+            CtrSlot* nxt = a.slots + ((level + 1) % 3) * kSlots;
+            for (int i = threadIdx.x; i < kSlots * 8; i += kBT)
+                __hip_atomic_store(&nxt[i / 8].v[i % 8], 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&a.flags[(level + 1) % 3], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        grid.sync();
+        if (threadIdx.x == 0) {
+            s_stop = __hip_atomic_load(&a.flags[level % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+            int64_t fe = 0;
+            for (int i = 0; i < kSlots; ++i)
+                fe += static_cast<int64_t>(
+                    __hip_atomic_load(&slots[i].v[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            s_front_edges = fe;
+        }
+        __syncthreads();
+        front_edges = s_front_edges;
+        if (s_stop) break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.levels_out = level;
+}
+
+__global__ void k_ms_seed(const int32_t* __restrict__ srcs, int32_t k, Bits* seen, Bits* front, int32_t* depth) {
+    const int i = threadIdx.x;
+    if (i < k) {
+        const int32_t v = srcs[i];
+        atomicOr(reinterpret_cast<unsigned long long*>(&seen[v].w[i >> 6]), 1ULL << (i & 63));
+        atomicOr(reinterpret_cast<unsigned long long*>(&front[v].w[i >> 6]), 1ULL << (i & 63));
+        depth[static_cast<int64_t>(v) * kPass + i] = 0;
+    }
+}
+
+// [n][kPass] vertex-major -> rows of `out` (row r = source r, stride n),
+// 64x64 tiles through LDS.
+__global__ void __launch_bounds__(256) k_ms_transpose(const int32_t* __restrict__ vm, int32_t n, int32_t k,
+                                                      int32_t* __restrict__ out) {
+    __shared__ int32_t t[64][65];
+    const int64_t v0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int s0 = blockIdx.y * 64;
+    for (int r = threadIdx.x / 64; r < 64; r += 4) {
+        const int64_t v = v0 + r;
+        const int s = s0 + (threadIdx.x & 63);
+        t[r][threadIdx.x & 63] = (v < n && s < kPass) ? vm[v * kPass + s] : 0;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x / 64; r < 64; r += 4) {
+        const int s = s0 + r;
+        const int64_t v = v0 + (threadIdx.x & 63);
+        if (s < k && v < n) out[static_cast<int64_t>(s) * n + v] = t[threadIdx.x & 63][r];
+    }
+}
+
+}  // namespace
+
+int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_out, const ghet_opts* o,
+                  ghet_run_stats* st) {
+    const int32_t n = g->n;
+    if (k < 0 || (k > 0 && !srcs)) return set_error(GHET_ERR_INVALID, "bad source list");
+    for (int32_t i = 0; i < k; ++i)
+        if (srcs[i] < 0 || srcs[i] >= n) return set_error(GHET_ERR_INVALID, "source %d out of range [0,%d)", srcs[i], n);
+    if (!g->out.present() || !g->in.present()) return set_error(GHET_ERR_INVALID, "batched BFS needs out- and in-CSR");
+    if (k == 0) return GHET_OK;
+    hipStream_t s = pick_stream(g, o);
+    void* p;
+    Bits *seen, *fa, *fb, *acc;
+    int32_t *depth, *flags, *d_src, *d_levels, *out_dev = nullptr;
+    CtrSlot* slots;
+    GHET_CHECK(g->ws.get(WS_MULTI_A, sizeof(Bits) * 4 * size_t(n), &p));
+    seen = static_cast<Bits*>(p);
+    fa = seen + n;
+    fb = fa + n;
+    acc = fb + n;
+    GHET_CHECK(g->ws.get(WS_MULTI_B, 4 * size_t(n) * kPass, &p)); depth = static_cast<int32_t*>(p);
+    // counters: [3][kSlots] per-level slots, [kSlots] totals, then flags[4], levels
+    constexpr size_t kCtrBytes = sizeof(CtrSlot) * kSlots * 4 + 64;
+    GHET_CHECK(g->ws.get(WS_COUNTERS, kCtrBytes, &p));
+    slots = static_cast<CtrSlot*>(p);
+    flags = reinterpret_cast<int32_t*>(slots + 4 * kSlots);
+    d_levels = flags + 4;
+    GHET_CHECK(g->ws.get(WS_MULTI_C, 4 * size_t(kPass), &p)); d_src = static_cast<int32_t*>(p);
+    if (depth_out) {
+        GHET_CHECK(g->ws.get(WS_MULTI_D, 4 * size_t(n) * std::min(k, kPass), &p));
+        out_dev = static_cast<int32_t*>(p);
+    }
+    int dev_blocks = 0, cus = 0;
+    GHET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&dev_blocks, reinterpret_cast<const void*>(k_msbfs), kBT, 0));
+    GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    const unsigned grid = static_cast<unsigned>(std::max(1, std::min(dev_blocks, 4) * cus));
+    const int64_t words = sizeof(Bits) / 8 * 4 * int64_t(n);
+
+    int64_t traversed = 0;
+    int32_t max_levels_seen = 0;
+    GHET_HIP(hipEventRecord(g->ev_begin, s));
+    for (int32_t p0 = 0; p0 < k; p0 += kPass) {
+        const int32_t kp = std::min(kPass, k - p0);
+        GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(seen), 0, words, s));
+        GHET_CHECK(fill<int32_t>(depth, kIntInf, int64_t(n) * kPass, s));
+        GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(slots), 0, kCtrBytes / 8, s));
+        GHET_HIP(hipMemcpyAsync(d_src, srcs + p0, 4 * size_t(kp), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(kPass), 0, s, d_src, kp, seen, fa, depth);
+        MsArgs a;
+        a.n = n;
+        a.out_rowptr = g->out.rowptr;
+        a.out_col = g->out.col;
+        a.in_rowptr = g->in.rowptr;
+        a.in_col = g->in.col;
+        a.seen = seen;
+        a.fa = fa;
+        a.fb = fb;
+        a.acc = acc;
+        a.depth = depth;
+        for (int w = 0; w < kWords; ++w) {
+            const int lo = w * 64, hi = std::min(kp, lo + 64);
+            a.full.w[w] = hi <= lo ? 0ULL : (hi - lo == 64 ? ~0ULL : ((1ULL << (hi - lo)) - 1));
+        }
+        a.flags = flags;
+        a.slots = slots;
+        // the seeds' distinct vertices form level 0's frontier
+        std::vector<int32_t> uniq(srcs + p0, srcs + p0 + kp);
+        std::sort(uniq.begin(), uniq.end());
+        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+        int64_t fe = 0;
+        for (int32_t v : uniq) fe += g->h_out_rowptr[v + 1] - g->h_out_rowptr[v];
+        for (int32_t i = 0; i < kp; ++i) traversed += g->h_out_rowptr[srcs[p0 + i] + 1] - g->h_out_rowptr[srcs[p0 + i]];
+        a.front_edges0 = fe;
+        a.push_limit = std::max<int64_t>(g->out.m / 20, 1);
+        a.max_levels = n + 1;
+        a.levels_out = d_levels;
+        void* args[] = {&a};
+        GHET_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_msbfs), dim3(grid), dim3(kBT), args, 0, s));
+        if (depth_out) {
+            dim3 tg(static_cast<unsigned>(ceil_div(n, 64)), static_cast<unsigned>(ceil_div(kp, 64)));
+            hipLaunchKernelGGL(k_ms_transpose, tg, dim3(256), 0, s, depth, n, kp, out_dev);
+            GHET_HIP(hipMemcpyAsync(depth_out + static_cast<int64_t>(p0) * n, out_dev, 4 * size_t(n) * kp,
+                                    hipMemcpyDeviceToHost, s));
+        }
+        GHET_HIP(hipMemcpyAsync(g->pinned, slots, kCtrBytes, hipMemcpyDeviceToHost, s));
+        GHET_HIP(hipStreamSynchronize(s));
+        const CtrSlot* hs = reinterpret_cast<const CtrSlot*>(g->pinned);
+        const int32_t lv = reinterpret_cast<const int32_t*>(hs + 4 * kSlots)[4];
+        max_levels_seen = std::max(max_levels_seen, lv);
+        traversed += static_cast<int64_t>(slot_sum(hs + 3 * kSlots, 1));
+    }
+    GHET_HIP(hipEventRecord(g->ev_end, s));
+    GHET_HIP(hipStreamSynchronize(s));
+    if (st) {
+        float ms = 0;
+        GHET_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
+        st->device_ms = ms;
+        st->levels = max_levels_seen;
+        st->edges_traversed = traversed;
+        st->extra[0] = grid;
+    }
+    return GHET_OK;
+}
+
+}  // namespace ghet

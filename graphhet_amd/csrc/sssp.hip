@@ -270,9 +270,10 @@ __global__ void k_pre_finish(int32_t* pre, int32_t n) {
 
 // Base bucket width: ~32 average weights per average out-degree (GPU near-far
 // favours wide buckets: a bucket should hold enough work to fill the chip).
+// The autotuner's class scales it by 2^(class-1).
 double sssp_base_delta(const ghet_graph* g) {
     const double avg_deg = g->n > 0 ? static_cast<double>(g->out.m) / g->n : 1.0;
-    return std::max(1e-12, 32.0 * 0.5 / std::max(avg_deg, 1.0));
+    return std::max(1e-12, 32.0 * 0.5 / std::max(avg_deg, 1.0)) * std::ldexp(1.0, g->tune.sssp_class - 1);
 }
 
 int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* pre_out, const ghet_opts* o,

@@ -89,13 +89,13 @@ __global__ void __launch_bounds__(kBT) k_dag(const int32_t* __restrict__ rowptr,
 // One wave per kEdgesPerTask consecutive edges (u, v) of X; for each, the
 // common out-neighbours w of u and v: cnt[w] += 1, cnt[u] += c, cnt[v] += c.
 __global__ void __launch_bounds__(kBT) k_tc_count(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                                  int32_t n, int64_t m, unsigned long long* cnt) {
+                                                  int32_t n, int64_t m, int ept, unsigned long long* cnt) {
     const int lane = lane_id();
-    const int64_t tasks = ceil_div(m, kEdgesPerTask);
+    const int64_t tasks = ceil_div(m, ept);
     const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
     const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
     for (int64_t t = wave_global; t < tasks; t += wave_stride) {
-        const int64_t e0 = t * kEdgesPerTask, e1 = min(e0 + kEdgesPerTask, m);
+        const int64_t e0 = t * ept, e1 = min(e0 + ept, m);
         int32_t u = upper_bound_i32(rowptr, static_cast<int64_t>(n) + 1, static_cast<int32_t>(e0)) - 1;
         int32_t ub = rowptr[u], ue = rowptr[u + 1];
         unsigned long long cu = 0;
@@ -178,6 +178,29 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
 
 }  // namespace
 
+int tc_prepare(ghet_graph* g, hipStream_t s) { return build_tc_layout(g, s); }
+
+// Enqueues the counting (fill, count, 6x scale) with no host synchronisation,
+// so it can be captured into a hipGraph.  ept: edges per wave task;
+// waves_per_cu: grid size in resident waves per CU.
+int enqueue_tc(ghet_graph* g, unsigned long long* cnt, int ept, int waves_per_cu, hipStream_t s) {
+    const int32_t n = g->n;
+    const TCLayout& L = g->tc;
+    const DevCSR& X = L.symmetric ? L.dag : g->out;
+    GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(cnt), 0ULL, n, s));
+    if (X.m > 0) {
+        const int64_t tasks = ceil_div(X.m, ept);
+        const int64_t cap = int64_t(256) * waves_per_cu / (kBT / 64);
+        hipLaunchKernelGGL(k_tc_count, dim3(grid_cap(std::min<int64_t>(ceil_div(tasks, kBT / 64), cap))), dim3(kBT), 0,
+                           s, X.rowptr, X.col, n, X.m, ept, cnt);
+    }
+    if (L.symmetric)
+        hipLaunchKernelGGL(k_scale_i64, dim3(grid_cap(std::min<int64_t>(ceil_div(n, 256), 8192))), dim3(256), 0, s, cnt,
+                           n, 6ULL);
+    GHET_HIP(hipGetLastError());
+    return GHET_OK;
+}
+
 int run_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* o, ghet_run_stats* st) {
     const int32_t n = g->n;
     if (!g->out.present()) return set_error(GHET_ERR_INVALID, "graph has no out-CSR");
@@ -190,15 +213,7 @@ int run_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* o, ghet_run_stats
     GHET_CHECK(g->ws.get(WS_TC_COUNT, 8 * size_t(n), &p));
     auto* cnt = static_cast<unsigned long long*>(p);
     GHET_HIP(hipEventRecord(g->ev_begin, s));
-    GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(cnt), 0ULL, n, s));
-    if (X.m > 0) {
-        const int64_t tasks = ceil_div(X.m, kEdgesPerTask);
-        hipLaunchKernelGGL(k_tc_count, dim3(grid_cap(std::min<int64_t>(ceil_div(tasks, kBT / 64), 256 * 32))),
-                           dim3(kBT), 0, s, X.rowptr, X.col, n, X.m, cnt);
-    }
-    if (L.symmetric)
-        hipLaunchKernelGGL(k_scale_i64, dim3(grid_cap(std::min<int64_t>(ceil_div(n, 256), 8192))), dim3(256), 0, s, cnt,
-                           n, 6ULL);
+    GHET_CHECK(enqueue_tc(g, cnt, g->tune.tc_ept, g->tune.tc_waves_per_cu, s));
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
     if (count_out) GHET_HIP(hipMemcpyAsync(count_out, cnt, 8 * size_t(n), hipMemcpyDeviceToHost, s));

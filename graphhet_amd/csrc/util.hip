@@ -54,11 +54,11 @@ __device__ T block_exclusive_scan(T x, T* lds_waves, T& total) {
     return wave_off + incl - x;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
-                                                               int32_t* __restrict__ aux) {
-    __shared__ int32_t lds[kScanThreads / 64];
+template <typename T>
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const T* __restrict__ in, int64_t n, T* __restrict__ aux) {
+    __shared__ T lds[kScanThreads / 64];
     int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
-    int32_t s = 0;
+    T s = 0;
     for (int j = 0; j < kScanItems; ++j) {
         int64_t i = base + j * kScanThreads + threadIdx.x;
         if (i < n) s += in[i];
@@ -67,22 +67,23 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int32_t* __r
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        int32_t t = 0;
+        T t = 0;
         for (int w = 0; w < kScanThreads / 64; ++w) t += lds[w];
         aux[blockIdx.x] = t;
     }
 }
 
-__global__ void __launch_bounds__(1024) k_scan_aux(int32_t* __restrict__ aux, int64_t nb) {
-    __shared__ int32_t lds[16];
-    __shared__ int32_t carry;
+template <typename T>
+__global__ void __launch_bounds__(1024) k_scan_aux(T* __restrict__ aux, int64_t nb) {
+    __shared__ T lds[16];
+    __shared__ T carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int64_t b = 0; b < nb; b += 1024) {
         int64_t i = b + threadIdx.x;
-        int32_t x = i < nb ? aux[i] : 0;
-        int32_t tot;
-        int32_t ex = block_exclusive_scan(x, lds, tot);
+        T x = i < nb ? aux[i] : 0;
+        T tot;
+        T ex = block_exclusive_scan(x, lds, tot);
         if (i < nb) aux[i] = carry + ex;
         __syncthreads();
         if (threadIdx.x == 0) carry += tot;
@@ -91,19 +92,20 @@ __global__ void __launch_bounds__(1024) k_scan_aux(int32_t* __restrict__ aux, in
     if (threadIdx.x == 0) aux[nb] = carry;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int32_t* in, int32_t* out, int64_t n,
-                                                              const int32_t* __restrict__ aux, int64_t nb) {
-    __shared__ int32_t lds[kScanThreads / 64];
+template <typename T>
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(const T* in, T* out, int64_t n, const T* __restrict__ aux,
+                                                              int64_t nb) {
+    __shared__ T lds[kScanThreads / 64];
     int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kScanItems;
-    int32_t v[kScanItems];
-    int32_t s = 0;
+    T v[kScanItems];
+    T s = 0;
     for (int j = 0; j < kScanItems; ++j) {
         int64_t i = base + j;
         v[j] = i < n ? in[i] : 0;
         s += v[j];
     }
-    int32_t tot;
-    int32_t run = block_exclusive_scan(s, lds, tot) + aux[blockIdx.x];
+    T tot;
+    T run = block_exclusive_scan(s, lds, tot) + aux[blockIdx.x];
     for (int j = 0; j < kScanItems; ++j) {
         int64_t i = base + j;
         if (i < n) out[i] = run;
@@ -114,17 +116,25 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const int32_t* in, 
 
 int64_t scan_aux_ints(int64_t n) { return ceil_div(std::max<int64_t>(n, 1), kScanTile) + 1; }
 
-int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* aux, hipStream_t s) {
+template <typename T>
+int exclusive_scan(const T* in, T* out, int64_t n, T* aux, hipStream_t s) {
     if (n <= 0) {
-        GHET_CHECK(fill<int32_t>(out, 0, 1, s));
+        GHET_CHECK(fill<T>(out, 0, 1, s));
         return GHET_OK;
     }
     int64_t nb = ceil_div(n, kScanTile);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in, n, aux);
-    hipLaunchKernelGGL(k_scan_aux, dim3(1), dim3(1024), 0, s, aux, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in, out, n, aux, nb);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in, n, aux);
+    hipLaunchKernelGGL(k_scan_aux<T>, dim3(1), dim3(1024), 0, s, aux, nb);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s, in, out, n, aux, nb);
     GHET_HIP(hipGetLastError());
     return GHET_OK;
+}
+
+int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* aux, hipStream_t s) {
+    return exclusive_scan<int32_t>(in, out, n, aux, s);
+}
+int exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, int64_t* aux, hipStream_t s) {
+    return exclusive_scan<int64_t>(in, out, n, aux, s);
 }
 
 // ------------------------------------------------------- reached stats ----

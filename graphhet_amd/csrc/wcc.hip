@@ -69,7 +69,7 @@ __global__ void k_wcc_gather(const int32_t* parent, const int32_t* idx, int32_t 
 // Link all neighbours from position r0 on; vertices whose root is `skip`
 // are left out.  Light rows go lane-serial, heavier rows wave-cooperative.
 __global__ void __launch_bounds__(256) k_wcc_full(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                                  int32_t n, int32_t r0, int32_t skip, int32_t* parent) {
+                                                  int32_t n, int32_t r0, int32_t skip, int light_max, int32_t* parent) {
     const int lane = lane_id();
     const int64_t groups = ceil_div(n, 64);
     const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256) k_wcc_full(const int32_t* __restrict__ ro
             e = rowptr[v + 1];
             act = b < e;
         }
-        const bool light = act && (e - b) <= 16;
+        const bool light = act && (e - b) <= light_max;
         if (light)
             for (int32_t i = b; i < e; ++i) link(static_cast<int32_t>(v), col[i], parent);
         uint64_t heavy = __ballot(act && !light);
@@ -154,9 +154,9 @@ int run_wcc(ghet_graph* g, int32_t* label_out, const ghet_opts* o, ghet_run_stat
             }
         }
         const unsigned gw = grid_for(ceil_div(n, 64), 4);
-        hipLaunchKernelGGL(k_wcc_full, dim3(gw), dim3(256), 0, s, A.rowptr, A.col, n, can_skip ? 2 : 0, skip, parent);
+        hipLaunchKernelGGL(k_wcc_full, dim3(gw), dim3(256), 0, s, A.rowptr, A.col, n, can_skip ? 2 : 0, skip, g->tune.wcc_light, parent);
         if (both)
-            hipLaunchKernelGGL(k_wcc_full, dim3(gw), dim3(256), 0, s, g->in.rowptr, g->in.col, n, 2, skip, parent);
+            hipLaunchKernelGGL(k_wcc_full, dim3(gw), dim3(256), 0, s, g->in.rowptr, g->in.col, n, 2, skip, g->tune.wcc_light, parent);
         hipLaunchKernelGGL(k_wcc_compress, dim3(grid_for(n)), dim3(256), 0, s, parent, n);
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
@@ -169,6 +169,35 @@ int run_wcc(ghet_graph* g, int32_t* label_out, const ghet_opts* o, ghet_run_stat
         st->device_ms = ms;
         st->levels = 1;
     }
+    return GHET_OK;
+}
+
+}  // namespace ghet
+
+namespace ghet {
+
+// WCC without the sampled-mode host round trip (no component skipping): two
+// sampled link rounds, compress, link everything, compress.  Nothing waits on
+// the host, so the sequence can be captured into a hipGraph.
+int enqueue_wcc_plain(ghet_graph* g, int32_t* parent, int light, int waves_per_cu, hipStream_t s) {
+    const int32_t n = g->n;
+    if (n == 0) return GHET_OK;
+    const bool use_all = g->all.present() && g->directed;
+    const bool both = !use_all && g->directed && g->in.present();
+    const DevCSR& A = use_all ? g->all : g->out;
+    hipLaunchKernelGGL(k_wcc_init, dim3(grid_for(n)), dim3(256), 0, s, parent, n);
+    for (int r = 0; r < 2; ++r) {
+        hipLaunchKernelGGL(k_wcc_sample, dim3(grid_for(n)), dim3(256), 0, s, A.rowptr, A.col, n, r, parent);
+        if (both)
+            hipLaunchKernelGGL(k_wcc_sample, dim3(grid_for(n)), dim3(256), 0, s, g->in.rowptr, g->in.col, n, r, parent);
+        hipLaunchKernelGGL(k_wcc_compress, dim3(grid_for(n)), dim3(256), 0, s, parent, n);
+    }
+    const unsigned gw = static_cast<unsigned>(
+        std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), int64_t(256) * waves_per_cu / 4)));
+    hipLaunchKernelGGL(k_wcc_full, dim3(gw), dim3(256), 0, s, A.rowptr, A.col, n, 2, -1, light, parent);
+    if (both) hipLaunchKernelGGL(k_wcc_full, dim3(gw), dim3(256), 0, s, g->in.rowptr, g->in.col, n, 2, -1, light, parent);
+    hipLaunchKernelGGL(k_wcc_compress, dim3(grid_for(n)), dim3(256), 0, s, parent, n);
+    GHET_HIP(hipGetLastError());
     return GHET_OK;
 }
 
