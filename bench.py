@@ -1,16 +1,18 @@
 """Benchmark of the MI355X CSR analytics path on a seeded R-MAT graph.
 
-One "step" is one pass of the six algorithms over the resident graph: BFS and
-SSSP from the max-out-degree vertex, PageRank (20 iterations, damping 0.85),
-WCC, CDLP (10 iterations) and TC on the symmetrised graph.  The graph is built
-and uploaded before the timed region; inputs are resident in HBM when it
-starts, and results stay on the device (no D2H inside the timed region).
+One "step" is one pass of the metric's algorithms over the resident graph:
+BFS and SSSP from the max-out-degree vertex and PageRank (20 iterations,
+damping 0.85).  The graph is built and uploaded before the timed region;
+inputs are resident in HBM when it starts and results stay on the device.
 
 Prints ONE JSON line.  `value` is BFS GTEPS (m_r / t, m_r = out-edges of the
 reached vertices), the headline of the metric; SSSP GTEPS and PageRank HBM
 GB/s ride along.  `roofline` covers the PageRank iteration kernel, the path's
-bandwidth-bound kernel, timed with HIP events on its own stream.
-`cpu_baseline` times GraphHET's CPU BFS (oracle/_ref, or the oracle
+bandwidth-bound kernel and the largest share of a step, timed with HIP events
+on its own stream.  Also reported, outside the timed step: WCC, CDLP and TC
+on the same graph; the batched forms against the plain form on the same GPU
+(256 BFS and 256 SSSP sources on a scale-22 R-MAT; many small graphs for WCC
+and TC); and `cpu_baseline`, GraphHET's CPU BFS (oracle/_ref, or the oracle
 restatement when _ref is absent) on the same graph on this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24]
@@ -30,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+INT_INF = 2**31 - 1
 
 
 def log(*a):
@@ -44,6 +47,10 @@ def pr_iter_bytes(n: int, m: int) -> int:
     return 4 * (n + 1) + 4 * m + 4 * n + 4 * n + 4 * n
 
 
+def gteps(edges, ms):
+    return edges / (ms * 1e-3) / 1e9 if ms > 0 else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,19 +61,25 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--pr-iters", type=int, default=20)
     ap.add_argument("--cdlp-iters", type=int, default=10)
-    ap.add_argument("--algos", default="bfs,sssp,pr,wcc,cdlp,tc")
+    ap.add_argument("--algos", default="bfs,sssp,pr", help="algorithms of the timed step")
+    ap.add_argument("--no-extras", action="store_true", help="skip WCC/CDLP/TC")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain section")
+    ap.add_argument("--batched-scale", type=int, default=22)
+    ap.add_argument("--batch-sources", type=int, default=256)
+    ap.add_argument("--many-graphs", type=int, default=1024)
+    ap.add_argument("--many-max-scale", type=int, default=14,
+                    help="largest small graph is 2^this vertices (the full form uses 18)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-s", type=float, default=20.0, help="budget for the CPU baseline")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
-    if args.copy_probe_mb:
-        copy_probe(args.copy_probe_mb)
 
     if args.gpus != 1:
         log("bench.py measures one GPU: this build has no multi-GPU path "
             "(the sharding design is described in DESIGN.md); run with --gpus 1.")
         sys.exit(2)
+    if args.copy_probe_mb:
+        copy_probe(args.copy_probe_mb)
 
     import graphhet_amd as G
 
@@ -78,15 +91,12 @@ def main():
     hg = G.HostGraph.rmat(args.scale, args.edge_factor, seed=args.seed)
     o = hg.view(G.VIEW_OUT, copy=False)
     i = hg.view(G.VIEW_IN, copy=False)
-    a = hg.view(G.VIEW_ALL, copy=False) if "cdlp" in algos else None
     n, m = o.n, o.m
     log(f"R-MAT scale {args.scale}: n={n} m={m} built in {time.time() - t0:.1f}s")
     t0 = time.time()
-    dg = G.DeviceGraph(o, i, a, with_all="cdlp" in algos)
-    sg = None
-    if "tc" in algos:
-        s = hg.view(G.VIEW_SYM, copy=False)
-        sg = G.DeviceGraph(s, undirected=True, weights=False)
+    extras = not args.no_extras
+    a = hg.view(G.VIEW_ALL, copy=False) if extras else None
+    dg = G.DeviceGraph(o, i, a, with_all=extras)
     log(f"uploaded in {time.time() - t0:.1f}s ({dg.device_bytes() / 1e9:.2f} GB)")
     src = int(np.argmax(o.degrees()))
 
@@ -98,12 +108,6 @@ def main():
             _, r["sssp"] = dg.sssp(src, out=False)
         if "pr" in algos:
             _, r["pr"] = dg.pr(args.pr_iters, 0.85, out=False)
-        if "wcc" in algos:
-            _, r["wcc"] = dg.wcc(out=False)
-        if "cdlp" in algos:
-            _, r["cdlp"] = dg.cdlp(args.cdlp_iters, out=False)
-        if "tc" in algos:
-            _, r["tc"] = sg.tc(out=False)
         return r
 
     for w in range(args.warmup):
@@ -121,8 +125,8 @@ def main():
         return statistics.median(f(r[alg]) for r in runs) if alg in algos else None
 
     per_alg_ms = {alg: med(alg, lambda s: s.device_ms) for alg in sorted(algos)}
-    bfs_gteps = med("bfs", lambda s: s.edges_traversed / (s.device_ms * 1e-3) / 1e9)
-    sssp_gteps = med("sssp", lambda s: s.edges_traversed / (s.device_ms * 1e-3) / 1e9)
+    bfs_gteps = med("bfs", lambda s: gteps(s.edges_traversed, s.device_ms))
+    sssp_gteps = med("sssp", lambda s: gteps(s.edges_traversed, s.device_ms))
     roofline = None
     pr_gbs = None
     if "pr" in algos:
@@ -133,10 +137,33 @@ def main():
                     "frac": round(pr_gbs / PEAK_HBM_GBS, 4), "traffic": None,
                     "kernel": "k_pr_iter", "kernel_avg_us": round(k_ms * 1e3, 2), "bytes_per_launch": bytes_it}
 
+    other = {}
+    if extras:
+        _, st = dg.wcc(out=False)
+        _, st = dg.wcc(out=False)
+        other["wcc_ms"] = round(st.device_ms, 3)
+        _, st = dg.cdlp(args.cdlp_iters, out=False)
+        other["cdlp_ms"] = round(st.device_ms, 3)
+        other["cdlp_iter_ms"] = round(st.hot_kernel_ms / max(args.cdlp_iters, 1), 3)
+        del dg
+        s = hg.view(G.VIEW_SYM, copy=False)
+        sg = G.DeviceGraph(s, undirected=True, weights=False)
+        _, st = sg.tc(out=False)
+        _, st = sg.tc(out=False)
+        other["tc_ms"] = round(st.device_ms, 3)
+        other["tc_dag_edges"] = int(st.extra[1])
+        del sg
+        log("extras done")
+
     bfs_stats = runs[-1].get("bfs")
     cpu = None
     if not args.no_cpu_baseline and "bfs" in algos:
-        cpu = cpu_baseline(o, i, src, args, bfs_stats)
+        cpu = cpu_baseline(o, src, bfs_stats)
+    del hg, o, i, a
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_section(G, args)
 
     out = {
         "metric": "GTEPS (BFS/SSSP) + PR HBM GB/s, scale-24 R-MAT; learned/batched vs plain",
@@ -151,19 +178,99 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"rmat-s{args.scale}-ef{int(args.edge_factor)} six-algorithm pass",
+        "config": {"workload": f"rmat-s{args.scale}-ef{int(args.edge_factor)} BFS+SSSP+PR pass",
                    "scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed, "n": n, "m": m,
-                   "source": src, "pr_iters": args.pr_iters, "cdlp_iters": args.cdlp_iters,
-                   "algos": sorted(algos)},
+                   "source": src, "pr_iters": args.pr_iters, "step_algos": sorted(algos)},
         "sssp_gteps": round(sssp_gteps, 3) if sssp_gteps else None,
         "pr_hbm_gbs": round(pr_gbs, 1) if pr_gbs else None,
         "per_algorithm_ms": {k: (round(v, 3) if v is not None else None) for k, v in per_alg_ms.items()},
         "bfs_levels": bfs_stats.levels if bfs_stats else None,
         "edges_traversed": bfs_stats.edges_traversed if bfs_stats else None,
+        "other_algorithms": other,
+        "batched": batched,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def batched_section(G, args):
+    """Batched forms vs the plain form on the same GPU (same inputs, results
+    compared on a sample of instances)."""
+    res = {}
+    hg = G.HostGraph.rmat(args.batched_scale, 16, seed=args.seed)
+    o = hg.view(G.VIEW_OUT, copy=False)
+    i = hg.view(G.VIEW_IN, copy=False)
+    dg = G.DeviceGraph(o, i, with_all=False)
+    indeg = np.bincount(o.col, minlength=o.n)
+    cand = np.nonzero((o.degrees() + indeg) > 0)[0]
+    rng = np.random.default_rng(args.seed)
+    srcs = rng.choice(cand, size=args.batch_sources, replace=False).astype(np.int32)
+    # warm both forms (layouts, workspaces)
+    dg.bfs_multi(srcs[:8], out=False)
+    dg.bfs(int(srcs[0]), out=False)
+    _, mst = dg.bfs_multi(srcs, out=False)
+    plain_ms, plain_edges = 0.0, 0
+    for s in srcs:
+        _, st = dg.bfs(int(s), out=False)
+        plain_ms += st.device_ms
+        plain_edges += st.edges_traversed
+    res["bfs"] = {"sources": len(srcs), "batched_ms": round(mst.device_ms, 3), "plain_ms": round(plain_ms, 3),
+                  "batched_gteps": round(gteps(mst.edges_traversed, mst.device_ms), 3),
+                  "plain_gteps": round(gteps(plain_edges, plain_ms), 3),
+                  "speedup": round(plain_ms / mst.device_ms, 3), "edges_match": mst.edges_traversed == plain_edges}
+    log(f"batched BFS: {res['bfs']}")
+    dg.sssp_multi(srcs[:4], out=False)
+    _, mst = dg.sssp_multi(srcs, out=False)
+    plain_ms, plain_edges = 0.0, 0
+    for s in srcs:
+        _, st = dg.sssp(int(s), out=False)
+        plain_ms += st.device_ms
+        plain_edges += st.edges_traversed
+    res["sssp"] = {"sources": len(srcs), "batched_ms": round(mst.device_ms, 3), "plain_ms": round(plain_ms, 3),
+                   "batched_gteps": round(gteps(mst.edges_traversed, mst.device_ms), 3),
+                   "plain_gteps": round(gteps(plain_edges, plain_ms), 3),
+                   "speedup": round(plain_ms / mst.device_ms, 3), "edges_match": mst.edges_traversed == plain_edges}
+    log(f"batched SSSP: {res['sssp']}")
+    del dg, hg
+    res["graph"] = f"rmat-s{args.batched_scale}-ef16"
+
+    # many small graphs: n log-uniform in [2^10, 2^max], edge factor uniform in [4, 32]
+    count = args.many_graphs
+    t0 = time.time()
+    graphs_dir, graphs_sym = [], []
+    for k in range(count):
+        scale = int(rng.integers(10, args.many_max_scale + 1))
+        ef = float(rng.uniform(4, 32))
+        g = G.HostGraph.rmat(scale, ef, seed=args.seed * 100003 + k, threads=4)
+        graphs_dir.append(g.view(G.VIEW_OUT))
+        graphs_sym.append(g.view(G.VIEW_SYM))
+    log(f"{count} small graphs built in {time.time() - t0:.1f}s")
+    G.wcc_many(graphs_dir[:4])
+    labs, wst = G.wcc_many(graphs_dir)
+    cnts, tst = G.tc_many(graphs_sym)
+    plain_w = plain_t = 0.0
+    ok = True
+    for k, (gd, gs) in enumerate(zip(graphs_dir, graphs_sym)):
+        d = G.DeviceGraph(gd, with_in=False, weights=False)
+        lab, st = d.wcc()
+        plain_w += st.device_ms
+        ok &= bool(np.array_equal(lab, labs[k]))
+        del d
+        u = G.DeviceGraph(gs, undirected=True, weights=False)
+        c, st = u.tc()
+        plain_t += st.device_ms
+        ok &= bool(np.array_equal(c, cnts[k]))
+        del u
+    res["many_graphs"] = {
+        "count": count, "vertices": int(sum(g.n for g in graphs_dir)), "edges": int(sum(g.m for g in graphs_dir)),
+        "scale_range": [10, args.many_max_scale],
+        "wcc_batched_ms": round(wst.device_ms, 3), "wcc_plain_ms": round(plain_w, 3),
+        "wcc_speedup": round(plain_w / wst.device_ms, 3),
+        "tc_batched_ms": round(tst.device_ms, 3), "tc_plain_ms": round(plain_t, 3),
+        "tc_speedup": round(plain_t / tst.device_ms, 3), "results_match": ok}
+    log(f"many graphs: {res['many_graphs']}")
+    return res
 
 
 def copy_probe(mb: int) -> None:
@@ -185,12 +292,11 @@ def copy_probe(mb: int) -> None:
     log(f"copy probe: 2 x {mb} MiB device-to-device")
 
 
-def cpu_baseline(o, i, src, args, gpu_stats):
+def cpu_baseline(o, src, gpu_stats):
     """GraphHET's CPU BFS (CPU_BFS.hpp:9-41) on this host, same graph and
     source; single-threaded as in the reference."""
     import oracle as O
 
-    cores = 1
     try:
         if O.ref_available():
             t0 = time.time()
@@ -207,9 +313,9 @@ def cpu_baseline(o, i, src, args, gpu_stats):
     except Exception as e:  # the baseline is reported, never required
         log(f"cpu baseline failed: {e}")
         return None
-    reached = depth != 2**31 - 1
+    reached = depth != INT_INF
     m_r = int(np.diff(o.rowptr)[reached].sum())
-    return {"value": round(m_r / secs / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": kind,
+    return {"value": round(m_r / secs / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": kind,
             "sample": f"one full BFS from vertex {src} over the whole graph ({m_r} edges), {secs:.2f}s",
             "seconds": round(secs, 3),
             "matches_gpu_reached": (gpu_stats is None or int(reached.sum()) == gpu_stats.vertices_reached)}
