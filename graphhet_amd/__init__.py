@@ -385,12 +385,14 @@ class DeviceGraph:
         _check(lib().ghet_tc(self._h, _ptr(cnt, C.c_int64), C.byref(_opts()), C.byref(st)))
         return cnt, st
 
-    def bfs_multi(self, srcs, out=True, schedule=SCHED_HEURISTIC):
+    def bfs_multi(self, srcs, out=True, schedule=SCHED_HEURISTIC, push_div=0):
         srcs = np.ascontiguousarray(srcs, dtype=np.int32)
         depth = np.empty((len(srcs), self.n), np.int32) if out else None
         st = RunStats()
+        o = _opts(schedule)
+        o.reserved[2] = push_div
         _check(lib().ghet_bfs_multi(self._h, len(srcs), _ptr(srcs, C.c_int32), _ptr(depth, C.c_int32),
-                                    C.byref(_opts(schedule)), C.byref(st)))
+                                    C.byref(o), C.byref(st)))
         return depth, st
 
     def autotune(self, cache_path: str | None = None) -> int:

@@ -12,8 +12,11 @@
 //         (atomicOr), then every vertex takes next = acc & ~seen;
 //   pull  every vertex not yet seen by all sources ORs front[u] over its
 //         in-neighbours, stopping as soon as nothing unseen is left.
-// Depths go to a vertex-major [n][256] block (a source's new bit at level L
-// writes L); the batch is transposed to source-major rows for the caller.
+// The traversal writes only the 32-byte set of new bits per (level, vertex)
+// into a level log (one coalesced store); depths are materialised afterwards
+// by a pass that reads a vertex's log column and writes source-major rows
+// with coalesced stores.  Levels past the log's depth fall back to direct
+// writes.
 #include <hip/hip_cooperative_groups.h>
 
 #include "graph_internal.hpp"
@@ -26,6 +29,7 @@ namespace {
 constexpr int kBT = 256;
 constexpr int kWords = 4;
 constexpr int kPass = 64 * kWords;  // sources per pass
+constexpr int kLogLevels = 32;      // levels recorded in the log
 
 struct Bits {
     uint64_t w[kWords];
@@ -35,11 +39,12 @@ struct MsArgs {
     int32_t n;
     const int32_t *out_rowptr, *out_col, *in_rowptr, *in_col;
     Bits *seen, *fa, *fb, *acc;
-    int32_t* depth;  // [n][kPass]
+    Bits* log;       // [kLogLevels][n]: bits first set at level L+1
+    int32_t* out;    // [kp][n] source-major depths (sources and deep levels written directly)
     Bits full;       // active sources of this pass
     int32_t* flags;  // [3] "something new at level L" (L % 3)
-    CtrSlot* slots;  // [3][kSlots] per level: v[0] next frontier out-edges; then [kSlots] totals: v[1]
-                     // traversed (source, edge) pairs
+    CtrSlot* slots;  // [3][kSlots] per level (v[0] next-frontier out-edges), then [kSlots] totals
+                     // (v[1] traversed (source, edge) pairs, v[2] edges scanned by pull, v[3] edges pushed)
     int64_t front_edges0;
     int64_t push_limit;
     int32_t max_levels;
@@ -60,7 +65,7 @@ __device__ __forceinline__ bool covers(const Bits& a, const Bits& seen, const Bi
     return c;
 }
 
-// New bits of v at `level`: update seen, write depths, count stats.
+// New bits of v at `level`: update seen, log them, count stats.
 __device__ __forceinline__ void settle(const MsArgs& a, int64_t v, const Bits& sn, const Bits& nb, int32_t level,
                                        unsigned long long& ne, unsigned long long& tr) {
     Bits s2;
@@ -70,13 +75,16 @@ __device__ __forceinline__ void settle(const MsArgs& a, int64_t v, const Bits& s
         pop += __popcll(nb.w[k]);
     }
     a.seen[v] = s2;
-    int32_t* row = a.depth + v * kPass;
-    for (int k = 0; k < kWords; ++k) {
-        uint64_t x = nb.w[k];
-        while (x) {
-            const int b = __ffsll(static_cast<unsigned long long>(x)) - 1;
-            x &= x - 1;
-            row[k * 64 + b] = level;
+    if (level <= kLogLevels) {
+        a.log[static_cast<int64_t>(level - 1) * a.n + v] = nb;
+    } else {
+        for (int k = 0; k < kWords; ++k) {
+            uint64_t x = nb.w[k];
+            while (x) {
+                const int b = __ffsll(static_cast<unsigned long long>(x)) - 1;
+                x &= x - 1;
+                a.out[static_cast<int64_t>(k * 64 + b) * a.n + v] = level;
+            }
         }
     }
     const unsigned long long d = a.out_rowptr[v + 1] - a.out_rowptr[v];
@@ -91,13 +99,15 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
     __shared__ int s_stop;
     const int lane = lane_id();
     const int64_t n = a.n;
+    CtrSlot* totals = a.slots + 3 * kSlots;
     int64_t front_edges = a.front_edges0;
     int32_t level = 0;
     for (; level < a.max_levels; ++level) {
         const Bits* front = (level & 1) ? a.fb : a.fa;
         Bits* next = (level & 1) ? a.fa : a.fb;
+        Bits* logl = level < kLogLevels ? a.log + static_cast<int64_t>(level) * n : nullptr;
         CtrSlot* slots = a.slots + (level % 3) * kSlots;
-        unsigned long long ne = 0, tr = 0;
+        unsigned long long ne = 0, tr = 0, scanned = 0, pushed = 0;
         bool any = false;
         if (front_edges < a.push_limit) {
             // push phase A: OR each frontier set into acc of its out-neighbours
@@ -112,6 +122,7 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                         e = a.out_rowptr[u + 1];
                     }
                 }
+                pushed += e - b;
                 const bool light = e - b <= 32;
                 if (light)
                     for (int32_t i = b; i < e; ++i) {
@@ -150,6 +161,7 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                     }
                 }
                 next[v] = nb;
+                if (logl && !any_bits(nb)) logl[v] = nb;
             }
         } else {
             // pull: every vertex not yet seen by all sources scans its in-list
@@ -170,6 +182,7 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                 if (todo && light)
                     for (int32_t i = b; i < e; ++i) {
                         const Bits f = front[a.in_col[i]];
+                        ++scanned;
                         for (int k = 0; k < kWords; ++k) acc.w[k] |= f.w[k];
                         if (covers(acc, sn, a.full)) break;
                     }
@@ -184,6 +197,7 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                     for (int32_t i0 = hb; i0 < he; i0 += 64) {
                         if (i0 + lane < he) {
                             const Bits f = front[a.in_col[i0 + lane]];
+                            ++scanned;
                             for (int k = 0; k < kWords; ++k) hacc.w[k] |= f.w[k];
                         }
                         hacc = wave_or(hacc);
@@ -197,6 +211,8 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                     if (any_bits(nb)) {
                         settle(a, v, sn, nb, level + 1, ne, tr);
                         any = true;
+                    } else if (logl) {
+                        logl[v] = nb;
                     }
                     next[v] = nb;
                 }
@@ -206,9 +222,13 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
             __hip_atomic_store(&a.flags[level % 3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ne = block_sum_u64<kBT>(ne, s_red);
         tr = block_sum_u64<kBT>(tr, s_red);
+        scanned = block_sum_u64<kBT>(scanned, s_red);
+        pushed = block_sum_u64<kBT>(pushed, s_red);
         if (threadIdx.x == 0) {
             slot_add(slots, 0, ne);
-            slot_add(a.slots + 3 * kSlots, 1, tr);
+            slot_add(totals, 1, tr);
+            slot_add(totals, 2, scanned);
+            slot_add(totals, 3, pushed);
         }
         if (blockIdx.x == 0) {
             // clear the flag and slots of level+1 (last read two levels ago)
@@ -234,33 +254,35 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.levels_out = level;
 }
 
-__global__ void k_ms_seed(const int32_t* __restrict__ srcs, int32_t k, Bits* seen, Bits* front, int32_t* depth) {
+__global__ void k_ms_seed(const int32_t* __restrict__ srcs, int32_t k, int32_t n, Bits* seen, Bits* front,
+                          int32_t* out) {
     const int i = threadIdx.x;
     if (i < k) {
         const int32_t v = srcs[i];
         atomicOr(reinterpret_cast<unsigned long long*>(&seen[v].w[i >> 6]), 1ULL << (i & 63));
         atomicOr(reinterpret_cast<unsigned long long*>(&front[v].w[i >> 6]), 1ULL << (i & 63));
-        depth[static_cast<int64_t>(v) * kPass + i] = 0;
+        out[static_cast<int64_t>(i) * n + v] = 0;
     }
 }
 
-// [n][kPass] vertex-major -> rows of `out` (row r = source r, stride n),
-// 64x64 tiles through LDS.
-__global__ void __launch_bounds__(256) k_ms_transpose(const int32_t* __restrict__ vm, int32_t n, int32_t k,
-                                                      int32_t* __restrict__ out) {
-    __shared__ int32_t t[64][65];
-    const int64_t v0 = static_cast<int64_t>(blockIdx.x) * 64;
-    const int s0 = blockIdx.y * 64;
-    for (int r = threadIdx.x / 64; r < 64; r += 4) {
-        const int64_t v = v0 + r;
-        const int s = s0 + (threadIdx.x & 63);
-        t[r][threadIdx.x & 63] = (v < n && s < kPass) ? vm[v * kPass + s] : 0;
-    }
-    __syncthreads();
-    for (int r = threadIdx.x / 64; r < 64; r += 4) {
-        const int s = s0 + r;
-        const int64_t v = v0 + (threadIdx.x & 63);
-        if (s < k && v < n) out[static_cast<int64_t>(s) * n + v] = t[threadIdx.x & 63][r];
+// Depths from the level log: thread (v, word k) reads log[0..levels)[v].w[k]
+// and, for each of the 64 sources of word k first reached within the log,
+// writes out[source][v] (consecutive threads = consecutive v: coalesced).
+__global__ void __launch_bounds__(256) k_ms_materialize(const Bits* __restrict__ log, int32_t levels, int32_t n,
+                                                        int32_t kp, int32_t* __restrict__ out) {
+    const int k = blockIdx.y;
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint64_t done = 0;
+    for (int32_t L = 0; L < levels; ++L) {
+        uint64_t x = log[static_cast<int64_t>(L) * n + v].w[k] & ~done;
+        done |= x;
+        while (x) {
+            const int b = __ffsll(static_cast<unsigned long long>(x)) - 1;
+            x &= x - 1;
+            const int s = k * 64 + b;
+            if (s < kp) out[static_cast<int64_t>(s) * n + v] = L + 1;
+        }
     }
 }
 
@@ -276,15 +298,17 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
     if (k == 0) return GHET_OK;
     hipStream_t s = pick_stream(g, o);
     void* p;
-    Bits *seen, *fa, *fb, *acc;
-    int32_t *depth, *flags, *d_src, *d_levels, *out_dev = nullptr;
+    Bits *seen, *fa, *fb, *acc, *log;
+    int32_t *out, *flags, *d_src, *d_levels;
     CtrSlot* slots;
     GHET_CHECK(g->ws.get(WS_MULTI_A, sizeof(Bits) * 4 * size_t(n), &p));
     seen = static_cast<Bits*>(p);
     fa = seen + n;
     fb = fa + n;
     acc = fb + n;
-    GHET_CHECK(g->ws.get(WS_MULTI_B, 4 * size_t(n) * kPass, &p)); depth = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_MULTI_B, sizeof(Bits) * kLogLevels * size_t(n), &p)); log = static_cast<Bits*>(p);
+    const int32_t kmax = std::min(k, kPass);
+    GHET_CHECK(g->ws.get(WS_MULTI_D, 4 * size_t(n) * kmax, &p)); out = static_cast<int32_t*>(p);
     // counters: [3][kSlots] per-level slots, [kSlots] totals, then flags[4], levels
     constexpr size_t kCtrBytes = sizeof(CtrSlot) * kSlots * 4 + 64;
     GHET_CHECK(g->ws.get(WS_COUNTERS, kCtrBytes, &p));
@@ -292,26 +316,24 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
     flags = reinterpret_cast<int32_t*>(slots + 4 * kSlots);
     d_levels = flags + 4;
     GHET_CHECK(g->ws.get(WS_MULTI_C, 4 * size_t(kPass), &p)); d_src = static_cast<int32_t*>(p);
-    if (depth_out) {
-        GHET_CHECK(g->ws.get(WS_MULTI_D, 4 * size_t(n) * std::min(k, kPass), &p));
-        out_dev = static_cast<int32_t*>(p);
-    }
     int dev_blocks = 0, cus = 0;
     GHET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&dev_blocks, reinterpret_cast<const void*>(k_msbfs), kBT, 0));
     GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
-    const unsigned grid = static_cast<unsigned>(std::max(1, std::min(dev_blocks, 4) * cus));
-    const int64_t words = sizeof(Bits) / 8 * 4 * int64_t(n);
+    const unsigned grid = static_cast<unsigned>(std::max(1, std::min(dev_blocks, 8) * cus));
+    const int64_t bits_words = sizeof(Bits) / 8 * 4 * int64_t(n);
+    const bool have_opts = o && o->size >= sizeof(ghet_opts);
+    const int64_t push_div = (have_opts && o->reserved[2] > 0) ? o->reserved[2] : 20;
 
-    int64_t traversed = 0;
+    int64_t traversed = 0, scanned = 0, pushed = 0;
     int32_t max_levels_seen = 0;
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     for (int32_t p0 = 0; p0 < k; p0 += kPass) {
         const int32_t kp = std::min(kPass, k - p0);
-        GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(seen), 0, words, s));
-        GHET_CHECK(fill<int32_t>(depth, kIntInf, int64_t(n) * kPass, s));
+        GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(seen), 0, bits_words, s));
+        GHET_CHECK(fill<int32_t>(out, kIntInf, int64_t(n) * kp, s));
         GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(slots), 0, kCtrBytes / 8, s));
         GHET_HIP(hipMemcpyAsync(d_src, srcs + p0, 4 * size_t(kp), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(kPass), 0, s, d_src, kp, seen, fa, depth);
+        hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(kPass), 0, s, d_src, kp, n, seen, fa, out);
         MsArgs a;
         a.n = n;
         a.out_rowptr = g->out.rowptr;
@@ -322,7 +344,8 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
         a.fa = fa;
         a.fb = fb;
         a.acc = acc;
-        a.depth = depth;
+        a.log = log;
+        a.out = out;
         for (int w = 0; w < kWords; ++w) {
             const int lo = w * 64, hi = std::min(kp, lo + 64);
             a.full.w[w] = hi <= lo ? 0ULL : (hi - lo == 64 ? ~0ULL : ((1ULL << (hi - lo)) - 1));
@@ -337,23 +360,28 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
         for (int32_t v : uniq) fe += g->h_out_rowptr[v + 1] - g->h_out_rowptr[v];
         for (int32_t i = 0; i < kp; ++i) traversed += g->h_out_rowptr[srcs[p0 + i] + 1] - g->h_out_rowptr[srcs[p0 + i]];
         a.front_edges0 = fe;
-        a.push_limit = std::max<int64_t>(g->out.m / 20, 1);
+        a.push_limit = std::max<int64_t>(g->out.m / push_div, 1);
         a.max_levels = n + 1;
         a.levels_out = d_levels;
         void* args[] = {&a};
         GHET_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_msbfs), dim3(grid), dim3(kBT), args, 0, s));
-        if (depth_out) {
-            dim3 tg(static_cast<unsigned>(ceil_div(n, 64)), static_cast<unsigned>(ceil_div(kp, 64)));
-            hipLaunchKernelGGL(k_ms_transpose, tg, dim3(256), 0, s, depth, n, kp, out_dev);
-            GHET_HIP(hipMemcpyAsync(depth_out + static_cast<int64_t>(p0) * n, out_dev, 4 * size_t(n) * kp,
-                                    hipMemcpyDeviceToHost, s));
-        }
         GHET_HIP(hipMemcpyAsync(g->pinned, slots, kCtrBytes, hipMemcpyDeviceToHost, s));
         GHET_HIP(hipStreamSynchronize(s));
         const CtrSlot* hs = reinterpret_cast<const CtrSlot*>(g->pinned);
         const int32_t lv = reinterpret_cast<const int32_t*>(hs + 4 * kSlots)[4];
         max_levels_seen = std::max(max_levels_seen, lv);
         traversed += static_cast<int64_t>(slot_sum(hs + 3 * kSlots, 1));
+        scanned += static_cast<int64_t>(slot_sum(hs + 3 * kSlots, 2));
+        pushed += static_cast<int64_t>(slot_sum(hs + 3 * kSlots, 3));
+        // the last level found nothing new; levels [0, lv) hold log entries
+        const int32_t logged = std::min(lv, kLogLevels);
+        if (logged > 0) {
+            dim3 mg(static_cast<unsigned>(ceil_div(n, 256)), static_cast<unsigned>(ceil_div(kp, 64)));
+            hipLaunchKernelGGL(k_ms_materialize, mg, dim3(256), 0, s, log, logged, n, kp, out);
+        }
+        if (depth_out)
+            GHET_HIP(hipMemcpyAsync(depth_out + static_cast<int64_t>(p0) * n, out, 4 * size_t(n) * kp,
+                                    hipMemcpyDeviceToHost, s));
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipStreamSynchronize(s));
@@ -364,6 +392,8 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
         st->levels = max_levels_seen;
         st->edges_traversed = traversed;
         st->extra[0] = grid;
+        st->extra[1] = static_cast<double>(scanned);
+        st->extra[2] = static_cast<double>(pushed);
     }
     return GHET_OK;
 }
