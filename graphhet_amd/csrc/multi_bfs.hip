@@ -180,10 +180,18 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                 }
                 const bool light = e - b <= 32;
                 if (todo && light)
-                    for (int32_t i = b; i < e; ++i) {
-                        const Bits f = front[a.in_col[i]];
-                        ++scanned;
-                        for (int k = 0; k < kWords; ++k) acc.w[k] |= f.w[k];
+                    for (int32_t i = b; i < e; i += 4) {
+                        // four gathers in flight per lane before the exit test
+                        int32_t u[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) u[j] = i + j < e ? a.in_col[i + j] : -1;
+                        Bits f[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) f[j] = u[j] >= 0 ? front[u[j]] : Bits{};
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            for (int k = 0; k < kWords; ++k) acc.w[k] |= f[j].w[k];
+                        scanned += min(4, e - i);
                         if (covers(acc, sn, a.full)) break;
                     }
                 uint64_t heavy = __ballot(!light);
@@ -322,7 +330,7 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
     const unsigned grid = static_cast<unsigned>(std::max(1, std::min(dev_blocks, 8) * cus));
     const int64_t bits_words = sizeof(Bits) / 8 * 4 * int64_t(n);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
-    const int64_t push_div = (have_opts && o->reserved[2] > 0) ? o->reserved[2] : 20;
+    const int64_t push_div = (have_opts && o->reserved[2] > 0) ? o->reserved[2] : 5;
 
     int64_t traversed = 0, scanned = 0, pushed = 0;
     int32_t max_levels_seen = 0;

@@ -163,6 +163,10 @@ typedef struct ghet_opts {
     ghet_level_trace* trace;/* optional per-level records                         */
     int32_t trace_cap;      /* capacity of `trace`                                */
     int32_t block_threads;  /* 0 = tuned default; else 64..1024 multiple of 64    */
+    /* reserved[0] bit0: PR without the FP64 normalisation (reference form)
+     * reserved[1] bit0: many-graph calls skip the autotuner
+     * reserved[2]     : batched BFS pushes while frontier out-edges < m / this
+     *                   (0 = default 5); other entries must be 0          */
     int32_t reserved[6];
 } ghet_opts;
 
