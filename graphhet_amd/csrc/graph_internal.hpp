@@ -53,11 +53,9 @@ struct PRLayout {
     int32_t num_tiles = 0;        // tiles of consecutive short rows
     int32_t num_long_chunks = 0;  // chunks of long rows
     int32_t num_long_rows = 0;
-    int32_t* tile_row = nullptr;     // [num_tiles] first row of each tile
-    int32_t* tile_end_ptr = nullptr; // [num_tiles] one past the last row (long rows are never inside)
-    int32_t* long_row = nullptr;     // [num_long_rows] row ids
-    int32_t* long_chunk0 = nullptr;  // [num_long_rows+1] first chunk of each long row
-    int32_t* long_done = nullptr;    // [num_long_rows] arrival counters (self-resetting)
+    int32_t num_items = 0;           // chunks first, then tiles
+    void* items = nullptr;           // [num_items] 32-byte work items (pr.hip)
+    void* long_rows = nullptr;       // [num_long_rows] int4 {row, first chunk, chunks, 0}
     int32_t* outdeg = nullptr;       // [n] out-degree of relabeled vertex r (sinks have 0)
     int32_t* perm = nullptr;         // [n] relabeled id -> caller's id (out-degree descending)
     int32_t* rowptr = nullptr;       // [n+1] relabeled in-CSR offsets

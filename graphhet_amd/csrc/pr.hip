@@ -20,12 +20,15 @@
 // packed into tiles of at most kTileNnz in-edges and kTileRows rows; a row
 // with more than kTileNnz in-edges is a "long row", split into kTileNnz-edge
 // chunks whose partial sums are combined in chunk order by the last chunk to
-// arrive.  One launch per iteration processes every tile and chunk:
-//   1. stage contrib[in_col[e]] for the tile's contiguous edge range in LDS
-//      (coalesced col reads, gathers that mostly hit L2/MALL);
-//   2. rows with <= 32 staged values: one lane each; longer rows: one wave;
+// arrive.  Chunks and tiles form one list of 32-byte work items.  One launch
+// per iteration walks the list with persistent blocks:
+//   1. gather contrib[in_col[e]] for the item's contiguous edge range (8 per
+//      lane, all issued before the first wait; the next item's column, row
+//      offset and out-degree loads are issued behind them);
+//   2. tiles stage the values in LDS -- rows with <= 32 values: one lane
+//      each; longer rows: one wave; chunks reduce in the block;
 //   3. write contrib'[v] = (float)(pr'[v] / outdeg(v)) for the next
-//      iteration, fold pr'[v] of sinks into a per-block FP64 partial, and on
+//      iteration, fold pr'[v] of sinks into a per-thread FP64 sum, and on
 //      the last iteration write pr'[v] in FP64.
 // A one-block kernel then sums the partials in index order into S.
 #include <cmath>
@@ -37,8 +40,9 @@ namespace {
 
 constexpr int kBT = 256;
 constexpr int kTileNnz = 2048;
-constexpr int kTileRows = 1024;
-constexpr int kStage = kTileNnz / kBT;  // staged values per thread
+constexpr int kTileRows = 512;
+constexpr int kStage = kTileNnz / kBT;    // staged values per thread
+constexpr int kRowsPT = kTileRows / kBT;  // tile rows per thread
 constexpr int kWaveRow = 32;            // rows with more staged values than this use a wave
 
 struct PRScalars {
@@ -104,141 +108,194 @@ __global__ void __launch_bounds__(1024) k_pr_reduce(const double* __restrict__ p
     if (threadIdx.x == 0) *out = s[0];
 }
 
-__device__ __forceinline__ void finish_vertex(int32_t v, float sum, double base, double damping,
-                                              const int32_t* __restrict__ outdeg, float* __restrict__ next,
-                                              double* __restrict__ rank_out, bool last, double& sink) {
-    const double pr = base + damping * static_cast<double>(sum);
-    const int32_t d = outdeg[v];
-    if (d > 0) next[v] = static_cast<float>(pr / d);
-    else sink += pr;
-    if (last) rank_out[v] = pr;
-}
+// A work item: a tile of whole rows (row_end > row) or one kTileNnz-edge
+// chunk of a long row (row_end < 0).  32 bytes, read with scalar loads.
+struct alignas(32) PRItem {
+    int32_t row;      // first row of the tile / the long row
+    int32_t row_end;  // one past the tile's last row; -1 for a chunk
+    int32_t e0, e1;   // edge range [e0, e1) in the relabeled in-CSR
+    int32_t chunk;    // chunk: global chunk index (slot in long_part)
+    int32_t pad[3];
+};
 
-// One iteration over the relabeled in-CSR: blocks [0, num_tiles) are tiles,
-// the rest are long-row chunks.
-__global__ void __launch_bounds__(kBT) k_pr_iter(
-    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
-    const int32_t* __restrict__ tile_beg, const int32_t* __restrict__ tile_end, int32_t num_tiles,
-    const int32_t* __restrict__ long_row, const int32_t* __restrict__ long_chunk0, int32_t num_long_rows,
-    int32_t* long_done, double* long_part, const float* __restrict__ contrib, float* __restrict__ next,
-    const PRScalars* __restrict__ scal, double nd, double damping, double* __restrict__ part,
-    double* __restrict__ rank_out, int last) {
-    __shared__ float s_val[kTileNnz];
-    __shared__ int32_t s_off[kTileRows + 1];
-    __shared__ double s_red[kBT / 64];
-    __shared__ float s_ks[kBT / 64], s_kc[kBT / 64];
-    __shared__ int32_t s_last;
-    const int tid = threadIdx.x;
-    const double base = scal->sink * damping / nd + (1.0 - damping) / nd;
-    double sink = 0;
-    const int32_t b = blockIdx.x;
-    if (b < num_tiles) {
-        const int32_t rb = tile_beg[b], re = tile_end[b];
-        const int32_t e0 = rowptr[rb], e1 = rowptr[re];
-        const int32_t nnz = e1 - e0, rows = re - rb;
-        // Stage: all column loads first, then all gathers (kStage in flight per lane).
-        int32_t idx[kStage];
-#pragma unroll
-        for (int j = 0; j < kStage; ++j) {
-            const int k = tid + j * kBT;
-            idx[j] = k < nnz ? col[e0 + k] : -1;
-        }
-        float val[kStage];
-#pragma unroll
-        for (int j = 0; j < kStage; ++j) val[j] = idx[j] >= 0 ? contrib[idx[j]] : 0.f;
-#pragma unroll
-        for (int j = 0; j < kStage; ++j) {
-            const int k = tid + j * kBT;
-            if (k < nnz) s_val[k] = val[j];
-        }
-        for (int r = tid; r <= rows; r += kBT) s_off[r] = rowptr[rb + r] - e0;
-        __syncthreads();
-        for (int r = tid; r < rows; r += kBT) {
-            const int32_t a = s_off[r], z = s_off[r + 1];
-            if (z - a > kWaveRow) continue;
-            Kahan32 acc;
-            for (int32_t i = a; i < z; ++i) acc.add(s_val[i]);
-            finish_vertex(rb + r, acc.value(), base, damping, outdeg, next, rank_out, last, sink);
-        }
-        // Rows with more staged values: wave w takes 64-row chunks w, w+4, ...
-        // and reduces each such row cooperatively, in row order.
-        const int wave = tid >> 6, lane = tid & 63;
-        for (int c0 = wave * 64; c0 < rows; c0 += kBT) {
-            const int r = c0 + lane;
-            const bool wide = r < rows && (s_off[r + 1] - s_off[r]) > kWaveRow;
-            uint64_t mask = __ballot(wide);
-            while (mask) {
-                const int rr = c0 + __ffsll(static_cast<unsigned long long>(mask)) - 1;
-                mask &= mask - 1;
-                const int32_t a = s_off[rr], z = s_off[rr + 1];
-                Kahan32 acc;
-                for (int32_t i = a + lane; i < z; i += 64) acc.add(s_val[i]);
-                acc = wave_kahan(acc);
-                if (lane == 0) finish_vertex(rb + rr, acc.value(), base, damping, outdeg, next, rank_out, last, sink);
-            }
-        }
-        double t = block_sum_f64(sink, s_red);
-        if (tid == 0) part[b] = t;
-        return;
-    }
-    // Long-row chunk.
-    const int32_t c = b - num_tiles;
-    int lo = 0, hi = num_long_rows;  // largest L with long_chunk0[L] <= c
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (long_chunk0[mid] <= c) lo = mid;
-        else hi = mid;
-    }
-    const int32_t L = lo, v = long_row[L];
-    const int32_t k = c - long_chunk0[L], nchunks = long_chunk0[L + 1] - long_chunk0[L];
-    const int32_t rs = rowptr[v], rz = rowptr[v + 1];
-    const int32_t e0 = rs + k * kTileNnz, e1 = min(e0 + kTileNnz, rz);
+// Register-resident loads of one work item: the staged column indices (lanes
+// past the end hold 0, a valid index, and are masked later), and for tiles
+// the row offsets and out-degrees of rows tid + j*kBT.
+struct PRStage {
     int32_t idx[kStage];
+    int32_t roff[kRowsPT];
+    int32_t od[kRowsPT];
+};
+
+__device__ __forceinline__ void pr_stage(const PRItem& it, bool valid, const int32_t* __restrict__ rowptr,
+                                         const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
+                                         PRStage& st) {
+    const int tid = threadIdx.x;
+    const int32_t nnz = valid ? it.e1 - it.e0 : 0;
+    // Branch-free: lanes past the end load element 0 (always allocated) and
+    // are masked where the values are used, so every load is unconditional
+    // and the compiler can wait for the gathers alone.
 #pragma unroll
     for (int j = 0; j < kStage; ++j) {
-        const int32_t i = e0 + tid + j * kBT;
-        idx[j] = i < e1 ? col[i] : -1;
+        const int k = tid + j * kBT;
+        st.idx[j] = col[k < nnz ? it.e0 + k : 0];
     }
-    Kahan32 acc;
+    const int32_t rows = (valid && it.row_end >= 0) ? it.row_end - it.row : 0;
 #pragma unroll
-    for (int j = 0; j < kStage; ++j)
-        if (idx[j] >= 0) acc.add(contrib[idx[j]]);
-    acc = wave_kahan(acc);
-    if ((tid & 63) == 0) {
-        s_ks[tid >> 6] = acc.s;
-        s_kc[tid >> 6] = acc.c;
+    for (int j = 0; j < kRowsPT; ++j) {
+        const int r = tid + j * kBT;
+        const int32_t v = r < rows ? it.row + r : 0;
+        st.roff[j] = rowptr[v];
+        st.od[j] = outdeg[v];
     }
-    __syncthreads();
-    if (tid == 0) {
-        Kahan32 t;
-        for (int w = 0; w < kBT / 64; ++w) {
-            Kahan32 o;
-            o.s = s_ks[w];
-            o.c = s_kc[w];
-            t.merge(o);
+}
+
+// One iteration over the relabeled in-CSR.  Persistent blocks walk the item
+// list with stride gridDim.x; the loads of item i+G (columns, row offsets,
+// out-degrees) are issued while the gathers of item i are in flight, so a
+// block always has two items' worth of memory requests outstanding and no
+// item pays more than one exposed round trip.  Every thread keeps its sink
+// mass in item order; part[blockIdx.x] gets the block's total.
+__global__ void __launch_bounds__(kBT) k_pr_iter(
+    const PRItem* __restrict__ items, int32_t num_items, const int32_t* __restrict__ rowptr,
+    const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg, double* __restrict__ long_part,
+    const float* __restrict__ contrib, float* __restrict__ next, const PRScalars* __restrict__ scal, double nd,
+    double damping, double* __restrict__ part, double* __restrict__ rank_out, int last) {
+    __shared__ float s_val[kTileNnz];
+    __shared__ int32_t s_off[kTileRows + 1];
+    __shared__ int32_t s_od[kTileRows];
+    __shared__ double s_red[kBT / 64];
+    __shared__ float s_ks[kBT / 64], s_kc[kBT / 64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const double base = scal->sink * damping / nd + (1.0 - damping) / nd;
+    const int32_t G = gridDim.x;
+    double sink = 0;
+    int32_t i = blockIdx.x;
+    PRItem cur = i < num_items ? items[i] : PRItem{};
+    PRStage st;
+    pr_stage(cur, i < num_items, rowptr, col, outdeg, st);
+    PRItem nxt = i + G < num_items ? items[i + G] : PRItem{};
+    for (; i < num_items; i += G) {
+        float val[kStage];
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) val[j] = contrib[st.idx[j]];
+        // the next item's loads overlap these gathers
+        PRStage sn;
+        pr_stage(nxt, i + G < num_items, rowptr, col, outdeg, sn);
+        const PRItem nxt2 = i + 2 * G < num_items ? items[i + 2 * G] : PRItem{};
+        const int32_t nnz = cur.e1 - cur.e0;
+        __syncthreads();  // the previous item's LDS readers are done
+        if (cur.row_end >= 0) {
+            const int32_t rows = cur.row_end - cur.row;
+#pragma unroll
+            for (int j = 0; j < kStage; ++j) {
+                const int k = tid + j * kBT;
+                if (k < nnz) s_val[k] = val[j];
+            }
+#pragma unroll
+            for (int j = 0; j < kRowsPT; ++j) {
+                const int r = tid + j * kBT;
+                if (r < rows) {
+                    s_off[r] = st.roff[j] - cur.e0;
+                    s_od[r] = st.od[j];
+                }
+            }
+            if (tid == 0) s_off[rows] = nnz;
+            __syncthreads();
+            // rows with few values: one lane each, in edge order
+#pragma unroll
+            for (int j = 0; j < kRowsPT; ++j) {
+                const int r = tid + j * kBT;
+                if (r < rows) {
+                    const int32_t a = s_off[r], z = s_off[r + 1];
+                    if (z - a <= kWaveRow) {
+                        Kahan32 acc;
+                        for (int32_t e = a; e < z; ++e) acc.add(s_val[e]);
+                        const double pr = base + damping * static_cast<double>(acc.value());
+                        if (st.od[j] > 0) next[cur.row + r] = static_cast<float>(pr / st.od[j]);
+                        else sink += pr;
+                        if (last) rank_out[cur.row + r] = pr;
+                    }
+                }
+            }
+            // rows with more values: wave w takes 64-row groups w, w+4, ...
+            for (int c0 = wave * 64; c0 < rows; c0 += kBT) {
+                const int r = c0 + lane;
+                const bool wide = r < rows && (s_off[r + 1] - s_off[r]) > kWaveRow;
+                uint64_t mask = __ballot(wide);
+                while (mask) {
+                    const int rr = c0 + __ffsll(static_cast<unsigned long long>(mask)) - 1;
+                    mask &= mask - 1;
+                    const int32_t a = s_off[rr], z = s_off[rr + 1];
+                    Kahan32 acc;
+                    for (int32_t e = a + lane; e < z; e += 64) acc.add(s_val[e]);
+                    acc = wave_kahan(acc);
+                    if (lane == 0) {
+                        const double pr = base + damping * static_cast<double>(acc.value());
+                        const int32_t d = s_od[rr];
+                        if (d > 0) next[cur.row + rr] = static_cast<float>(pr / d);
+                        else sink += pr;
+                        if (last) rank_out[cur.row + rr] = pr;
+                    }
+                }
+            }
+        } else {
+            // chunk of a long row: compensated block sum, then the last chunk
+            // to arrive combines the row's partials in chunk order
+            Kahan32 acc;
+#pragma unroll
+            for (int j = 0; j < kStage; ++j)
+                if (tid + j * kBT < nnz) acc.add(val[j]);
+            acc = wave_kahan(acc);
+            if (lane == 0) {
+                s_ks[wave] = acc.s;
+                s_kc[wave] = acc.c;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                Kahan32 t;
+                for (int w = 0; w < kBT / 64; ++w) {
+                    Kahan32 o;
+                    o.s = s_ks[w];
+                    o.c = s_kc[w];
+                    t.merge(o);
+                }
+                long_part[cur.chunk] = static_cast<double>(t.s) + static_cast<double>(t.c);
+            }
         }
-        long_part[c] = static_cast<double>(t.s) + static_cast<double>(t.c);
-        __threadfence();
-        const int32_t arrived = atomicAdd(&long_done[L], 1);
-        s_last = (arrived == nchunks - 1);
+        cur = nxt;
+        nxt = nxt2;
+        st = sn;
     }
-    __syncthreads();
-    if (tid == 0) {
-        if (s_last) {
-            __threadfence();
-            double total = 0;
-            const int32_t c0 = long_chunk0[L];
-            for (int32_t j = 0; j < nchunks; ++j)
-                total += __hip_atomic_load(&long_part[c0 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            long_done[L] = 0;
-            const double pr = base + damping * total;
-            const int32_t d = outdeg[v];
-            if (d > 0) next[v] = static_cast<float>(pr / d);
-            else sink += pr;
-            if (last) rank_out[v] = pr;
-        }
-        part[b] = sink;
+    const double t = block_sum_f64(sink, s_red);
+    if (tid == 0) part[blockIdx.x] = t;
+}
+
+// Long rows: sum each row's chunk partials in chunk order and finish the
+// vertex (one thread per long row).  Runs after k_pr_iter in stream order,
+// which makes every partial visible without device-scope fences -- a fence
+// per chunk would write back and invalidate the XCD's L2 and with it the
+// hot contributions.  Sink mass goes to part[blockIdx.x].
+__global__ void __launch_bounds__(kBT) k_pr_long(const int4* __restrict__ long_rows, int32_t num_long_rows,
+                                                 const double* __restrict__ long_part,
+                                                 const int32_t* __restrict__ outdeg, float* __restrict__ next,
+                                                 const PRScalars* __restrict__ scal, double nd, double damping,
+                                                 double* __restrict__ part, double* __restrict__ rank_out, int last) {
+    __shared__ double s_red[kBT / 64];
+    const double base = scal->sink * damping / nd + (1.0 - damping) / nd;
+    double sink = 0;
+    for (int32_t L = blockIdx.x * kBT + threadIdx.x; L < num_long_rows; L += gridDim.x * kBT) {
+        const int4 lr = long_rows[L];  // {row, first chunk, chunks, 0}
+        double total = 0;
+        for (int32_t j = 0; j < lr.z; ++j) total += long_part[lr.y + j];
+        const double pr = base + damping * total;
+        const int32_t d = outdeg[lr.x];
+        if (d > 0) next[lr.x] = static_cast<float>(pr / d);
+        else sink += pr;
+        if (last) rank_out[lr.x] = pr;
     }
+    const double t = block_sum_f64(sink, s_red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 __global__ void __launch_bounds__(kBT) k_pr_sum(const double* __restrict__ rank, int32_t n, double* __restrict__ part) {
@@ -302,15 +359,27 @@ int build_pr_layout(ghet_graph* g, hipStream_t s) {
         rp[r + 1] = rp[r] + (irp[v + 1] - irp[v]);
         odp[r] = od[v];
     }
-    // Tiles of consecutive relabeled rows; rows above kTileNnz become chunked long rows.
-    std::vector<int32_t> tb, te, lrow, lch0;
-    int32_t chunks = 0;
+    // Work items: every long row (more than kTileNnz in-edges) split into
+    // kTileNnz-edge chunks, then tiles of consecutive short rows.
+    std::vector<PRItem> chunk_items, tile_items;
+    std::vector<int4> long_meta;
+    int32_t long_rows = 0;
     for (int32_t r = 0; r < n;) {
         const int32_t d = rp[r + 1] - rp[r];
         if (d > kTileNnz) {
-            lrow.push_back(r);
-            lch0.push_back(chunks);
-            chunks += static_cast<int32_t>(ceil_div(d, kTileNnz));
+            const int32_t first = static_cast<int32_t>(chunk_items.size());
+            const int32_t nch = static_cast<int32_t>(ceil_div(d, kTileNnz));
+            for (int32_t k = 0; k < nch; ++k) {
+                PRItem it{};
+                it.row = r;
+                it.row_end = -1;
+                it.e0 = rp[r] + k * kTileNnz;
+                it.e1 = std::min(rp[r + 1], it.e0 + kTileNnz);
+                it.chunk = first + k;
+                chunk_items.push_back(it);
+            }
+            long_meta.push_back(make_int4(r, first, nch, 0));
+            ++long_rows;
             ++r;
             continue;
         }
@@ -322,24 +391,29 @@ int build_pr_layout(ghet_graph* g, hipStream_t s) {
             nnz += dr;
             ++r;
         }
-        tb.push_back(first);
-        te.push_back(r);
+        PRItem it{};
+        it.row = first;
+        it.row_end = r;
+        it.e0 = rp[first];
+        it.e1 = rp[r];
+        tile_items.push_back(it);
     }
-    lch0.push_back(chunks);
-    L.num_tiles = static_cast<int32_t>(tb.size());
-    L.num_long_rows = static_cast<int32_t>(lrow.size());
-    L.num_long_chunks = chunks;
+    L.num_tiles = static_cast<int32_t>(tile_items.size());
+    L.num_long_rows = long_rows;
+    L.num_long_chunks = static_cast<int32_t>(chunk_items.size());
+    L.num_items = L.num_tiles + L.num_long_chunks;
+    std::vector<PRItem> all_items(chunk_items);
+    all_items.insert(all_items.end(), tile_items.begin(), tile_items.end());
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t bytes = al(4 * tb.size()) + al(4 * te.size()) + al(4 * lrow.size()) + al(4 * lch0.size()) +
-                         al(4 * lrow.size()) + 3 * al(4 * size_t(n)) + al(4 * (size_t(n) + 1)) + al(4 * size_t(m)) + 256;
+    const size_t bytes = al(sizeof(PRItem) * all_items.size()) + al(sizeof(int4) * long_meta.size()) +
+                         3 * al(4 * size_t(n)) + al(4 * (size_t(n) + 1)) + al(4 * size_t(m)) + 256;
     GHET_HIP(hipMalloc(&L.mem, bytes));
     char* p = static_cast<char*>(L.mem);
+    L.items = p;
+    p += al(sizeof(PRItem) * all_items.size());
     auto take = [&](size_t cnt) { int32_t* q = reinterpret_cast<int32_t*>(p); p += al(4 * cnt); return q; };
-    L.tile_row = take(tb.size());
-    L.tile_end_ptr = take(te.size());
-    L.long_row = take(lrow.size());
-    L.long_chunk0 = take(lch0.size());
-    L.long_done = take(lrow.size());
+    L.long_rows = p;
+    p += al(sizeof(int4) * long_meta.size());
     L.outdeg = take(size_t(n));
     L.perm = take(size_t(n));
     int32_t* d_inv = take(size_t(n));
@@ -349,15 +423,14 @@ int build_pr_layout(ghet_graph* g, hipStream_t s) {
         if (!src.empty()) GHET_HIP(hipMemcpyAsync(dst, src.data(), 4 * src.size(), hipMemcpyHostToDevice, s));
         return GHET_OK;
     };
-    GHET_CHECK(up(L.tile_row, tb));
-    GHET_CHECK(up(L.tile_end_ptr, te));
-    GHET_CHECK(up(L.long_row, lrow));
-    GHET_CHECK(up(L.long_chunk0, lch0));
+    if (!all_items.empty())
+        GHET_HIP(hipMemcpyAsync(L.items, all_items.data(), sizeof(PRItem) * all_items.size(), hipMemcpyHostToDevice, s));
     GHET_CHECK(up(L.outdeg, odp));
     GHET_CHECK(up(L.perm, perm));
     GHET_CHECK(up(d_inv, inv));
     GHET_CHECK(up(L.rowptr, rp));
-    if (!lrow.empty()) GHET_CHECK(fill<int32_t>(L.long_done, 0, static_cast<int64_t>(lrow.size()), s));
+    if (!long_meta.empty())
+        GHET_HIP(hipMemcpyAsync(L.long_rows, long_meta.data(), sizeof(int4) * long_meta.size(), hipMemcpyHostToDevice, s));
     if (n > 0) {
         const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBT / 64), 65535)));
         hipLaunchKernelGGL(k_permute_rows, dim3(grid), dim3(kBT), 0, s, L.perm, d_inv, g->in.rowptr, g->in.col, L.rowptr,
@@ -380,8 +453,15 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     const bool normalise = !(o && o->size >= sizeof(ghet_opts) && (o->reserved[0] & 1));
     GHET_CHECK(build_pr_layout(g, s));
     PRLayout& L = g->pr;
-    const int32_t blocks = L.num_tiles + L.num_long_chunks;
+    // persistent grid: as many blocks as stay resident, at most one per item
+    int occ = 0, cus = 0;
+    GHET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_pr_iter), kBT, 0));
+    GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    const int32_t blocks = static_cast<int32_t>(
+        std::max<int64_t>(1, std::min<int64_t>(int64_t(std::max(occ, 1)) * std::max(cus, 1), L.num_items)));
     const int32_t init_blocks = static_cast<int32_t>(std::min<int64_t>(ceil_div(n, kBT), 4096));
+    // k_pr_long's sink partials follow k_pr_iter's in the same buffer
+    const int32_t long_blocks = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(L.num_long_rows, kBT), 1024)));
     void* p;
     float *ca, *cb;
     double *rank, *out, *part, *long_part;
@@ -390,7 +470,7 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     GHET_CHECK(g->ws.get(WS_LABEL_A, 4 * size_t(n), &p)); cb = static_cast<float*>(p);
     GHET_CHECK(g->ws.get(WS_PR_RANK, 8 * size_t(n), &p)); rank = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_PR_OUT, 8 * size_t(n), &p)); out = static_cast<double*>(p);
-    GHET_CHECK(g->ws.get(WS_PR_PART, 8 * size_t(std::max(blocks, init_blocks) + 1), &p)); part = static_cast<double*>(p);
+    GHET_CHECK(g->ws.get(WS_PR_PART, 8 * size_t(std::max(blocks + long_blocks, init_blocks) + 1), &p)); part = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_PR_LONGPART, 8 * size_t(L.num_long_chunks + 1), &p)); long_part = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_COUNTERS, 256, &p)); scal = static_cast<PRScalars*>(p);
     const double inv_n = 1.0 / n;
@@ -406,11 +486,14 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     for (int32_t it = 0; it < iters; ++it) {
         const int last = (it == iters - 1);
         if (timing) GHET_HIP(hipEventRecord(kev[2 * it], s));
-        hipLaunchKernelGGL(k_pr_iter, dim3(blocks), dim3(kBT), 0, s, L.rowptr, L.col, L.outdeg, L.tile_row,
-                           L.tile_end_ptr, L.num_tiles, L.long_row, L.long_chunk0, L.num_long_rows, L.long_done,
-                           long_part, ca, cb, scal, static_cast<double>(n), damping, part, rank, last);
+        hipLaunchKernelGGL(k_pr_iter, dim3(blocks), dim3(kBT), 0, s, static_cast<const PRItem*>(L.items), L.num_items,
+                           L.rowptr, L.col, L.outdeg, long_part, ca, cb, scal, static_cast<double>(n),
+                           damping, part, rank, last);
         if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
-        hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, blocks, &scal->sink);
+        hipLaunchKernelGGL(k_pr_long, dim3(long_blocks), dim3(kBT), 0, s, static_cast<const int4*>(L.long_rows),
+                           L.num_long_rows, long_part, L.outdeg, cb, scal, static_cast<double>(n), damping,
+                           part + blocks, rank, last);
+        hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, blocks + long_blocks, &scal->sink);
         std::swap(ca, cb);
     }
     const int norm = (normalise && iters > 0) ? 1 : 0;
