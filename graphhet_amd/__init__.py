@@ -81,7 +81,7 @@ EXPORTS = [
     "ghet_bfs", "ghet_bfs_pre", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_cdlp", "ghet_tc",
     "ghet_bfs_multi", "ghet_sssp_multi", "ghet_wcc_many", "ghet_tc_many",
     "ghet_sched_load", "ghet_sched_save", "ghet_sched_clear", "ghet_sched_set_params", "ghet_sched_get_params",
-    "ghet_mlp_forward", "ghet_mlp_train_step", "ghet_autotune",
+    "ghet_mlp_forward", "ghet_mlp_forward_host", "ghet_mlp_train_step", "ghet_autotune",
 ]
 
 _lib = None
@@ -134,6 +134,7 @@ def lib() -> C.CDLL:
         "ghet_sched_set_params": (C.c_int, [C.c_int, f32p]),
         "ghet_sched_get_params": (C.c_int, [C.c_int, f32p]),
         "ghet_mlp_forward": (C.c_int, [C.c_int, C.c_int32, f32p, f32p]),
+        "ghet_mlp_forward_host": (C.c_int, [C.c_int, C.c_int32, f32p, f32p]),
         "ghet_mlp_train_step": (C.c_int, [C.c_int, C.c_int32, f32p, i32p, C.c_float, f32p, f32p]),
         "ghet_autotune": (C.c_int, [vp, C.c_char_p, i32p]),
     }
@@ -447,3 +448,67 @@ def tc_many(csrs: list[Csr], tune: bool = True):
     _check(lib().ghet_tc_many(len(csrs), _ptr(voff, C.c_int32), _ptr(rowptr, C.c_int32), _ptr(col, C.c_int32),
                               _ptr(cnt, C.c_int64), C.byref(_many_opts(tune)), C.byref(st)))
     return [cnt[voff[i]:voff[i + 1]] for i in range(len(csrs))], st
+
+
+# --- learned schedule networks (16 -> 64 -> 64 -> 4, FP16 weights) -----------
+
+NET_BFS, NET_SSSP = 0, 1
+MLP_IN, MLP_H, MLP_OUT = 16, 64, 4
+MLP_PARAMS = MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_OUT * MLP_H + MLP_OUT
+
+
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """He-initialised FP32 parameters in the library's layout
+    (W1[64][16], b1, W2[64][64], b2, W3[4][64], b3)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((MLP_IN, MLP_H), (MLP_H, MLP_H), (MLP_H, MLP_OUT)):
+        parts.append((rng.standard_normal((fan_out, fan_in)) * np.sqrt(2.0 / fan_in)).ravel())
+        parts.append(np.zeros(fan_out))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def sched_set_params(kind: int, params: np.ndarray) -> None:
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    if p.size != MLP_PARAMS:
+        raise ValueError(f"expected {MLP_PARAMS} parameters, got {p.size}")
+    _check(lib().ghet_sched_set_params(kind, _ptr(p, C.c_float)), allow_fallback=False)
+
+
+def sched_get_params(kind: int) -> np.ndarray:
+    p = np.empty(MLP_PARAMS, np.float32)
+    _check(lib().ghet_sched_get_params(kind, _ptr(p, C.c_float)), allow_fallback=False)
+    return p
+
+
+def sched_save(path: str) -> None:
+    _check(lib().ghet_sched_save(path.encode()), allow_fallback=False)
+
+
+def sched_load(path: str) -> bool:
+    """Installs the networks in `path`; False (heuristic stays) if unusable."""
+    return _check(lib().ghet_sched_load(path.encode())) == OK
+
+
+def sched_clear() -> None:
+    _check(lib().ghet_sched_clear())
+
+
+def mlp_forward(kind: int, x: np.ndarray, host: bool = False) -> np.ndarray:
+    """Logits [B,4] of the installed network (GPU kernel, or the host mirror)."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+    y = np.empty((x.shape[0], MLP_OUT), np.float32)
+    fn = lib().ghet_mlp_forward_host if host else lib().ghet_mlp_forward
+    _check(fn(kind, x.shape[0], _ptr(x, C.c_float), _ptr(y, C.c_float)), allow_fallback=False)
+    return y
+
+
+def mlp_train_step(kind: int, x: np.ndarray, labels: np.ndarray, lr: float, want_grad: bool = False):
+    """One Adam step on the GPU; returns (mean loss, gradient or None)."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    loss = C.c_float()
+    g = np.empty(MLP_PARAMS, np.float32) if want_grad else None
+    _check(lib().ghet_mlp_train_step(kind, x.shape[0], _ptr(x, C.c_float), _ptr(lab, C.c_int32), lr,
+                                     C.byref(loss), _ptr(g, C.c_float)), allow_fallback=False)
+    return float(loss.value), g

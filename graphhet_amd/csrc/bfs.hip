@@ -102,6 +102,7 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
     __shared__ unsigned long long s_red[kBT / 64];
     const int tid = threadIdx.x;
     if (tid == 0) s_cnt = 0;
+    __syncthreads();  // counter zeroed before any wave appends
     unsigned long long cnt = 0, od = 0, id = 0;
     auto on_item = [&](int32_t v) {
         ++cnt;
@@ -302,6 +303,7 @@ __global__ void __launch_bounds__(kBT) k_bitmap_to_queue(int32_t n, const uint32
     __shared__ int32_t s_out[kOut];
     __shared__ int32_t s_cnt, s_base;
     if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();  // counter zeroed before any wave appends
     auto nop = [](int32_t) {};
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < n; base += static_cast<int64_t>(gridDim.x) * kBT) {
         const int64_t v = base + threadIdx.x;

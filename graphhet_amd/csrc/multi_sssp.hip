@@ -97,6 +97,7 @@ __global__ void __launch_bounds__(kBT) k_mss_relax(const uint64_t* __restrict__ 
         s_nc = 0;
         s_fc = 0;
     }
+    __syncthreads();  // counters zeroed before any wave appends
     unsigned long long od = 0;
     auto on_near = [&](uint64_t it) {
         const int32_t v = item_v(it);
@@ -177,6 +178,7 @@ __global__ void __launch_bounds__(kBT) k_mss_split(const uint64_t* __restrict__ 
         s_nc = 0;
         s_fc = 0;
     }
+    __syncthreads();  // counters zeroed before any wave appends
     unsigned long long od = 0, dmin = kInfBits;
     auto on_near = [&](uint64_t it) {
         const int32_t v = item_v(it);

@@ -26,6 +26,69 @@ void sched_begin(SchedState* s, int schedule, int kind, int64_t n, int64_t m, in
     }
 }
 
+namespace {
+
+constexpr int kIn = GHET_MLP_IN, kH = GHET_MLP_H, kOut = GHET_MLP_OUT;
+constexpr int kB1 = kH * kIn, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH, kB3 = kW3 + kOut * kH;
+
+// Both variants compute every output as bias, then fused multiply-adds over
+// the inputs in index order -- the GPU kernel's order -- so their results
+// are bit-identical to it; the loops run over outputs innermost so the FMA
+// build vectorises.  The bodies are spelled out twice because the FMA one
+// must be compiled for the avx2+fma target as a whole.
+#pragma GCC push_options
+#pragma GCC target("avx2,fma")
+void forward_fma(const float* wt, const float* x, float* y) {
+    float h1[kH], h2[kH];
+    for (int j = 0; j < kH; ++j) h1[j] = wt[kB1 + j];
+    for (int k = 0; k < kIn; ++k)
+        for (int j = 0; j < kH; ++j) h1[j] = __builtin_fmaf(wt[k * kH + j], x[k], h1[j]);
+    for (int j = 0; j < kH; ++j) h1[j] = h1[j] > 0.f ? h1[j] : 0.f;
+    for (int j = 0; j < kH; ++j) h2[j] = wt[kB2 + j];
+    for (int i = 0; i < kH; ++i)
+        for (int j = 0; j < kH; ++j) h2[j] = __builtin_fmaf(wt[kW2 + i * kH + j], h1[i], h2[j]);
+    for (int j = 0; j < kH; ++j) h2[j] = h2[j] > 0.f ? h2[j] : 0.f;
+    for (int o = 0; o < kOut; ++o) y[o] = wt[kB3 + o];
+    for (int j = 0; j < kH; ++j)
+        for (int o = 0; o < kOut; ++o) y[o] = __builtin_fmaf(wt[kW3 + j * kOut + o], h2[j], y[o]);
+}
+#pragma GCC pop_options
+
+void forward_libm(const float* wt, const float* x, float* y) {
+    float h1[kH], h2[kH];
+    for (int j = 0; j < kH; ++j) h1[j] = wt[kB1 + j];
+    for (int k = 0; k < kIn; ++k)
+        for (int j = 0; j < kH; ++j) h1[j] = std::fma(wt[k * kH + j], x[k], h1[j]);
+    for (int j = 0; j < kH; ++j) h1[j] = h1[j] > 0.f ? h1[j] : 0.f;
+    for (int j = 0; j < kH; ++j) h2[j] = wt[kB2 + j];
+    for (int i = 0; i < kH; ++i)
+        for (int j = 0; j < kH; ++j) h2[j] = std::fma(wt[kW2 + i * kH + j], h1[i], h2[j]);
+    for (int j = 0; j < kH; ++j) h2[j] = h2[j] > 0.f ? h2[j] : 0.f;
+    for (int o = 0; o < kOut; ++o) y[o] = wt[kB3 + o];
+    for (int j = 0; j < kH; ++j)
+        for (int o = 0; o < kOut; ++o) y[o] = std::fma(wt[kW3 + j * kOut + o], h2[j], y[o]);
+}
+
+}  // namespace
+
+void mlp_forward_host_t(const float* wt, const float* x, float* y) {
+    static const bool has_fma = __builtin_cpu_supports("fma") && __builtin_cpu_supports("avx2");
+    if (has_fma) forward_fma(wt, x, y);
+    else forward_libm(wt, x, y);
+}
+
+void mlp_transpose(const float* w, float* wt) {
+    for (int j = 0; j < kH; ++j)
+        for (int k = 0; k < kIn; ++k) wt[k * kH + j] = w[j * kIn + k];
+    for (int j = 0; j < kH; ++j) wt[kB1 + j] = w[kB1 + j];
+    for (int j = 0; j < kH; ++j)
+        for (int i = 0; i < kH; ++i) wt[kW2 + i * kH + j] = w[kW2 + j * kH + i];
+    for (int j = 0; j < kH; ++j) wt[kB2 + j] = w[kB2 + j];
+    for (int o = 0; o < kOut; ++o)
+        for (int j = 0; j < kH; ++j) wt[kW3 + j * kOut + o] = w[kW3 + o * kH + j];
+    for (int o = 0; o < kOut; ++o) wt[kB3 + o] = w[kB3 + o];
+}
+
 static inline float lg(double x) { return static_cast<float>(std::log2(1.0 + std::max(0.0, x))); }
 
 void sched_features(const LevelInfo& li, float* f) {

@@ -42,5 +42,10 @@ int sched_pick_sssp(SchedState* s, const LevelInfo& li);
 void sched_features(const LevelInfo& li, float* f);
 // The heuristic's own choice (used as fallback and as the training baseline).
 int heuristic_bfs(const LevelInfo& li);
+// Host forward pass of a schedule network over its parameters in transposed
+// layout (mlp_transpose of the FP16-rounded parameters); bit-identical to the
+// GPU forward kernel.
+void mlp_transpose(const float* w, float* wt);
+void mlp_forward_host_t(const float* wt, const float* x, float* y);
 
 }  // namespace ghet

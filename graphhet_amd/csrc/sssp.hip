@@ -100,6 +100,7 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
         s_nc = 0;
         s_fc = 0;
     }
+    __syncthreads();  // counters zeroed before any wave appends
     unsigned long long od = 0;
     auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
     auto nop = [](int32_t) {};
@@ -176,6 +177,7 @@ __global__ void __launch_bounds__(kBT) k_far_split(const int32_t* __restrict__ f
         s_nc = 0;
         s_fc = 0;
     }
+    __syncthreads();  // counters zeroed before any wave appends
     unsigned long long od = 0, dmin = kInfBits;
     auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
     auto nop = [](int32_t) {};

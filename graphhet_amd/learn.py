@@ -1,0 +1,266 @@
+"""Learned schedule: per-level trace collection and training of the schedule
+networks on the GPU (the library's own forward/backward/Adam kernels).
+
+BFS.  A BFS visits the same vertex sets level by level whichever direction
+computes each level, so one source run under each forced schedule (push,
+pull, bitmap) yields, for every level, the device time of every mode at
+identical frontier features.  The labels are the cheapest mode path through
+the levels (dynamic programming over those times plus the cost of changing
+frontier representation).  The "previous mode" features of a sample are
+those of the label path (the mode the network would have picked one level
+earlier), level 0 starting from push as the library does.
+
+SSSP.  The bucket width changes which vertices settle in which phase, so
+runs under different widths are not level-aligned.  Each source is run at
+the four width classes (delta = base * 2^(class-1)); every phase record of
+the base-width run is labelled with the class whose whole run was fastest.
+
+Outputs never depend on the networks: every schedule computes the same
+depths and distances, which the tests check bit for bit.
+
+Command line (runs on a GPU):
+    python -m graphhet_amd.learn --out graphhet_amd/sched_net.txt
+trains both networks on R-MAT graphs of scale 18-22 with a in [0.45, 0.65]
+(b = c = (1-a)*0.19/0.43, d = the rest) and edge factor in [8, 32].
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import (DeviceGraph, HostGraph, MLP_IN, MLP_OUT, NET_BFS, NET_SSSP, SCHED_BITMAP, SCHED_HEURISTIC,
+               SCHED_LEARNED, SCHED_PULL, SCHED_PUSH, mlp_forward, mlp_init, mlp_train_step, sched_save,
+               sched_set_params)
+
+FORCED = ((0, SCHED_PUSH), (1, SCHED_PULL), (2, SCHED_BITMAP))
+PREV_MODE = slice(9, 12)  # one-hot of the previous level's mode in the feature vector
+
+
+def rmat_params(a: float) -> tuple[float, float, float]:
+    """(a, b, c) with b = c = (1-a)*0.19/0.43 (d keeps the rest)."""
+    bc = (1.0 - a) * 0.19 / 0.43
+    return a, bc, bc
+
+
+def pick_sources(hg: HostGraph, k: int, seed: int) -> np.ndarray:
+    """k distinct sources among vertices with at least one out-edge."""
+    o = hg.view(0, copy=False)
+    cand = np.nonzero(o.degrees() > 0)[0]
+    rng = np.random.default_rng(seed)
+    return rng.choice(cand, size=min(k, len(cand)), replace=False).astype(np.int32)
+
+
+def _trace_arrays(trace):
+    f = np.array([list(t.features) for t in trace], np.float32).reshape(-1, MLP_IN)
+    ms = np.array([t.ms for t in trace], np.float64)
+    return f, ms
+
+
+def switch_ms(n: int) -> dict:
+    """Extra device time (ms) of changing frontier representation between
+    levels on an n-vertex graph: a push level after a bottom-up or bitmap
+    level first turns the bitmap into a queue (a pass over n bits and two
+    launches); a bottom-up or bitmap level after a push level first scatters
+    the queue into a cleared bitmap.  Fitted to MI355X runs at scales 20-22
+    (0.031 ms at 2^20 vertices, 0.05 ms at 2^22 for bitmap -> queue)."""
+    s = n / float(1 << 20)
+    b2q = 0.025 + 0.006 * s
+    q2b = 0.005 + 0.004 * s
+    return {(0, 1): q2b, (0, 2): q2b, (1, 0): b2q, (2, 0): b2q}
+
+
+def label_path(ms: np.ndarray, n: int) -> np.ndarray:
+    """Cheapest mode sequence over the levels (dynamic programming over the
+    per-level mode times [L,3] plus representation-switch costs); level 0
+    starts from a queue holding the source."""
+    sw = switch_ms(n)
+    L = ms.shape[0]
+    cost = ms[0] + np.array([sw.get((0, m), 0.0) for m in range(3)])
+    back = np.zeros((L, 3), np.int64)
+    for lv in range(1, L):
+        new = np.empty(3)
+        for m in range(3):
+            c = [cost[p] + sw.get((p, m), 0.0) for p in range(3)]
+            back[lv, m] = int(np.argmin(c))
+            new[m] = c[back[lv, m]] + ms[lv, m]
+        cost = new
+    path = np.empty(L, np.int64)
+    path[-1] = int(np.argmin(cost))
+    for lv in range(L - 1, 0, -1):
+        path[lv - 1] = back[lv, path[lv]]
+    return path
+
+
+def regret_weights(T: np.ndarray) -> np.ndarray:
+    """Sampling weight of a level: log of its worst-to-best mode time ratio
+    (a wrong pick at level 0 can cost 1000x), plus a floor."""
+    return np.log(T.max(axis=1) / np.maximum(T.min(axis=1), 1e-6)) + 0.05
+
+
+def bfs_samples(dg: DeviceGraph, sources, reps: int = 2):
+    """Features [N,16] and fastest-mode labels [N] from forced-mode runs;
+    also the per-level mode times [N,3]."""
+    X, Y, T = [], [], []
+    for src in sources:
+        src = int(src)
+        times, feats = [], None
+        for mode, sched in FORCED:
+            best = None
+            for _ in range(reps):
+                _, st, tr = dg.bfs(src, schedule=sched, out=False, trace_cap=256)
+                f, ms = _trace_arrays(tr)
+                best = ms if best is None else np.minimum(best, ms)
+                if mode == 0:
+                    feats = f
+            times.append(best)
+        L = min(len(t) for t in times)
+        ms = np.stack([t[:L] for t in times], axis=1)
+        lab = label_path(ms, dg.n)
+        f = feats[:L].copy()
+        prev = np.concatenate([[0], lab[:-1]])
+        f[:, PREV_MODE] = 0.0
+        f[np.arange(L), 9 + prev] = 1.0
+        X.append(f)
+        Y.append(lab)
+        T.append(ms)
+    return np.concatenate(X), np.concatenate(Y).astype(np.int32), np.concatenate(T)
+
+
+def sssp_samples(dg: DeviceGraph, sources):
+    """Phase features of the base-width run labelled with the fastest class,
+    and the class regret of each run as the samples' weight."""
+    X, Y, W = [], [], []
+    for src in sources:
+        src = int(src)
+        _, st0, tr = dg.sssp(src, out=False, trace_cap=1024)
+        base = st0.extra[0]
+        totals = []
+        for cls in range(MLP_OUT):
+            _, st = dg.sssp(src, out=False, delta=base * 2.0 ** (cls - 1))
+            totals.append(st.device_ms)
+        f, _ = _trace_arrays(tr)
+        X.append(f)
+        Y.append(np.full(len(f), int(np.argmin(totals)), np.int32))
+        W.append(np.full(len(f), np.log(max(totals) / max(min(totals), 1e-6)) + 0.05))
+    return np.concatenate(X), np.concatenate(Y), np.concatenate(W)
+
+
+def train(kind: int, X: np.ndarray, y: np.ndarray, steps: int = 1500, lr: float = 2e-3, batch: int = 256,
+          seed: int = 0, weights: np.ndarray | None = None) -> list[float]:
+    """Fresh network, Adam on the GPU; minibatches drawn with probability
+    proportional to `weights` (uniform if None).  Returns the loss history."""
+    sched_set_params(kind, mlp_init(seed))
+    rng = np.random.default_rng(seed)
+    prob = None if weights is None else weights / weights.sum()
+    hist = []
+    for _ in range(steps):
+        idx = rng.choice(len(X), size=min(batch, len(X)), p=prob)
+        loss, _ = mlp_train_step(kind, X[idx], y[idx], lr)
+        hist.append(loss)
+    return hist
+
+
+def accuracy(kind: int, X: np.ndarray, y: np.ndarray, classes: int) -> float:
+    logits = mlp_forward(kind, X)[:, :classes]
+    return float(np.mean(np.argmax(logits, axis=1) == y))
+
+
+def compare_bfs(dg: DeviceGraph, sources) -> dict:
+    """Learned vs heuristic vs reference-rule device time over the sources;
+    depths must be identical."""
+    from . import SCHED_REFERENCE
+    tot = {"heuristic": 0.0, "learned": 0.0, "reference": 0.0}
+    same = True
+    for src in sources:
+        src = int(src)
+        d0, s0 = dg.bfs(src, schedule=SCHED_HEURISTIC)
+        d1, s1 = dg.bfs(src, schedule=SCHED_LEARNED)
+        d2, s2 = dg.bfs(src, schedule=SCHED_REFERENCE)
+        same &= bool(np.array_equal(d0, d1) and np.array_equal(d0, d2))
+        tot["heuristic"] += s0.device_ms
+        tot["learned"] += s1.device_ms
+        tot["reference"] += s2.device_ms
+    tot["identical"] = same
+    tot["gain_vs_heuristic"] = tot["heuristic"] / max(tot["learned"], 1e-9)
+    tot["gain_vs_reference"] = tot["reference"] / max(tot["learned"], 1e-9)
+    return tot
+
+
+def compare_sssp(dg: DeviceGraph, sources) -> dict:
+    tot = {"heuristic": 0.0, "learned": 0.0}
+    same = True
+    for src in sources:
+        src = int(src)
+        d0, s0 = dg.sssp(src, schedule=SCHED_HEURISTIC)
+        d1, s1 = dg.sssp(src, schedule=SCHED_LEARNED)
+        same &= bool(np.array_equal(d0, d1))
+        tot["heuristic"] += s0.device_ms
+        tot["learned"] += s1.device_ms
+    tot["identical"] = same
+    tot["gain_vs_heuristic"] = tot["heuristic"] / max(tot["learned"], 1e-9)
+    return tot
+
+
+def trace_set(scales, graphs_per_scale: int, sources: int, seed: int, log=print):
+    """Traces from R-MAT graphs with a in [0.45, 0.65], ef in [8, 32]:
+    BFS (features, labels, weights) and SSSP (features, labels, weights)."""
+    rng = np.random.default_rng(seed)
+    bx, by, bw, sx, sy, sw = [], [], [], [], [], []
+    for sc in scales:
+        for gi in range(graphs_per_scale):
+            a = float(rng.uniform(0.45, 0.65))
+            ef = float(rng.uniform(8, 32))
+            gseed = int(rng.integers(1, 2**31))
+            t0 = time.time()
+            hg = HostGraph.rmat(sc, ef, *rmat_params(a), seed=gseed)
+            dg = DeviceGraph.from_host(hg, with_all=False)
+            srcs = pick_sources(hg, sources, gseed)
+            X, y, T = bfs_samples(dg, srcs)
+            bx.append(X)
+            by.append(y)
+            bw.append(regret_weights(T))
+            X, y, w = sssp_samples(dg, srcs[: max(1, sources // 2)])
+            sx.append(X)
+            sy.append(y)
+            sw.append(w)
+            log(f"traces: scale {sc} a={a:.3f} ef={ef:.1f}: {len(by[-1])} BFS levels "
+                f"(labels {np.bincount(by[-1], minlength=3).tolist()}), {len(sy[-1])} SSSP phases "
+                f"(class {int(sy[-1][0])}) in {time.time() - t0:.1f}s")
+            dg.close()
+            hg.close()
+    cat = np.concatenate
+    return (cat(bx), cat(by), cat(bw)), (cat(sx), cat(sy), cat(sw))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--out", default=None, help="schedule file to write")
+    ap.add_argument("--scales", default="18,19,20,21,22")
+    ap.add_argument("--graphs", type=int, default=2, help="graphs per scale")
+    ap.add_argument("--sources", type=int, default=8, help="sources per graph")
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--seed", type=int, default=7)
+    args = ap.parse_args(argv)
+    scales = [int(s) for s in args.scales.split(",")]
+    (bx, by, bw), (sx, sy, sw) = trace_set(scales, args.graphs, args.sources, args.seed,
+                                           log=lambda s: print(s, flush=True))
+    hb = train(NET_BFS, bx, by, steps=args.steps, seed=args.seed, weights=bw)
+    hs = train(NET_SSSP, sx, sy, steps=args.steps, seed=args.seed + 1, weights=sw)
+    report = {
+        "bfs_samples": int(len(by)), "bfs_label_counts": np.bincount(by, minlength=3).tolist(),
+        "bfs_loss": [hb[0], hb[-1]], "bfs_train_accuracy": accuracy(NET_BFS, bx, by, 3),
+        "sssp_samples": int(len(sy)), "sssp_label_counts": np.bincount(sy, minlength=4).tolist(),
+        "sssp_loss": [hs[0], hs[-1]], "sssp_train_accuracy": accuracy(NET_SSSP, sx, sy, 4),
+    }
+    print(json.dumps(report), flush=True)
+    if args.out:
+        sched_save(args.out)
+        print(f"wrote {args.out}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

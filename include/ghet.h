@@ -270,6 +270,9 @@ int ghet_sched_set_params(int kind, const float* params);
 int ghet_sched_get_params(int kind, float* params);
 /* Forward pass on the GPU: x [batch,16] FP32 -> logits [batch,4] FP32. */
 int ghet_mlp_forward(int kind, int32_t batch, const float* x, float* logits);
+/* The host mirror of the same forward pass (bit-identical logits); it makes
+ * the per-level decisions inside ghet_bfs / ghet_sssp. */
+int ghet_mlp_forward_host(int kind, int32_t batch, const float* x, float* logits);
 /* One training step on the GPU (softmax cross-entropy, Adam).  Returns the
  * mean loss in *loss and, if grad_out != NULL, the FP32 gradient of the
  * loss w.r.t. the parameters before the update (GHET_MLP_PARAMS floats). */
