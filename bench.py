@@ -47,6 +47,26 @@ def pr_iter_bytes(n: int, m: int) -> int:
     return 4 * (n + 1) + 4 * m + 4 * n + 4 * n + 4 * n
 
 
+def measured_traffic(workload: str, seed: int):
+    """HBM bytes per k_pr_iter launch measured by the rocprofv3 PMC runs
+    recorded under profiles/ for this workload (None if not measured)."""
+    path = os.path.join(ROOT, "profiles", f"k_pr_iter__{workload}.traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#"):
+                continue
+            for tok in line.split():
+                if "=" in tok:
+                    k, v = tok.split("=", 1)
+                    kv[k] = v
+    if kv.get("workload") != workload or kv.get("seed") != str(seed):
+        return None
+    return int(kv["traffic_bytes"])
+
+
 def gteps(edges, ms):
     return edges / (ms * 1e-3) / 1e9 if ms > 0 else None
 
@@ -70,9 +90,13 @@ def main():
     ap.add_argument("--many-max-scale", type=int, default=14,
                     help="largest small graph is 2^this vertices (the full form uses 18)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-heuristic section")
+    ap.add_argument("--sched-file", default=os.path.join(ROOT, "graphhet_amd", "sched_net.txt"),
+                    help="trained schedule networks (python -m graphhet_amd.learn)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
+    sys.modules[__name__]._args = args
 
     if args.gpus != 1:
         log("bench.py measures one GPU: this build has no multi-GPU path "
@@ -133,9 +157,14 @@ def main():
         k_ms = med("pr", lambda s: s.hot_kernel_ms / max(s.hot_kernel_launches, 1))
         bytes_it = pr_iter_bytes(n, m)
         pr_gbs = bytes_it / (k_ms * 1e-3) / 1e9
+        traffic = measured_traffic(f"rmat-s{args.scale}-ef{int(args.edge_factor)}", args.seed)
         roofline = {"bound": "hbm", "achieved": round(pr_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(pr_gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                    "frac": round(pr_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
                     "kernel": "k_pr_iter", "kernel_avg_us": round(k_ms * 1e3, 2), "bytes_per_launch": bytes_it}
+
+    learned = None
+    if not args.no_learned:
+        learned = learned_section(G, dg, hg, src)
 
     other = {}
     if extras:
@@ -187,11 +216,39 @@ def main():
         "bfs_levels": bfs_stats.levels if bfs_stats else None,
         "edges_traversed": bfs_stats.edges_traversed if bfs_stats else None,
         "other_algorithms": other,
+        "learned": learned,
         "batched": batched,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def learned_section(G, dg, hg, src):
+    """Learned schedule vs the heuristic (and GraphHET's n/100 rule) on this
+    graph, which the networks never saw (they are trained on scales 18-22).
+    Outputs are compared bit for bit."""
+    from graphhet_amd import learn
+    args = sys.modules[__name__]._args
+    if not G.sched_load(args.sched_file):
+        log(f"no usable schedule file at {args.sched_file}: learned section skipped")
+        return None
+    srcs = np.concatenate([[src], learn.pick_sources(hg, 8, 24)]).astype(np.int32)
+    learn.compare_bfs(dg, srcs[:2])  # warm-up
+    b = learn.compare_bfs(dg, srcs)
+    s = learn.compare_sssp(dg, srcs[:4])
+    _, st = dg.bfs(src, schedule=G.SCHED_LEARNED, out=False)
+    log("learned section done")
+    return {
+        "sources": len(srcs),
+        "bfs_ms": {k: round(b[k], 3) for k in ("heuristic", "learned", "reference")},
+        "bfs_gain_vs_heuristic": round(b["gain_vs_heuristic"], 3),
+        "bfs_gain_vs_reference_rule": round(b["gain_vs_reference"], 3),
+        "sssp_ms": {k: round(s[k], 3) for k in ("heuristic", "learned")},
+        "sssp_gain_vs_heuristic": round(s["gain_vs_heuristic"], 3),
+        "identical": bool(b["identical"] and s["identical"]),
+        "bfs_gteps_learned_headline_source": round(gteps(st.edges_traversed, st.device_ms), 3),
+    }
 
 
 def batched_section(G, args):
