@@ -7,7 +7,11 @@
 // bit-identical to per-source runs whatever the interleaving.
 //
 // Work items are (source, vertex) pairs packed as (s << 32) | v; distances
-// and stamps are source-major [K][n].  A round expands the concatenated near
+// and stamps are vertex-major [n][K]: the K sources' values of one vertex
+// share cache lines, and R-MAT frontiers of different sources meet at the
+// same hubs in the same rounds, so a pass of K sources touches far fewer
+// lines than K separate searches.  The caller's [k][n] rows are produced by
+// one tiled transpose at the end of a pass.  A round expands the concatenated near
 // queues of all K sources edge-balanced (int64 degree scan, 1024-edge tiles),
 // so even when each single search has a thin frontier the batch fills the chip.
 #include <cmath>
@@ -34,6 +38,8 @@ constexpr int kItems = 4;
 constexpr int kTile = kBT * kItems;
 constexpr int kOut = 2 * kTile;
 constexpr int kMaxBlocks = 2048;
+constexpr int kMaxK = 256;     // sources per pass (vertex-major stride)
+constexpr int kDefaultK = 16;
 
 __device__ __forceinline__ int64_t upper_bound_i64(const int64_t* a, int64_t len, int64_t key) {
     int64_t lo = 0, hi = len;
@@ -52,7 +58,7 @@ __global__ void k_mss_seed(const int32_t* __restrict__ srcs, int32_t k, int32_t 
                            int32_t* stamp_near) {
     const int i = threadIdx.x;
     if (i < k) {
-        const int64_t idx = static_cast<int64_t>(i) * n + srcs[i];
+        const int64_t idx = static_cast<int64_t>(srcs[i]) * k + i;
         dist[idx] = 0.0;
         stamp_near[idx] = 0;
         queue[i] = (static_cast<uint64_t>(i) << 32) | static_cast<uint32_t>(srcs[i]);
@@ -80,7 +86,7 @@ __global__ void k_item_deg(const uint64_t* __restrict__ q, int64_t F, const int3
 __global__ void __launch_bounds__(kBT) k_mss_relax(const uint64_t* __restrict__ q, int64_t F,
                                                    const int64_t* __restrict__ scan, int64_t E,
                                                    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                                   const double* __restrict__ wts, int32_t n, double* dist,
+                                                   const double* __restrict__ wts, int32_t ks, double* dist,
                                                    double threshold, int32_t* stamp_near, int32_t next_round,
                                                    int32_t* stamp_far, int32_t far_gen, uint64_t* __restrict__ next_q,
                                                    uint64_t* far_list, int far_sel, MssCtr* ctr) {
@@ -140,14 +146,15 @@ __global__ void __launch_bounds__(kBT) k_mss_relax(const uint64_t* __restrict__ 
                     it = q[i0 + j];
                     pos = rowptr[item_v(it)] + (e - scan[i0 + j]);
                 }
-                const int64_t sb = item_s(it) * n;
+                const int64_t sb = item_s(it);
                 const int32_t v = col[pos];
-                const double nd = dist[sb + item_v(it)] + wts[pos];
-                if (nd < dist[sb + v]) {
-                    const double old = atomic_min_nonneg(&dist[sb + v], nd);
+                const int64_t pv = static_cast<int64_t>(v) * ks + sb;
+                const double nd = dist[static_cast<int64_t>(item_v(it)) * ks + sb] + wts[pos];
+                if (nd < dist[pv]) {
+                    const double old = atomic_min_nonneg(&dist[pv], nd);
                     if (nd < old) {
-                        if (nd < threshold) to_near = atomicExch(&stamp_near[sb + v], next_round) != next_round;
-                        else to_far = atomicExch(&stamp_far[sb + v], far_gen) != far_gen;
+                        if (nd < threshold) to_near = atomicExch(&stamp_near[pv], next_round) != next_round;
+                        else to_far = atomicExch(&stamp_far[pv], far_gen) != far_gen;
                     }
                 }
                 out = (it & 0xFFFFFFFF00000000ULL) | static_cast<uint32_t>(v);
@@ -165,7 +172,7 @@ __global__ void __launch_bounds__(kBT) k_mss_relax(const uint64_t* __restrict__ 
 }
 
 __global__ void __launch_bounds__(kBT) k_mss_split(const uint64_t* __restrict__ far_in, int64_t count,
-                                                   const double* __restrict__ dist, int32_t n, double old_thr,
+                                                   const double* __restrict__ dist, int32_t ks, double old_thr,
                                                    double new_thr, int32_t* stamp_near, int32_t next_round,
                                                    int32_t* stamp_far, int32_t new_gen, uint64_t* __restrict__ near_q,
                                                    uint64_t* far_out, int other, const int32_t* __restrict__ rowptr,
@@ -192,7 +199,7 @@ __global__ void __launch_bounds__(kBT) k_mss_split(const uint64_t* __restrict__ 
         uint64_t it = 0;
         if (i < count) {
             it = far_in[i];
-            const int64_t idx = item_s(it) * n + item_v(it);
+            const int64_t idx = static_cast<int64_t>(item_v(it)) * ks + item_s(it);
             const double d = dist[idx];
             if (d >= old_thr) {
                 if (d < new_thr) {
@@ -224,16 +231,32 @@ __global__ void __launch_bounds__(kBT) k_mss_split(const uint64_t* __restrict__ 
     }
 }
 
+// [n][ks] -> [ks][n] through a 64x64 LDS tile (coalesced on both sides).
+__global__ void __launch_bounds__(256) k_mss_transpose(const double* __restrict__ in, int32_t n, int32_t ks,
+                                                       double* __restrict__ out) {
+    __shared__ double t[64][65];
+    const int64_t v0 = static_cast<int64_t>(blockIdx.x) * 64, s0 = static_cast<int64_t>(blockIdx.y) * 64;
+    for (int r = threadIdx.y; r < 64; r += 4) {
+        const int64_t v = v0 + r, sidx = s0 + threadIdx.x;
+        if (v < n && sidx < ks) t[r][threadIdx.x] = in[v * ks + sidx];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 64; r += 4) {
+        const int64_t sidx = s0 + r, v = v0 + threadIdx.x;
+        if (v < n && sidx < ks) out[sidx * n + v] = t[threadIdx.x][r];
+    }
+}
+
 // Reached-vertex statistics over the batch: edges of (source, vertex) pairs
 // with a finite distance.
-__global__ void __launch_bounds__(256) k_mss_reached(const double* __restrict__ dist, int64_t total, int32_t n,
+__global__ void __launch_bounds__(256) k_mss_reached(const double* __restrict__ dist, int64_t total, int32_t ks,
                                                      double inf, const int32_t* __restrict__ rowptr, CtrSlot* slots) {
     __shared__ unsigned long long s_red[4];
     unsigned long long e = 0;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x)
         if (dist[i] != inf) {
-            const int32_t v = static_cast<int32_t>(i % n);
+            const int32_t v = static_cast<int32_t>(i / ks);
             e += rowptr[v + 1] - rowptr[v];
         }
     e = block_sum_u64<256>(e, s_red);
@@ -254,9 +277,14 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     hipStream_t s = pick_stream(g, o);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
     const double delta = (have_opts && o->sssp_delta > 0) ? o->sssp_delta : sssp_base_delta(g);
-    // Sources per pass: bounded by a 16 GB working set (dist, two stamps, four queues ~ 52 B per pair).
-    const int64_t budget = int64_t(16) << 30;
-    const int32_t K = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>({64, k, budget / (52 * int64_t(n))})));
+    // Sources per pass: 16 by default (fastest measured on R-MAT s22: 16 ->
+    // 1692 ms, 64 -> 1944 ms, 256 -> 2140 ms for 256 sources; wider passes add
+    // same-line atomic contention and rounds), at most kMaxK, bounded by a
+    // 64 GB working set (dist, two stamps, four queues ~ 52 B per pair).
+    const int64_t budget = int64_t(64) << 30;
+    const int64_t kcap = (have_opts && o->reserved[3] > 0) ? o->reserved[3] : kDefaultK;
+    const int32_t K = static_cast<int32_t>(
+        std::max<int64_t>(1, std::min<int64_t>({kcap, int64_t(kMaxK), int64_t(k), budget / (52 * int64_t(n))})));
     const int64_t KN = int64_t(K) * n;
     void* p;
     double* dist;
@@ -289,7 +317,7 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
         GHET_CHECK(fill<int32_t>(stamp_far, -1, kn, s));
         GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(ctr), 0, kCtrWords, s));
         GHET_HIP(hipMemcpyAsync(d_src, srcs + p0, 4 * size_t(kp), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_mss_seed, dim3(1), dim3(64), 0, s, d_src, kp, n, dist, qa, stamp_near);
+        hipLaunchKernelGGL(k_mss_seed, dim3(1), dim3(kMaxK), 0, s, d_src, kp, n, dist, qa, stamp_near);
         int64_t F = kp, E = 0;
         for (int32_t i = 0; i < kp; ++i) E += g->h_out_rowptr[srcs[p0 + i] + 1] - g->h_out_rowptr[srcs[p0 + i]];
         int32_t round = 0, far_gen = 0;
@@ -306,7 +334,7 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
                 if (E > 0) {
                     const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), kMaxBlocks));
                     hipLaunchKernelGGL(k_mss_relax, dim3(grid), dim3(kBT), 0, s, qa, F, scan, E, g->out.rowptr,
-                                       g->out.col, g->out.w, n, dist, threshold, stamp_near, next_round, stamp_far,
+                                       g->out.col, g->out.w, kp, dist, threshold, stamp_near, next_round, stamp_far,
                                        far_gen, qb, far[sel], sel, ctr);
                 }
                 GHET_HIP(hipGetLastError());
@@ -327,7 +355,7 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
                 hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, other);
                 const int32_t next_round = round + 1;
                 const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, kBT), 2048)));
-                hipLaunchKernelGGL(k_mss_split, dim3(grid), dim3(kBT), 0, s, far[sel], far_count, dist, n, old_thr,
+                hipLaunchKernelGGL(k_mss_split, dim3(grid), dim3(kBT), 0, s, far[sel], far_count, dist, kp, old_thr,
                                    new_thr, stamp_near, next_round, stamp_far, far_gen + 1, qa, far[other], other,
                                    g->out.rowptr, ctr);
                 GHET_HIP(hipGetLastError());
@@ -352,11 +380,17 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
         }
         if (st) {
             hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, -1);
-            hipLaunchKernelGGL(k_mss_reached, dim3(2048), dim3(256), 0, s, dist, kn, n, inf, g->out.rowptr, ctr->slots);
+            hipLaunchKernelGGL(k_mss_reached, dim3(2048), dim3(256), 0, s, dist, kn, kp, inf, g->out.rowptr, ctr->slots);
             GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(MssCtr), hipMemcpyDeviceToHost, s));
         }
-        if (dist_out)
-            GHET_HIP(hipMemcpyAsync(dist_out + int64_t(p0) * n, dist, 8 * size_t(kn), hipMemcpyDeviceToHost, s));
+        if (dist_out) {
+            // vertex-major [n][kp] -> the caller's source-major [kp][n], staged in the (now idle) queue
+            double* tr = reinterpret_cast<double*>(qa);
+            hipLaunchKernelGGL(k_mss_transpose, dim3(static_cast<unsigned>(ceil_div(n, 64)), static_cast<unsigned>(ceil_div(kp, 64))),
+                               dim3(64, 4), 0, s, dist, n, kp, tr);
+            GHET_HIP(hipGetLastError());
+            GHET_HIP(hipMemcpyAsync(dist_out + int64_t(p0) * n, tr, 8 * size_t(kn), hipMemcpyDeviceToHost, s));
+        }
         GHET_HIP(hipStreamSynchronize(s));
         if (st) traversed += static_cast<int64_t>(slot_sum(hc->slots, 2));
     }
