@@ -14,9 +14,10 @@
 //   d <= 32      one lane, labels staged in LDS, pairwise counting;
 //   d <= 256     one wave, labels in LDS, each lane counts its labels;
 //   d <= 2048    one workgroup, open-addressing hash table in LDS (4096 slots);
-//   d > 2048     several workgroups per vertex insert into a global hash table
-//                (2^ceil(log2(2d)) slots), then one workgroup takes the argmax
-//                and clears the table for the next iteration.
+//   d > 2048     one workgroup per 4096-edge chunk counts in an LDS hash table,
+//                then merges (label, count) pairs into the vertex's global
+//                hash table (2^ceil(log2(2d)) slots); one workgroup takes the
+//                argmax and clears the table for the next iteration.
 // Winner = max count, ties to the smallest label: the maximum of the key
 // (count << 32) | (0xFFFFFFFF - label).
 #include "graph_internal.hpp"
@@ -26,7 +27,8 @@ namespace {
 
 constexpr int kBT = 256;
 constexpr int kSmall = 32, kWave = 256, kLds = 2048, kLdsSlots = 4096;
-constexpr int kHeavyChunk = 8192;
+constexpr int kHeavyChunk = 4096;  // edges per heavy-vertex chunk
+constexpr int kHeavySlots = 8192;  // LDS hash slots of a chunk (load <= 1/2)
 constexpr int32_t kEmpty = -1;
 
 __device__ __forceinline__ uint64_t pack(uint32_t count, int32_t label) {
@@ -144,26 +146,50 @@ __global__ void __launch_bounds__(kBT) k_cdlp_lds(const int32_t* __restrict__ li
     }
 }
 
-// Heavy vertices: chunk c inserts the labels of edges [beg, beg+kHeavyChunk)
-// of heavy vertex hv into its global table.
+// Heavy vertices: chunk c counts the labels of edges [beg, beg+kHeavyChunk)
+// of heavy vertex hv in an LDS hash table first, then adds one (label,
+// count) pair per distinct label to the vertex's global table.  Once labels
+// converge most of a hub's neighbours share a few labels; counting edge by
+// edge in global memory would pile every one of them onto one counter.
 __global__ void __launch_bounds__(kBT) k_cdlp_heavy_count(
     const int32_t* __restrict__ list, const int64_t* __restrict__ base, const int32_t* __restrict__ chunk_vtx,
     const int32_t* __restrict__ chunk_beg, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
     const int32_t* __restrict__ lab, int32_t* tkeys, uint32_t* tcnts) {
+    __shared__ int32_t s_keys[kHeavySlots];
+    __shared__ uint32_t s_cnts[kHeavySlots];
     const int32_t hv = chunk_vtx[blockIdx.x];
     const int32_t v = list[hv];
     const int32_t beg = chunk_beg[blockIdx.x], end = min(beg + kHeavyChunk, rowptr[v + 1]);
+    for (int j = threadIdx.x; j < kHeavySlots; j += kBT) {
+        s_keys[j] = kEmpty;
+        s_cnts[j] = 0;
+    }
+    __syncthreads();
+    for (int32_t j = beg + threadIdx.x; j < end; j += kBT) {
+        const int32_t x = lab[col[j]];
+        uint32_t h = hash_slot(x, kHeavySlots - 1);
+        for (;;) {
+            const int32_t prev = atomicCAS(&s_keys[h], kEmpty, x);
+            if (prev == kEmpty || prev == x) {
+                atomicAdd(&s_cnts[h], 1u);
+                break;
+            }
+            h = (h + 1) & (kHeavySlots - 1);
+        }
+    }
+    __syncthreads();
     const int64_t b0 = base[hv];
     const uint32_t mask = static_cast<uint32_t>(base[hv + 1] - b0 - 1);
     int32_t* keys = tkeys + b0;
     uint32_t* cnts = tcnts + b0;
-    for (int32_t j = beg + threadIdx.x; j < end; j += kBT) {
-        const int32_t x = lab[col[j]];
+    for (int j = threadIdx.x; j < kHeavySlots; j += kBT) {
+        const int32_t x = s_keys[j];
+        if (x == kEmpty) continue;
         uint32_t h = hash_slot(x, mask);
         for (;;) {
-            int32_t prev = atomicCAS(&keys[h], kEmpty, x);
+            const int32_t prev = atomicCAS(&keys[h], kEmpty, x);
             if (prev == kEmpty || prev == x) {
-                atomicAdd(&cnts[h], 1u);
+                atomicAdd(&cnts[h], s_cnts[j]);
                 break;
             }
             h = (h + 1) & mask;
