@@ -74,7 +74,7 @@ class DevicePtrs(C.Structure):
 
 # Every symbol include/ghet.h declares (checked by the CPU tests).
 EXPORTS = [
-    "ghet_last_error", "ghet_abi_version", "ghet_device_count",
+    "ghet_last_error", "ghet_abi_version", "ghet_device_count", "ghet_device_mem_info",
     "ghet_gen_rmat", "ghet_gen_er", "ghet_host_graph_from_edges", "ghet_host_graph_view", "ghet_host_graph_free",
     "ghet_graph_create", "ghet_graph_wrap_device", "ghet_graph_destroy", "ghet_graph_device_ptrs",
     "ghet_graph_device_bytes",
@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
         "ghet_last_error": (C.c_char_p, []),
         "ghet_abi_version": (C.c_int, []),
         "ghet_device_count": (C.c_int, []),
+        "ghet_device_mem_info": (C.c_int, [P(C.c_size_t), P(C.c_size_t)]),
         "ghet_gen_rmat": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
                                     P(vp)]),
         "ghet_gen_er": (C.c_int, [C.c_int32, C.c_int64, C.c_uint64, C.c_int, P(vp)]),

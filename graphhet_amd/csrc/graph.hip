@@ -144,6 +144,15 @@ int ghet_device_count(void) {
     return c;
 }
 
+int ghet_device_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    clear_error();
+    if (!free_bytes || !total_bytes) return set_error(GHET_ERR_INVALID, "NULL output");
+    *free_bytes = *total_bytes = 0;
+    if (ghet_device_count() < 1) return set_error(GHET_ERR_NODEVICE, "no GPU visible");
+    GHET_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return GHET_OK;
+}
+
 int ghet_graph_create(int32_t n, const int32_t* out_rowptr, const int32_t* out_col, const double* out_w,
                       const int32_t* in_rowptr, const int32_t* in_col, const double* in_w,
                       const int32_t* all_rowptr, const int32_t* all_col, uint32_t flags, ghet_graph** out) {

@@ -43,6 +43,9 @@ const char* ghet_last_error(void);
 int ghet_abi_version(void);
 /* Number of visible GPUs (0 when none); never initialises more than HIP's runtime. */
 int ghet_device_count(void);
+/* Free and total device memory of the current GPU (replaces the LDBC driver's
+ * cudaMemGetInfo calls, LDBC_GPU_csr.cu:74,116). */
+int ghet_device_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ------------------------------------------------------------------------ */
 /* Host-side graphs (no GPU needed)                                          */
