@@ -387,14 +387,20 @@ class DeviceGraph:
         _check(lib().ghet_tc(self._h, _ptr(cnt, C.c_int64), C.byref(_opts()), C.byref(st)))
         return cnt, st
 
-    def bfs_multi(self, srcs, out=True, schedule=SCHED_HEURISTIC, push_div=0):
+    def bfs_multi(self, srcs, out=True, schedule=SCHED_HEURISTIC, push_div=0, trace_cap=0):
+        """Depths [k, n] of k sources in one batched launch; with trace_cap,
+        also the first pass's per-level records (mode, features[0] = frontier
+        out-edges / m, device ms)."""
         srcs = np.ascontiguousarray(srcs, dtype=np.int32)
         depth = np.empty((len(srcs), self.n), np.int32) if out else None
         st = RunStats()
-        o = _opts(schedule)
+        tr = (LevelTrace * trace_cap)() if trace_cap else None
+        o = _opts(schedule, trace=tr)
         o.reserved[2] = push_div
         _check(lib().ghet_bfs_multi(self._h, len(srcs), _ptr(srcs, C.c_int32), _ptr(depth, C.c_int32),
                                     C.byref(o), C.byref(st)))
+        if trace_cap:
+            return depth, st, [tr[i] for i in range(st.trace_len)]
         return depth, st
 
     def autotune(self, cache_path: str | None = None) -> int:

@@ -78,6 +78,17 @@ struct CDLPLayout {
     void* mem = nullptr;
 };
 
+// Batched BFS: vertices whose in-lists are too long for one wave, and their
+// in-edges cut into fixed-size chunks any wave can take.
+struct MSLayout {
+    bool built = false;
+    int32_t num_heavy = 0;
+    int32_t num_chunks = 0;
+    int32_t* heavy = nullptr;  // [num_heavy] vertex ids
+    int2* chunks = nullptr;    // [num_chunks] {vertex, first in-edge}
+    void* mem = nullptr;
+};
+
 // Degree-ordered DAG of a symmetric graph for TC.
 struct TCLayout {
     bool built = false;
@@ -113,6 +124,7 @@ struct ghet_graph {
     ghet::PRLayout pr;
     ghet::CDLPLayout cdlp;
     ghet::TCLayout tc;
+    ghet::MSLayout ms;
     ghet::Tuning tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;

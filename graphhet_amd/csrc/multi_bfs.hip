@@ -11,7 +11,11 @@
 //   push  frontier vertices OR their set into acc[w] of each out-neighbour
 //         (atomicOr), then every vertex takes next = acc & ~seen;
 //   pull  every vertex not yet seen by all sources ORs front[u] over its
-//         in-neighbours, stopping as soon as nothing unseen is left.
+//         in-neighbours, stopping as soon as nothing unseen is left; in-lists
+//         longer than 256 are cut into 2048-edge chunks that any wave takes
+//         (OR into acc[v], settled after a grid barrier) -- R-MAT hubs sit at
+//         low ids, and one wave walking all of them serialised every pull
+//         level on its memory round trips.
 // The traversal writes only the 32-byte set of new bits per (level, vertex)
 // into a level log (one coalesced store); depths are materialised afterwards
 // by a pass that reads a vertex's log column and writes source-major rows
@@ -30,6 +34,8 @@ constexpr int kBT = 256;
 constexpr int kWords = 4;
 constexpr int kPass = 64 * kWords;  // sources per pass
 constexpr int kLogLevels = 32;      // levels recorded in the log
+constexpr int kPullHeavy = 256;     // pull: longer in-lists are split into chunks
+constexpr int kChunkE = 2048;       // in-edges per chunk
 
 struct Bits {
     uint64_t w[kWords];
@@ -49,6 +55,14 @@ struct MsArgs {
     int64_t push_limit;
     int32_t max_levels;
     int32_t* levels_out;
+    // optional per-level record (block 0 writes it after each level's
+    // barrier): {wall-clock stamp, mode (0 push, 1 pull), frontier out-edges}
+    unsigned long long* lvl_rec;
+    int32_t lvl_cap;
+    const int32_t* heavy;  // [num_heavy] vertices with in-degree > kPullHeavy
+    int32_t num_heavy;
+    const int2* chunks;    // [num_chunks] {vertex, first in-edge} of their in-lists
+    int32_t num_chunks;
 };
 
 __device__ __forceinline__ bool any_bits(const Bits& b) { return (b.w[0] | b.w[1] | b.w[2] | b.w[3]) != 0; }
@@ -102,6 +116,7 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
     CtrSlot* totals = a.slots + 3 * kSlots;
     int64_t front_edges = a.front_edges0;
     int32_t level = 0;
+    if (a.lvl_rec && blockIdx.x == 0 && threadIdx.x == 0) a.lvl_rec[0] = wall_clock64();  // start stamp
     for (; level < a.max_levels; ++level) {
         const Bits* front = (level & 1) ? a.fb : a.fa;
         Bits* next = (level & 1) ? a.fa : a.fb;
@@ -164,7 +179,8 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                 if (logl && !any_bits(nb)) logl[v] = nb;
             }
         } else {
-            // pull: every vertex not yet seen by all sources scans its in-list
+            // pull: every vertex not yet seen by all sources scans its in-list;
+            // in-lists longer than kPullHeavy are left to the chunk pass below
             for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kBT; b0 < n; b0 += static_cast<int64_t>(gridDim.x) * kBT) {
                 const int64_t v = b0 + threadIdx.x;
                 Bits sn = {}, acc = {};
@@ -172,11 +188,10 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                 bool todo = false;
                 if (v < n) {
                     sn = a.seen[v];
-                    todo = !covers(Bits{}, sn, a.full);
-                    if (todo) {
-                        b = a.in_rowptr[v];
-                        e = a.in_rowptr[v + 1];
-                    }
+                    b = a.in_rowptr[v];
+                    e = a.in_rowptr[v + 1];
+                    todo = e - b <= kPullHeavy && !covers(Bits{}, sn, a.full);
+                    if (!todo) e = b;
                 }
                 const bool light = e - b <= 32;
                 if (todo && light)
@@ -225,6 +240,52 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
                     next[v] = nb;
                 }
             }
+            // Long in-lists: any wave takes any chunk (a hub's list would
+            // otherwise serialise on the one wave that owns the hub), ORs the
+            // frontier sets of its edges and adds the new bits to acc[v].
+            const int64_t wave_id = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+            const int64_t num_waves = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+            for (int64_t c = wave_id; c < a.num_chunks; c += num_waves) {
+                const int2 ch = a.chunks[c];
+                const Bits sn = a.seen[ch.x];
+                if (covers(Bits{}, sn, a.full)) continue;  // wave-uniform
+                const int32_t e1 = min(ch.y + kChunkE, a.in_rowptr[ch.x + 1]);
+                Bits acc = {};
+                for (int32_t i = ch.y + lane; i < e1; i += 4 * 64) {
+                    Bits f[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int32_t ij = i + j * 64;
+                        f[j] = ij < e1 ? front[a.in_col[ij]] : Bits{};
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        for (int k = 0; k < kWords; ++k) acc.w[k] |= f[j].w[k];
+                }
+                if (lane == 0) scanned += e1 - ch.y;
+                acc = wave_or(acc);
+                if (lane < kWords) {
+                    const uint64_t add = acc.w[lane] & ~sn.w[lane] & a.full.w[lane];
+                    if (add) atomicOr(reinterpret_cast<unsigned long long*>(&a.acc[ch.x].w[lane]), add);
+                }
+            }
+            grid.sync();
+            // settle the long-list vertices from acc (and clear it for the next level)
+            for (int64_t h = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; h < a.num_heavy;
+                 h += static_cast<int64_t>(gridDim.x) * kBT) {
+                const int32_t v = a.heavy[h];
+                const Bits ac = a.acc[v];
+                if (!any_bits(ac)) continue;
+                a.acc[v] = Bits{};
+                const Bits sn = a.seen[v];
+                Bits nb;
+                for (int k = 0; k < kWords; ++k) nb.w[k] = ac.w[k] & ~sn.w[k] & a.full.w[k];
+                if (any_bits(nb)) {
+                    settle(a, v, sn, nb, level + 1, ne, tr);
+                    next[v] = nb;
+                    any = true;
+                }
+            }
         }
         if (__ballot(any) && lane == 0)
             __hip_atomic_store(&a.flags[level % 3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -256,6 +317,12 @@ __global__ void __launch_bounds__(kBT) k_msbfs(MsArgs a) {
             s_front_edges = fe;
         }
         __syncthreads();
+        if (a.lvl_rec && blockIdx.x == 0 && threadIdx.x == 0 && level + 1 < a.lvl_cap) {
+            unsigned long long* r = a.lvl_rec + 3 * static_cast<int64_t>(level + 1);
+            r[0] = wall_clock64();
+            r[1] = front_edges < a.push_limit ? 0ULL : 1ULL;
+            r[2] = static_cast<unsigned long long>(front_edges);
+        }
         front_edges = s_front_edges;
         if (s_stop) break;
     }
@@ -294,6 +361,32 @@ __global__ void __launch_bounds__(256) k_ms_materialize(const Bits* __restrict__
     }
 }
 
+// Long in-lists of the in-CSR and their chunks (once per graph).
+int build_ms_layout(ghet_graph* g, hipStream_t s) {
+    MSLayout& L = g->ms;
+    if (L.built) return GHET_OK;
+    const std::vector<int32_t>& rp = g->h_in_rowptr;
+    std::vector<int32_t> heavy;
+    std::vector<int2> chunks;
+    for (int32_t v = 0; v < g->n; ++v) {
+        if (rp[v + 1] - rp[v] <= kPullHeavy) continue;
+        heavy.push_back(v);
+        for (int32_t e = rp[v]; e < rp[v + 1]; e += kChunkE) chunks.push_back(make_int2(v, e));
+    }
+    const size_t hb = (4 * heavy.size() + 255) & ~size_t(255);
+    GHET_HIP(hipMalloc(&L.mem, hb + sizeof(int2) * chunks.size() + 256));
+    L.heavy = static_cast<int32_t*>(L.mem);
+    L.chunks = reinterpret_cast<int2*>(static_cast<char*>(L.mem) + hb);
+    if (!heavy.empty()) GHET_HIP(hipMemcpyAsync(L.heavy, heavy.data(), 4 * heavy.size(), hipMemcpyHostToDevice, s));
+    if (!chunks.empty())
+        GHET_HIP(hipMemcpyAsync(L.chunks, chunks.data(), sizeof(int2) * chunks.size(), hipMemcpyHostToDevice, s));
+    GHET_HIP(hipStreamSynchronize(s));
+    L.num_heavy = static_cast<int32_t>(heavy.size());
+    L.num_chunks = static_cast<int32_t>(chunks.size());
+    L.built = true;
+    return GHET_OK;
+}
+
 }  // namespace
 
 int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_out, const ghet_opts* o,
@@ -305,6 +398,7 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
     if (!g->out.present() || !g->in.present()) return set_error(GHET_ERR_INVALID, "batched BFS needs out- and in-CSR");
     if (k == 0) return GHET_OK;
     hipStream_t s = pick_stream(g, o);
+    GHET_CHECK(build_ms_layout(g, s));
     void* p;
     Bits *seen, *fa, *fb, *acc, *log;
     int32_t *out, *flags, *d_src, *d_levels;
@@ -323,14 +417,22 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
     slots = static_cast<CtrSlot*>(p);
     flags = reinterpret_cast<int32_t*>(slots + 4 * kSlots);
     d_levels = flags + 4;
-    GHET_CHECK(g->ws.get(WS_MULTI_C, 4 * size_t(kPass), &p)); d_src = static_cast<int32_t*>(p);
+    constexpr int kRecCap = 256;  // level records (slot 0 = start stamp)
+    GHET_CHECK(g->ws.get(WS_MULTI_C, 4 * size_t(kPass) + 24 * kRecCap, &p));
+    d_src = static_cast<int32_t*>(p);
+    unsigned long long* d_rec = reinterpret_cast<unsigned long long*>(d_src + kPass);
+    ghet_level_trace* trace = (o && o->size >= sizeof(ghet_opts)) ? o->trace : nullptr;
+    const int trace_cap = trace ? o->trace_cap : 0;
+    int trace_len = 0;
+    int clock_khz = 0;
+    if (trace) GHET_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, g->device));
     int dev_blocks = 0, cus = 0;
     GHET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&dev_blocks, reinterpret_cast<const void*>(k_msbfs), kBT, 0));
     GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
     const unsigned grid = static_cast<unsigned>(std::max(1, std::min(dev_blocks, 8) * cus));
     const int64_t bits_words = sizeof(Bits) / 8 * 4 * int64_t(n);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
-    const int64_t push_div = (have_opts && o->reserved[2] > 0) ? o->reserved[2] : 5;
+    const int64_t push_div = (have_opts && o->reserved[2] > 0) ? o->reserved[2] : 16;
 
     int64_t traversed = 0, scanned = 0, pushed = 0;
     int32_t max_levels_seen = 0;
@@ -371,13 +473,33 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
         a.push_limit = std::max<int64_t>(g->out.m / push_div, 1);
         a.max_levels = n + 1;
         a.levels_out = d_levels;
+        a.lvl_rec = (trace && p0 == 0) ? d_rec : nullptr;
+        a.lvl_cap = kRecCap;
+        a.heavy = g->ms.heavy;
+        a.num_heavy = g->ms.num_heavy;
+        a.chunks = g->ms.chunks;
+        a.num_chunks = g->ms.num_chunks;
         void* args[] = {&a};
         GHET_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_msbfs), dim3(grid), dim3(kBT), args, 0, s));
         GHET_HIP(hipMemcpyAsync(g->pinned, slots, kCtrBytes, hipMemcpyDeviceToHost, s));
+        unsigned long long* h_rec = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(g->pinned) + kCtrBytes);
+        static_assert(kCtrBytes + 24 * kRecCap <= kPinnedBytes, "pinned buffer");
+        if (a.lvl_rec) GHET_HIP(hipMemcpyAsync(h_rec, d_rec, 24 * kRecCap, hipMemcpyDeviceToHost, s));
         GHET_HIP(hipStreamSynchronize(s));
         const CtrSlot* hs = reinterpret_cast<const CtrSlot*>(g->pinned);
         const int32_t lv = reinterpret_cast<const int32_t*>(hs + 4 * kSlots)[4];
         max_levels_seen = std::max(max_levels_seen, lv);
+        if (a.lvl_rec && clock_khz > 0) {
+            // first pass: one record per level (mode, frontier out-edges / m, device ms)
+            for (int32_t L = 0; L < lv && L + 1 < kRecCap && trace_len < trace_cap; ++L) {
+                ghet_level_trace& t = trace[trace_len++];
+                std::memset(&t, 0, sizeof(t));
+                const unsigned long long* r = h_rec + 3 * (L + 1);
+                t.mode = static_cast<int32_t>(r[1]);
+                t.features[0] = static_cast<float>(static_cast<double>(r[2]) / std::max<int64_t>(g->out.m, 1));
+                t.ms = static_cast<float>(static_cast<double>(r[0] - h_rec[3 * L]) / clock_khz);
+            }
+        }
         traversed += static_cast<int64_t>(slot_sum(hs + 3 * kSlots, 1));
         scanned += static_cast<int64_t>(slot_sum(hs + 3 * kSlots, 2));
         pushed += static_cast<int64_t>(slot_sum(hs + 3 * kSlots, 3));
@@ -402,6 +524,7 @@ int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_
         st->extra[0] = grid;
         st->extra[1] = static_cast<double>(scanned);
         st->extra[2] = static_cast<double>(pushed);
+        st->trace_len = trace_len;
     }
     return GHET_OK;
 }

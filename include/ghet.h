@@ -169,7 +169,7 @@ typedef struct ghet_opts {
     /* reserved[0] bit0: PR without the FP64 normalisation (reference form)
      * reserved[1] bit0: many-graph calls skip the autotuner
      * reserved[2]     : batched BFS pushes while frontier out-edges < m / this
-     *                   (0 = default 5)
+     *                   (0 = default 16)
      * reserved[3]     : batched SSSP sources per pass (0 = 16, at most 256)
      * other entries must be 0                                             */
     int32_t reserved[6];
