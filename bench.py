@@ -4,6 +4,10 @@ One "step" is one pass of the metric's algorithms over the resident graph:
 BFS and SSSP from the max-out-degree vertex and PageRank (20 iterations,
 damping 0.85).  The graph is built and uploaded before the timed region;
 inputs are resident in HBM when it starts and results stay on the device.
+BFS and SSSP use the trained schedule networks (graphhet_amd/sched_net.txt,
+trained on scales 18-22, so this graph is held out) when they load, else the
+heuristic; the heuristic ("plain") form is timed beside it on the same
+source and reported under `plain_heuristic`.
 
 Prints ONE JSON line.  `value` is BFS GTEPS (m_r / t, m_r = out-edges of the
 reached vertices), the headline of the metric; SSSP GTEPS and PageRank HBM
@@ -96,7 +100,6 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
-    sys.modules[__name__]._args = args
 
     if args.gpus != 1:
         log("bench.py measures one GPU: this build has no multi-GPU path "
@@ -123,13 +126,20 @@ def main():
     dg = G.DeviceGraph(o, i, a, with_all=extras)
     log(f"uploaded in {time.time() - t0:.1f}s ({dg.device_bytes() / 1e9:.2f} GB)")
     src = int(np.argmax(o.degrees()))
+    # The step traverses with the trained schedule networks when they load
+    # (they never saw this graph: trained on scales 18-22); the heuristic
+    # ("plain") form is timed beside it.  Outputs are identical either way.
+    sched = G.SCHED_HEURISTIC
+    if not args.no_learned and G.sched_load(args.sched_file):
+        sched = G.SCHED_LEARNED
+    sched_name = "learned" if sched == G.SCHED_LEARNED else "heuristic"
 
     def step():
         r = {}
         if "bfs" in algos:
-            _, r["bfs"] = dg.bfs(src, out=False)
+            _, r["bfs"] = dg.bfs(src, schedule=sched, out=False)
         if "sssp" in algos:
-            _, r["sssp"] = dg.sssp(src, out=False)
+            _, r["sssp"] = dg.sssp(src, schedule=sched, out=False)
         if "pr" in algos:
             _, r["pr"] = dg.pr(args.pr_iters, 0.85, out=False)
         return r
@@ -162,8 +172,15 @@ def main():
                     "frac": round(pr_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
                     "kernel": "k_pr_iter", "kernel_avg_us": round(k_ms * 1e3, 2), "bytes_per_launch": bytes_it}
 
+    plain = {}
+    if sched != G.SCHED_HEURISTIC:
+        for alg, fn in (("bfs", dg.bfs), ("sssp", dg.sssp)):
+            if alg in algos:
+                sts = [fn(src, schedule=G.SCHED_HEURISTIC, out=False)[1] for _ in range(3)]
+                ms = statistics.median(x.device_ms for x in sts)
+                plain[alg] = {"ms": round(ms, 3), "gteps": round(gteps(sts[0].edges_traversed, ms), 3)}
     learned = None
-    if not args.no_learned:
+    if sched == G.SCHED_LEARNED:
         learned = learned_section(G, dg, hg, src)
 
     other = {}
@@ -209,7 +226,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"rmat-s{args.scale}-ef{int(args.edge_factor)} BFS+SSSP+PR pass",
                    "scale": args.scale, "edge_factor": args.edge_factor, "seed": args.seed, "n": n, "m": m,
-                   "source": src, "pr_iters": args.pr_iters, "step_algos": sorted(algos)},
+                   "source": src, "pr_iters": args.pr_iters, "step_algos": sorted(algos),
+                   "schedule": sched_name},
+        "plain_heuristic": plain or None,
         "sssp_gteps": round(sssp_gteps, 3) if sssp_gteps else None,
         "pr_hbm_gbs": round(pr_gbs, 1) if pr_gbs else None,
         "per_algorithm_ms": {k: (round(v, 3) if v is not None else None) for k, v in per_alg_ms.items()},
@@ -229,15 +248,10 @@ def learned_section(G, dg, hg, src):
     graph, which the networks never saw (they are trained on scales 18-22).
     Outputs are compared bit for bit."""
     from graphhet_amd import learn
-    args = sys.modules[__name__]._args
-    if not G.sched_load(args.sched_file):
-        log(f"no usable schedule file at {args.sched_file}: learned section skipped")
-        return None
     srcs = np.concatenate([[src], learn.pick_sources(hg, 8, 24)]).astype(np.int32)
     learn.compare_bfs(dg, srcs[:2])  # warm-up
     b = learn.compare_bfs(dg, srcs)
     s = learn.compare_sssp(dg, srcs[:4])
-    _, st = dg.bfs(src, schedule=G.SCHED_LEARNED, out=False)
     log("learned section done")
     return {
         "sources": len(srcs),
@@ -247,7 +261,6 @@ def learned_section(G, dg, hg, src):
         "sssp_ms": {k: round(s[k], 3) for k in ("heuristic", "learned")},
         "sssp_gain_vs_heuristic": round(s["gain_vs_heuristic"], 3),
         "identical": bool(b["identical"] and s["identical"]),
-        "bfs_gteps_learned_headline_source": round(gteps(st.edges_traversed, st.device_ms), 3),
     }
 
 
