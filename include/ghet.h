@@ -170,7 +170,7 @@ typedef struct ghet_opts {
      * reserved[1] bit0: many-graph calls skip the autotuner
      * reserved[2]     : batched BFS pushes while frontier out-edges < m / this
      *                   (0 = default 16)
-     * reserved[3]     : batched SSSP sources per pass (0 = 16, at most 256)
+     * reserved[3]     : batched SSSP sources per pass (0 = 32, at most 32)
      * other entries must be 0                                             */
     int32_t reserved[6];
 } ghet_opts;
