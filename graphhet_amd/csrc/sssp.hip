@@ -337,7 +337,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         delta = base_delta * std::ldexp(1.0, cls - 1);
         threshold = delta;
     }
-    int64_t rounds = 0;
+    int64_t rounds = 0, relaxed = 0;
     for (;;) {
         // relaxation rounds of the current bucket
         if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k0, s));
@@ -347,6 +347,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
                                g->out.rowptr, scan);
             GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
             const int32_t next_round = round + 1;
+            relaxed += E;
             if (E > 0) {
                 unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), kMaxBlocks));
                 hipLaunchKernelGGL(k_relax, dim3(grid), dim3(kBT), 0, s, qa, F, scan, static_cast<int32_t>(E),
@@ -447,6 +448,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         st->trace_len = trace_len;
         st->extra[0] = base_delta;
         st->extra[1] = phases;
+        st->extra[2] = static_cast<double>(relaxed);  // edges relaxed over all rounds
         st->status_flags = sched.fallback ? 1 : 0;
     }
     return sched.fallback ? GHET_WARN_FALLBACK : GHET_OK;
