@@ -47,12 +47,19 @@ def test_bfs_multi_matches_single(rmat14):
 
 def test_sssp_multi_matches_single(rmat14):
     hg, o, i, dg = rmat14
-    srcs = sources(o, 70, 2)  # two passes of up to 64 sources
+    srcs = sources(o, 70, 2)  # three passes of up to 32 sources (32 + 32 + 6)
+    srcs[5] = srcs[6]  # one vertex carrying two sources of a pass
+    srcs[-1] = int(np.argmin(o.degrees()))  # possibly a sink / isolated vertex
     dist, st = dg.sssp_multi(srcs)
+    tot = 0
     for j in range(len(srcs)):
-        ref, _ = dg.sssp(int(srcs[j]))
+        ref, rs = dg.sssp(int(srcs[j]))
         np.testing.assert_array_equal(dist[j], ref, err_msg=f"source #{j}")
+        tot += rs.edges_traversed
+    assert st.edges_traversed == tot
     np.testing.assert_array_equal(dist[3], O.sssp(o.rowptr, o.col, o.w, int(srcs[3])))
+    dist7, _ = dg.sssp_multi(srcs, per_pass=7)  # pass width changes nothing
+    np.testing.assert_array_equal(dist7, dist)
 
 
 def small_graphs(count, seed, sym):
