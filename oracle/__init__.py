@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
             "oracle_sssp_parent": [I32, i32p, i32p, f64p, f64p, F64, I32, i32p],
             "oracle_wcc": [I32, i32p, i32p, I32, i32p],
             "oracle_pr": [I32, i32p, i32p, i32p, F64, I32, I32, f64p],
+            "oracle_pr_step": [I32, i32p, i32p, i32p, F64, I32, f64p, f64p],
             "oracle_cdlp": [I32, i32p, i32p, i32p, i32p, I32, I32, i32p],
             "oracle_tc": [I32, i32p, i32p, I32, I32, i64p],
         })
@@ -146,6 +147,16 @@ def pr(in_rowptr, in_col, out_rowptr, damping=0.85, iters=10, threads: int = 100
     n = len(irp) - 1
     out = np.empty(n, np.float64)
     lib().oracle_pr(n, _p(irp, I32), _p(ic, I32), _p(orp, I32), damping, iters, threads, _p(out, F64))
+    return out
+
+
+def pr_step(in_rowptr, in_col, out_rowptr, rank, damping=0.85, threads: int = 100) -> np.ndarray:
+    """One PageRank iteration from `rank` (the oracle's operation order)."""
+    irp, ic, orp = _a(in_rowptr, np.int32), _a(in_col, np.int32), _a(out_rowptr, np.int32)
+    r = _a(rank, np.float64)
+    n = len(irp) - 1
+    out = np.empty(n, np.float64)
+    lib().oracle_pr_step(n, _p(irp, I32), _p(ic, I32), _p(orp, I32), damping, threads, _p(r, F64), _p(out, F64))
     return out
 
 

@@ -203,6 +203,29 @@ void oracle_pr(int32_t n, const int32_t* in_rowptr, const int32_t* in_col, const
     for (int32_t i = 0; i < n; ++i) rank_out[i] = rank[i];
 }
 
+// One iteration of oracle_pr (same operations, same order): next from rank.
+// Lets a checker compare every iteration of a run.
+void oracle_pr_step(int32_t n, const int32_t* in_rowptr, const int32_t* in_col, const int32_t* out_rowptr,
+                    double damp, int32_t threads, const double* rank, double* next) {
+    const double teleport = (1 - damp) / n;
+    double sink_sum = 0;
+    for (int32_t i = 0; i < n; ++i)
+        if (out_rowptr[i + 1] == out_rowptr[i]) sink_sum += rank[i];
+    const double common_add = (sink_sum * damp / n) + teleport;
+    run_chunks(100, threads, [&](int q) {
+        const int64_t start = static_cast<int64_t>(q) * n / 100;
+        const int64_t end = std::min<int64_t>(static_cast<int64_t>(n) - 1, static_cast<int64_t>(q + 1) * n / 100);
+        for (int64_t v = start; v <= end; ++v) {
+            double sum = 0;
+            for (int32_t e = in_rowptr[v]; e < in_rowptr[v + 1]; ++e) {
+                const int32_t u = in_col[e];
+                sum += rank[u] / static_cast<double>(out_rowptr[u + 1] - out_rowptr[u]);
+            }
+            next[v] = common_add + damp * sum;
+        }
+    });
+}
+
 // CPU_Community_Detection.hpp:12-64: synchronous; counts labels of in- then
 // out-neighbours with multiplicity in a hash map; the result is the label with
 // the largest count, ties to the smaller label, and a vertex without

@@ -1,0 +1,140 @@
+"""Parity at full size: R-MAT scale 24 (and an Erdos-Renyi graph with the same
+vertex and edge counts for PageRank).  Running the oracle's BFS/SSSP/WCC/TC
+on 263M edges takes too long here, so the exact results are checked through
+properties that determine them completely, vectorised over every edge:
+
+  BFS   depth(src) = 0; every edge u->v with u reached has depth(v) <=
+        depth(u) + 1; every reached v != src has an in-edge from depth(v) - 1;
+        unreached = INT_MAX.  Only the true depths satisfy all three.
+  SSSP  dist(src) = 0; every edge has dist(v) <= fl(dist(u) + w); every
+        reached v != src has an in-edge with dist(v) == fl(dist(u) + w);
+        unreached = DBL_MAX.  These are the equations of the rounded
+        min-plus fixpoint that every label-correcting order reaches.
+  WCC   labels equal the smallest vertex id of scipy's weak components.
+  TC    6 * t(x) on sampled vertices of degree <= 64 (pairs of neighbours
+        that are adjacent), and the total is 18 * (number of triangles).
+  PR    every one of 20 iterations against the FP64 oracle stepping from
+        the same start: L1 <= 1e-6 and max-abs reported; the reference form
+        (no normalisation) and the normalised form.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
+
+import graphhet_amd as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+INT_MAX = 2**31 - 1
+DBL_MAX = np.finfo(np.float64).max
+
+
+@pytest.fixture(scope="module")
+def rmat24():
+    hg = G.HostGraph.rmat(24, 16, seed=1)
+    o, i = hg.view(G.VIEW_OUT, copy=False), hg.view(G.VIEW_IN, copy=False)
+    dg = G.DeviceGraph(o, i, with_all=False)
+    yield hg, o, i, dg
+    dg.close()
+
+
+def _edge_sources(o):
+    return np.repeat(np.arange(o.n, dtype=np.int32), np.diff(o.rowptr))
+
+
+def test_bfs_s24_exact_by_invariants(rmat24):
+    hg, o, i, dg = rmat24
+    src = int(np.argmax(o.degrees()))
+    for sched in (G.SCHED_HEURISTIC, G.SCHED_REFERENCE):
+        d, st = dg.bfs(src, schedule=sched)
+        assert d[src] == 0
+        eu = _edge_sources(o)
+        du, dv = d[eu], d[o.col]
+        fin = du != INT_MAX
+        assert np.all(dv[fin].astype(np.int64) <= du[fin].astype(np.int64) + 1)
+        has_parent = np.zeros(o.n, bool)
+        has_parent[o.col[fin & (dv.astype(np.int64) == du.astype(np.int64) + 1)]] = True
+        reached = d != INT_MAX
+        reached[src] = False
+        assert np.all(has_parent[reached])
+        assert st.edges_traversed == int(np.diff(o.rowptr)[d != INT_MAX].sum())
+        del eu, du, dv, fin
+
+
+def test_sssp_s24_exact_by_invariants(rmat24):
+    hg, o, i, dg = rmat24
+    src = int(np.argmax(o.degrees()))
+    d, st = dg.sssp(src)
+    assert d[src] == 0.0
+    eu = _edge_sources(o)
+    du = d[eu]
+    fin = du != DBL_MAX
+    nd = du[fin] + o.w[fin]  # IEEE FP64 addition, as on the GPU
+    dv = d[o.col[fin]]
+    assert np.all(dv <= nd)
+    has_parent = np.zeros(o.n, bool)
+    has_parent[o.col[fin][dv == nd]] = True
+    reached = d != DBL_MAX
+    reached[src] = False
+    assert np.all(has_parent[reached])
+
+
+def test_wcc_s24_min_id_of_weak_components(rmat24):
+    hg, o, i, dg = rmat24
+    lab, _ = dg.wcc()
+    n = o.n
+    a = sp.csr_matrix((np.ones(o.m, np.int8), o.col, o.rowptr), shape=(n, n))
+    ncomp, comp = connected_components(a, directed=True, connection="weak")
+    minid = np.full(ncomp, n, np.int64)
+    np.minimum.at(minid, comp, np.arange(n))
+    np.testing.assert_array_equal(lab, minid[comp].astype(np.int32))
+
+
+def test_tc_s24_sampled_vertices(rmat24):
+    hg, o, i, dg = rmat24
+    s = hg.view(G.VIEW_SYM, copy=False)
+    sg = G.DeviceGraph(s, undirected=True, weights=False)
+    cnt, _ = sg.tc()
+    sg.close()
+    deg = np.diff(s.rowptr)
+    assert int(cnt.sum()) % 18 == 0
+    cand = np.nonzero((deg >= 2) & (deg <= 64))[0]
+    rng = np.random.default_rng(7)
+    for x in rng.choice(cand, size=min(3000, len(cand)), replace=False):
+        nb = s.col[s.rowptr[x]:s.rowptr[x + 1]]
+        t = 0
+        for a in nb:  # neighbours of x adjacent to a, counted once per pair
+            na = s.col[s.rowptr[a]:s.rowptr[a + 1]]
+            t += int(np.count_nonzero(np.isin(nb[nb > a], na, assume_unique=True)))
+        assert int(cnt[x]) == 6 * t, int(x)
+
+
+@pytest.mark.parametrize("kind", ["rmat", "er"])
+def test_pr_s24_every_iteration_vs_fp64_oracle(rmat24, kind):
+    if kind == "rmat":
+        hg, o, i, dg = rmat24
+        er = None
+    else:
+        n = rmat24[1].n
+        er = G.HostGraph.er(n, rmat24[1].m, seed=2)
+        o, i = er.view(G.VIEW_OUT, copy=False), er.view(G.VIEW_IN, copy=False)
+        dg = G.DeviceGraph(o, i, with_all=False, weights=False)
+    n = o.n
+    ref = np.full(n, 1.0 / n)
+    rows = []
+    for it in range(1, 21):
+        ref = O.pr_step(i.rowptr, i.col, o.rowptr, ref, 0.85, threads=16)
+        raw, _ = dg.pr(it, 0.85, normalise=False)
+        nrm, _ = dg.pr(it, 0.85)
+        l1 = float(np.abs(raw - ref).sum())
+        mx = float(np.abs(raw - ref).max())
+        l1n = float(np.abs(nrm - ref / ref.sum()).sum())
+        rows.append((it, l1, mx, l1n))
+        assert l1 <= 1e-6 and l1n <= 1e-6, (kind, it, l1, l1n)
+    print(f"\nPR {kind} s24 per iteration: (iter, L1, max-abs, L1 normalised)")
+    for r in rows:
+        print("  %2d  %.3e  %.3e  %.3e" % r)
+    if er is not None:
+        dg.close()

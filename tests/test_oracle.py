@@ -129,6 +129,18 @@ def test_pr_mass_is_conserved(golden):
     assert abs(r.sum() - 1.0) < 1e-12
 
 
+def test_pr_step_is_one_iteration_of_pr(golden):
+    """pr_step (used to check every iteration of a GPU run) composes to pr."""
+    g = golden
+    n = int(g["n"])
+    r = np.full(n, 1.0 / n)
+    for k in range(1, 11):
+        r = O.pr_step(g["in_rowptr"], g["in_col"], g["out_rowptr"], r, 0.85)
+        if k in (1, 10):
+            np.testing.assert_array_equal(r, O.pr(g["in_rowptr"], g["in_col"], g["out_rowptr"], 0.85, k))
+    np.testing.assert_array_equal(r, g["pr"])  # the reference's own 10-iteration output
+
+
 def _cdlp_brute(n, orp, ocol, irp, icol, iters):
     lab = list(range(n))
     for _ in range(iters):
