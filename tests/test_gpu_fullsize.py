@@ -1,5 +1,5 @@
-"""Parity at full size: R-MAT scale 24 (and an Erdos-Renyi graph with the same
-vertex and edge counts for PageRank).  Running the oracle's BFS/SSSP/WCC/TC
+"""Parity at full size: R-MAT scale 24 and an Erdos-Renyi graph with the same
+vertex and edge counts (TC on the R-MAT graph).  Running the oracle's BFS/SSSP/WCC/TC
 on 263M edges takes too long here, so the exact results are checked through
 properties that determine them completely, vectorised over every edge:
 
@@ -31,21 +31,44 @@ INT_MAX = 2**31 - 1
 DBL_MAX = np.finfo(np.float64).max
 
 
+class _Graphs:
+    """R-MAT s24 and the Erdos-Renyi graph with its n and m, built on first use."""
+
+    def __init__(self):
+        self.cache = {}
+
+    def __call__(self, kind):
+        if kind not in self.cache:
+            if kind == "rmat":
+                hg = G.HostGraph.rmat(24, 16, seed=1)
+            else:
+                r = self("rmat")[1]
+                hg = G.HostGraph.er(r.n, r.m, seed=2)
+            o, i = hg.view(G.VIEW_OUT, copy=False), hg.view(G.VIEW_IN, copy=False)
+            self.cache[kind] = (hg, o, i, G.DeviceGraph(o, i, with_all=False))
+        return self.cache[kind]
+
+
 @pytest.fixture(scope="module")
-def rmat24():
-    hg = G.HostGraph.rmat(24, 16, seed=1)
-    o, i = hg.view(G.VIEW_OUT, copy=False), hg.view(G.VIEW_IN, copy=False)
-    dg = G.DeviceGraph(o, i, with_all=False)
-    yield hg, o, i, dg
-    dg.close()
+def graphs():
+    g = _Graphs()
+    yield g
+    for hg, o, i, dg in g.cache.values():
+        dg.close()
+
+
+@pytest.fixture(scope="module")
+def rmat24(graphs):
+    return graphs("rmat")
 
 
 def _edge_sources(o):
     return np.repeat(np.arange(o.n, dtype=np.int32), np.diff(o.rowptr))
 
 
-def test_bfs_s24_exact_by_invariants(rmat24):
-    hg, o, i, dg = rmat24
+@pytest.mark.parametrize("kind", ["rmat", "er"])
+def test_bfs_s24_exact_by_invariants(graphs, kind):
+    hg, o, i, dg = graphs(kind)
     src = int(np.argmax(o.degrees()))
     for sched in (G.SCHED_HEURISTIC, G.SCHED_REFERENCE):
         d, st = dg.bfs(src, schedule=sched)
@@ -63,8 +86,9 @@ def test_bfs_s24_exact_by_invariants(rmat24):
         del eu, du, dv, fin
 
 
-def test_sssp_s24_exact_by_invariants(rmat24):
-    hg, o, i, dg = rmat24
+@pytest.mark.parametrize("kind", ["rmat", "er"])
+def test_sssp_s24_exact_by_invariants(graphs, kind):
+    hg, o, i, dg = graphs(kind)
     src = int(np.argmax(o.degrees()))
     d, st = dg.sssp(src)
     assert d[src] == 0.0
@@ -81,8 +105,9 @@ def test_sssp_s24_exact_by_invariants(rmat24):
     assert np.all(has_parent[reached])
 
 
-def test_wcc_s24_min_id_of_weak_components(rmat24):
-    hg, o, i, dg = rmat24
+@pytest.mark.parametrize("kind", ["rmat", "er"])
+def test_wcc_s24_min_id_of_weak_components(graphs, kind):
+    hg, o, i, dg = graphs(kind)
     lab, _ = dg.wcc()
     n = o.n
     a = sp.csr_matrix((np.ones(o.m, np.int8), o.col, o.rowptr), shape=(n, n))
@@ -112,15 +137,8 @@ def test_tc_s24_sampled_vertices(rmat24):
 
 
 @pytest.mark.parametrize("kind", ["rmat", "er"])
-def test_pr_s24_every_iteration_vs_fp64_oracle(rmat24, kind):
-    if kind == "rmat":
-        hg, o, i, dg = rmat24
-        er = None
-    else:
-        n = rmat24[1].n
-        er = G.HostGraph.er(n, rmat24[1].m, seed=2)
-        o, i = er.view(G.VIEW_OUT, copy=False), er.view(G.VIEW_IN, copy=False)
-        dg = G.DeviceGraph(o, i, with_all=False, weights=False)
+def test_pr_s24_every_iteration_vs_fp64_oracle(graphs, kind):
+    hg, o, i, dg = graphs(kind)
     n = o.n
     ref = np.full(n, 1.0 / n)
     rows = []
@@ -136,5 +154,3 @@ def test_pr_s24_every_iteration_vs_fp64_oracle(rmat24, kind):
     print(f"\nPR {kind} s24 per iteration: (iter, L1, max-abs, L1 normalised)")
     for r in rows:
         print("  %2d  %.3e  %.3e  %.3e" % r)
-    if er is not None:
-        dg.close()
