@@ -19,7 +19,8 @@
 //                hash table (2^ceil(log2(2d)) slots); one workgroup takes the
 //                argmax and clears the table for the next iteration.
 // Winner = max count, ties to the smallest label: the maximum of the key
-// (count << 32) | (0xFFFFFFFF - label).
+// (count << 32) | (0xFFFFFFFF - label).  The first iteration, from identity
+// labels, has a closed form and skips the counting (k_cdlp_first_*).
 #include "graph_internal.hpp"
 
 namespace ghet {
@@ -223,6 +224,114 @@ __global__ void __launch_bounds__(kBT) k_cdlp_heavy_pick(const int32_t* __restri
     }
 }
 
+// First iteration.  Labels start as vertex ids, and a vertex's all-list is
+// its in-list then its out-list, each sorted and duplicate-free, so a label
+// is counted twice exactly when that neighbour is both an in- and an
+// out-neighbour and once otherwise.  The mode is therefore the smallest
+// neighbour present in both lists if there is one (count 2), else the
+// smallest neighbour (all counts 1); a vertex without neighbours keeps its
+// id.  Undirected graphs (all-list = out-list, in_rowptr == nullptr) take
+// the smallest neighbour.  Same result as the counting kernels, no hashing.
+__device__ __forceinline__ bool contains_sorted(const int32_t* __restrict__ a, int32_t len, int32_t x) {
+    int32_t lo = 0, hi = len;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        const int32_t y = a[mid];
+        if (y == x) return true;
+        if (y < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return false;
+}
+
+__device__ __forceinline__ int32_t first_fallback(const int32_t* __restrict__ col, int32_t b, int32_t e, int32_t di,
+                                                  int32_t v) {
+    if (e == b) return v;
+    if (di == 0 || di == e - b) return col[b];  // one list: its first element
+    return min(col[b], col[b + di]);
+}
+
+__global__ void __launch_bounds__(kBT) k_cdlp_first_small(const int32_t* __restrict__ list, int32_t count,
+                                                          const int32_t* __restrict__ rowptr,
+                                                          const int32_t* __restrict__ col,
+                                                          const int32_t* __restrict__ in_rowptr,
+                                                          int32_t* __restrict__ out) {
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; k < count;
+         k += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int32_t v = list[k];
+        const int32_t b = rowptr[v], e = rowptr[v + 1];
+        const int32_t di = in_rowptr ? in_rowptr[v + 1] - in_rowptr[v] : 0;
+        int32_t best = -1;
+        for (int32_t i = b, j = b + di; i < b + di && j < e;) {  // merge: first common = smallest
+            const int32_t x = col[i], y = col[j];
+            if (x == y) {
+                best = x;
+                break;
+            }
+            if (x < y) ++i;
+            else ++j;
+        }
+        out[v] = best >= 0 ? best : first_fallback(col, b, e, di, v);
+    }
+}
+
+// One wave per vertex; in-list scanned in 64-element steps, each lane
+// binary-searching its element in the out-list; the first hit is the smallest.
+__global__ void __launch_bounds__(kBT) k_cdlp_first_wave(const int32_t* __restrict__ list, int32_t count,
+                                                         const int32_t* __restrict__ rowptr,
+                                                         const int32_t* __restrict__ col,
+                                                         const int32_t* __restrict__ in_rowptr,
+                                                         int32_t* __restrict__ out) {
+    const int lane = lane_id();
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t waves = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    for (int64_t k = wave; k < count; k += waves) {
+        const int32_t v = list[k];
+        const int32_t b = rowptr[v], e = rowptr[v + 1];
+        const int32_t di = in_rowptr ? in_rowptr[v + 1] - in_rowptr[v] : 0;
+        int32_t best = -1;
+        for (int32_t c0 = 0; c0 < di && di < e - b; c0 += 64) {
+            const int32_t i = c0 + lane;
+            const bool hit = i < di && contains_sorted(col + b + di, e - b - di, col[b + i]);
+            const uint64_t m = __ballot(hit);
+            if (m) {
+                best = col[b + c0 + __ffsll(static_cast<unsigned long long>(m)) - 1];
+                break;
+            }
+        }
+        if (lane == 0) out[v] = best >= 0 ? best : first_fallback(col, b, e, di, v);
+    }
+}
+
+// One workgroup per heavy vertex; the in-list in 256-element steps.
+__global__ void __launch_bounds__(kBT) k_cdlp_first_block(const int32_t* __restrict__ list, int32_t count,
+                                                          const int32_t* __restrict__ rowptr,
+                                                          const int32_t* __restrict__ col,
+                                                          const int32_t* __restrict__ in_rowptr,
+                                                          int32_t* __restrict__ out) {
+    __shared__ int32_t s_best;
+    for (int64_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const int32_t v = list[k];
+        const int32_t b = rowptr[v], e = rowptr[v + 1];
+        const int32_t di = in_rowptr ? in_rowptr[v + 1] - in_rowptr[v] : 0;
+        if (threadIdx.x == 0) s_best = INT32_MAX;
+        __syncthreads();
+        for (int32_t c0 = 0; c0 < di && di < e - b; c0 += kBT) {
+            const int32_t i = c0 + static_cast<int32_t>(threadIdx.x);
+            if (i < di) {
+                const int32_t x = col[b + i];
+                if (contains_sorted(col + b + di, e - b - di, x)) atomicMin(&s_best, x);
+            }
+            __syncthreads();
+            const bool done = s_best != INT32_MAX;
+            __syncthreads();
+            if (done) break;
+        }
+        if (threadIdx.x == 0) out[v] = s_best != INT32_MAX ? s_best : first_fallback(col, b, e, di, v);
+        __syncthreads();
+    }
+}
+
 int build_cdlp_layout(ghet_graph* g) {
     CDLPLayout& L = g->cdlp;
     if (L.built) return GHET_OK;
@@ -304,10 +413,30 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_label_init, dim3(grid_cap(ceil_div(n, 256))), dim3(256), 0, s, la, lb, n);
+    // the identity-label first iteration has a closed form (see k_cdlp_first_*);
+    // it needs the in-degrees to split each all-list, or an undirected graph
+    const bool first_closed = !g->directed || (g->in.present() && g->in.rowptr != g->all.rowptr);
+    const int32_t* in_rp = g->directed ? g->in.rowptr : nullptr;
     for (int32_t it = 0; it < iters; ++it) {
         if (timing) GHET_HIP(hipEventRecord(kev[2 * it], s));
         const int32_t* A = g->all.rowptr;
         const int32_t* C = g->all.col;
+        if (it == 0 && first_closed) {
+            // vertices with no neighbours are in no list and keep their id (lb holds ids)
+            if (L.count[0])
+                hipLaunchKernelGGL(k_cdlp_first_small, dim3(grid_cap(ceil_div(L.count[0], kBT))), dim3(kBT), 0, s,
+                                   L.list[0], L.count[0], A, C, in_rp, lb);
+            for (int c = 1; c <= 2; ++c)
+                if (L.count[c])
+                    hipLaunchKernelGGL(k_cdlp_first_wave, dim3(grid_cap(ceil_div(L.count[c], kBT / 64))), dim3(kBT), 0,
+                                       s, L.list[c], L.count[c], A, C, in_rp, lb);
+            if (L.count[3])
+                hipLaunchKernelGGL(k_cdlp_first_block, dim3(grid_cap(L.count[3])), dim3(kBT), 0, s, L.list[3],
+                                   L.count[3], A, C, in_rp, lb);
+            if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
+            std::swap(la, lb);
+            continue;
+        }
         if (L.count[0])
             hipLaunchKernelGGL(k_cdlp_small, dim3(grid_cap(ceil_div(L.count[0], kBT))), dim3(kBT), 0, s, L.list[0],
                                L.count[0], A, C, la, lb);
