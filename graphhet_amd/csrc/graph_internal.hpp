@@ -31,7 +31,7 @@ enum WsSlot {
     WS_COUNTERS, WS_DIST, WS_STAMP_NEAR, WS_STAMP_FAR, WS_FAR_A, WS_FAR_B, WS_PARENT,
     WS_PR_CONTRIB, WS_PR_RANK, WS_PR_PART, WS_PR_LONGPART, WS_LABEL_A, WS_LABEL_B, WS_HASH,
     WS_TC_COUNT, WS_PRE, WS_PR_OUT, WS_MULTI_A, WS_MULTI_B, WS_MULTI_C, WS_MULTI_D, WS_MISC,
-    WS_NUM_SLOTS
+    WS_SSSP_LIST, WS_SSSP_START, WS_NUM_SLOTS
 };
 
 struct Workspace {
@@ -80,6 +80,15 @@ struct CDLPLayout {
 
 // Batched BFS: vertices whose in-lists are too long for one wave, and their
 // in-edges cut into fixed-size chunks any wave can take.
+// SSSP: the out-CSR with each row's edges ordered by weight (ties by target),
+// so the edges with d[u] + w < thr form a prefix of the row.
+struct SsspLayout {
+    bool built = false;
+    int32_t* col = nullptr;  // [m]
+    double* w = nullptr;     // [m]
+    void* mem = nullptr;
+};
+
 struct MSLayout {
     bool built = false;
     int32_t num_heavy = 0;
@@ -125,6 +134,8 @@ struct ghet_graph {
     ghet::CDLPLayout cdlp;
     ghet::TCLayout tc;
     ghet::MSLayout ms;
+    ghet::SsspLayout sl;
+    double mean_w = -1.0;  // mean out-edge weight (SSSP bucket width), reduced on first use
     ghet::Tuning tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;

@@ -22,7 +22,7 @@
 #include "graph_internal.hpp"
 
 namespace ghet {
-double sssp_base_delta(const ghet_graph* g);
+int sssp_base_delta(ghet_graph* g, double* out);
 namespace {
 
 __global__ void k_localize(int32_t* __restrict__ lab, int32_t n, const int32_t* __restrict__ voff, int32_t count) {
@@ -354,7 +354,8 @@ int ghet_autotune(ghet_graph* g, const char* cache_path, int32_t* best_block_out
                 if (g->h_out_rowptr[v + 1] - g->h_out_rowptr[v] > g->h_out_rowptr[src + 1] - g->h_out_rowptr[src]) src = v;
             double best = 1e300;
             g->tune.sssp_class = 1;
-            const double base0 = sssp_base_delta(g);
+            double base0 = 0;
+            GHET_CHECK(sssp_base_delta(g, &base0));
             for (int cls = 0; cls < 4; ++cls) {
                 ghet_opts o{};
                 o.size = sizeof(o);

@@ -29,7 +29,7 @@
 
 namespace ghet {
 
-double sssp_base_delta(const ghet_graph* g);
+int sssp_base_delta(ghet_graph* g, double* out);
 
 namespace {
 
@@ -297,7 +297,14 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     if (k == 0 || n == 0) return GHET_OK;
     hipStream_t s = pick_stream(g, o);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
-    const double delta = (have_opts && o->sssp_delta > 0) ? o->sssp_delta : sssp_base_delta(g);
+    // The batched form relaxes whole rows each round (no light/heavy split),
+    // so it favours buckets 8x wider than the single-source base.
+    double delta = 0;
+    if (have_opts && o->sssp_delta > 0) delta = o->sssp_delta;
+    else {
+        GHET_CHECK(sssp_base_delta(g, &delta));
+        delta *= 8.0;
+    }
     // Sources per pass: up to 32 (the mask width), or the caller's choice.
     const int64_t kcap = (have_opts && o->reserved[3] > 0) ? o->reserved[3] : kMaxK;
     const int32_t K = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>({kcap, int64_t(kMaxK), int64_t(k)})));

@@ -1,4 +1,5 @@
-// Single-source shortest paths in FP64 on gfx950: near-far bucketing.
+// Single-source shortest paths in FP64 on gfx950: near-far bucketing with a
+// light/heavy edge split.
 //
 // Reference: csr_sssp (GPU_SSSP_csr.cuh:253-319) is frontier Bellman-Ford with
 // an 8-byte (u<<32|e) candidate queue, two launches and five syncs per round;
@@ -10,11 +11,18 @@
 // minimum over walks of the left-to-right rounded path sum.  The bucket width
 // therefore changes speed, never a bit of the output.
 //
-// Each round expands the near queue edge-balanced (degree scan + 2048-edge
-// tiles); an improved vertex goes to the next near queue when its new distance
-// is below the threshold, else to the far list.  Per-vertex round/generation
-// stamps keep both lists duplicate-free.  When the near queue drains, the far
-// list is split at threshold + delta.
+// A bucket is the distance range below the threshold `thr` not yet settled.
+// Each round expands the near queue edge-balanced (degree scan + 1024-edge
+// tiles) but relaxes only the edges with d[u] + w < thr (light for the current
+// d[u]); an improved vertex joins the next near queue.  Per-vertex round stamps
+// keep the queue duplicate-free, and a vertex's first entry into a bucket also
+// appends it to the bucket list.  When the near queue drains, every distance
+// below thr is final, so one heavy pass over the bucket list relaxes the
+// remaining edges (d[u] + w >= thr) exactly once and sends improved vertices
+// to the far list (generation stamps keep it duplicate-free).  A vertex that
+// re-enters the near queue k times inside a bucket thus gathers its heavy
+// edges once instead of k times.  The far list is then split at thr + delta;
+// the vertices below it form the next bucket list and its first near queue.
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -27,11 +35,15 @@ namespace {
 
 // Queue positions (one atomic per LDS flush) plus spread sums:
 // slot v[0] out-edges of the next near queue, v[1] bits of the smallest kept
-// far distance (atomicMin on non-negative doubles' bit patterns).
+// far distance (atomicMin on non-negative doubles' bit patterns), v[2]
+// out-edges of the vertices appended to the bucket list.  list_count is the
+// running length of the bucket lists (all buckets share one n-entry buffer:
+// every vertex is listed in exactly one bucket) and is never reset.
 struct SsspCtr {
     int32_t near_count;
     int32_t far_count[2];
-    int32_t pad[13];
+    int32_t list_count;
+    int32_t pad[12];
     CtrSlot slots[kSlots];
 };
 constexpr int kCtrWords = sizeof(SsspCtr) / 8;
@@ -53,11 +65,12 @@ __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_
     return lo;
 }
 
-__global__ void k_sssp_seed(double* dist, int32_t* queue, int32_t* stamp_near, int32_t src) {
+__global__ void k_sssp_seed(double* dist, int32_t* list, int32_t* stamp_near, int32_t src, SsspCtr* ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         dist[src] = 0.0;
-        queue[0] = src;
+        list[0] = src;
         stamp_near[src] = 0;
+        ctr->list_count = 1;
     }
 }
 
@@ -72,38 +85,66 @@ __global__ void k_ctr_reset(SsspCtr* c, int other) {
     }
 }
 
-__global__ void k_frontier_deg(const int32_t* __restrict__ q, int32_t F, const int32_t* __restrict__ rowptr,
-                               int32_t* __restrict__ deg) {
+// Per queue entry u: the row segment a pass relaxes.  Rows are ordered by
+// weight, so fl(d[u] + w) is non-decreasing along the row and the edges with
+// d[u] + w < thr are a prefix (binary search).  Round: the prefix; heavy: the
+// rest.  cnt feeds the scan, start the relax kernel.
+template <bool kHeavy>
+__global__ void k_row_split(const int32_t* __restrict__ q, int32_t F, const int32_t* __restrict__ rowptr,
+                            const double* __restrict__ w, const double* __restrict__ dist, double threshold,
+                            int32_t* __restrict__ cnt, int32_t* __restrict__ start) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < F) {
-        int32_t u = q[i];
-        deg[i] = rowptr[u + 1] - rowptr[u];
+        const int32_t u = q[i];
+        const int32_t b = rowptr[u], e = rowptr[u + 1];
+        const double du = dist[u];
+        int32_t lo = b, hi = e;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (du + w[mid] < threshold) lo = mid + 1;
+            else hi = mid;
+        }
+        cnt[i] = kHeavy ? e - lo : lo - b;
+        start[i] = kHeavy ? lo : b;
     }
 }
 
-// One round: persistent blocks over kTile-edge tiles of the near queue.
+// One pass, persistent blocks over kTile-edge tiles of queue q; entry j
+// covers edges [start[j], start[j] + scan[j+1] - scan[j]) and scan[F] is the
+// pass's edge count (E_max, from the host, only bounds the grid).
+// Round (kHeavy false): relaxes the edges with d[u] + w < thr; an improved v
+// joins next_q (stamp_near != next_round) and, on its first entry into this
+// bucket (old stamp < r0, the bucket's first round), the bucket list.
+// Heavy (kHeavy true): q is the drained bucket's list; relaxes the edges with
+// d[u] + w >= thr and sends improved vertices to far list far_sel.
+template <bool kHeavy>
 __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, int32_t F,
-                                               const int32_t* __restrict__ scan, int32_t E,
+                                               const int32_t* __restrict__ scan,
+                                               const int32_t* __restrict__ start,
                                                const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                const double* __restrict__ wts, double* dist, double threshold,
-                                               int32_t* stamp_near, int32_t next_round, int32_t* stamp_far,
-                                               int32_t far_gen, int32_t* __restrict__ next_q, int32_t* far_list,
-                                               int far_sel, SsspCtr* ctr) {
+                                               int32_t* stamp_near, int32_t next_round, int32_t r0,
+                                               int32_t* __restrict__ next_q, int32_t* __restrict__ list,
+                                               int32_t* stamp_far, int32_t far_gen, int32_t* far_list, int far_sel,
+                                               SsspCtr* ctr) {
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
     __shared__ int32_t s_src[kTile];
-    __shared__ int32_t s_near[kOut], s_far[kOut];
-    __shared__ int32_t s_i0, s_span, s_nc, s_fc, s_base;
+    __shared__ int32_t s_qa[kOut], s_qb[kOut];  // round: near, list; heavy: far
+    __shared__ int32_t s_i0, s_span, s_na, s_nb, s_base;
     __shared__ unsigned long long s_red[kBT / 64];
     const int tid = threadIdx.x;
     if (tid == 0) {
-        s_nc = 0;
-        s_fc = 0;
+        s_na = 0;
+        s_nb = 0;
     }
     __syncthreads();  // counters zeroed before any wave appends
-    unsigned long long od = 0;
+    unsigned long long od = 0, ld = 0;
     auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
+    auto on_list = [&](int32_t v) { ld += rowptr[v + 1] - rowptr[v]; };
     auto nop = [](int32_t) {};
+    const int32_t E = scan[F];
+    if (blockIdx.x == 0 && tid == 0) slot_add(ctr->slots, 3, static_cast<unsigned long long>(E));
     for (int64_t tile = blockIdx.x; tile * kTile < E; tile += gridDim.x) {
         const int32_t t0 = static_cast<int32_t>(tile * kTile);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
@@ -118,16 +159,15 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
         const bool use_lds = span <= kTile;
         if (use_lds) {
             for (int j = tid; j < span; j += kBT) {
-                int32_t u = q[i0 + j];
                 s_scan[j] = scan[i0 + j];
-                s_start[j] = rowptr[u];
-                s_src[j] = u;
+                s_start[j] = start[i0 + j];
+                s_src[j] = q[i0 + j];
             }
         }
         __syncthreads();
         for (int k = 0; k < kItems; ++k) {
             const int32_t e = t0 + k * kBT + tid;
-            bool to_near = false, to_far = false;
+            bool push_a = false, push_b = false;
             int32_t v = 0;
             if (e < t1) {
                 int32_t pos, u;
@@ -138,37 +178,57 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
                 } else {
                     int j = upper_bound_i32(scan + i0, span, e) - 1;
                     u = q[i0 + j];
-                    pos = rowptr[u] + (e - scan[i0 + j]);
+                    pos = start[i0 + j] + (e - scan[i0 + j]);
                 }
-                v = col[pos];
                 const double nd = dist[u] + wts[pos];
-                if (nd < dist[v]) {
-                    const double old = atomic_min_nonneg(&dist[v], nd);
-                    if (nd < old) {
-                        if (nd < threshold) to_near = atomicExch(&stamp_near[v], next_round) != next_round;
-                        else to_far = atomicExch(&stamp_far[v], far_gen) != far_gen;
+                if (kHeavy ? nd >= threshold : nd < threshold) {
+                    v = col[pos];
+                    if (nd < dist[v]) {
+                        const double old = atomic_min_nonneg(&dist[v], nd);
+                        if (nd < old) {
+                            if (kHeavy) {
+                                push_a = atomicExch(&stamp_far[v], far_gen) != far_gen;
+                            } else {
+                                const int32_t prev = atomicExch(&stamp_near[v], next_round);
+                                push_a = prev != next_round;
+                                push_b = prev < r0;
+                            }
+                        }
                     }
                 }
             }
-            lq_push(s_near, &s_nc, to_near, v);
-            lq_push(s_far, &s_fc, to_far, v);
+            lq_push(s_qa, &s_na, push_a, v);
+            if (!kHeavy) lq_push(s_qb, &s_nb, push_b, v);
         }
-        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, next_q, kOut - kTile, on_near);
-        lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[far_sel], far_list, kOut - kTile, nop);
+        if (kHeavy) {
+            lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->far_count[far_sel], far_list, kOut - kTile, nop);
+        } else {
+            lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->near_count, next_q, kOut - kTile, on_near);
+            lq_flush<kBT>(s_qb, &s_nb, &s_base, &ctr->list_count, list, kOut - kTile, on_list);
+        }
     }
-    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, next_q, 0, on_near);
-    lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[far_sel], far_list, 0, nop);
-    od = block_sum_u64<kBT>(od, s_red);
-    if (tid == 0) slot_add(ctr->slots, 0, od);
+    if (kHeavy) {
+        lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->far_count[far_sel], far_list, 0, nop);
+    } else {
+        lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->near_count, next_q, 0, on_near);
+        lq_flush<kBT>(s_qb, &s_nb, &s_base, &ctr->list_count, list, 0, on_list);
+        od = block_sum_u64<kBT>(od, s_red);
+        ld = block_sum_u64<kBT>(ld, s_red);
+        if (tid == 0) {
+            slot_add(ctr->slots, 0, od);
+            slot_add(ctr->slots, 2, ld);
+        }
+    }
 }
 
 // Splits far list `sel` at new_thr: entries below old_thr are stale (already
-// settled at their current distance), [old_thr, new_thr) go to the near queue,
-// the rest to far list 1-sel with the new generation stamp.
+// settled at their current distance), [old_thr, new_thr) are appended to the
+// bucket list (which is also the new bucket's first near queue), the rest go
+// to far list 1-sel with the new generation stamp.
 __global__ void __launch_bounds__(kBT) k_far_split(const int32_t* __restrict__ far_in, int32_t count,
                                                    const double* __restrict__ dist, double old_thr, double new_thr,
                                                    int32_t* stamp_near, int32_t next_round, int32_t* stamp_far,
-                                                   int32_t new_gen, int32_t* __restrict__ near_q, int32_t* far_out,
+                                                   int32_t new_gen, int32_t* __restrict__ list, int32_t* far_out,
                                                    int other, const int32_t* __restrict__ rowptr, SsspCtr* ctr) {
     __shared__ int32_t s_near[kOut], s_far[kOut];
     __shared__ int32_t s_nc, s_fc, s_base;
@@ -203,10 +263,10 @@ __global__ void __launch_bounds__(kBT) k_far_split(const int32_t* __restrict__ f
         }
         lq_push(s_near, &s_nc, to_near, v);
         lq_push(s_far, &s_fc, keep, v);
-        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, near_q, kOut - kBT, on_near);
+        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, kOut - kBT, on_near);
         lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, kOut - kBT, nop);
     }
-    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, near_q, 0, on_near);
+    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, 0, on_near);
     lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, 0, nop);
     od = block_sum_u64<kBT>(od, s_red);
     dmin = wave_min(dmin);
@@ -270,12 +330,86 @@ __global__ void k_pre_finish(int32_t* pre, int32_t n) {
 
 }  // namespace
 
-// Base bucket width: ~32 average weights per average out-degree (GPU near-far
-// favours wide buckets: a bucket should hold enough work to fill the chip).
-// The autotuner's class scales it by 2^(class-1).
-double sssp_base_delta(const ghet_graph* g) {
+// Orders each out-row by (weight, target) into g->sl.  Built once per graph on
+// the first SSSP; the host sorts the rows in parallel (per-row index sort).
+static int build_sssp_layout(ghet_graph* g) {
+    SsspLayout& L = g->sl;
+    if (L.built) return GHET_OK;
+    const int64_t m = g->out.m;
+    const int32_t n = g->n;
+    const size_t col_bytes = (4 * size_t(m) + 255) & ~size_t(255);
+    GHET_HIP(hipMalloc(&L.mem, col_bytes + 8 * size_t(std::max<int64_t>(m, 1))));
+    L.col = static_cast<int32_t*>(L.mem);
+    L.w = reinterpret_cast<double*>(static_cast<char*>(L.mem) + col_bytes);
+    if (m > 0) {
+        std::vector<int32_t> hcol(m);
+        std::vector<double> hw(m);
+        GHET_HIP(hipMemcpy(hcol.data(), g->out.col, 4 * size_t(m), hipMemcpyDeviceToHost));
+        GHET_HIP(hipMemcpy(hw.data(), g->out.w, 8 * size_t(m), hipMemcpyDeviceToHost));
+        const int32_t* rp = g->h_out_rowptr.data();
+        std::vector<int32_t> scol(m);
+        std::vector<double> sw(m);
+        parallel_for(n, hw_threads(0), [&](int64_t b, int64_t e, int) {
+            std::vector<int32_t> idx;
+            for (int64_t r = b; r < e; ++r) {
+                const int32_t r0 = rp[r], len = rp[r + 1] - rp[r];
+                idx.resize(len);
+                for (int32_t k = 0; k < len; ++k) idx[k] = r0 + k;
+                std::sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
+                    return hw[x] < hw[y] || (hw[x] == hw[y] && hcol[x] < hcol[y]);
+                });
+                for (int32_t k = 0; k < len; ++k) {
+                    scol[r0 + k] = hcol[idx[k]];
+                    sw[r0 + k] = hw[idx[k]];
+                }
+            }
+        });
+        GHET_HIP(hipMemcpy(L.col, scol.data(), 4 * size_t(m), hipMemcpyHostToDevice));
+        GHET_HIP(hipMemcpy(L.w, sw.data(), 8 * size_t(m), hipMemcpyHostToDevice));
+    }
+    L.built = true;
+    return GHET_OK;
+}
+
+__global__ void __launch_bounds__(256) k_wsum(const double* __restrict__ w, int64_t m, double* part) {
+    __shared__ double s_red[4];
+    double x = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < m; i += static_cast<int64_t>(gridDim.x) * 256)
+        x += w[i];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+// Base bucket width: 4 mean weights per mean out-degree, so a bucket spans
+// about four light edges of an average row; light/heavy splitting keeps the
+// rest of each row out of the bucket's rounds.  The autotuner's class scales
+// it by 2^(class-1).  The mean weight is reduced once per graph.
+int sssp_base_delta(ghet_graph* g, double* out) {
+    if (g->mean_w < 0) {
+        const int64_t m = g->out.m;
+        double mean = 1.0;
+        if (m > 0 && g->out.w) {
+            constexpr int kParts = 1024;
+            double* part;
+            GHET_HIP(hipMalloc(&part, sizeof(double) * kParts));
+            hipLaunchKernelGGL(k_wsum, dim3(kParts), dim3(256), 0, g->stream, g->out.w, m, part);
+            std::vector<double> hp(kParts);
+            const hipError_t e1 = hipMemcpyAsync(hp.data(), part, sizeof(double) * kParts, hipMemcpyDeviceToHost, g->stream);
+            const hipError_t e2 = hipStreamSynchronize(g->stream);
+            hipFree(part);
+            GHET_HIP(e1);
+            GHET_HIP(e2);
+            double sum = 0;
+            for (double x : hp) sum += x;
+            mean = sum / static_cast<double>(m);
+        }
+        g->mean_w = mean > 0 ? mean : 1.0;
+    }
     const double avg_deg = g->n > 0 ? static_cast<double>(g->out.m) / g->n : 1.0;
-    return std::max(1e-12, 32.0 * 0.5 / std::max(avg_deg, 1.0)) * std::ldexp(1.0, g->tune.sssp_class - 1);
+    *out = std::max(1e-12, 4.0 * g->mean_w / std::max(avg_deg, 1.0)) * std::ldexp(1.0, g->tune.sssp_class - 1);
+    return GHET_OK;
 }
 
 int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* pre_out, const ghet_opts* o,
@@ -287,13 +421,15 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     hipStream_t s = pick_stream(g, o);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
     const int schedule = have_opts ? o->schedule : GHET_SCHED_HEURISTIC;
-    const double base_delta = (have_opts && o->sssp_delta > 0) ? o->sssp_delta : sssp_base_delta(g);
+    double base_delta = 0;
+    if (have_opts && o->sssp_delta > 0) base_delta = o->sssp_delta;
+    else GHET_CHECK(sssp_base_delta(g, &base_delta));
     ghet_level_trace* trace = have_opts ? o->trace : nullptr;
     const int trace_cap = trace ? o->trace_cap : 0;
     int trace_len = 0;
 
     double* dist;
-    int32_t *qa, *qb, *scan, *aux, *stamp_near, *stamp_far, *far[2];
+    int32_t *qa, *qb, *scan, *aux, *stamp_near, *stamp_far, *far[2], *list, *qstart;
     SsspCtr* ctr;
     void* p;
     GHET_CHECK(g->ws.get(WS_DIST, 8 * size_t(n), &p)); dist = static_cast<double*>(p);
@@ -301,6 +437,8 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     GHET_CHECK(g->ws.get(WS_QUEUE_B, 4 * size_t(n), &p)); qb = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_FAR_A, 4 * size_t(n), &p)); far[0] = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_FAR_B, 4 * size_t(n), &p)); far[1] = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_SSSP_LIST, 4 * size_t(n), &p)); list = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_SSSP_START, 4 * size_t(n), &p)); qstart = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_STAMP_NEAR, 4 * size_t(n), &p)); stamp_near = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_STAMP_FAR, 4 * size_t(n), &p)); stamp_far = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
@@ -308,6 +446,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(SsspCtr), &p)); ctr = static_cast<SsspCtr*>(p);
     const SsspCtr* hc = reinterpret_cast<const SsspCtr*>(g->pinned);
 
+    GHET_CHECK(build_sssp_layout(g));
     SchedState sched;
     sched_begin(&sched, schedule, GHET_NET_SSSP, n, g->out.m, g->out.m);
 
@@ -316,11 +455,17 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     GHET_CHECK(fill<int32_t>(stamp_near, -1, n, s));
     GHET_CHECK(fill<int32_t>(stamp_far, -1, n, s));
     GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(ctr), 0, kCtrWords, s));
-    hipLaunchKernelGGL(k_sssp_seed, dim3(1), dim3(64), 0, s, dist, qa, stamp_near, src);
+    hipLaunchKernelGGL(k_sssp_seed, dim3(1), dim3(64), 0, s, dist, list, stamp_near, src, ctr);
 
+    // The near queue of a bucket's first round is its list segment; later
+    // rounds alternate between qa and qb.
+    const int32_t* cur = list;
+    int32_t* nxt = qa;
     int32_t F = 1;
     int64_t E = g->h_out_rowptr[src + 1] - g->h_out_rowptr[src];
-    int32_t round = 0, far_gen = 0, phases = 0;
+    int32_t list_base = 0, list_len = 1, list_count = 1;  // the current bucket's segment, all segments
+    int64_t list_edges = E;
+    int32_t round = 0, r0 = 0, far_gen = 0, phases = 0;
     int sel = 0;
     int32_t far_count = 0;
     double threshold = 0.0;
@@ -337,32 +482,63 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         delta = base_delta * std::ldexp(1.0, cls - 1);
         threshold = delta;
     }
+    // Edge-balanced pass over queue q of F vertices whose rows hold at most
+    // E_max edges in all (the grid bound); the pass covers each row's light
+    // prefix (round) or heavy rest (heavy pass).
+    const SsspLayout& L = g->sl;
+    auto launch_pass = [&](bool heavy, const int32_t* q, int32_t Fq, int64_t E_max, int32_t next_round) -> int {
+        const unsigned gq = static_cast<unsigned>(ceil_div(Fq, 256));
+        if (heavy)
+            hipLaunchKernelGGL(k_row_split<true>, dim3(gq), dim3(256), 0, s, q, Fq, g->out.rowptr, L.w, dist,
+                               threshold, scan, qstart);
+        else
+            hipLaunchKernelGGL(k_row_split<false>, dim3(gq), dim3(256), 0, s, q, Fq, g->out.rowptr, L.w, dist,
+                               threshold, scan, qstart);
+        GHET_CHECK(exclusive_scan_i32(scan, scan, Fq, aux, s));
+        const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
+        if (heavy)
+            hipLaunchKernelGGL(k_relax<true>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
+                               L.w, dist, threshold, stamp_near, next_round, r0, nxt, list, stamp_far, far_gen,
+                               far[sel], sel, ctr);
+        else
+            hipLaunchKernelGGL(k_relax<false>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
+                               L.w, dist, threshold, stamp_near, next_round, r0, nxt, list, stamp_far, far_gen,
+                               far[sel], sel, ctr);
+        GHET_HIP(hipGetLastError());
+        GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(SsspCtr), hipMemcpyDeviceToHost, s));
+        GHET_HIP(hipStreamSynchronize(s));
+        return GHET_OK;
+    };
     int64_t rounds = 0, relaxed = 0;
     for (;;) {
-        // relaxation rounds of the current bucket
+        // relaxation rounds of the current bucket (light edges only)
         if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k0, s));
         while (F > 0) {
             hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, -1);
-            hipLaunchKernelGGL(k_frontier_deg, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s, qa, F,
-                               g->out.rowptr, scan);
-            GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
             const int32_t next_round = round + 1;
-            relaxed += E;
             if (E > 0) {
-                unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), kMaxBlocks));
-                hipLaunchKernelGGL(k_relax, dim3(grid), dim3(kBT), 0, s, qa, F, scan, static_cast<int32_t>(E),
-                                   g->out.rowptr, g->out.col, g->out.w, dist, threshold, stamp_near, next_round,
-                                   stamp_far, far_gen, qb, far[sel], sel, ctr);
+                GHET_CHECK(launch_pass(false, cur, F, E, next_round));
+                relaxed += static_cast<int64_t>(slot_sum(hc->slots, 3));
+                F = hc->near_count;
+                E = static_cast<int64_t>(slot_sum(hc->slots, 0));
+                list_edges += static_cast<int64_t>(slot_sum(hc->slots, 2));
+                list_count = hc->list_count;
+                list_len = list_count - list_base;
+            } else {
+                F = 0;
             }
-            GHET_HIP(hipGetLastError());
-            GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(SsspCtr), hipMemcpyDeviceToHost, s));
-            GHET_HIP(hipStreamSynchronize(s));
-            F = hc->near_count;
-            E = static_cast<int64_t>(slot_sum(hc->slots, 0));
-            far_count = hc->far_count[sel];
-            std::swap(qa, qb);
+            cur = nxt;
+            nxt = (nxt == qa) ? qb : qa;
             round = next_round;
             ++rounds;
+        }
+        // heavy pass: the bucket's distances are final; its remaining edges
+        // all land at or above the threshold
+        if (list_edges > 0) {
+            hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, -1);
+            GHET_CHECK(launch_pass(true, list + list_base, list_len, list_edges, round));
+            relaxed += static_cast<int64_t>(slot_sum(hc->slots, 3));
+            far_count = hc->far_count[sel];
         }
         if (trace && trace_len < trace_cap) {
             GHET_HIP(hipEventRecord(g->ev_k1, s));
@@ -389,18 +565,20 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
             delta = base_delta * std::ldexp(1.0, cls - 1);
         }
         double new_thr = std::max(threshold + delta, std::nextafter(threshold, HUGE_VAL));
+        list_base = list_count;
         for (;;) {
             const int other = 1 - sel;
             hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, other);
             const int32_t next_round = round + 1;
             unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, kBT), 1024)));
             hipLaunchKernelGGL(k_far_split, dim3(grid), dim3(kBT), 0, s, far[sel], far_count, dist, old_thr, new_thr,
-                               stamp_near, next_round, stamp_far, far_gen + 1, qa, far[other], other, g->out.rowptr,
+                               stamp_near, next_round, stamp_far, far_gen + 1, list, far[other], other, g->out.rowptr,
                                ctr);
             GHET_HIP(hipGetLastError());
             GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(SsspCtr), hipMemcpyDeviceToHost, s));
             GHET_HIP(hipStreamSynchronize(s));
-            F = hc->near_count;
+            list_count = hc->list_count;
+            F = list_count - list_base;
             E = static_cast<int64_t>(slot_sum(hc->slots, 0));
             far_count = hc->far_count[other];
             unsigned long long mb = kInfBits;
@@ -410,6 +588,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
             sel = other;
             ++far_gen;
             round = next_round;
+            r0 = next_round;
             if (F > 0 || far_count == 0) break;
             // nothing in the next bucket: jump to the smallest far distance
             // (strictly above it, so the split always makes progress)
@@ -417,6 +596,9 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
             new_thr = std::max(far_min + delta, std::nextafter(far_min, HUGE_VAL));
         }
         threshold = new_thr;
+        list_len = F;
+        list_edges = E;
+        cur = list + list_base;
         if (F == 0 && far_count == 0) break;
     }
     if (pre_out) {
@@ -448,7 +630,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         st->trace_len = trace_len;
         st->extra[0] = base_delta;
         st->extra[1] = phases;
-        st->extra[2] = static_cast<double>(relaxed);  // edges relaxed over all rounds
+        st->extra[2] = static_cast<double>(relaxed);  // edges scanned over all passes
         st->status_flags = sched.fallback ? 1 : 0;
     }
     return sched.fallback ? GHET_WARN_FALLBACK : GHET_OK;
