@@ -298,7 +298,7 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     hipStream_t s = pick_stream(g, o);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
     // The batched form relaxes whole rows each round (no light/heavy split),
-    // so it favours buckets 8x wider than the single-source base.
+    // so it favours buckets 8x wider than the single-source base at scale 22.
     double delta = 0;
     if (have_opts && o->sssp_delta > 0) delta = o->sssp_delta;
     else {

@@ -154,13 +154,16 @@ int sched_pick_bfs(SchedState* s, const LevelInfo& li) {
 
 int sched_pick_sssp(SchedState* s, const LevelInfo& li) {
     if (s->schedule == GHET_SCHED_LEARNED && s->use_net) {
-        float x[GHET_TRACE_FEATURES], y[GHET_MLP_OUT];
-        sched_features(li, x);
-        net_forward_host(GHET_NET_SSSP, x, y);
-        int best = 0;
-        for (int k = 1; k < GHET_MLP_OUT; ++k)
-            if (y[k] > y[best]) best = k;
-        return best;
+        if (s->sssp_cls < 0) {
+            float x[GHET_TRACE_FEATURES], y[GHET_MLP_OUT];
+            sched_features(li, x);
+            net_forward_host(GHET_NET_SSSP, x, y);
+            int best = 0;
+            for (int k = 1; k < GHET_MLP_OUT; ++k)
+                if (y[k] > y[best]) best = k;
+            s->sssp_cls = best;
+        }
+        return s->sssp_cls;
     }
     return 1;  // the base width
 }

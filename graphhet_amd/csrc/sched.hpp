@@ -31,12 +31,14 @@ struct SchedState {
     bool use_net = false;   // learned network installed and requested
     bool fallback = false;  // learned requested but unavailable
     int64_t n = 0, m = 0, m_in = 0;
+    int sssp_cls = -1;      // learned SSSP class, picked once per run
 };
 
 void sched_begin(SchedState* s, int schedule, int kind, int64_t n, int64_t m, int64_t m_in);
 // Picks the BFS mode of the next level.
 int sched_pick_bfs(SchedState* s, const LevelInfo& li);
 // Picks an SSSP bucket-width multiplier class in [0, 4): delta = base * 2^(class-1).
+// The learned schedule picks it from the first phase's features and keeps it.
 int sched_pick_sssp(SchedState* s, const LevelInfo& li);
 // The 16 features the networks read (also written to traces).
 void sched_features(const LevelInfo& li, float* f);

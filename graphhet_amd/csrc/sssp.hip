@@ -384,7 +384,9 @@ __global__ void __launch_bounds__(256) k_wsum(const double* __restrict__ w, int6
 
 // Base bucket width: 4 mean weights per mean out-degree, so a bucket spans
 // about four light edges of an average row; light/heavy splitting keeps the
-// rest of each row out of the bucket's rounds.  The autotuner's class scales
+// rest of each row out of the bucket's rounds.  Graphs below 2^22 vertices
+// get wider buckets (x2 per factor 4 in n, at most x4): their buckets hold
+// too little work to pay a pass's fixed cost.  The autotuner's class scales
 // it by 2^(class-1).  The mean weight is reduced once per graph.
 int sssp_base_delta(ghet_graph* g, double* out) {
     if (g->mean_w < 0) {
@@ -408,7 +410,9 @@ int sssp_base_delta(ghet_graph* g, double* out) {
         g->mean_w = mean > 0 ? mean : 1.0;
     }
     const double avg_deg = g->n > 0 ? static_cast<double>(g->out.m) / g->n : 1.0;
-    *out = std::max(1e-12, 4.0 * g->mean_w / std::max(avg_deg, 1.0)) * std::ldexp(1.0, g->tune.sssp_class - 1);
+    const double small = std::min(2.0, std::max(0.0, (22.0 - std::log2(std::max<double>(g->n, 1))) / 2.0));
+    *out = std::max(1e-12, 4.0 * g->mean_w / std::max(avg_deg, 1.0)) * std::exp2(small) *
+           std::ldexp(1.0, g->tune.sssp_class - 1);
     return GHET_OK;
 }
 

@@ -129,22 +129,27 @@ def bfs_samples(dg: DeviceGraph, sources, reps: int = 2):
     return np.concatenate(X), np.concatenate(Y).astype(np.int32), np.concatenate(T)
 
 
-def sssp_samples(dg: DeviceGraph, sources):
-    """Phase features of the base-width run labelled with the fastest class,
-    and the class regret of each run as the samples' weight."""
+def sssp_samples(dg: DeviceGraph, sources, tie: float = 1.03):
+    """First-phase features of each run (the learned schedule picks the class
+    there and keeps it), labelled with the fastest whole-run class; the base
+    class wins when it is within `tie` of the fastest (timing noise).  The
+    class regret of the run is the sample's weight."""
     X, Y, W = [], [], []
     for src in sources:
         src = int(src)
-        _, st0, tr = dg.sssp(src, out=False, trace_cap=1024)
+        _, st0, tr = dg.sssp(src, out=False, trace_cap=1)
         base = st0.extra[0]
         totals = []
         for cls in range(MLP_OUT):
             _, st = dg.sssp(src, out=False, delta=base * 2.0 ** (cls - 1))
             totals.append(st.device_ms)
         f, _ = _trace_arrays(tr)
-        X.append(f)
-        Y.append(np.full(len(f), int(np.argmin(totals)), np.int32))
-        W.append(np.full(len(f), np.log(max(totals) / max(min(totals), 1e-6)) + 0.05))
+        best = int(np.argmin(totals))
+        if totals[1] <= tie * totals[best]:
+            best = 1
+        X.append(f[:1])
+        Y.append(np.array([best], np.int32))
+        W.append(np.array([np.log(max(totals) / max(min(totals), 1e-6)) + 0.05]))
     return np.concatenate(X), np.concatenate(Y), np.concatenate(W)
 
 
@@ -222,13 +227,13 @@ def trace_set(scales, graphs_per_scale: int, sources: int, seed: int, log=print)
             bx.append(X)
             by.append(y)
             bw.append(regret_weights(T))
-            X, y, w = sssp_samples(dg, srcs[: max(1, sources // 2)])
+            X, y, w = sssp_samples(dg, srcs)
             sx.append(X)
             sy.append(y)
             sw.append(w)
             log(f"traces: scale {sc} a={a:.3f} ef={ef:.1f}: {len(by[-1])} BFS levels "
-                f"(labels {np.bincount(by[-1], minlength=3).tolist()}), {len(sy[-1])} SSSP phases "
-                f"(class {int(sy[-1][0])}) in {time.time() - t0:.1f}s")
+                f"(labels {np.bincount(by[-1], minlength=3).tolist()}), SSSP runs by class "
+                f"{np.bincount(sy[-1], minlength=MLP_OUT).tolist()} in {time.time() - t0:.1f}s")
             dg.close()
             hg.close()
     cat = np.concatenate
@@ -238,7 +243,7 @@ def trace_set(scales, graphs_per_scale: int, sources: int, seed: int, log=print)
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", default=None, help="schedule file to write")
-    ap.add_argument("--scales", default="18,19,20,21,22")
+    ap.add_argument("--scales", default="18,19,20,21,22,23,24")
     ap.add_argument("--graphs", type=int, default=2, help="graphs per scale")
     ap.add_argument("--sources", type=int, default=8, help="sources per graph")
     ap.add_argument("--steps", type=int, default=1500)
