@@ -1,6 +1,6 @@
 // Batched SSSP: up to 32 sources per pass advanced together, one launch per
 // round for the whole batch (near-far buckets with a threshold shared by the
-// batch).
+// batch, light/heavy edge split as in the single-source form).
 //
 // No reference counterpart: the reference runs one csr_sssp per source
 // (GPU_SSSP_csr.cuh:253).  Every source converges to the same fixpoint as its
@@ -10,16 +10,21 @@
 //
 // Vertex queues with source masks.  A round's frontier is a queue of DISTINCT
 // vertices; vertex u carries a 32-bit mask of the sources whose distance at
-// u improved below the threshold.  One degree scan over the queue balances
-// the round's edges for all sources at once, and each edge (u -> v, w) is
-// read once and relaxed for every source in u's mask.  Distances are
-// vertex-major [n][K] (the K values of a vertex share cache lines).  An
-// improvement ORs the source's bit into v's next-round mask (below the
-// threshold) or its far mask (above); the 0 -> nonzero transition of a mask
-// appends v to the next queue / the far list, which replaces per-pair
-// stamps.  When the near queue drains, the far list is split at the next
-// threshold per (vertex, source) bit; bits of entries already below the old
-// threshold are stale and dropped.
+// u improved below the threshold.  One scan over the queue balances the
+// round's edges for all sources at once, and each edge (u -> v, w) is read
+// once and relaxed for every source in u's mask.  Distances are vertex-major
+// [n][K] (the K values of a vertex share cache lines).  Rows are ordered by
+// weight (the single-source layout), so a round reads only the prefix of u's
+// row that is light for the smallest masked distance, and relaxes per source
+// only the edges with d_s[u] + w < thr; an improvement ORs the source's bit
+// into v's next-round mask, whose 0 -> nonzero transition appends v to the
+// next queue.  It also ORs the bit into v's bucket mask, whose 0 -> nonzero
+// transition appends v to the bucket list.  When the near queue drains, one
+// heavy pass over the bucket list relaxes, per bucket-mask source, the edges
+// with d_s[u] + w >= thr (now final) and sends improvements to the far list
+// through far masks.  The far list is then split at the next threshold per
+// (vertex, source) bit; bits of entries already below the old threshold are
+// stale and dropped.
 //
 // The caller's [k][n] rows are produced by one tiled transpose per pass.
 #include <cmath>
@@ -30,14 +35,18 @@
 namespace ghet {
 
 int sssp_base_delta(ghet_graph* g, double* out);
+int build_sssp_layout(ghet_graph* g);
 
 namespace {
 
+// v[0] out-edges of the next near queue, v[1] bits of the smallest kept far
+// distance, v[2] out-edges appended to the bucket list, v[3] edges scanned.
 struct MssCtr {
     int32_t near_count;
     int32_t far_count[2];
-    int32_t pad[13];
-    CtrSlot slots[kSlots];  // v[0] out-edges of the next near queue, v[1] bits of the smallest kept far distance
+    int32_t list_count;
+    int32_t pad[12];
+    CtrSlot slots[kSlots];
 };
 constexpr int kCtrWords = sizeof(MssCtr) / 8;
 constexpr unsigned long long kInfBits = 0x7FF0000000000000ULL;
@@ -59,32 +68,63 @@ __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_
     return lo;
 }
 
-// Seeds: q[j] = distinct source vertex, mask[q[j]] = its sources, dist 0.
+// Seeds: q[j] = distinct source vertex with its sources qmask[j]: distance 0,
+// near and bucket masks, and the bucket list (= the first near queue).
 __global__ void k_mss_seed(const int32_t* __restrict__ q, const uint32_t* __restrict__ qmask, int32_t F, int32_t ks,
-                           double* dist, uint32_t* mask) {
+                           double* dist, uint32_t* mask, uint32_t* bmask, int32_t* list, MssCtr* ctr) {
     for (int j = threadIdx.x; j < F; j += blockDim.x) {
         const int32_t v = q[j];
         const uint32_t m = qmask[j];
         mask[v] = m;
+        bmask[v] = m;
+        list[j] = v;
         for (uint32_t b = m; b; b &= b - 1) dist[static_cast<int64_t>(v) * ks + (__ffs(b) - 1)] = 0.0;
     }
+    if (threadIdx.x == 0) ctr->list_count = F;
 }
 
+// Resets the near queue and the spread sums; also far list `other` (split)
+// and the bucket list (split: a new bucket starts).
 __global__ void k_mss_reset(MssCtr* c, int other) {
     unsigned long long* w = reinterpret_cast<unsigned long long*>(c->slots);
     for (int i = threadIdx.x; i < kSlots * 8; i += blockDim.x) w[i] = (i % 8 == 1) ? kInfBits : 0ULL;
     if (threadIdx.x == 0) {
         c->near_count = 0;
-        if (other >= 0) c->far_count[other] = 0;
+        if (other >= 0) {
+            c->far_count[other] = 0;
+            c->list_count = 0;
+        }
     }
 }
 
-__global__ void k_qdeg(const int32_t* __restrict__ q, int32_t F, const int32_t* __restrict__ rowptr,
-                       int32_t* __restrict__ deg) {
+// Per queue entry u with source mask m: the row segment a pass reads.  Round:
+// the prefix with min_s d_s[u] + w < thr (it holds every source's light
+// edges); heavy: the rest from the prefix end of max_s d_s[u] (it holds every
+// source's heavy edges).
+template <bool kHeavy>
+__global__ void k_mss_rows(const int32_t* __restrict__ q, int32_t F, const uint32_t* __restrict__ qmask,
+                           const int32_t* __restrict__ rowptr, const double* __restrict__ w,
+                           const double* __restrict__ dist, int32_t ks, double threshold, int32_t* __restrict__ cnt,
+                           int32_t* __restrict__ start) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < F;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int32_t v = q[i];
-        deg[i] = rowptr[v + 1] - rowptr[v];
+        const int32_t u = q[i];
+        const uint32_t m = qmask[u];
+        const double* du = dist + static_cast<int64_t>(u) * ks;
+        double d = kHeavy ? 0.0 : HUGE_VAL;
+        for (uint32_t b = m; b; b &= b - 1) {
+            const double x = du[__ffs(b) - 1];
+            d = kHeavy ? fmax(d, x) : fmin(d, x);
+        }
+        const int32_t rb = rowptr[u], re = rowptr[u + 1];
+        int32_t lo = rb, hi = re;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (d + w[mid] < threshold) lo = mid + 1;
+            else hi = mid;
+        }
+        cnt[i] = kHeavy ? re - lo : lo - rb;
+        start[i] = kHeavy ? lo : rb;
     }
 }
 
@@ -95,30 +135,44 @@ __global__ void k_clear_mask(const int32_t* __restrict__ q, int32_t F, uint32_t*
         mask[q[i]] = 0u;
 }
 
-// One round: persistent blocks over 2048-edge tiles of the queue's edges.
+// One pass, persistent blocks over 2048-edge tiles; entry j of queue q covers
+// edges [start[j], start[j] + scan[j+1] - scan[j]) of the weight-ordered rows
+// and scan[F] is the pass's edge count.
+// Round (kHeavy false): per source in mask_cur[u], relaxes the edges with
+// d_s[u] + w < thr; improvements go to v's next-round mask (0 -> nonzero
+// appends v to next_q) and its bucket mask (0 -> nonzero appends v to list).
+// Heavy (kHeavy true): q is the drained bucket's list and mask_cur the
+// bucket masks; relaxes the edges with d_s[u] + w >= thr, improvements go to
+// v's far mask (0 -> nonzero appends v to the far list).
+template <bool kHeavy>
 __global__ void __launch_bounds__(kBT) k_mss_relax(const int32_t* __restrict__ q, int32_t F,
-                                                   const int32_t* __restrict__ scan, int32_t E,
+                                                   const int32_t* __restrict__ scan,
+                                                   const int32_t* __restrict__ start,
                                                    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                    const double* __restrict__ wts, int32_t ks, double* dist,
                                                    double threshold, const uint32_t* __restrict__ mask_cur,
-                                                   uint32_t* mask_next, uint32_t* fmask, int32_t* __restrict__ next_q,
+                                                   uint32_t* mask_next, uint32_t* bmask, uint32_t* fmask,
+                                                   int32_t* __restrict__ next_q, int32_t* __restrict__ list,
                                                    int32_t* far_list, int far_sel, MssCtr* ctr) {
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
     __shared__ int32_t s_src[kTile];
     __shared__ uint32_t s_mask[kTile];
-    __shared__ int32_t s_near[kOut], s_far[kOut];
-    __shared__ int32_t s_i0, s_span, s_nc, s_fc, s_base;
+    __shared__ int32_t s_qa[kOut], s_qb[kOut];  // round: near, list; heavy: far
+    __shared__ int32_t s_i0, s_span, s_na, s_nb, s_base;
     __shared__ unsigned long long s_red[kBT / 64];
     const int tid = threadIdx.x;
     if (tid == 0) {
-        s_nc = 0;
-        s_fc = 0;
+        s_na = 0;
+        s_nb = 0;
     }
     __syncthreads();  // counters zeroed before any wave appends
-    unsigned long long od = 0;
+    unsigned long long od = 0, ld = 0;
     auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
+    auto on_list = [&](int32_t v) { ld += rowptr[v + 1] - rowptr[v]; };
     auto nop = [](int32_t) {};
+    const int32_t E = scan[F];
+    if (blockIdx.x == 0 && tid == 0) slot_add(ctr->slots, 3, static_cast<unsigned long long>(E));
     for (int64_t tile = blockIdx.x; tile * kTile < E; tile += gridDim.x) {
         const int32_t t0 = static_cast<int32_t>(tile * kTile);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
@@ -135,14 +189,14 @@ __global__ void __launch_bounds__(kBT) k_mss_relax(const int32_t* __restrict__ q
             for (int j = tid; j < span; j += kBT) {
                 const int32_t u = q[i0 + j];
                 s_scan[j] = scan[i0 + j];
-                s_start[j] = rowptr[u];
+                s_start[j] = start[i0 + j];
                 s_src[j] = u;
                 s_mask[j] = mask_cur[u];
             }
         __syncthreads();
         for (int k = 0; k < kItems; ++k) {
             const int32_t e = t0 + k * kBT + tid;
-            bool to_near = false, to_far = false;
+            bool push_a = false, push_b = false;
             int32_t v = 0;
             if (e < t1) {
                 int32_t pos, u;
@@ -155,48 +209,66 @@ __global__ void __launch_bounds__(kBT) k_mss_relax(const int32_t* __restrict__ q
                 } else {
                     const int j = upper_bound_i32(scan + i0, span, e) - 1;
                     u = q[i0 + j];
-                    pos = rowptr[u] + (e - scan[i0 + j]);
+                    pos = start[i0 + j] + (e - scan[i0 + j]);
                     m = mask_cur[u];
                 }
                 v = col[pos];
                 const double w = wts[pos];
                 const double* du = dist + static_cast<int64_t>(u) * ks;
                 double* dv = dist + static_cast<int64_t>(v) * ks;
-                uint32_t near_bits = 0, far_bits = 0;
+                uint32_t bits = 0;
                 for (uint32_t b = m; b; b &= b - 1) {
                     const int s = __ffs(b) - 1;
                     const double nd = du[s] + w;
-                    if (nd < dv[s]) {
+                    if ((kHeavy ? nd >= threshold : nd < threshold) && nd < dv[s]) {
                         const double old = atomic_min_nonneg(&dv[s], nd);
-                        if (nd < old) {
-                            if (nd < threshold) near_bits |= 1u << s;
-                            else far_bits |= 1u << s;
-                        }
+                        if (nd < old) bits |= 1u << s;
                     }
                 }
-                if (near_bits) to_near = atomicOr(&mask_next[v], near_bits) == 0u;
-                if (far_bits) to_far = atomicOr(&fmask[v], far_bits) == 0u;
+                if (bits) {
+                    if (kHeavy) {
+                        push_a = atomicOr(&fmask[v], bits) == 0u;
+                    } else {
+                        push_a = atomicOr(&mask_next[v], bits) == 0u;
+                        push_b = atomicOr(&bmask[v], bits) == 0u;
+                    }
+                }
             }
-            lq_push(s_near, &s_nc, to_near, v);
-            lq_push(s_far, &s_fc, to_far, v);
+            lq_push(s_qa, &s_na, push_a, v);
+            if (!kHeavy) lq_push(s_qb, &s_nb, push_b, v);
         }
-        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, next_q, kOut - kTile, on_near);
-        lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[far_sel], far_list, kOut - kTile, nop);
+        if (kHeavy) {
+            lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->far_count[far_sel], far_list, kOut - kTile, nop);
+        } else {
+            lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->near_count, next_q, kOut - kTile, on_near);
+            lq_flush<kBT>(s_qb, &s_nb, &s_base, &ctr->list_count, list, kOut - kTile, on_list);
+        }
     }
-    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, next_q, 0, on_near);
-    lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[far_sel], far_list, 0, nop);
-    od = block_sum_u64<kBT>(od, s_red);
-    if (tid == 0) slot_add(ctr->slots, 0, od);
+    if (kHeavy) {
+        lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->far_count[far_sel], far_list, 0, nop);
+    } else {
+        lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->near_count, next_q, 0, on_near);
+        lq_flush<kBT>(s_qb, &s_nb, &s_base, &ctr->list_count, list, 0, on_list);
+        od = block_sum_u64<kBT>(od, s_red);
+        ld = block_sum_u64<kBT>(ld, s_red);
+        if (tid == 0) {
+            slot_add(ctr->slots, 0, od);
+            slot_add(ctr->slots, 2, ld);
+        }
+    }
 }
 
 // Splits far list `far_in` at new_thr: per (vertex, source) bit, entries below
 // old_thr are stale (settled in an earlier bucket) and dropped, entries below
-// new_thr move to the near queue, the rest stay far.
+// new_thr start the next bucket, the rest stay far.  A vertex with bits in
+// the new bucket is appended to the bucket list, which is also the bucket's
+// first near queue (its near and bucket masks get those bits; both masks are
+// all-zero here and far-list entries are distinct).
 __global__ void __launch_bounds__(kBT) k_mss_split(const int32_t* __restrict__ far_in, int32_t count,
                                                    const double* __restrict__ dist, int32_t ks, double old_thr,
                                                    double new_thr, uint32_t* fmask, uint32_t* mask_near,
-                                                   int32_t* __restrict__ near_q, int32_t* far_out, int other,
-                                                   const int32_t* __restrict__ rowptr, MssCtr* ctr) {
+                                                   uint32_t* bmask, int32_t* __restrict__ list, int32_t* far_out,
+                                                   int other, const int32_t* __restrict__ rowptr, MssCtr* ctr) {
     __shared__ int32_t s_near[kOut], s_far[kOut];
     __shared__ int32_t s_nc, s_fc, s_base;
     __shared__ unsigned long long s_red[kBT / 64];
@@ -232,14 +304,18 @@ __global__ void __launch_bounds__(kBT) k_mss_split(const int32_t* __restrict__ f
             }
             fmask[v] = kb;
             keep = kb != 0u;
-            if (nb) to_near = atomicOr(&mask_near[v], nb) == 0u;
+            if (nb) {
+                mask_near[v] = nb;
+                bmask[v] = nb;
+                to_near = true;
+            }
         }
         lq_push(s_near, &s_nc, to_near, v);
         lq_push(s_far, &s_fc, keep, v);
-        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, near_q, kOut - kBT, on_near);
+        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, kOut - kBT, on_near);
         lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, kOut - kBT, nop);
     }
-    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->near_count, near_q, 0, on_near);
+    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, 0, on_near);
     lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, 0, nop);
     od = block_sum_u64<kBT>(od, s_red);
     dmin = wave_min(dmin);
@@ -297,13 +373,13 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     if (k == 0 || n == 0) return GHET_OK;
     hipStream_t s = pick_stream(g, o);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
-    // The batched form relaxes whole rows each round (no light/heavy split),
-    // so it favours buckets 8x wider than the single-source base at scale 22.
+    // A round carries up to 32 sources, so buckets twice the single-source
+    // base width fill the chip (measured best at scale 22, 256 sources).
     double delta = 0;
     if (have_opts && o->sssp_delta > 0) delta = o->sssp_delta;
     else {
         GHET_CHECK(sssp_base_delta(g, &delta));
-        delta *= 8.0;
+        delta *= 2.0;
     }
     // Sources per pass: up to 32 (the mask width), or the caller's choice.
     const int64_t kcap = (have_opts && o->reserved[3] > 0) ? o->reserved[3] : kMaxK;
@@ -311,19 +387,24 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     const int64_t KN = int64_t(K) * n;
     void* p;
     double* dist;
-    uint32_t *mask_a, *mask_b, *fmask, *d_qmask;
-    int32_t *qa, *qb, *far[2], *scan, *aux, *d_q;
+    uint32_t *mask_a, *mask_b, *fmask, *bmask, *d_qmask;
+    int32_t *qa, *qb, *far[2], *list, *qstart, *scan, *aux, *d_q;
     MssCtr* ctr;
+    GHET_CHECK(build_sssp_layout(g));
+    const SsspLayout& L = g->sl;
     GHET_CHECK(g->ws.get(WS_MULTI_A, 8 * size_t(KN), &p)); dist = static_cast<double*>(p);
-    GHET_CHECK(g->ws.get(WS_MULTI_B, 12 * size_t(n), &p));
+    GHET_CHECK(g->ws.get(WS_MULTI_B, 16 * size_t(n), &p));
     mask_a = static_cast<uint32_t*>(p);
     mask_b = mask_a + n;
     fmask = mask_b + n;
-    GHET_CHECK(g->ws.get(WS_MULTI_C, 16 * size_t(n), &p));
+    bmask = fmask + n;
+    GHET_CHECK(g->ws.get(WS_MULTI_C, 24 * size_t(n), &p));
     qa = static_cast<int32_t*>(p);
     qb = qa + n;
     far[0] = qb + n;
     far[1] = far[0] + n;
+    list = far[1] + n;
+    qstart = list + n;
     GHET_CHECK(g->ws.get(WS_MULTI_D, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(MssCtr), &p)); ctr = static_cast<MssCtr*>(p);
@@ -337,13 +418,13 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     }
     const MssCtr* hc = reinterpret_cast<const MssCtr*>(g->pinned);
 
-    int64_t traversed = 0, rounds = 0;
+    int64_t traversed = 0, rounds = 0, scanned = 0;
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     for (int32_t p0 = 0; p0 < k; p0 += K) {
         const int32_t kp = std::min(K, k - p0);
         const int64_t kn = int64_t(kp) * n;
         GHET_CHECK(fill<double>(dist, inf, kn, s));
-        GHET_CHECK(fill<uint32_t>(mask_a, 0u, 3 * int64_t(n), s));
+        GHET_CHECK(fill<uint32_t>(mask_a, 0u, 4 * int64_t(n), s));
         GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(ctr), 0, kCtrWords, s));
         // the pass's distinct source vertices and their source masks
         std::vector<int32_t> hq;
@@ -361,42 +442,80 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
         int32_t F = static_cast<int32_t>(hq.size());
         int64_t E = 0;
         for (int32_t v : hq) E += g->h_out_rowptr[v + 1] - g->h_out_rowptr[v];
+        int32_t list_len = F;
+        int64_t list_edges = E;
         // pageable sources: hipMemcpyAsync stages them before returning
         GHET_HIP(hipMemcpyAsync(d_q, hq.data(), 4 * hq.size(), hipMemcpyHostToDevice, s));
         GHET_HIP(hipMemcpyAsync(d_qmask, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice, s));
-        GHET_HIP(hipMemcpyAsync(qa, d_q, 4 * hq.size(), hipMemcpyDeviceToDevice, s));
         uint32_t *mcur = mask_a, *mnext = mask_b;
-        hipLaunchKernelGGL(k_mss_seed, dim3(1), dim3(64), 0, s, d_q, d_qmask, F, kp, dist, mcur);
+        hipLaunchKernelGGL(k_mss_seed, dim3(1), dim3(64), 0, s, d_q, d_qmask, F, kp, dist, mcur, bmask, list, ctr);
+        // a bucket's first near queue is its list; later rounds alternate qa/qb
+        const int32_t* cur = list;
+        int32_t* nxt = qa;
         int sel = 0;
         int64_t far_count = 0;
         double threshold = delta;
+        // Edge-balanced pass over queue q (F entries, masks qm, rows holding
+        // at most E_max edges: the grid bound).
+        auto pass = [&](bool heavy, const int32_t* q, int32_t Fq, const uint32_t* qm, int64_t E_max) -> int {
+            const unsigned vgrid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(Fq, 256), 65535)));
+            if (heavy)
+                hipLaunchKernelGGL(k_mss_rows<true>, dim3(vgrid), dim3(256), 0, s, q, Fq, qm, g->out.rowptr, L.w, dist,
+                                   kp, threshold, scan, qstart);
+            else
+                hipLaunchKernelGGL(k_mss_rows<false>, dim3(vgrid), dim3(256), 0, s, q, Fq, qm, g->out.rowptr, L.w, dist,
+                                   kp, threshold, scan, qstart);
+            GHET_CHECK(exclusive_scan_i32(scan, scan, Fq, aux, s));
+            const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
+            if (heavy)
+                hipLaunchKernelGGL(k_mss_relax<true>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr,
+                                   L.col, L.w, kp, dist, threshold, qm, mnext, bmask, fmask, nxt, list, far[sel], sel,
+                                   ctr);
+            else
+                hipLaunchKernelGGL(k_mss_relax<false>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr,
+                                   L.col, L.w, kp, dist, threshold, qm, mnext, bmask, fmask, nxt, list, far[sel], sel,
+                                   ctr);
+            // the pass's masks are consumed: clear them for reuse
+            hipLaunchKernelGGL(k_clear_mask, dim3(vgrid), dim3(256), 0, s, q, Fq, const_cast<uint32_t*>(qm));
+            GHET_HIP(hipGetLastError());
+            GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(MssCtr), hipMemcpyDeviceToHost, s));
+            GHET_HIP(hipStreamSynchronize(s));
+            scanned += static_cast<int64_t>(slot_sum(hc->slots, 3));
+            return GHET_OK;
+        };
         for (;;) {
             while (F > 0) {
                 hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, -1);
-                const unsigned vgrid =
-                    static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(F, 256), 65535)));
-                hipLaunchKernelGGL(k_qdeg, dim3(vgrid), dim3(256), 0, s, qa, F, g->out.rowptr, scan);
-                GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
                 if (E > 0) {
-                    const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), kMaxBlocks));
-                    hipLaunchKernelGGL(k_mss_relax, dim3(grid), dim3(kBT), 0, s, qa, F, scan, static_cast<int32_t>(E),
-                                       g->out.rowptr, g->out.col, g->out.w, kp, dist, threshold, mcur, mnext, fmask,
-                                       qb, far[sel], sel, ctr);
+                    GHET_CHECK(pass(false, cur, F, mcur, E));
+                    F = hc->near_count;
+                    E = static_cast<int64_t>(slot_sum(hc->slots, 0));
+                    list_edges += static_cast<int64_t>(slot_sum(hc->slots, 2));
+                    list_len = hc->list_count;
+                } else {
+                    hipLaunchKernelGGL(k_clear_mask, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s,
+                                       cur, F, mcur);
+                    F = 0;
                 }
-                hipLaunchKernelGGL(k_clear_mask, dim3(vgrid), dim3(256), 0, s, qa, F, mcur);
-                GHET_HIP(hipGetLastError());
-                GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(MssCtr), hipMemcpyDeviceToHost, s));
-                GHET_HIP(hipStreamSynchronize(s));
-                F = hc->near_count;
-                E = static_cast<int64_t>(slot_sum(hc->slots, 0));
-                far_count = hc->far_count[sel];
-                std::swap(qa, qb);
+                cur = nxt;
+                nxt = (nxt == qa) ? qb : qa;
                 std::swap(mcur, mnext);
                 ++rounds;
             }
+            // heavy pass over the drained bucket (also clears its bucket masks)
+            if (list_len > 0) {
+                hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, -1);
+                if (list_edges > 0) {
+                    GHET_CHECK(pass(true, list, list_len, bmask, list_edges));
+                    far_count = hc->far_count[sel];
+                } else {
+                    hipLaunchKernelGGL(k_clear_mask, dim3(static_cast<unsigned>(ceil_div(list_len, 256))), dim3(256),
+                                       0, s, list, list_len, bmask);
+                }
+            }
             if (far_count == 0) break;
-            // the near queue drained: split the far list at the next threshold
-            // (into the empty queue qa and its all-zero mask mcur)
+            // split the far list at the next threshold into a new bucket list
+            // (its near masks go to mcur, all-zero after the last round)
             double old_thr = threshold;
             double new_thr = std::max(threshold + delta, std::nextafter(threshold, HUGE_VAL));
             for (;;) {
@@ -405,11 +524,12 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
                 const unsigned grid =
                     static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, kBT), 2048)));
                 hipLaunchKernelGGL(k_mss_split, dim3(grid), dim3(kBT), 0, s, far[sel], static_cast<int32_t>(far_count),
-                                   dist, kp, old_thr, new_thr, fmask, mcur, qa, far[other], other, g->out.rowptr, ctr);
+                                   dist, kp, old_thr, new_thr, fmask, mcur, bmask, list, far[other], other,
+                                   g->out.rowptr, ctr);
                 GHET_HIP(hipGetLastError());
                 GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(MssCtr), hipMemcpyDeviceToHost, s));
                 GHET_HIP(hipStreamSynchronize(s));
-                F = hc->near_count;
+                F = hc->list_count;
                 E = static_cast<int64_t>(slot_sum(hc->slots, 0));
                 far_count = hc->far_count[other];
                 unsigned long long mb = kInfBits;
@@ -424,6 +544,10 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
                 new_thr = std::max(far_min + delta, std::nextafter(far_min, HUGE_VAL));
             }
             threshold = new_thr;
+            list_len = F;
+            list_edges = E;
+            cur = list;
+            nxt = qa;
             if (F == 0 && far_count == 0) break;
         }
         if (st) {
@@ -451,6 +575,7 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
         st->edges_traversed = traversed;
         st->extra[0] = delta;
         st->extra[1] = K;
+        st->extra[2] = static_cast<double>(scanned);  // edges scanned over all passes
     }
     return GHET_OK;
 }

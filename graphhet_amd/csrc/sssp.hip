@@ -332,7 +332,7 @@ __global__ void k_pre_finish(int32_t* pre, int32_t n) {
 
 // Orders each out-row by (weight, target) into g->sl.  Built once per graph on
 // the first SSSP; the host sorts the rows in parallel (per-row index sort).
-static int build_sssp_layout(ghet_graph* g) {
+int build_sssp_layout(ghet_graph* g) {
     SsspLayout& L = g->sl;
     if (L.built) return GHET_OK;
     const int64_t m = g->out.m;
