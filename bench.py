@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-heuristic section")
     ap.add_argument("--sched-file", default=os.path.join(ROOT, "graphhet_amd", "sched_net.txt"),
                     help="trained schedule networks (python -m graphhet_amd.learn)")
+    ap.add_argument("--graph-cache", default=os.environ.get("GHET_GRAPH_CACHE", ""),
+                    help="directory of binary CSR caches for the R-MAT graphs (default: $GHET_GRAPH_CACHE, "
+                         "empty = always generate)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -115,7 +118,7 @@ def main():
         sys.exit(1)
     algos = set(args.algos.split(","))
     t0 = time.time()
-    hg = G.HostGraph.rmat(args.scale, args.edge_factor, seed=args.seed)
+    hg = G.HostGraph.rmat_cached(args.graph_cache, args.scale, args.edge_factor, seed=args.seed)
     o = hg.view(G.VIEW_OUT, copy=False)
     i = hg.view(G.VIEW_IN, copy=False)
     n, m = o.n, o.m
@@ -268,7 +271,7 @@ def batched_section(G, args):
     """Batched forms vs the plain form on the same GPU (same inputs, results
     compared on a sample of instances)."""
     res = {}
-    hg = G.HostGraph.rmat(args.batched_scale, 16, seed=args.seed)
+    hg = G.HostGraph.rmat_cached(args.graph_cache, args.batched_scale, 16, seed=args.seed)
     o = hg.view(G.VIEW_OUT, copy=False)
     i = hg.view(G.VIEW_IN, copy=False)
     dg = G.DeviceGraph(o, i, with_all=False)

@@ -18,7 +18,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libghet.so")
 
-OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_NODEVICE, ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
+OK, ERR_INVALID, ERR_NOMEM, ERR_HIP, ERR_NODEVICE, ERR_UNSUPPORTED, ERR_IO = 0, 1, 2, 3, 4, 5, 6
 WARN_FALLBACK = 100
 
 GRAPH_UNDIRECTED, GRAPH_NO_WEIGHTS, GRAPH_NO_IN, GRAPH_NO_ALL = 1, 2, 4, 8
@@ -76,6 +76,7 @@ class DevicePtrs(C.Structure):
 EXPORTS = [
     "ghet_last_error", "ghet_abi_version", "ghet_device_count", "ghet_device_mem_info",
     "ghet_gen_rmat", "ghet_gen_er", "ghet_host_graph_from_edges", "ghet_host_graph_view", "ghet_host_graph_free",
+    "ghet_host_graph_save", "ghet_host_graph_load",
     "ghet_graph_create", "ghet_graph_wrap_device", "ghet_graph_destroy", "ghet_graph_device_ptrs",
     "ghet_graph_device_bytes",
     "ghet_bfs", "ghet_bfs_pre", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_cdlp", "ghet_tc",
@@ -108,6 +109,8 @@ def lib() -> C.CDLL:
                                     P(vp)]),
         "ghet_gen_er": (C.c_int, [C.c_int32, C.c_int64, C.c_uint64, C.c_int, P(vp)]),
         "ghet_host_graph_from_edges": (C.c_int, [C.c_int32, C.c_int64, i32p, i32p, f64p, C.c_int, P(vp)]),
+        "ghet_host_graph_save": (C.c_int, [vp, C.c_char_p]),
+        "ghet_host_graph_load": (C.c_int, [C.c_char_p, C.c_int, P(vp)]),
         "ghet_host_graph_view": (C.c_int, [vp, C.c_int, P(CsrView)]),
         "ghet_host_graph_free": (None, [vp]),
         "ghet_graph_create": (C.c_int, [C.c_int32, i32p, i32p, f64p, i32p, i32p, f64p, i32p, i32p, C.c_uint32,
@@ -208,6 +211,33 @@ class HostGraph:
         _check(lib().ghet_host_graph_from_edges(n, len(src), _ptr(src, C.c_int32), _ptr(dst, C.c_int32),
                                                 _ptr(wa, C.c_double), threads, C.byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def load(cls, path: str, threads: int = 0) -> "HostGraph":
+        """Reads a binary CSR cache (`path` targets, `path`.w weights)."""
+        h = C.c_void_p()
+        _check(lib().ghet_host_graph_load(os.fsencode(path), threads, C.byref(h)))
+        return cls(h.value)
+
+    def save(self, path: str) -> None:
+        """Writes the out-adjacency as a binary CSR cache (`path`, `path`.w)."""
+        _check(lib().ghet_host_graph_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def rmat_cached(cls, cache_dir: str | None, scale: int, edge_factor: float = 16, a: float = 0.57,
+                    b: float = 0.19, c: float = 0.19, seed: int = 1, threads: int = 0) -> "HostGraph":
+        """R-MAT through a binary cache in `cache_dir` (generated and saved on
+        a miss; no cache when cache_dir is None or empty)."""
+        if not cache_dir:
+            return cls.rmat(scale, edge_factor, a, b, c, seed, threads)
+        name = f"rmat-s{scale}-ef{edge_factor:g}-a{a:g}-b{b:g}-c{c:g}-seed{seed}.csr"
+        path = os.path.join(cache_dir, name)
+        if os.path.exists(path) and os.path.exists(path + ".w"):
+            return cls.load(path, threads)
+        g = cls.rmat(scale, edge_factor, a, b, c, seed, threads)
+        os.makedirs(cache_dir, exist_ok=True)
+        g.save(path)
+        return g
 
     def view(self, kind: int = VIEW_OUT, copy: bool = True) -> Csr:
         v = CsrView()

@@ -7,7 +7,13 @@
 // lists ascending, duplicates merged with the last weight winning -- is built
 // by counting sort + per-row sort, threaded, so scale-24 graphs build in
 // seconds.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <atomic>
+#include <cerrno>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -482,3 +488,174 @@ ghet_host_graph* host_graph_adopt(int32_t n, const int32_t* rowptr, const int32_
 }
 
 }  // namespace ghet
+
+// ------------------------------------------------------------ CSR cache ----
+// Two files in the vector-of-vectors layout (int32 row count, then per row an
+// int32 length and its elements): targets in `path`, weights in `path`.w.
+// Row u's elements start at byte 4 + 4(u+1) + sizeof(T) * rowptr[u], so once
+// the target file's headers give rowptr, every other offset is known and the
+// copies and checks run in parallel.
+namespace ghet {
+namespace {
+
+struct MappedFile {
+    const unsigned char* p = nullptr;
+    size_t size = 0;
+    int fd = -1;
+    ~MappedFile() {
+        if (p && size) munmap(const_cast<unsigned char*>(p), size);
+        if (fd >= 0) close(fd);
+    }
+    // 0 ok, 1 missing, -1 other failure
+    int open_ro(const std::string& path) {
+        fd = ::open(path.c_str(), O_RDONLY);
+        if (fd < 0) return errno == ENOENT ? 1 : -1;
+        struct stat sb;
+        if (fstat(fd, &sb) != 0) return -1;
+        size = static_cast<size_t>(sb.st_size);
+        if (size == 0) return 0;
+        void* m = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (m == MAP_FAILED) {
+            size = 0;
+            return -1;
+        }
+        p = static_cast<const unsigned char*>(m);
+        return 0;
+    }
+    int32_t i32(size_t off) const {
+        int32_t x;
+        std::memcpy(&x, p + off, 4);
+        return x;
+    }
+};
+
+template <typename T>
+int save_rows(const std::string& path, int32_t n, const int32_t* rowptr, const T* data) {
+    const std::string tmp = path + ".tmp." + std::to_string(static_cast<long long>(getpid()));
+    std::FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return set_error(GHET_ERR_IO, "Unable to open file %s", tmp.c_str());
+    std::vector<char> buf(size_t(1) << 22);
+    std::setvbuf(f, buf.data(), _IOFBF, buf.size());
+    bool ok = std::fwrite(&n, 4, 1, f) == 1;
+    for (int32_t u = 0; ok && u < n; ++u) {
+        const int32_t len = rowptr[u + 1] - rowptr[u];
+        ok = std::fwrite(&len, 4, 1, f) == 1;
+        if (ok && len > 0) ok = std::fwrite(data + rowptr[u], sizeof(T), static_cast<size_t>(len), f) == size_t(len);
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        return set_error(GHET_ERR_IO, "Unable to write file %s", path.c_str());
+    }
+    return GHET_OK;
+}
+
+}  // namespace
+}  // namespace ghet
+
+extern "C" {
+
+int ghet_host_graph_save(ghet_host_graph* g, const char* path) {
+    clear_error();
+    if (!g || !path) return set_error(GHET_ERR_INVALID, "NULL argument");
+    const std::string p(path);
+    const int32_t* rp = g->out_rowptr.data();
+    int st = save_rows<int32_t>(p, g->n, rp, g->out_col.data());
+    if (st != GHET_OK) return st;
+    return save_rows<double>(p + ".w", g->n, rp, g->out_w.data());
+}
+
+int ghet_host_graph_load(const char* path, int threads, ghet_host_graph** out) {
+    clear_error();
+    if (!path || !out) return set_error(GHET_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    const std::string p(path);
+    MappedFile tf, wf;
+    if (tf.open_ro(p) != 0) return set_error(GHET_ERR_IO, "Unable to open file %s", path);
+    const int wst = wf.open_ro(p + ".w");
+    if (wst < 0) return set_error(GHET_ERR_IO, "Unable to open file %s.w", path);
+    const bool have_w = wst == 0;
+    if (tf.size < 4) return set_error(GHET_ERR_IO, "Unable to read file %s (no row count)", path);
+    const int32_t n = tf.i32(0);
+    if (n < 0) return set_error(GHET_ERR_IO, "Unable to read file %s (row count %d)", path, n);
+    // headers of the target file: row lengths -> rowptr
+    std::vector<int32_t> rowptr(static_cast<size_t>(n) + 1, 0);
+    size_t pos = 4;
+    int64_t m = 0;
+    for (int32_t u = 0; u < n; ++u) {
+        if (pos + 4 > tf.size) return set_error(GHET_ERR_IO, "Unable to read file %s (row %d header)", path, u);
+        const int32_t len = tf.i32(pos);
+        if (len < 0 || pos + 4 + 4 * size_t(len) > tf.size)
+            return set_error(GHET_ERR_IO, "Unable to read file %s (row %d length %d)", path, u, len);
+        m += len;
+        if (m > INT32_MAX) return set_error(GHET_ERR_UNSUPPORTED, "graph has more than 2^31-1 edges");
+        rowptr[u + 1] = static_cast<int32_t>(m);
+        pos += 4 + 4 * size_t(len);
+    }
+    if (pos != tf.size) return set_error(GHET_ERR_IO, "Unable to read file %s (%zu trailing bytes)", path, tf.size - pos);
+    if (have_w) {
+        const size_t want = 4 + 4 * size_t(n) + 8 * size_t(m);
+        if (wf.size != want || (wf.size >= 4 && wf.i32(0) != n))
+            return set_error(GHET_ERR_IO, "Unable to read file %s.w (does not match %s)", path, path);
+    }
+    auto* g = new (std::nothrow) ghet_host_graph();
+    if (!g) return set_error(GHET_ERR_NOMEM, "host allocation failed");
+    std::unique_ptr<ghet_host_graph> guard(g);
+    g->n = n;
+    g->threads = hw_threads(threads);
+    const int T = g->threads;
+    g->out_rowptr = std::move(rowptr);
+    const int32_t* rp = g->out_rowptr.data();
+    g->out_col.resize(static_cast<size_t>(m));
+    g->out_w.resize(static_cast<size_t>(m));
+    std::atomic<int> bad_target{-1}, bad_w{-1};
+    std::atomic<bool> canonical{true};
+    parallel_for(n, T, [&](int64_t b, int64_t e, int) {
+        bool canon = true;
+        for (int64_t u = b; u < e; ++u) {
+            const int32_t r0 = rp[u], len = rp[u + 1] - rp[u];
+            const size_t toff = 4 + 4 * size_t(u + 1) + 4 * size_t(r0);
+            if (len > 0) std::memcpy(&g->out_col[r0], tf.p + toff, 4 * size_t(len));
+            for (int32_t k = 0; k < len; ++k) {
+                const int32_t v = g->out_col[r0 + k];
+                if (v < 0 || v >= n) bad_target.store(static_cast<int>(u));
+                if (k > 0 && v <= g->out_col[r0 + k - 1]) canon = false;
+            }
+            if (have_w) {
+                const size_t woff = 4 + 4 * size_t(u + 1) + 8 * size_t(r0);
+                if (wf.i32(woff - 4) != len) bad_w.store(static_cast<int>(u));
+                else if (len > 0) std::memcpy(&g->out_w[r0], wf.p + woff, 8 * size_t(len));
+            } else {
+                for (int32_t k = 0; k < len; ++k) g->out_w[r0 + k] = 1.0;
+            }
+        }
+        if (!canon) canonical.store(false);
+    });
+    if (bad_target.load() >= 0)
+        return set_error(GHET_ERR_IO, "Unable to read file %s (row %d has a target outside [0,%d))", path,
+                         bad_target.load(), n);
+    if (bad_w.load() >= 0)
+        return set_error(GHET_ERR_IO, "Unable to read file %s.w (row %d length differs)", path, bad_w.load());
+    if (!canonical.load()) {
+        // unsorted or repeated targets: rebuild as an edge list in file order
+        std::vector<uint64_t> pairs(static_cast<size_t>(m));
+        parallel_for(n, T, [&](int64_t b, int64_t e, int) {
+            for (int64_t u = b; u < e; ++u)
+                for (int32_t k = rp[u]; k < rp[u + 1]; ++k)
+                    pairs[k] = (static_cast<uint64_t>(u) << 32) | static_cast<uint32_t>(g->out_col[k]);
+        });
+        std::vector<double> w = std::move(g->out_w);
+        std::vector<int64_t> offs;
+        std::vector<uint64_t> keys;
+        int st = bucket_by_source(g, pairs, offs, keys);
+        if (st != GHET_OK) return st;
+        pairs.clear();
+        pairs.shrink_to_fit();
+        st = finish_rows(g, offs, keys, [&](uint32_t k) { return w[k]; });
+        if (st != GHET_OK) return st;
+    }
+    *out = guard.release();
+    return GHET_OK;
+}
+
+}  // extern "C"

@@ -35,6 +35,7 @@ typedef enum ghet_status {
     GHET_ERR_HIP = 3,         /* HIP runtime error (message in ghet_last_error) */
     GHET_ERR_NODEVICE = 4,    /* no GPU visible to this process */
     GHET_ERR_UNSUPPORTED = 5, /* input the path does not handle (e.g. > 2^31-1 edges) */
+    GHET_ERR_IO = 6,          /* a file could not be opened, read or written, or is malformed */
     GHET_WARN_FALLBACK = 100  /* learned schedule unavailable: heuristic used, results valid */
 } ghet_status;
 
@@ -87,6 +88,20 @@ int ghet_host_graph_from_edges(int32_t n, int64_t m, const int32_t* src, const i
 /* Views are built on first request and owned by the graph. */
 int ghet_host_graph_view(ghet_host_graph* g, int kind, ghet_csr_view* out);
 void ghet_host_graph_free(ghet_host_graph* g);
+
+/* Binary CSR cache: the out-adjacency in the vector-of-vectors layout of
+ * binary_save_vector_of_vectors (binary_save_read_vector_of_vectors.hpp:10-36,
+ * reproduced in include/CPU_adj_list/binary_save_read_vector_of_vectors.hpp):
+ * an int32 row count, then per row an int32 length and that many elements.
+ * `path` holds the targets (int32), `path` + ".w" the weights (double), so
+ * each file reads back with binary_read_vector_of_vectors<int> / <double>.
+ * Load checks both files against each other and the targets against the row
+ * count.  Rows already ascending without duplicates are adopted as they are;
+ * others are canonicalised as ghet_host_graph_from_edges does (sorted, the
+ * later entry's weight wins).  A missing ".w" file gives unit weights.
+ * I/O failures return GHET_ERR_IO with the file name in ghet_last_error. */
+int ghet_host_graph_save(ghet_host_graph* g, const char* path);
+int ghet_host_graph_load(const char* path, int threads, ghet_host_graph** out);
 
 /* ------------------------------------------------------------------------ */
 /* Device graphs                                                             */

@@ -1,5 +1,6 @@
 """Parity at full size: R-MAT scale 24 and an Erdos-Renyi graph with the same
-vertex and edge counts (TC on the R-MAT graph).  Running the oracle's BFS/SSSP/WCC/TC
+vertex and edge counts (TC on the R-MAT graph).  With GHET_GRAPH_CACHE set,
+the R-MAT graph goes through the binary CSR cache in that directory.  Running the oracle's BFS/SSSP/WCC/TC
 on 263M edges takes too long here, so the exact results are checked through
 properties that determine them completely, vectorised over every edge:
 
@@ -17,6 +18,8 @@ properties that determine them completely, vectorised over every edge:
         the same start: L1 <= 1e-6 and max-abs reported; the reference form
         (no normalisation) and the normalised form.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -40,7 +43,7 @@ class _Graphs:
     def __call__(self, kind):
         if kind not in self.cache:
             if kind == "rmat":
-                hg = G.HostGraph.rmat(24, 16, seed=1)
+                hg = G.HostGraph.rmat_cached(os.environ.get("GHET_GRAPH_CACHE"), 24, 16, seed=1)
             else:
                 r = self("rmat")[1]
                 hg = G.HostGraph.er(r.n, r.m, seed=2)
