@@ -52,9 +52,10 @@ def pr_iter_bytes(n: int, m: int) -> int:
 
 
 def measured_traffic(workload: str, seed: int):
-    """HBM bytes per k_pr_iter launch measured by the rocprofv3 PMC runs
-    recorded under profiles/ for this workload (None if not measured)."""
-    path = os.path.join(ROOT, "profiles", f"k_pr_iter__{workload}.traffic.txt")
+    """HBM bytes per PR iteration (k_pr_iter + k_pr_long) measured by the
+    rocprofv3 PMC runs recorded under profiles/ for this workload (None if
+    not measured)."""
+    path = os.path.join(ROOT, "profiles", f"pr_iteration__{workload}.traffic.txt")
     if not os.path.exists(path):
         return None
     kv = {}
@@ -173,7 +174,8 @@ def main():
         traffic = measured_traffic(f"rmat-s{args.scale}-ef{int(args.edge_factor)}", args.seed)
         roofline = {"bound": "hbm", "achieved": round(pr_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(pr_gbs / PEAK_HBM_GBS, 4), "traffic": traffic,
-                    "kernel": "k_pr_iter", "kernel_avg_us": round(k_ms * 1e3, 2), "bytes_per_launch": bytes_it}
+                    "kernel": "k_pr_iter+k_pr_long", "kernel_avg_us": round(k_ms * 1e3, 2),
+                    "bytes_per_launch": bytes_it}
 
     plain = {}
     if sched != G.SCHED_HEURISTIC:

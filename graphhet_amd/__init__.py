@@ -398,11 +398,15 @@ class DeviceGraph:
         _check(lib().ghet_wcc(self._h, _ptr(lab, C.c_int32), C.byref(_opts()), C.byref(st)))
         return lab, st
 
-    def pr(self, iters: int = 10, damping: float = 0.85, normalise: bool = True, out=True):
+    def pr(self, iters: int = 10, damping: float = 0.85, normalise: bool = True, out=True, split=None):
+        """split = (log2 hot sources, log2 window) forces the source-window
+        layout (None: automatic)."""
         r = np.empty(self.n, np.float64) if out else None
         st = RunStats()
-        _check(lib().ghet_pr(self._h, iters, damping, _ptr(r, C.c_double),
-                             C.byref(_opts(flags=0 if normalise else 1)), C.byref(st)))
+        o = _opts(flags=0 if normalise else 1)
+        if split is not None:
+            o.reserved[4] = int(split[0]) | (int(split[1]) << 8)
+        _check(lib().ghet_pr(self._h, iters, damping, _ptr(r, C.c_double), C.byref(o), C.byref(st)))
         return r, st
 
     def cdlp(self, iters: int = 10, out=True):

@@ -50,16 +50,20 @@ struct Workspace {
 // PageRank tile layout over the in-CSR (built once per graph).
 struct PRLayout {
     bool built = false;
-    int32_t num_tiles = 0;        // tiles of consecutive short rows
-    int32_t num_long_chunks = 0;  // chunks of long rows
-    int32_t num_long_rows = 0;
-    int32_t num_items = 0;           // chunks first, then tiles
-    void* items = nullptr;           // [num_items] 32-byte work items (pr.hip)
-    void* long_rows = nullptr;       // [num_long_rows] int4 {row, first chunk, chunks, 0}
-    int32_t* outdeg = nullptr;       // [n] out-degree of relabeled vertex r (sinks have 0)
-    int32_t* perm = nullptr;         // [n] relabeled id -> caller's id (out-degree descending)
-    int32_t* rowptr = nullptr;       // [n+1] relabeled in-CSR offsets
-    int32_t* col = nullptr;          // [m] relabeled in-CSR sources (relabeled ids)
+    int32_t hot = 0, window = 0;  // source split the layout was built with
+    int32_t num_tiles = 0;        // tiles of consecutive short rows or segments
+    int32_t num_long_chunks = 0;  // chunks of long rows and long segments
+    int32_t num_long_rows = 0;    // rows finished from partials (k_pr_long)
+    int32_t num_slots = 0;        // partial sums written per iteration
+    int32_t num_segments = 0;     // cold (row, source window) segments
+    int32_t num_items = 0;        // hot items, then one group per cold window
+    void* items = nullptr;        // [num_items] 32-byte work items (pr.hip)
+    void* long_rows = nullptr;    // [num_long_rows] int4 {row, first slot, slots, 0}
+    int32_t* outdeg = nullptr;    // [n] out-degree of relabeled vertex r (sinks have 0)
+    int32_t* perm = nullptr;      // [n] relabeled id -> caller's id (out-degree descending)
+    int32_t* rowptr = nullptr;    // [n+S+1] offsets: n hot row parts, then S cold segments
+    int32_t* slot = nullptr;      // [n+S] partial slot of a tile row / segment, -1: finish in place
+    int32_t* col = nullptr;       // [m] relabeled sources: hot parts row-major, then cold segments
     void* mem = nullptr;
 };
 

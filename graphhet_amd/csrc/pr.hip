@@ -16,21 +16,26 @@
 // scale 22, against ~57% in the original numbering.  Results are written back
 // in the caller's numbering by a final scatter.
 //
-// Layout (built once per graph from the in-offsets): consecutive rows are
-// packed into tiles of at most kTileNnz in-edges and kTileRows rows; a row
-// with more than kTileNnz in-edges is a "long row", split into kTileNnz-edge
-// chunks whose partial sums are combined in chunk order by the last chunk to
-// arrive.  Chunks and tiles form one list of 32-byte work items.  One launch
-// per iteration walks the list with persistent blocks:
-//   1. gather contrib[in_col[e]] for the item's contiguous edge range (8 per
+// Layout (built once per graph from the in-offsets): each row's sources are
+// split into a hot part (relabeled ids below `hot`) and cold segments (one per
+// source window).  The row space is the n hot parts followed by the segments
+// (window-major); consecutive short entries are packed into tiles of at most
+// kTileNnz edges and kTileRows entries, and an entry with more than kTileNnz
+// edges is split into kTileNnz-edge chunks.  Chunks and tiles form one list
+// of 32-byte work items: the hot phase first, then one group per window.
+// One launch per iteration walks the list with persistent blocks:
+//   1. gather contrib[col[e]] for the item's contiguous edge range (8 per
 //      lane, all issued before the first wait; the next item's column, row
-//      offset and out-degree loads are issued behind them);
+//      offset, out-degree and slot loads are issued behind them);
 //   2. tiles stage the values in LDS -- rows with <= 32 values: one lane
 //      each; longer rows: one wave; chunks reduce in the block;
-//   3. write contrib'[v] = (float)(pr'[v] / outdeg(v)) for the next
-//      iteration, fold pr'[v] of sinks into a per-thread FP64 sum, and on
-//      the last iteration write pr'[v] in FP64.
-// A one-block kernel then sums the partials in index order into S.
+//   3. a row without partials writes contrib'[v] = (float)(pr'[v] /
+//      outdeg(v)) for the next iteration, folds pr'[v] of sinks into a
+//      per-thread FP64 sum, and on the last iteration writes pr'[v] in FP64;
+//      chunks, cold segments and the hot parts of rows with cold sources
+//      write FP64 partials to their slots instead.
+// k_pr_long then finishes the rows with partials (slots summed in order), and
+// a one-block kernel sums the sink partials in index order into S.
 #include <algorithm>
 #include <cmath>
 
@@ -121,16 +126,17 @@ struct alignas(32) PRItem {
 
 // Register-resident loads of one work item: the staged column indices (lanes
 // past the end hold 0, a valid index, and are masked later), and for tiles
-// the row offsets and out-degrees of rows tid + j*kBT.
+// the row offsets, out-degrees and partial slots of rows tid + j*kBT.
 struct PRStage {
     int32_t idx[kStage];
     int32_t roff[kRowsPT];
     int32_t od[kRowsPT];
+    int32_t sl[kRowsPT];
 };
 
 __device__ __forceinline__ void pr_stage(const PRItem& it, bool valid, const int32_t* __restrict__ rowptr,
                                          const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg,
-                                         PRStage& st) {
+                                         const int32_t* __restrict__ slot, int32_t n, PRStage& st) {
     const int tid = threadIdx.x;
     const int32_t nnz = valid ? it.e1 - it.e0 : 0;
     // Branch-free: lanes past the end load element 0 (always allocated) and
@@ -147,24 +153,30 @@ __device__ __forceinline__ void pr_stage(const PRItem& it, bool valid, const int
         const int r = tid + j * kBT;
         const int32_t v = r < rows ? it.row + r : 0;
         st.roff[j] = rowptr[v];
-        st.od[j] = outdeg[v];
+        st.od[j] = outdeg[v < n ? v : 0];  // segments (v >= n) never finish a vertex
+        st.sl[j] = slot[v];
     }
 }
 
 // One iteration over the relabeled in-CSR.  Persistent blocks walk the item
 // list with stride gridDim.x; the loads of item i+G (columns, row offsets,
-// out-degrees) are issued while the gathers of item i are in flight, so a
-// block always has two items' worth of memory requests outstanding and no
-// item pays more than one exposed round trip.  Every thread keeps its sink
-// mass in item order; part[blockIdx.x] gets the block's total.
+// out-degrees, slots) are issued while the gathers of item i are in flight,
+// so a block always has two items' worth of memory requests outstanding and
+// no item pays more than one exposed round trip.  A tile row with slot -1
+// finishes its vertex; any other tile row (the hot part of a vertex with cold
+// sources, or a cold segment) and every chunk store a partial sum for
+// k_pr_long.  Every thread keeps its sink mass in item order;
+// part[blockIdx.x] gets the block's total.
 __global__ void __launch_bounds__(kBT) k_pr_iter(
     const PRItem* __restrict__ items, int32_t num_items, const int32_t* __restrict__ rowptr,
-    const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg, double* __restrict__ long_part,
-    const float* __restrict__ contrib, float* __restrict__ next, const PRScalars* __restrict__ scal, double nd,
-    double damping, double* __restrict__ part, double* __restrict__ rank_out, int last) {
+    const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ slot, int32_t n,
+    double* __restrict__ long_part, const float* __restrict__ contrib, float* __restrict__ next,
+    const PRScalars* __restrict__ scal, double nd, double damping, double* __restrict__ part,
+    double* __restrict__ rank_out, int last) {
     __shared__ float s_val[kTileNnz];
     __shared__ int32_t s_off[kTileRows + 1];
     __shared__ int32_t s_od[kTileRows];
+    __shared__ int32_t s_sl[kTileRows];
     __shared__ double s_red[kBT / 64];
     __shared__ float s_ks[kBT / 64], s_kc[kBT / 64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -174,7 +186,7 @@ __global__ void __launch_bounds__(kBT) k_pr_iter(
     int32_t i = blockIdx.x;
     PRItem cur = i < num_items ? items[i] : PRItem{};
     PRStage st;
-    pr_stage(cur, i < num_items, rowptr, col, outdeg, st);
+    pr_stage(cur, i < num_items, rowptr, col, outdeg, slot, n, st);
     PRItem nxt = i + G < num_items ? items[i + G] : PRItem{};
     for (; i < num_items; i += G) {
         float val[kStage];
@@ -182,7 +194,7 @@ __global__ void __launch_bounds__(kBT) k_pr_iter(
         for (int j = 0; j < kStage; ++j) val[j] = contrib[st.idx[j]];
         // the next item's loads overlap these gathers
         PRStage sn;
-        pr_stage(nxt, i + G < num_items, rowptr, col, outdeg, sn);
+        pr_stage(nxt, i + G < num_items, rowptr, col, outdeg, slot, n, sn);
         const PRItem nxt2 = i + 2 * G < num_items ? items[i + 2 * G] : PRItem{};
         const int32_t nnz = cur.e1 - cur.e0;
         __syncthreads();  // the previous item's LDS readers are done
@@ -199,6 +211,7 @@ __global__ void __launch_bounds__(kBT) k_pr_iter(
                 if (r < rows) {
                     s_off[r] = st.roff[j] - cur.e0;
                     s_od[r] = st.od[j];
+                    s_sl[r] = st.sl[j];
                 }
             }
             if (tid == 0) s_off[rows] = nnz;
@@ -212,10 +225,14 @@ __global__ void __launch_bounds__(kBT) k_pr_iter(
                     if (z - a <= kWaveRow) {
                         Kahan32 acc;
                         for (int32_t e = a; e < z; ++e) acc.add(s_val[e]);
-                        const double pr = base + damping * static_cast<double>(acc.value());
-                        if (st.od[j] > 0) next[cur.row + r] = static_cast<float>(pr / st.od[j]);
-                        else sink += pr;
-                        if (last) rank_out[cur.row + r] = pr;
+                        if (st.sl[j] >= 0) {
+                            long_part[st.sl[j]] = static_cast<double>(acc.s) + static_cast<double>(acc.c);
+                        } else {
+                            const double pr = base + damping * static_cast<double>(acc.value());
+                            if (st.od[j] > 0) next[cur.row + r] = static_cast<float>(pr / st.od[j]);
+                            else sink += pr;
+                            if (last) rank_out[cur.row + r] = pr;
+                        }
                     }
                 }
             }
@@ -232,11 +249,16 @@ __global__ void __launch_bounds__(kBT) k_pr_iter(
                     for (int32_t e = a + lane; e < z; e += 64) acc.add(s_val[e]);
                     acc = wave_kahan(acc);
                     if (lane == 0) {
-                        const double pr = base + damping * static_cast<double>(acc.value());
-                        const int32_t d = s_od[rr];
-                        if (d > 0) next[cur.row + rr] = static_cast<float>(pr / d);
-                        else sink += pr;
-                        if (last) rank_out[cur.row + rr] = pr;
+                        const int32_t sl = s_sl[rr];
+                        if (sl >= 0) {
+                            long_part[sl] = static_cast<double>(acc.s) + static_cast<double>(acc.c);
+                        } else {
+                            const double pr = base + damping * static_cast<double>(acc.value());
+                            const int32_t d = s_od[rr];
+                            if (d > 0) next[cur.row + rr] = static_cast<float>(pr / d);
+                            else sink += pr;
+                            if (last) rank_out[cur.row + rr] = pr;
+                        }
                     }
                 }
             }
@@ -272,11 +294,12 @@ __global__ void __launch_bounds__(kBT) k_pr_iter(
     if (tid == 0) part[blockIdx.x] = t;
 }
 
-// Long rows: sum each row's chunk partials in chunk order and finish the
-// vertex (one thread per long row).  Runs after k_pr_iter in stream order,
-// which makes every partial visible without device-scope fences -- a fence
-// per chunk would write back and invalidate the XCD's L2 and with it the
-// hot contributions.  Sink mass goes to part[blockIdx.x].
+// Rows finished from partials (long rows and rows with cold sources): sum
+// each row's slots in slot order and finish the vertex (one thread per row).
+// Runs after k_pr_iter in stream order, which makes every partial visible
+// without device-scope fences -- a fence per partial would write back and
+// invalidate the XCD's L2 and with it the hot contributions.  Sink mass goes
+// to part[blockIdx.x].
 __global__ void __launch_bounds__(kBT) k_pr_long(const int4* __restrict__ long_rows, int32_t num_long_rows,
                                                  const double* __restrict__ long_part,
                                                  const int32_t* __restrict__ outdeg, float* __restrict__ next,
@@ -286,7 +309,7 @@ __global__ void __launch_bounds__(kBT) k_pr_long(const int4* __restrict__ long_r
     const double base = scal->sink * damping / nd + (1.0 - damping) / nd;
     double sink = 0;
     for (int32_t L = blockIdx.x * kBT + threadIdx.x; L < num_long_rows; L += gridDim.x * kBT) {
-        const int4 lr = long_rows[L];  // {row, first chunk, chunks, 0}
+        const int4 lr = long_rows[L];  // {row, first slot, slots, 0}
         double total = 0;
         for (int32_t j = 0; j < lr.z; ++j) total += long_part[lr.y + j];
         const double pr = base + damping * total;
@@ -334,13 +357,45 @@ __global__ void __launch_bounds__(kBT) k_permute_rows(const int32_t* __restrict_
     }
 }
 
-int build_pr_layout(ghet_graph* g, hipStream_t s) {
+// Relabeled sources below `hot` (the highest out-degrees) are "hot": their
+// FP32 contributions stay in every XCD's L2.  A row's other sources are cut
+// at multiples of `window` into cold segments; the hot parts of all rows are
+// processed first, then the cold segments window by window, each writing a
+// partial that k_pr_long adds to the row's hot partial.  Keeping cold
+// gathers out of the hot phase stops them evicting hot lines.  Automatic
+// split: none while the whole contribution array fits in 8 MB, else 2^19 hot
+// sources (2 MB) and 2^23-id windows (a single cold window up to scale 24).
+// Measured per iteration on R-MAT scale 24 (k_pr_iter + k_pr_long): no split
+// 1839 us; hot 2^19 with windows 2^20 1754, 2^21 1719, 2^23 1697; hot 2^18
+// or 2^20 are slower.
+void pr_split(int32_t n, int32_t code, int32_t* hot, int32_t* window) {
+    if (code > 0) {
+        *hot = static_cast<int32_t>(std::min<int64_t>(int64_t(1) << std::min(code & 0xFF, 30), n));
+        *window = static_cast<int32_t>(int64_t(1) << std::min(std::max((code >> 8) & 0xFF, 1), 30));
+    } else if (n > (1 << 21)) {
+        *hot = 1 << 19;
+        *window = 1 << 23;
+    } else {
+        *hot = n;
+        *window = 1 << 20;
+    }
+}
+
+int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     PRLayout& L = g->pr;
-    if (L.built) return GHET_OK;
     const int32_t n = g->n;
+    int32_t H = 0, W = 0;
+    pr_split(n, split_code, &H, &W);
+    if (L.built && L.hot == H && L.window == W) return GHET_OK;
+    if (L.mem) {
+        GHET_HIP(hipStreamSynchronize(s));
+        GHET_HIP(hipFree(L.mem));
+        L = PRLayout{};
+    }
     const int64_t m = g->in.m;
     const std::vector<int32_t>& irp = g->h_in_rowptr;
     const std::vector<int32_t>& orp = g->h_out_rowptr;
+    const int T = hw_threads(0);
     // Counting sort of vertices by out-degree, descending, stable by id.
     std::vector<int32_t> od(static_cast<size_t>(n)), perm(static_cast<size_t>(n)), inv(static_cast<size_t>(n));
     int32_t maxd = 0;
@@ -360,96 +415,239 @@ int build_pr_layout(ghet_graph* g, hipStream_t s) {
         rp[r + 1] = rp[r] + (irp[v + 1] - irp[v]);
         odp[r] = od[v];
     }
-    // Work items: every long row (more than kTileNnz in-edges) split into
-    // kTileNnz-edge chunks, then tiles of consecutive short rows.
-    std::vector<PRItem> chunk_items, tile_items;
-    std::vector<int4> long_meta;
-    int32_t long_rows = 0;
-    for (int32_t r = 0; r < n;) {
-        const int32_t d = rp[r + 1] - rp[r];
-        if (d > kTileNnz) {
-            const int32_t first = static_cast<int32_t>(chunk_items.size());
-            const int32_t nch = static_cast<int32_t>(ceil_div(d, kTileNnz));
-            for (int32_t k = 0; k < nch; ++k) {
-                PRItem it{};
-                it.row = r;
-                it.row_end = -1;
-                it.e0 = rp[r] + k * kTileNnz;
-                it.e1 = std::min(rp[r + 1], it.e0 + kTileNnz);
-                it.chunk = first + k;
-                chunk_items.push_back(it);
-            }
-            long_meta.push_back(make_int4(r, first, nch, 0));
-            ++long_rows;
-            ++r;
-            continue;
+    // Relabeled in-lists (device permute), then every row's sources ascending
+    // on the host: hot sources first, then the cold windows in order, each a
+    // contiguous run; neighbouring lanes of a tile share cache lines.
+    int32_t* d_scratch = nullptr;
+    std::vector<int32_t> hcol(static_cast<size_t>(m));
+    if (m > 0) {
+        GHET_HIP(hipMalloc(&d_scratch, 4 * (3 * size_t(n) + 1 + size_t(m))));
+        int32_t* d_perm = d_scratch;
+        int32_t* d_inv = d_perm + n;
+        int32_t* d_rp = d_inv + n;
+        int32_t* d_col = d_rp + n + 1;
+        hipError_t e = hipMemcpyAsync(d_perm, perm.data(), 4 * size_t(n), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_inv, inv.data(), 4 * size_t(n), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_rp, rp.data(), 4 * (size_t(n) + 1), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBT / 64), 65535)));
+            hipLaunchKernelGGL(k_permute_rows, dim3(grid), dim3(kBT), 0, s, d_perm, d_inv, g->in.rowptr, g->in.col, d_rp,
+                               d_col, n);
+            e = hipGetLastError();
         }
-        const int32_t first = r;
-        int32_t nnz = 0;
-        while (r < n && r - first < kTileRows) {
-            const int32_t dr = rp[r + 1] - rp[r];
-            if (dr > kTileNnz || nnz + dr > kTileNnz) break;
-            nnz += dr;
-            ++r;
-        }
-        PRItem it{};
-        it.row = first;
-        it.row_end = r;
-        it.e0 = rp[first];
-        it.e1 = rp[r];
-        tile_items.push_back(it);
+        if (e == hipSuccess) e = hipMemcpyAsync(hcol.data(), d_col, 4 * size_t(m), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        hipFree(d_scratch);
+        GHET_HIP(e);
+        parallel_for(n, T, [&](int64_t b, int64_t e2, int) {
+            for (int64_t r = b; r < e2; ++r) std::sort(hcol.begin() + rp[r], hcol.begin() + rp[r + 1]);
+        });
     }
-    L.num_tiles = static_cast<int32_t>(tile_items.size());
-    L.num_long_rows = long_rows;
-    L.num_long_chunks = static_cast<int32_t>(chunk_items.size());
-    L.num_items = L.num_tiles + L.num_long_chunks;
-    std::vector<PRItem> all_items(chunk_items);
-    all_items.insert(all_items.end(), tile_items.begin(), tile_items.end());
+    const int32_t NW = n > H ? static_cast<int32_t>(ceil_div(n - H, W)) : 0;
+    // boundary k of row r: first edge with source >= H + k*W (k = 0..NW)
+    auto bound = [&](int32_t r, int32_t k) -> int32_t {
+        const int64_t key = int64_t(H) + int64_t(k) * W;
+        if (key >= n) return rp[r + 1];
+        return static_cast<int32_t>(std::lower_bound(hcol.begin() + rp[r], hcol.begin() + rp[r + 1],
+                                                     static_cast<int32_t>(key)) - hcol.begin());
+    };
+    auto nslots_of = [](int32_t len) -> int32_t {
+        return len > kTileNnz ? static_cast<int32_t>(ceil_div(len, kTileNnz)) : 1;
+    };
+    // Pass 1 (thread chunks of rows): hot length, slots, and per window the
+    // number of segments and cold edges of the chunk.
+    std::vector<int32_t> hot_len(static_cast<size_t>(n)), nslots(static_cast<size_t>(n));
+    std::vector<int64_t> cnt_seg(size_t(T) * std::max(NW, 1), 0), cnt_edge(size_t(T) * std::max(NW, 1), 0);
+    parallel_for(n, T, [&](int64_t b, int64_t e, int t) {
+        for (int64_t r = b; r < e; ++r) {
+            int32_t prev = bound(static_cast<int32_t>(r), 0);
+            const int32_t hl = prev - rp[r];
+            hot_len[r] = hl;
+            int32_t cold_slots = 0;
+            for (int32_t w = 0; w < NW; ++w) {
+                const int32_t nb = bound(static_cast<int32_t>(r), w + 1);
+                const int32_t len = nb - prev;
+                if (len > 0) {
+                    cold_slots += nslots_of(len);
+                    ++cnt_seg[size_t(t) * NW + w];
+                    cnt_edge[size_t(t) * NW + w] += len;
+                }
+                prev = nb;
+            }
+            if (cold_slots > 0) nslots[r] = nslots_of(hl) + cold_slots;
+            else nslots[r] = hl > kTileNnz ? nslots_of(hl) : 0;
+        }
+    });
+    // window-major numbering: segment and cold-edge bases per (window, chunk)
+    const int64_t num_hot = [&] {
+        int64_t t = 0;
+        for (int32_t r = 0; r < n; ++r) t += hot_len[r];
+        return t;
+    }();
+    std::vector<int64_t> seg_base(size_t(T) * std::max(NW, 1)), edge_base(size_t(T) * std::max(NW, 1));
+    int64_t S = 0, ce = 0;
+    for (int32_t w = 0; w < NW; ++w)
+        for (int t = 0; t < T; ++t) {
+            seg_base[size_t(t) * NW + w] = S;
+            edge_base[size_t(t) * NW + w] = ce;
+            S += cnt_seg[size_t(t) * NW + w];
+            ce += cnt_edge[size_t(t) * NW + w];
+        }
+    if (n + S >= INT32_MAX) return set_error(GHET_ERR_UNSUPPORTED, "PageRank layout: too many segments");
+    std::vector<int32_t> slot_base(static_cast<size_t>(n) + 1, 0);
+    for (int32_t r = 0; r < n; ++r) slot_base[r + 1] = slot_base[r] + nslots[r];
+    const int32_t num_slots = slot_base[n];
+    // Pass 2: hot parts row-major, cold segments window-major; offsets,
+    // slots and the combined column array.
+    std::vector<int32_t> rp_all(static_cast<size_t>(n + S) + 1), slot_all(static_cast<size_t>(n + S));
+    std::vector<int32_t> col_all(static_cast<size_t>(m));
+    {
+        int64_t acc = 0;
+        for (int32_t r = 0; r < n; ++r) {
+            rp_all[r] = static_cast<int32_t>(acc);
+            acc += hot_len[r];
+        }
+        rp_all[n] = static_cast<int32_t>(acc);
+    }
+    parallel_for(n, T, [&](int64_t b, int64_t e, int t) {
+        std::vector<int64_t> sc(seg_base.begin() + size_t(t) * std::max(NW, 1),
+                                seg_base.begin() + size_t(t) * std::max(NW, 1) + std::max(NW, 1));
+        std::vector<int64_t> ec(edge_base.begin() + size_t(t) * std::max(NW, 1),
+                                edge_base.begin() + size_t(t) * std::max(NW, 1) + std::max(NW, 1));
+        for (int64_t r = b; r < e; ++r) {
+            const int32_t hl = hot_len[r];
+            std::copy(hcol.begin() + rp[r], hcol.begin() + rp[r] + hl, col_all.begin() + rp_all[r]);
+            const bool partial = nslots[r] > 0;
+            slot_all[r] = partial ? slot_base[r] : -1;
+            int32_t next_slot = slot_base[r] + (partial ? nslots_of(hl) : 0);
+            int32_t prev = rp[r] + hl;
+            for (int32_t w = 0; w < NW; ++w) {
+                const int32_t nb = bound(static_cast<int32_t>(r), w + 1);
+                const int32_t len = nb - prev;
+                if (len > 0) {
+                    const int64_t sid = sc[w]++;
+                    const int64_t off = num_hot + ec[w];
+                    ec[w] += len;
+                    std::copy(hcol.begin() + prev, hcol.begin() + nb, col_all.begin() + off);
+                    rp_all[n + sid] = static_cast<int32_t>(off);
+                    slot_all[n + sid] = next_slot;
+                    next_slot += nslots_of(len);
+                }
+                prev = nb;
+            }
+        }
+    });
+    rp_all[n + S] = static_cast<int32_t>(m);
+    hcol.clear();
+    hcol.shrink_to_fit();
+    // Work items over the combined row space [0, n+S): chunks of long rows
+    // or segments (more than kTileNnz edges) write their slots; tiles pack
+    // consecutive short ones.  Hot items first, then one group per window.
+    std::vector<PRItem> items;
+    std::vector<int4> combine;
+    auto emit = [&](int32_t lo, int32_t hi) {  // rows [lo, hi) of the combined space
+        for (int32_t r = lo; r < hi;) {
+            const int32_t d = rp_all[r + 1] - rp_all[r];
+            if (d > kTileNnz) {
+                const int32_t nch = static_cast<int32_t>(ceil_div(d, kTileNnz));
+                for (int32_t k = 0; k < nch; ++k) {
+                    PRItem it{};
+                    it.row = r;
+                    it.row_end = -1;
+                    it.e0 = rp_all[r] + k * kTileNnz;
+                    it.e1 = std::min(rp_all[r + 1], it.e0 + kTileNnz);
+                    it.chunk = slot_all[r] + k;
+                    items.push_back(it);
+                }
+                ++r;
+                continue;
+            }
+            const int32_t first = r;
+            int32_t nnz = 0;
+            while (r < hi && r - first < kTileRows) {
+                const int32_t dr = rp_all[r + 1] - rp_all[r];
+                if (dr > kTileNnz || nnz + dr > kTileNnz) break;
+                nnz += dr;
+                ++r;
+            }
+            PRItem it{};
+            it.row = first;
+            it.row_end = r;
+            it.e0 = rp_all[first];
+            it.e1 = rp_all[r];
+            items.push_back(it);
+        }
+    };
+    emit(0, n);
+    std::vector<int64_t> wseg(static_cast<size_t>(NW) + 1, 0);
+    for (int32_t w = 0; w < NW; ++w) wseg[w + 1] = wseg[w] + [&] {
+        int64_t c = 0;
+        for (int t = 0; t < T; ++t) c += cnt_seg[size_t(t) * NW + w];
+        return c;
+    }();
+    for (int32_t w = 0; w < NW; ++w) emit(static_cast<int32_t>(n + wseg[w]), static_cast<int32_t>(n + wseg[w + 1]));
+    int32_t chunks = 0, tiles = 0;
+    for (const PRItem& it : items) (it.row_end < 0 ? chunks : tiles) += 1;
+    for (int32_t r = 0; r < n; ++r)
+        if (nslots[r] > 0) combine.push_back(make_int4(r, slot_base[r], nslots[r], 0));
+    // Every index a launch will use, checked once here: item edge ranges and
+    // rows, slots of chunks, tile rows and combine entries.
+    {
+        const int64_t rows_all = n + S;
+        bool ok = rp_all[0] == 0 && rp_all[rows_all] == m;
+        for (int64_t r = 0; ok && r < rows_all; ++r)
+            ok = rp_all[r] <= rp_all[r + 1] && slot_all[r] >= -1 && slot_all[r] < num_slots &&
+                 (r < n || slot_all[r] >= 0);
+        for (const PRItem& it : items) {
+            if (!ok) break;
+            ok = it.e0 >= 0 && it.e0 <= it.e1 && it.e1 <= m && it.row >= 0 && it.row < rows_all &&
+                 (it.row_end < 0 ? (it.chunk >= 0 && it.chunk < num_slots && it.e1 - it.e0 <= kTileNnz)
+                                 : (it.row_end > it.row && it.row_end <= rows_all && it.row_end - it.row <= kTileRows &&
+                                    it.e0 == rp_all[it.row] && it.e1 == rp_all[it.row_end] && it.e1 - it.e0 <= kTileNnz));
+        }
+        for (const int4& c : combine) {
+            if (!ok) break;
+            ok = c.x >= 0 && c.x < n && c.y >= 0 && c.z > 0 && int64_t(c.y) + c.z <= num_slots;
+        }
+        if (!ok) return set_error(GHET_ERR_INVALID, "PageRank layout failed its bounds check");
+    }
+    L.num_tiles = tiles;
+    L.num_long_chunks = chunks;
+    L.num_long_rows = static_cast<int32_t>(combine.size());
+    L.num_slots = num_slots;
+    L.num_segments = static_cast<int32_t>(S);
+    L.num_items = static_cast<int32_t>(items.size());
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t bytes = al(sizeof(PRItem) * all_items.size()) + al(sizeof(int4) * long_meta.size()) +
-                         3 * al(4 * size_t(n)) + al(4 * (size_t(n) + 1)) + al(4 * size_t(m)) + 256;
+    const size_t rows_all = static_cast<size_t>(n + S);
+    const size_t bytes = al(sizeof(PRItem) * items.size()) + al(sizeof(int4) * combine.size()) +
+                         2 * al(4 * size_t(n)) + al(4 * (rows_all + 1)) + al(4 * std::max<size_t>(rows_all, 1)) +
+                         al(4 * size_t(m)) + 256;
     GHET_HIP(hipMalloc(&L.mem, bytes));
     char* p = static_cast<char*>(L.mem);
     L.items = p;
-    p += al(sizeof(PRItem) * all_items.size());
-    auto take = [&](size_t cnt) { int32_t* q = reinterpret_cast<int32_t*>(p); p += al(4 * cnt); return q; };
+    p += al(sizeof(PRItem) * items.size());
     L.long_rows = p;
-    p += al(sizeof(int4) * long_meta.size());
+    p += al(sizeof(int4) * combine.size());
+    auto take = [&](size_t cnt) { int32_t* q = reinterpret_cast<int32_t*>(p); p += al(4 * cnt); return q; };
     L.outdeg = take(size_t(n));
     L.perm = take(size_t(n));
-    int32_t* d_inv = take(size_t(n));
-    L.rowptr = take(size_t(n) + 1);
+    L.rowptr = take(rows_all + 1);
+    L.slot = take(std::max<size_t>(rows_all, 1));
     L.col = take(size_t(m));
-    auto up = [&](int32_t* dst, const std::vector<int32_t>& src) -> int {
-        if (!src.empty()) GHET_HIP(hipMemcpyAsync(dst, src.data(), 4 * src.size(), hipMemcpyHostToDevice, s));
+    auto up = [&](void* dst, const void* src, size_t b) -> int {
+        if (b) GHET_HIP(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, s));
         return GHET_OK;
     };
-    if (!all_items.empty())
-        GHET_HIP(hipMemcpyAsync(L.items, all_items.data(), sizeof(PRItem) * all_items.size(), hipMemcpyHostToDevice, s));
-    GHET_CHECK(up(L.outdeg, odp));
-    GHET_CHECK(up(L.perm, perm));
-    GHET_CHECK(up(d_inv, inv));
-    GHET_CHECK(up(L.rowptr, rp));
-    if (!long_meta.empty())
-        GHET_HIP(hipMemcpyAsync(L.long_rows, long_meta.data(), sizeof(int4) * long_meta.size(), hipMemcpyHostToDevice, s));
-    if (n > 0) {
-        const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBT / 64), 65535)));
-        hipLaunchKernelGGL(k_permute_rows, dim3(grid), dim3(kBT), 0, s, L.perm, d_inv, g->in.rowptr, g->in.col, L.rowptr,
-                           L.col, n);
-    }
-    GHET_HIP(hipGetLastError());
+    GHET_CHECK(up(L.items, items.data(), sizeof(PRItem) * items.size()));
+    GHET_CHECK(up(L.long_rows, combine.data(), sizeof(int4) * combine.size()));
+    GHET_CHECK(up(L.outdeg, odp.data(), 4 * size_t(n)));
+    GHET_CHECK(up(L.perm, perm.data(), 4 * size_t(n)));
+    GHET_CHECK(up(L.rowptr, rp_all.data(), 4 * (rows_all + 1)));
+    GHET_CHECK(up(L.slot, slot_all.data(), 4 * rows_all));
+    GHET_CHECK(up(L.col, col_all.data(), 4 * size_t(m)));
     GHET_HIP(hipStreamSynchronize(s));
-    // Sources of every row ascending, hot (small relabeled ids) first: the
-    // lanes of a chunk then gather neighbouring contributions that share
-    // cache lines, which the memory pipeline merges into fewer requests.
-    if (m > 0) {
-        std::vector<int32_t> hcol(static_cast<size_t>(m));
-        GHET_HIP(hipMemcpy(hcol.data(), L.col, 4 * size_t(m), hipMemcpyDeviceToHost));
-        parallel_for(n, hw_threads(0), [&](int64_t b, int64_t e, int) {
-            for (int64_t r = b; r < e; ++r) std::sort(hcol.begin() + rp[r], hcol.begin() + rp[r + 1]);
-        });
-        GHET_HIP(hipMemcpy(L.col, hcol.data(), 4 * size_t(m), hipMemcpyHostToDevice));
-    }
+    L.hot = H;
+    L.window = W;
     L.built = true;
     return GHET_OK;
 }
@@ -463,7 +661,8 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     if (n == 0) return GHET_OK;
     hipStream_t s = pick_stream(g, o);
     const bool normalise = !(o && o->size >= sizeof(ghet_opts) && (o->reserved[0] & 1));
-    GHET_CHECK(build_pr_layout(g, s));
+    const int32_t split_code = (o && o->size >= sizeof(ghet_opts)) ? o->reserved[4] : 0;
+    GHET_CHECK(build_pr_layout(g, s, split_code));
     PRLayout& L = g->pr;
     // persistent grid: as many blocks as stay resident, at most one per item
     int occ = 0, cus = 0;
@@ -483,7 +682,7 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     GHET_CHECK(g->ws.get(WS_PR_RANK, 8 * size_t(n), &p)); rank = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_PR_OUT, 8 * size_t(n), &p)); out = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_PR_PART, 8 * size_t(std::max(blocks + long_blocks, init_blocks) + 1), &p)); part = static_cast<double*>(p);
-    GHET_CHECK(g->ws.get(WS_PR_LONGPART, 8 * size_t(L.num_long_chunks + 1), &p)); long_part = static_cast<double*>(p);
+    GHET_CHECK(g->ws.get(WS_PR_LONGPART, 8 * size_t(L.num_slots + 1), &p)); long_part = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_COUNTERS, 256, &p)); scal = static_cast<PRScalars*>(p);
     const double inv_n = 1.0 / n;
     const bool timing = st != nullptr;
@@ -499,12 +698,12 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
         const int last = (it == iters - 1);
         if (timing) GHET_HIP(hipEventRecord(kev[2 * it], s));
         hipLaunchKernelGGL(k_pr_iter, dim3(blocks), dim3(kBT), 0, s, static_cast<const PRItem*>(L.items), L.num_items,
-                           L.rowptr, L.col, L.outdeg, long_part, ca, cb, scal, static_cast<double>(n),
+                           L.rowptr, L.col, L.outdeg, L.slot, n, long_part, ca, cb, scal, static_cast<double>(n),
                            damping, part, rank, last);
-        if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
         hipLaunchKernelGGL(k_pr_long, dim3(long_blocks), dim3(kBT), 0, s, static_cast<const int4*>(L.long_rows),
                            L.num_long_rows, long_part, L.outdeg, cb, scal, static_cast<double>(n), damping,
                            part + blocks, rank, last);
+        if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
         hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, blocks + long_blocks, &scal->sink);
         std::swap(ca, cb);
     }

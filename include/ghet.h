@@ -186,6 +186,9 @@ typedef struct ghet_opts {
      * reserved[2]     : batched BFS pushes while frontier out-edges < m / this
      *                   (0 = default 16)
      * reserved[3]     : batched SSSP sources per pass (0 = 32, at most 32)
+     * reserved[4]     : PR source windows: 0 = automatic, else
+     *                   log2(hot sources) | log2(window) << 8 (the layout
+     *                   is rebuilt when this changes)
      * other entries must be 0                                             */
     int32_t reserved[6];
 } ghet_opts;

@@ -198,3 +198,36 @@ def test_hub_heavy_graph():
     np.testing.assert_array_equal(dg.wcc()[0], O.wcc(o.rowptr, o.col))
     s = hg.view(G.VIEW_SYM)
     np.testing.assert_array_equal(G.DeviceGraph(s, undirected=True, weights=False).tc()[0], O.tc(s.rowptr, s.col))
+
+
+@pytest.mark.parametrize("split", [(0, 1), (4, 6), (9, 10), (12, 12)])
+def test_pr_source_windows_match_oracle(medium, split):
+    """Forced hot/window splits of the relabeled sources (log2 hot sources,
+    log2 window): hot parts and cold segments become partial slots summed by
+    k_pr_long; every split must meet the same tolerance as the default."""
+    o, i, dg = medium["o"], medium["i"], medium["dg"]
+    ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 20)
+    r, st = dg.pr(20, 0.85, split=split)
+    err = np.abs(r - ref)
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (split, err.sum(), err.max())
+    r2, _ = dg.pr(20, 0.85, split=split)
+    np.testing.assert_array_equal(r, r2)  # deterministic under a split
+    r0, _ = dg.pr(20, 0.85)  # back to the automatic layout
+    err = np.abs(r0 - ref)
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS
+
+
+def test_pr_source_windows_with_long_segments():
+    """A hub whose in-list spans windows of more than 2048 sources: cold
+    segments longer than a tile are chunked into several slots."""
+    n = 20000
+    src = list(range(1, n)) + [0] * (n - 1) + [1] * 3000 + list(range(2, 3002))
+    dst = [0] * (n - 1) + list(range(1, n)) + list(range(2, 3002)) + [1] * 3000
+    hg = G.HostGraph.from_edges(n, src, dst)
+    o, i = hg.view(G.VIEW_OUT), hg.view(G.VIEW_IN)
+    dg = G.DeviceGraph(o, i)
+    ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 10)
+    for split in ((2, 13), (6, 12), (0, 14)):
+        r, _ = dg.pr(10, 0.85, split=split)
+        err = np.abs(r - ref)
+        assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (split, err.sum(), err.max())
