@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-heuristic section")
     ap.add_argument("--sched-file", default=os.path.join(ROOT, "graphhet_amd", "sched_net.txt"),
                     help="trained schedule networks (python -m graphhet_amd.learn)")
+    ap.add_argument("--gen", choices=("device", "host"), default="device",
+                    help="R-MAT generation on the GPU (default) or on the host")
     ap.add_argument("--graph-cache", default=os.environ.get("GHET_GRAPH_CACHE", ""),
                     help="directory of binary CSR caches for the R-MAT graphs (default: $GHET_GRAPH_CACHE, "
                          "empty = always generate)")
@@ -119,20 +121,15 @@ def main():
         sys.exit(1)
     algos = set(args.algos.split(","))
     t0 = time.time()
-    hg = G.HostGraph.rmat_cached(args.graph_cache, args.scale, args.edge_factor, seed=args.seed)
-    o = hg.view(G.VIEW_OUT, copy=False)
-    i = hg.view(G.VIEW_IN, copy=False)
-    n, m = o.n, o.m
-    log(f"R-MAT scale {args.scale}: n={n} m={m} built in {time.time() - t0:.1f}s")
-    t0 = time.time()
     extras = not args.no_extras
-    a = hg.view(G.VIEW_ALL, copy=False) if extras else None
-    dg = G.DeviceGraph(o, i, a, with_all=extras)
-    log(f"uploaded in {time.time() - t0:.1f}s ({dg.device_bytes() / 1e9:.2f} GB)")
+    dg, o, hg = make_graph(G, args, args.scale, args.edge_factor, with_all=extras)
+    n, m = o.n, o.m
+    log(f"R-MAT scale {args.scale}: n={n} m={m} on the GPU in {time.time() - t0:.1f}s "
+        f"({args.gen} generation, {dg.device_bytes() / 1e9:.2f} GB)")
     src = int(np.argmax(o.degrees()))
     # The step traverses with the trained schedule networks when they load
-    # (they never saw this graph: trained on scales 18-22); the heuristic
-    # ("plain") form is timed beside it.  Outputs are identical either way.
+    # (trained on other R-MAT graphs, scales 18-24); the heuristic ("plain")
+    # form is timed beside it.  Outputs are identical either way.
     sched = G.SCHED_HEURISTIC
     if not args.no_learned and G.sched_load(args.sched_file):
         sched = G.SCHED_LEARNED
@@ -186,7 +183,7 @@ def main():
                 plain[alg] = {"ms": round(ms, 3), "gteps": round(gteps(sts[0].edges_traversed, ms), 3)}
     learned = None
     if sched == G.SCHED_LEARNED:
-        learned = learned_section(G, dg, hg, src)
+        learned = learned_section(G, dg, o, src)
 
     other = {}
     if extras:
@@ -197,6 +194,8 @@ def main():
         other["cdlp_ms"] = round(st.device_ms, 3)
         other["cdlp_iter_ms"] = round(st.hot_kernel_ms / max(args.cdlp_iters, 1), 3)
         del dg
+        if hg is None:
+            hg = G.HostGraph.from_csr(o)
         s = hg.view(G.VIEW_SYM, copy=False)
         sg = G.DeviceGraph(s, undirected=True, weights=False)
         _, st = sg.tc(out=False)
@@ -210,7 +209,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and "bfs" in algos:
         cpu = cpu_baseline(o, src, bfs_stats)
-    del hg, o, i, a
+    del hg, o
 
     batched = None
     if not args.no_batched:
@@ -248,12 +247,27 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def learned_section(G, dg, hg, src):
+def make_graph(G, args, scale, edge_factor, with_all):
+    """The R-MAT input as (device graph, host out-CSR, host graph or None).
+    Device generation builds it in GPU memory (bit-identical to the host
+    generator) and downloads the out-CSR for the host-side work; host
+    generation, or a graph cache, builds and uploads it."""
+    if args.gen == "device" and not args.graph_cache:
+        dg = G.DeviceGraph.rmat(scale, edge_factor, seed=args.seed, with_all=with_all)
+        return dg, dg.download(G.VIEW_OUT), None
+    hg = G.HostGraph.rmat_cached(args.graph_cache, scale, edge_factor, seed=args.seed)
+    o = hg.view(G.VIEW_OUT, copy=False)
+    i = hg.view(G.VIEW_IN, copy=False)
+    a = hg.view(G.VIEW_ALL, copy=False) if with_all else None
+    return G.DeviceGraph(o, i, a, with_all=with_all), o, hg
+
+
+def learned_section(G, dg, o, src):
     """Learned schedule vs the heuristic (and GraphHET's n/100 rule) on this
-    graph, which the networks never saw (they are trained on scales 18-22).
-    Outputs are compared bit for bit."""
+    graph, which the networks never saw (they are trained on other R-MAT
+    graphs, scales 18-24).  Outputs are compared bit for bit."""
     from graphhet_amd import learn
-    srcs = np.concatenate([[src], learn.pick_sources(hg, 8, 24)]).astype(np.int32)
+    srcs = np.concatenate([[src], learn.pick_sources(o, 8, 24)]).astype(np.int32)
     learn.compare_bfs(dg, srcs[:2])  # warm-up
     b = learn.compare_bfs(dg, srcs)
     s = learn.compare_sssp(dg, srcs[:4])
@@ -273,10 +287,7 @@ def batched_section(G, args):
     """Batched forms vs the plain form on the same GPU (same inputs, results
     compared on a sample of instances)."""
     res = {}
-    hg = G.HostGraph.rmat_cached(args.graph_cache, args.batched_scale, 16, seed=args.seed)
-    o = hg.view(G.VIEW_OUT, copy=False)
-    i = hg.view(G.VIEW_IN, copy=False)
-    dg = G.DeviceGraph(o, i, with_all=False)
+    dg, o, hg = make_graph(G, args, args.batched_scale, 16, with_all=False)
     indeg = np.bincount(o.col, minlength=o.n)
     cand = np.nonzero((o.degrees() + indeg) > 0)[0]
     rng = np.random.default_rng(args.seed)

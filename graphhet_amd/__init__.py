@@ -76,9 +76,9 @@ class DevicePtrs(C.Structure):
 EXPORTS = [
     "ghet_last_error", "ghet_abi_version", "ghet_device_count", "ghet_device_mem_info",
     "ghet_gen_rmat", "ghet_gen_er", "ghet_host_graph_from_edges", "ghet_host_graph_view", "ghet_host_graph_free",
-    "ghet_host_graph_save", "ghet_host_graph_load",
+    "ghet_host_graph_save", "ghet_host_graph_load", "ghet_host_graph_from_csr",
     "ghet_graph_create", "ghet_graph_wrap_device", "ghet_graph_destroy", "ghet_graph_device_ptrs",
-    "ghet_graph_device_bytes",
+    "ghet_graph_device_bytes", "ghet_graph_gen_rmat", "ghet_graph_download",
     "ghet_bfs", "ghet_bfs_pre", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_cdlp", "ghet_tc",
     "ghet_bfs_multi", "ghet_sssp_multi", "ghet_wcc_many", "ghet_tc_many",
     "ghet_sched_load", "ghet_sched_save", "ghet_sched_clear", "ghet_sched_set_params", "ghet_sched_get_params",
@@ -110,6 +110,10 @@ def lib() -> C.CDLL:
         "ghet_gen_er": (C.c_int, [C.c_int32, C.c_int64, C.c_uint64, C.c_int, P(vp)]),
         "ghet_host_graph_from_edges": (C.c_int, [C.c_int32, C.c_int64, i32p, i32p, f64p, C.c_int, P(vp)]),
         "ghet_host_graph_save": (C.c_int, [vp, C.c_char_p]),
+        "ghet_host_graph_from_csr": (C.c_int, [C.c_int32, i32p, i32p, f64p, C.c_int, P(vp)]),
+        "ghet_graph_gen_rmat": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                          C.c_uint32, P(vp)]),
+        "ghet_graph_download": (C.c_int, [vp, C.c_int, i32p, i32p, f64p]),
         "ghet_host_graph_load": (C.c_int, [C.c_char_p, C.c_int, P(vp)]),
         "ghet_host_graph_view": (C.c_int, [vp, C.c_int, P(CsrView)]),
         "ghet_host_graph_free": (None, [vp]),
@@ -210,6 +214,17 @@ class HostGraph:
         h = C.c_void_p()
         _check(lib().ghet_host_graph_from_edges(n, len(src), _ptr(src, C.c_int32), _ptr(dst, C.c_int32),
                                                 _ptr(wa, C.c_double), threads, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, csr: Csr, threads: int = 0) -> "HostGraph":
+        """From CSR arrays (e.g. DeviceGraph.download()); canonical rows are adopted."""
+        rp = np.ascontiguousarray(csr.rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(csr.col, dtype=np.int32)
+        w = None if csr.w is None else np.ascontiguousarray(csr.w, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib().ghet_host_graph_from_csr(csr.n, _ptr(rp, C.c_int32), _ptr(col, C.c_int32), _ptr(w, C.c_double),
+                                              threads, C.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -330,6 +345,32 @@ class DeviceGraph:
         inn = hg.view(VIEW_IN, copy=False)
         all_ = hg.view(VIEW_ALL, copy=False) if with_all else None
         return cls(out, inn, all_, weights=weights, with_all=with_all)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: float = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1, weights: bool = True, with_in: bool = True, with_all: bool = True) -> "DeviceGraph":
+        """R-MAT generated on the GPU: the graph of HostGraph.rmat with the same
+        arguments, bit for bit, without a host copy."""
+        flags = (0 if weights else GRAPH_NO_WEIGHTS) | (0 if with_in else GRAPH_NO_IN) | \
+                (0 if with_all else GRAPH_NO_ALL)
+        h = C.c_void_p()
+        _check(lib().ghet_graph_gen_rmat(scale, edge_factor, a, b, c, seed, flags, C.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.n = 1 << scale
+        return self
+
+    def download(self, kind: int = VIEW_OUT, weights: bool = True) -> Csr:
+        """Copies a device CSR (VIEW_OUT, VIEW_IN or VIEW_ALL) to host arrays."""
+        p = self.device_ptrs()
+        m = {VIEW_OUT: p.m_out, VIEW_IN: p.m_in, VIEW_ALL: p.m_all}[kind]
+        rp = np.empty(p.n + 1, np.int32)
+        col = np.empty(m, np.int32)
+        has_w = weights and kind != VIEW_ALL and bool(p.out_w if kind == VIEW_OUT else p.in_w)
+        w = np.empty(m, np.float64) if has_w else None
+        _check(lib().ghet_graph_download(self._h, kind, _ptr(rp, C.c_int32), _ptr(col, C.c_int32),
+                                         _ptr(w, C.c_double)))
+        return Csr(p.n, rp, col, w)
 
     @property
     def handle(self):

@@ -131,6 +131,57 @@ void destroy_graph(ghet_graph* g) {
 }
 
 }  // namespace
+
+// A directed graph owning an arena laid out as ghet_graph_create lays it out
+// (out-CSR and weights, in-CSR and weights, all-CSR; absent parts have size
+// 0 and null pointers).  The caller fills the device arrays and the host
+// copies of the offsets.
+int graph_alloc_arena(int32_t n, int64_t m_out, bool want_in, int64_t m_in, bool want_all, int64_t m_all,
+                      bool want_w, ghet_graph** out) {
+    *out = nullptr;
+    auto* g = new ghet_graph();
+    g->n = n;
+    g->directed = true;
+    int st = init_runtime(g);
+    if (st != GHET_OK) {
+        destroy_graph(g);
+        return st;
+    }
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    const size_t o_or = take(4 * (size_t(n) + 1)), o_oc = take(4 * size_t(m_out));
+    const size_t o_ow = want_w ? take(8 * size_t(m_out)) : 0;
+    size_t o_ir = 0, o_ic = 0, o_iw = 0, o_ar = 0, o_ac = 0;
+    if (want_in) {
+        o_ir = take(4 * (size_t(n) + 1));
+        o_ic = take(4 * size_t(m_in));
+        if (want_w) o_iw = take(8 * size_t(m_in));
+    }
+    if (want_all) {
+        o_ar = take(4 * (size_t(n) + 1));
+        o_ac = take(4 * size_t(m_all));
+    }
+    hipError_t e = hipMalloc(&g->arena, std::max<size_t>(off, 256));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        destroy_graph(g);
+        return set_error(GHET_ERR_NOMEM, "device arena of %zu bytes: %s", off, hipGetErrorString(e));
+    }
+    g->arena_bytes = off;
+    char* base = static_cast<char*>(g->arena);
+    g->out = {n, m_out, reinterpret_cast<int32_t*>(base + o_or), reinterpret_cast<int32_t*>(base + o_oc),
+              want_w ? reinterpret_cast<double*>(base + o_ow) : nullptr};
+    if (want_in)
+        g->in = {n, m_in, reinterpret_cast<int32_t*>(base + o_ir), reinterpret_cast<int32_t*>(base + o_ic),
+                 want_w ? reinterpret_cast<double*>(base + o_iw) : nullptr};
+    if (want_all)
+        g->all = {n, m_all, reinterpret_cast<int32_t*>(base + o_ar), reinterpret_cast<int32_t*>(base + o_ac), nullptr};
+    *out = g;
+    return GHET_OK;
+}
+
+void graph_free(ghet_graph* g) { destroy_graph(g); }
+
 }  // namespace ghet
 
 using namespace ghet;

@@ -529,6 +529,29 @@ struct MappedFile {
     }
 };
 
+// Rows with unsorted or repeated targets: rebuilt as an edge list in row
+// order (sorted, the later entry's weight kept), like
+// ghet_host_graph_from_edges.
+int rebuild_rows(ghet_host_graph* g) {
+    const int32_t n = g->n;
+    const int64_t m = g->out_rowptr[n];
+    const int32_t* rp = g->out_rowptr.data();
+    std::vector<uint64_t> pairs(static_cast<size_t>(m));
+    parallel_for(n, g->threads, [&](int64_t b, int64_t e, int) {
+        for (int64_t u = b; u < e; ++u)
+            for (int32_t k = rp[u]; k < rp[u + 1]; ++k)
+                pairs[k] = (static_cast<uint64_t>(u) << 32) | static_cast<uint32_t>(g->out_col[k]);
+    });
+    std::vector<double> w = std::move(g->out_w);
+    std::vector<int64_t> offs;
+    std::vector<uint64_t> keys;
+    int st = bucket_by_source(g, pairs, offs, keys);
+    if (st != GHET_OK) return st;
+    pairs.clear();
+    pairs.shrink_to_fit();
+    return finish_rows(g, offs, keys, [&](uint32_t k) { return w[k]; });
+}
+
 template <typename T>
 int save_rows(const std::string& path, int32_t n, const int32_t* rowptr, const T* data) {
     const std::string tmp = path + ".tmp." + std::to_string(static_cast<long long>(getpid()));
@@ -637,21 +660,47 @@ int ghet_host_graph_load(const char* path, int threads, ghet_host_graph** out) {
     if (bad_w.load() >= 0)
         return set_error(GHET_ERR_IO, "Unable to read file %s.w (row %d length differs)", path, bad_w.load());
     if (!canonical.load()) {
-        // unsorted or repeated targets: rebuild as an edge list in file order
-        std::vector<uint64_t> pairs(static_cast<size_t>(m));
-        parallel_for(n, T, [&](int64_t b, int64_t e, int) {
-            for (int64_t u = b; u < e; ++u)
-                for (int32_t k = rp[u]; k < rp[u + 1]; ++k)
-                    pairs[k] = (static_cast<uint64_t>(u) << 32) | static_cast<uint32_t>(g->out_col[k]);
-        });
-        std::vector<double> w = std::move(g->out_w);
-        std::vector<int64_t> offs;
-        std::vector<uint64_t> keys;
-        int st = bucket_by_source(g, pairs, offs, keys);
+        const int st = rebuild_rows(g);
         if (st != GHET_OK) return st;
-        pairs.clear();
-        pairs.shrink_to_fit();
-        st = finish_rows(g, offs, keys, [&](uint32_t k) { return w[k]; });
+    }
+    *out = guard.release();
+    return GHET_OK;
+}
+
+int ghet_host_graph_from_csr(int32_t n, const int32_t* rowptr, const int32_t* col, const double* w, int threads,
+                             ghet_host_graph** out) {
+    clear_error();
+    if (!out) return set_error(GHET_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (n < 0 || !rowptr) return set_error(GHET_ERR_INVALID, "bad CSR");
+    if (rowptr[0] != 0) return set_error(GHET_ERR_INVALID, "rowptr[0] = %d, expected 0", rowptr[0]);
+    for (int64_t u = 0; u < n; ++u)
+        if (rowptr[u + 1] < rowptr[u]) return set_error(GHET_ERR_INVALID, "offsets not monotone at vertex %lld", (long long)u);
+    const int64_t m = rowptr[n];
+    if (m > 0 && !col) return set_error(GHET_ERR_INVALID, "col is NULL");
+    auto* g = new (std::nothrow) ghet_host_graph();
+    if (!g) return set_error(GHET_ERR_NOMEM, "host allocation failed");
+    std::unique_ptr<ghet_host_graph> guard(g);
+    g->n = n;
+    g->threads = hw_threads(threads);
+    g->out_rowptr.assign(rowptr, rowptr + n + 1);
+    g->out_col.assign(col, col + m);
+    if (w) g->out_w.assign(w, w + m);
+    else g->out_w.assign(static_cast<size_t>(m), 1.0);
+    std::atomic<int64_t> bad{-1};
+    std::atomic<bool> canonical{true};
+    parallel_for(n, g->threads, [&](int64_t b, int64_t e, int) {
+        bool canon = true;
+        for (int64_t u = b; u < e; ++u)
+            for (int32_t k = rowptr[u]; k < rowptr[u + 1]; ++k) {
+                if (col[k] < 0 || col[k] >= n) bad.store(k);
+                if (k > rowptr[u] && col[k] <= col[k - 1]) canon = false;
+            }
+        if (!canon) canonical.store(false);
+    });
+    if (bad.load() >= 0) return set_error(GHET_ERR_INVALID, "column index out of range at edge %lld", (long long)bad.load());
+    if (!canonical.load()) {
+        const int st = rebuild_rows(g);
         if (st != GHET_OK) return st;
     }
     *out = guard.release();

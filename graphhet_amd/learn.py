@@ -45,9 +45,10 @@ def rmat_params(a: float) -> tuple[float, float, float]:
     return a, bc, bc
 
 
-def pick_sources(hg: HostGraph, k: int, seed: int) -> np.ndarray:
-    """k distinct sources among vertices with at least one out-edge."""
-    o = hg.view(0, copy=False)
+def pick_sources(hg, k: int, seed: int) -> np.ndarray:
+    """k distinct sources among vertices with at least one out-edge (hg: a
+    HostGraph or an out-CSR)."""
+    o = hg.view(0, copy=False) if isinstance(hg, HostGraph) else hg
     cand = np.nonzero(o.degrees() > 0)[0]
     rng = np.random.default_rng(seed)
     return rng.choice(cand, size=min(k, len(cand)), replace=False).astype(np.int32)

@@ -85,6 +85,11 @@ int ghet_gen_er(int32_t n, int64_t m, uint64_t seed, int threads, ghet_host_grap
 /* Build from an edge list (duplicates: later entry's weight wins; self-loops kept). */
 int ghet_host_graph_from_edges(int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
                                const double* w, int threads, ghet_host_graph** out);
+/* Build from a CSR (row offsets n+1, targets, optional weights): rows already
+ * ascending without duplicates are adopted; others are canonicalised as
+ * ghet_host_graph_from_edges does (the later entry's weight wins). */
+int ghet_host_graph_from_csr(int32_t n, const int32_t* rowptr, const int32_t* col, const double* w,
+                             int threads, ghet_host_graph** out);
 /* Views are built on first request and owned by the graph. */
 int ghet_host_graph_view(ghet_host_graph* g, int kind, ghet_csr_view* out);
 void ghet_host_graph_free(ghet_host_graph* g);
@@ -148,6 +153,20 @@ typedef struct ghet_device_ptrs {
 int ghet_graph_device_ptrs(const ghet_graph* g, ghet_device_ptrs* out);
 /* Device bytes held by the graph (CSR + layouts + workspace). */
 int64_t ghet_graph_device_bytes(const ghet_graph* g);
+/* On-GPU R-MAT: the graph ghet_gen_rmat(scale, edge_factor, a, b, c, seed)
+ * followed by ghet_graph_create would give, bit for bit (same samples, loops
+ * dropped, duplicates merged with the latest sample's weight, in-rows in
+ * ascending source order, all-rows = in-list then out-list), generated in
+ * device memory without a host copy.  flags: GHET_GRAPH_NO_WEIGHTS,
+ * GHET_GRAPH_NO_IN, GHET_GRAPH_NO_ALL (UNDIRECTED is rejected).  No reference
+ * counterpart: the reference generates and converts on the host
+ * (CPU_adj_list.hpp:110-122, GPU_csr.hpp:282-339). */
+int ghet_graph_gen_rmat(int scale, double edge_factor, double a, double b, double c, uint64_t seed,
+                        uint32_t flags, ghet_graph** out);
+/* Copies a device CSR (GHET_VIEW_OUT, _IN or _ALL) into caller buffers of
+ * n+1 offsets and m entries; any pointer may be NULL; weights exist for OUT
+ * and IN only.  Sizes: ghet_graph_device_ptrs. */
+int ghet_graph_download(const ghet_graph* g, int kind, int32_t* rowptr, int32_t* col, double* w);
 
 /* ------------------------------------------------------------------------ */
 /* Options and statistics                                                    */

@@ -128,3 +128,26 @@ def test_header_and_library_interoperate(tmp_path):
     out = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "cache selftest ok" in out.stdout
+
+
+def test_device_generator_rejects_bad_arguments_on_the_host():
+    """Argument checks run before any device work (so they hold without a GPU)."""
+    for args in ((0, 16, 0.57, 0.19, 0.19, 1, 0), (10, -1, 0.57, 0.19, 0.19, 1, 0),
+                 (10, 16, 0.7, 0.2, 0.2, 1, 0), (10, 16, 0.57, 0.19, 0.19, 1, G.GRAPH_UNDIRECTED)):
+        h = G.C.c_void_p()
+        assert G.lib().ghet_graph_gen_rmat(*args, G.C.byref(h)) == G.ERR_INVALID
+
+
+def test_from_csr_adopts_canonical_rows_and_canonicalises_others():
+    hg = G.HostGraph.rmat(11, 8, seed=6)
+    o = hg.view(G.VIEW_OUT)
+    _same(G.HostGraph.from_csr(o).view(G.VIEW_OUT), o)
+    _same(G.HostGraph.from_csr(o, threads=2).view(G.VIEW_SYM), hg.view(G.VIEW_SYM))
+    # unsorted row with a repeat: row 0 = [2, 1, 2] -> [1, 2] with the later weight of 2
+    c = G.Csr(3, np.array([0, 3, 3, 4], np.int32), np.array([2, 1, 2, 0], np.int32), np.array([0.1, 0.2, 0.3, 0.4]))
+    g = G.HostGraph.from_csr(c).view(G.VIEW_OUT)
+    np.testing.assert_array_equal(g.rowptr, [0, 2, 2, 3])
+    np.testing.assert_array_equal(g.col, [1, 2, 0])
+    np.testing.assert_array_equal(g.w, [0.2, 0.3, 0.4])
+    with pytest.raises(G.GhetError):
+        G.HostGraph.from_csr(G.Csr(2, np.array([0, 1, 1], np.int32), np.array([5], np.int32), None))
