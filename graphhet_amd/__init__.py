@@ -79,6 +79,8 @@ EXPORTS = [
     "ghet_host_graph_save", "ghet_host_graph_load", "ghet_host_graph_from_csr",
     "ghet_graph_create", "ghet_graph_wrap_device", "ghet_graph_destroy", "ghet_graph_device_ptrs",
     "ghet_graph_device_bytes", "ghet_graph_gen_rmat", "ghet_graph_download",
+    "ghet_dyn_create", "ghet_dyn_from_graph", "ghet_dyn_update", "ghet_dyn_edges", "ghet_dyn_snapshot",
+    "ghet_dyn_destroy",
     "ghet_bfs", "ghet_bfs_pre", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_cdlp", "ghet_tc",
     "ghet_bfs_multi", "ghet_sssp_multi", "ghet_wcc_many", "ghet_tc_many",
     "ghet_sched_load", "ghet_sched_save", "ghet_sched_clear", "ghet_sched_set_params", "ghet_sched_get_params",
@@ -114,6 +116,12 @@ def lib() -> C.CDLL:
         "ghet_graph_gen_rmat": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                           C.c_uint32, P(vp)]),
         "ghet_graph_download": (C.c_int, [vp, C.c_int, i32p, i32p, f64p]),
+        "ghet_dyn_create": (C.c_int, [C.c_int32, i32p, i32p, f64p, C.c_int, P(vp)]),
+        "ghet_dyn_from_graph": (C.c_int, [vp, P(vp)]),
+        "ghet_dyn_update": (C.c_int, [vp, C.c_int64, i32p, i32p, f64p, vp]),
+        "ghet_dyn_edges": (C.c_int64, [vp]),
+        "ghet_dyn_snapshot": (C.c_int, [vp, C.c_uint32, P(vp)]),
+        "ghet_dyn_destroy": (None, [vp]),
         "ghet_host_graph_load": (C.c_int, [C.c_char_p, C.c_int, P(vp)]),
         "ghet_host_graph_view": (C.c_int, [vp, C.c_int, P(CsrView)]),
         "ghet_host_graph_free": (None, [vp]),
@@ -492,6 +500,68 @@ class DeviceGraph:
         _check(lib().ghet_sssp_multi(self._h, len(srcs), _ptr(srcs, C.c_int32), inf, _ptr(dist, C.c_double),
                                      C.byref(o), C.byref(st)))
         return dist, st
+
+
+class DynGraph:
+    """A dynamic graph on the GPU (ghet_dyn handle): batched inserts, weight
+    changes and deletes (weight 0), and CSR snapshots for the analytics."""
+
+    def __init__(self, handle, n: int):
+        self._h = handle
+        self.n = n
+
+    @classmethod
+    def create(cls, csr: Csr, threads: int = 0) -> "DynGraph":
+        rp = np.ascontiguousarray(csr.rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(csr.col, dtype=np.int32)
+        w = None if csr.w is None else np.ascontiguousarray(csr.w, dtype=np.float64)
+        h = C.c_void_p()
+        _check(lib().ghet_dyn_create(csr.n, _ptr(rp, C.c_int32), _ptr(col, C.c_int32), _ptr(w, C.c_double), threads,
+                                     C.byref(h)))
+        return cls(h, csr.n)
+
+    @classmethod
+    def from_graph(cls, dg: "DeviceGraph") -> "DynGraph":
+        h = C.c_void_p()
+        _check(lib().ghet_dyn_from_graph(dg.handle, C.byref(h)))
+        return cls(h, dg.n)
+
+    def update(self, src, dst, w=None) -> RunStats:
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        dst = np.ascontiguousarray(dst, dtype=np.int32)
+        wa = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+        st = RunStats()
+        _check(lib().ghet_dyn_update(self._h, len(src), _ptr(src, C.c_int32), _ptr(dst, C.c_int32),
+                                     _ptr(wa, C.c_double), C.byref(st)))
+        return st
+
+    def delete(self, src, dst) -> RunStats:
+        return self.update(src, dst, np.zeros(len(src)))
+
+    @property
+    def m(self) -> int:
+        return lib().ghet_dyn_edges(self._h)
+
+    def snapshot(self, weights: bool = True, with_in: bool = True, with_all: bool = True) -> "DeviceGraph":
+        flags = (0 if weights else GRAPH_NO_WEIGHTS) | (0 if with_in else GRAPH_NO_IN) | \
+                (0 if with_all else GRAPH_NO_ALL)
+        h = C.c_void_p()
+        _check(lib().ghet_dyn_snapshot(self._h, flags, C.byref(h)))
+        g = DeviceGraph.__new__(DeviceGraph)
+        g._h = h
+        g.n = self.n
+        return g
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().ghet_dyn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def pack_graphs(csrs: list[Csr]):

@@ -183,6 +183,16 @@ struct DevBuf {
 };
 
 }  // namespace
+
+// all-CSR (per vertex its in-list, then its out-list) from device out- and
+// in-CSRs, on stream s.
+void launch_all_csr(const int32_t* orp, const int32_t* ocol, const int32_t* irp, const int32_t* icol, int32_t n,
+                    int32_t* arp, int32_t* acol, hipStream_t s) {
+    hipLaunchKernelGGL(k_all_rowptr, dim3(grid_for(int64_t(n) + 1)), dim3(256), 0, s, orp, irp, n, arp);
+    hipLaunchKernelGGL(k_all_fill, dim3(grid_for(int64_t(n) * 64)), dim3(256), 0, s, orp, ocol, irp, icol, arp, n,
+                       acol);
+}
+
 }  // namespace ghet
 
 using namespace ghet;
@@ -297,10 +307,7 @@ int ghet_graph_gen_rmat(int scale, double edge_factor, double a, double b, doubl
     }
     // 6. all-CSR
     if (want_all) {
-        hipLaunchKernelGGL(k_all_rowptr, dim3(grid_for(int64_t(n) + 1)), dim3(256), 0, s, g->out.rowptr,
-                           g->in.rowptr, n, g->all.rowptr);
-        hipLaunchKernelGGL(k_all_fill, dim3(grid_for(int64_t(n) * 64)), dim3(256), 0, s, g->out.rowptr, g->out.col,
-                           g->in.rowptr, g->in.col, g->all.rowptr, n, g->all.col);
+        launch_all_csr(g->out.rowptr, g->out.col, g->in.rowptr, g->in.col, n, g->all.rowptr, g->all.col, s);
         GHET_HIP(hipGetLastError());
     }
     // host copies of the offsets (the layout builders read them)

@@ -168,6 +168,7 @@ int ghet_graph_gen_rmat(int scale, double edge_factor, double a, double b, doubl
  * and IN only.  Sizes: ghet_graph_device_ptrs. */
 int ghet_graph_download(const ghet_graph* g, int kind, int32_t* rowptr, int32_t* col, double* w);
 
+
 /* ------------------------------------------------------------------------ */
 /* Options and statistics                                                    */
 /* ------------------------------------------------------------------------ */
@@ -324,6 +325,35 @@ int ghet_mlp_train_step(int kind, int32_t batch, const float* x, const int32_t* 
 /* Autotuner: times candidate block shapes for the batched launches and the
  * SSSP bucket width on graph g; results are cached in `cache_path` (may be NULL). */
 int ghet_autotune(ghet_graph* g, const char* cache_path, int32_t* best_block_out);
+
+/* ------------------------------------------------------------------------ */
+/* Dynamic graphs                                                            */
+/* ------------------------------------------------------------------------ */
+/* The role of GraphHET's GPMA+ backend (GPU_gpma.hpp: class GPMA 42-73,
+ * init_csr_gpma 928-941, init_gpma_from_csr 882-926, update_gpma 756-824):
+ * a device edge store that takes batches of (src, dst, weight) updates with
+ * GPMA's meaning -- weight 0 deletes the edge (absent: no-op), an existing
+ * edge takes the new weight, any other edge is inserted -- and hands the
+ * analytics a CSR.  Within one batch the last occurrence of an edge wins
+ * (GPMA leaves repeated keys in one batch undefined).  Weight 0 cannot be
+ * stored: ghet_dyn_create drops such edges and ghet_dyn_from_graph rejects
+ * a graph that has them. */
+typedef struct ghet_dyn ghet_dyn;
+/* From a host CSR (rows canonicalised as ghet_host_graph_from_csr does). */
+int ghet_dyn_create(int32_t n, const int32_t* rowptr, const int32_t* col, const double* w, int threads,
+                    ghet_dyn** out);
+/* From a device graph with its in-CSR (no host copy). */
+int ghet_dyn_from_graph(const ghet_graph* g, ghet_dyn** out);
+/* Applies one batch (w NULL: every weight 1.0).  stats: device_ms,
+ * edges_traversed = distinct updates, levels = pool compactions, extra[0..3]
+ * = inserted, deleted, modified, relocated slots. */
+int ghet_dyn_update(ghet_dyn* d, int64_t count, const int32_t* src, const int32_t* dst, const double* w,
+                    ghet_run_stats* stats);
+int64_t ghet_dyn_edges(const ghet_dyn* d);
+/* A ghet_graph holding the current edges (flags as ghet_graph_create:
+ * NO_WEIGHTS, NO_IN, NO_ALL); every analytics entry point runs on it. */
+int ghet_dyn_snapshot(ghet_dyn* d, uint32_t flags, ghet_graph** out);
+void ghet_dyn_destroy(ghet_dyn* d);
 
 #ifdef __cplusplus
 }
