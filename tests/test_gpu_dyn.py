@@ -148,3 +148,20 @@ def test_from_device_graph_and_no_op_batches():
         dyn.update(np.array([0], np.int32), np.array([o.n], np.int32))
     with pytest.raises(G.GhetError):
         dyn.update(np.array([0], np.int32), np.array([1], np.int32), np.array([-1.0]))
+
+
+def test_gpma_drop_in_driver():
+    """graphhet_amd/drivers/GPU_gpma_example.cpp: the reference's GPMA example
+    on the drop-in headers (include/GPU_gpma), then batches of update_gpma
+    checked against the CSR drop-in on a graph_structure with the same
+    changes, for all six analytics."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(G.LIB_PATH), "..", "bin", "GPU_gpma_example")
+    if not os.path.exists(exe):
+        pytest.skip("drivers not built")
+    out = subprocess.run([exe, "--check"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "gpma selftest ok" in out.stdout
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and "Community Detection result:" in out.stdout, out.stdout + out.stderr
