@@ -90,6 +90,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip WCC/CDLP/TC")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain section")
     ap.add_argument("--batched-scale", type=int, default=22)
+    ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic-graph update section")
     ap.add_argument("--batch-sources", type=int, default=256)
     ap.add_argument("--many-graphs", type=int, default=1024)
     ap.add_argument("--many-max-scale", type=int, default=14,
@@ -214,6 +215,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_section(G, args)
+    dynamic = None
+    if not args.no_dynamic:
+        dynamic = dynamic_section(G, args)
 
     out = {
         "metric": "GTEPS (BFS/SSSP) + PR HBM GB/s, scale-24 R-MAT; learned/batched vs plain",
@@ -241,10 +245,42 @@ def main():
         "other_algorithms": other,
         "learned": learned,
         "batched": batched,
+        "dynamic": dynamic,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def dynamic_section(G, args):
+    """Dynamic store on the batched-section graph: batches of new random edges
+    inserted, then the same edges deleted (as GraphHET's GPMA benchmark does
+    with 1000-edge batches), and a snapshot of the whole graph.  Update rates
+    count distinct edge updates per second of device time (upload of the
+    batch included)."""
+    res = {"graph": f"rmat-s{args.batched_scale}-ef16"}
+    dg = G.DeviceGraph.rmat(args.batched_scale, 16, seed=args.seed, with_all=False)
+    dyn = G.DynGraph.from_graph(dg)
+    del dg
+    n = 1 << args.batched_scale
+    rng = np.random.default_rng(args.seed + 7)
+    for size in (1000, 65536, 1 << 20):
+        s = rng.integers(0, n, size).astype(np.int32)
+        d = rng.integers(0, n, size).astype(np.int32)
+        w = rng.random(size) + 1e-3
+        ins = dyn.update(s, d, w)
+        dele = dyn.delete(s, d)
+        res[f"batch_{size}"] = {"insert_ms": round(ins.device_ms, 3), "delete_ms": round(dele.device_ms, 3),
+                                "inserted": int(ins.extra[0]), "deleted": int(dele.extra[1]),
+                                "insert_mups": round(ins.edges_traversed / (ins.device_ms * 1e3), 2),
+                                "delete_mups": round(dele.edges_traversed / (dele.device_ms * 1e3), 2)}
+        log(f"dynamic batch {size}: {res[f'batch_{size}']}")
+    t0 = time.perf_counter()
+    g = dyn.snapshot(with_all=False)
+    res["snapshot_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+    res["edges"] = dyn.m
+    del g, dyn
+    return res
 
 
 def make_graph(G, args, scale, edge_factor, with_all):
