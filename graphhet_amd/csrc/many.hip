@@ -22,7 +22,7 @@
 #include "graph_internal.hpp"
 
 namespace ghet {
-int sssp_base_delta(ghet_graph* g, double* out);
+int sssp_base_delta(ghet_graph* g, double* out, bool size_term);
 namespace {
 
 __global__ void k_localize(int32_t* __restrict__ lab, int32_t n, const int32_t* __restrict__ voff, int32_t count) {
@@ -355,7 +355,7 @@ int ghet_autotune(ghet_graph* g, const char* cache_path, int32_t* best_block_out
             double best = 1e300;
             g->tune.sssp_class = 1;
             double base0 = 0;
-            GHET_CHECK(sssp_base_delta(g, &base0));
+            GHET_CHECK(sssp_base_delta(g, &base0, true));
             for (int cls = 0; cls < 4; ++cls) {
                 ghet_opts o{};
                 o.size = sizeof(o);

@@ -34,7 +34,7 @@
 
 namespace ghet {
 
-int sssp_base_delta(ghet_graph* g, double* out);
+int sssp_base_delta(ghet_graph* g, double* out, bool size_term);
 int build_sssp_layout(ghet_graph* g);
 
 namespace {
@@ -373,12 +373,13 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     if (k == 0 || n == 0) return GHET_OK;
     hipStream_t s = pick_stream(g, o);
     const bool have_opts = o && o->size >= sizeof(ghet_opts);
-    // A round carries up to 32 sources, so buckets twice the single-source
-    // base width fill the chip (measured best at scale 22, 256 sources).
+    // A round carries up to 32 sources, so twice the single-source base
+    // width without its small-graph term fills the chip (measured best at
+    // scale 22, 256 sources).
     double delta = 0;
     if (have_opts && o->sssp_delta > 0) delta = o->sssp_delta;
     else {
-        GHET_CHECK(sssp_base_delta(g, &delta));
+        GHET_CHECK(sssp_base_delta(g, &delta, false));
         delta *= 2.0;
     }
     // Sources per pass: up to 32 (the mask width), or the caller's choice.

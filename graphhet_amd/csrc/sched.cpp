@@ -152,11 +152,21 @@ int sched_pick_bfs(SchedState* s, const LevelInfo& li) {
     }
 }
 
+// The SSSP network sees only scale-free features: the absolute sizes (log
+// frontier, frontier edges, unvisited, unvisited in-edges, frontier / n, log
+// n) are zeroed, so that a network trained on small graphs transfers to large
+// ones; the base width's size term carries the scale.  The trainer applies
+// the same mask (graphhet_amd/learn.py, SSSP_MASKED).
+void sssp_mask_features(float* f) {
+    for (int k : {0, 1, 2, 3, 4, 13}) f[k] = 0.f;
+}
+
 int sched_pick_sssp(SchedState* s, const LevelInfo& li) {
     if (s->schedule == GHET_SCHED_LEARNED && s->use_net) {
         if (s->sssp_cls < 0) {
             float x[GHET_TRACE_FEATURES], y[GHET_MLP_OUT];
             sched_features(li, x);
+            sssp_mask_features(x);
             net_forward_host(GHET_NET_SSSP, x, y);
             int best = 0;
             for (int k = 1; k < GHET_MLP_OUT; ++k)

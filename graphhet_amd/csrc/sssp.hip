@@ -384,11 +384,12 @@ __global__ void __launch_bounds__(256) k_wsum(const double* __restrict__ w, int6
 
 // Base bucket width: 4 mean weights per mean out-degree, so a bucket spans
 // about four light edges of an average row; light/heavy splitting keeps the
-// rest of each row out of the bucket's rounds.  Graphs below 2^22 vertices
-// get wider buckets (x2 per factor 4 in n, at most x4): their buckets hold
-// too little work to pay a pass's fixed cost.  The autotuner's class scales
+// rest of each row out of the bucket's rounds.  With size_term, graphs below
+// 2^24 vertices get wider buckets, x2^(0.8 (24 - log2 n)) up to x64: their
+// buckets hold too little work to pay a pass's fixed cost (fitted to the
+// best measured widths on R-MAT scales 18-24).  The autotuner's class scales
 // it by 2^(class-1).  The mean weight is reduced once per graph.
-int sssp_base_delta(ghet_graph* g, double* out) {
+int sssp_base_delta(ghet_graph* g, double* out, bool size_term) {
     if (g->mean_w < 0) {
         const int64_t m = g->out.m;
         double mean = 1.0;
@@ -410,7 +411,8 @@ int sssp_base_delta(ghet_graph* g, double* out) {
         g->mean_w = mean > 0 ? mean : 1.0;
     }
     const double avg_deg = g->n > 0 ? static_cast<double>(g->out.m) / g->n : 1.0;
-    const double small = std::min(2.0, std::max(0.0, (22.0 - std::log2(std::max<double>(g->n, 1))) / 2.0));
+    const double small =
+        size_term ? std::min(6.0, std::max(0.0, 0.8 * (24.0 - std::log2(std::max<double>(g->n, 1))))) : 0.0;
     *out = std::max(1e-12, 4.0 * g->mean_w / std::max(avg_deg, 1.0)) * std::exp2(small) *
            std::ldexp(1.0, g->tune.sssp_class - 1);
     return GHET_OK;
@@ -427,7 +429,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     const int schedule = have_opts ? o->schedule : GHET_SCHED_HEURISTIC;
     double base_delta = 0;
     if (have_opts && o->sssp_delta > 0) base_delta = o->sssp_delta;
-    else GHET_CHECK(sssp_base_delta(g, &base_delta));
+    else GHET_CHECK(sssp_base_delta(g, &base_delta, true));
     ghet_level_trace* trace = have_opts ? o->trace : nullptr;
     const int trace_cap = trace ? o->trace_cap : 0;
     int trace_len = 0;
