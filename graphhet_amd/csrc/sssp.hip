@@ -109,6 +109,71 @@ __global__ void k_row_split(const int32_t* __restrict__ q, int32_t F, const int3
     }
 }
 
+// Small queues (at most kSmallMax entries): one 1024-thread block resets the
+// pass's counters (as k_ctr_reset with no far list), splits the rows (as
+// k_row_split) and scans the counts, in place of five launches.
+constexpr int kSmallBT = 1024;
+constexpr int kSmallItems = 4;
+constexpr int kSmallMax = kSmallBT * kSmallItems;
+
+template <bool kHeavy>
+__global__ void __launch_bounds__(kSmallBT) k_row_split_small(const int32_t* __restrict__ q, int32_t F,
+                                                              const int32_t* __restrict__ rowptr,
+                                                              const double* __restrict__ w,
+                                                              const double* __restrict__ dist, double threshold,
+                                                              int32_t* __restrict__ scan, int32_t* __restrict__ start,
+                                                              SsspCtr* ctr) {
+    __shared__ int32_t s_wave[kSmallBT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned long long* words = reinterpret_cast<unsigned long long*>(ctr->slots);
+    for (int i = tid; i < kSlots * 8; i += kSmallBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
+    if (tid == 0) ctr->near_count = 0;
+    int32_t c[kSmallItems], st[kSmallItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+        const int i = tid * kSmallItems + k;
+        c[k] = 0;
+        st[k] = 0;
+        if (i < F) {
+            const int32_t u = q[i];
+            const int32_t b = rowptr[u], e = rowptr[u + 1];
+            const double du = dist[u];
+            int32_t lo = b, hi = e;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (du + w[mid] < threshold) lo = mid + 1;
+                else hi = mid;
+            }
+            c[k] = kHeavy ? e - lo : lo - b;
+            st[k] = kHeavy ? lo : b;
+        }
+        sum += c[k];
+    }
+    int32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int32_t run = incl - sum, total = 0;
+    for (int v = 0; v < kSmallBT / 64; ++v) {
+        if (v < wave) run += s_wave[v];
+        total += s_wave[v];
+    }
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+        const int i = tid * kSmallItems + k;
+        if (i < F) {
+            scan[i] = run;
+            start[i] = st[k];
+            run += c[k];
+        }
+    }
+    if (tid == 0) scan[F] = total;
+}
+
 // One pass, persistent blocks over kTile-edge tiles of queue q; entry j
 // covers edges [start[j], start[j] + scan[j+1] - scan[j]) and scan[F] is the
 // pass's edge count (E_max, from the host, only bounds the grid).
@@ -492,15 +557,26 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     // E_max edges in all (the grid bound); the pass covers each row's light
     // prefix (round) or heavy rest (heavy pass).
     const SsspLayout& L = g->sl;
+    // The pass also resets the counters it fills.
     auto launch_pass = [&](bool heavy, const int32_t* q, int32_t Fq, int64_t E_max, int32_t next_round) -> int {
-        const unsigned gq = static_cast<unsigned>(ceil_div(Fq, 256));
-        if (heavy)
-            hipLaunchKernelGGL(k_row_split<true>, dim3(gq), dim3(256), 0, s, q, Fq, g->out.rowptr, L.w, dist,
-                               threshold, scan, qstart);
-        else
-            hipLaunchKernelGGL(k_row_split<false>, dim3(gq), dim3(256), 0, s, q, Fq, g->out.rowptr, L.w, dist,
-                               threshold, scan, qstart);
-        GHET_CHECK(exclusive_scan_i32(scan, scan, Fq, aux, s));
+        if (Fq <= kSmallMax) {
+            if (heavy)
+                hipLaunchKernelGGL(k_row_split_small<true>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, g->out.rowptr, L.w,
+                                   dist, threshold, scan, qstart, ctr);
+            else
+                hipLaunchKernelGGL(k_row_split_small<false>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, g->out.rowptr, L.w,
+                                   dist, threshold, scan, qstart, ctr);
+        } else {
+            hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, -1);
+            const unsigned gq = static_cast<unsigned>(ceil_div(Fq, 256));
+            if (heavy)
+                hipLaunchKernelGGL(k_row_split<true>, dim3(gq), dim3(256), 0, s, q, Fq, g->out.rowptr, L.w, dist,
+                                   threshold, scan, qstart);
+            else
+                hipLaunchKernelGGL(k_row_split<false>, dim3(gq), dim3(256), 0, s, q, Fq, g->out.rowptr, L.w, dist,
+                                   threshold, scan, qstart);
+            GHET_CHECK(exclusive_scan_i32(scan, scan, Fq, aux, s));
+        }
         const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
         if (heavy)
             hipLaunchKernelGGL(k_relax<true>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
@@ -520,7 +596,6 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         // relaxation rounds of the current bucket (light edges only)
         if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k0, s));
         while (F > 0) {
-            hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, -1);
             const int32_t next_round = round + 1;
             if (E > 0) {
                 GHET_CHECK(launch_pass(false, cur, F, E, next_round));
@@ -541,7 +616,6 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         // heavy pass: the bucket's distances are final; its remaining edges
         // all land at or above the threshold
         if (list_edges > 0) {
-            hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, -1);
             GHET_CHECK(launch_pass(true, list + list_base, list_len, list_edges, round));
             relaxed += static_cast<int64_t>(slot_sum(hc->slots, 3));
             far_count = hc->far_count[sel];
