@@ -159,6 +159,23 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
     block_report(cnt, od, id, ctr, s_red);
 }
 
+// Bits of the vertices without in-edges (and of the padding past n): no BFS
+// level can reach them, so the visited bitmap starts with them set and pull
+// levels skip them.  Depths are untouched (they stay unreached unless one is
+// the source), and the level statistics are counted on the host.
+__global__ void __launch_bounds__(kBT) k_skip_mask(int32_t n, const int32_t* __restrict__ in_rowptr, int64_t words,
+                                                   uint32_t* __restrict__ mask) {
+    for (int64_t wd = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; wd < words;
+         wd += static_cast<int64_t>(gridDim.x) * kBT) {
+        uint32_t m = 0;
+        for (int b = 0; b < 32; ++b) {
+            const int64_t v = wd * 32 + b;
+            if (v >= n || in_rowptr[v + 1] == in_rowptr[v]) m |= 1u << b;
+        }
+        mask[wd] = m;
+    }
+}
+
 // PULL: bottom-up over the in-CSR.  Each block iteration covers 256
 // consecutive vertices; wave w owns 64 of them = bitmap words 2g, 2g+1, so the
 // next frontier and the visited bits are written without atomics.
@@ -397,9 +414,17 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     int trace_len = 0;
     const BfsCtr* hc = reinterpret_cast<const BfsCtr*>(g->pinned);
 
+    if (have_in && !g->bfs_skip) {
+        GHET_HIP(hipMalloc(reinterpret_cast<void**>(&g->bfs_skip), 4 * size_t(words)));
+        hipLaunchKernelGGL(k_skip_mask, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(words, kBT), 4096))),
+                           dim3(kBT), 0, s, n, g->in.rowptr, words, g->bfs_skip);
+        GHET_HIP(hipGetLastError());
+    }
+
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     GHET_CHECK(fill<int32_t>(depth, kIntInf, n, s));
-    GHET_CHECK(fill<uint32_t>(visited, 0u, words, s));
+    if (have_in) GHET_HIP(hipMemcpyAsync(visited, g->bfs_skip, 4 * size_t(words), hipMemcpyDeviceToDevice, s));
+    else GHET_CHECK(fill<uint32_t>(visited, 0u, words, s));
     hipLaunchKernelGGL(k_seed, dim3(1), dim3(64), 0, s, depth, visited, qa, src);
 
     int32_t F = 1;

@@ -119,6 +119,7 @@ void destroy_graph(ghet_graph* g) {
     if (g->tc.mem) hipFree(g->tc.mem);
     if (g->ms.mem) hipFree(g->ms.mem);
     if (g->sl.mem) hipFree(g->sl.mem);
+    if (g->bfs_skip) hipFree(g->bfs_skip);
     if (g->arena) hipFree(g->arena);
     if (g->pinned) hipHostFree(g->pinned);
     if (g->ev_begin) hipEventDestroy(g->ev_begin);

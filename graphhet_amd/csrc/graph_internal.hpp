@@ -140,6 +140,7 @@ struct ghet_graph {
     ghet::MSLayout ms;
     ghet::SsspLayout sl;
     double mean_w = -1.0;  // mean out-edge weight (SSSP bucket width), reduced on first use
+    uint32_t* bfs_skip = nullptr;  // BFS: bit set for vertices no pull can reach (no in-edges), built on first use
     ghet::Tuning tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
