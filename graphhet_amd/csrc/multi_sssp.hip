@@ -128,6 +128,78 @@ __global__ void k_mss_rows(const int32_t* __restrict__ q, int32_t F, const uint3
     }
 }
 
+// Small queues (at most kSmallMax entries): one 1024-thread block resets the
+// pass's counters (as k_mss_reset with no far list), computes the row
+// segments (as k_mss_rows) and scans the counts.
+constexpr int kSmallBT = 1024;
+constexpr int kSmallItems = 4;
+constexpr int kSmallMax = kSmallBT * kSmallItems;
+
+template <bool kHeavy>
+__global__ void __launch_bounds__(kSmallBT) k_mss_rows_small(const int32_t* __restrict__ q, int32_t F,
+                                                             const uint32_t* __restrict__ qmask,
+                                                             const int32_t* __restrict__ rowptr,
+                                                             const double* __restrict__ w,
+                                                             const double* __restrict__ dist, int32_t ks,
+                                                             double threshold, int32_t* __restrict__ scan,
+                                                             int32_t* __restrict__ start, MssCtr* ctr) {
+    __shared__ int32_t s_wave[kSmallBT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned long long* words = reinterpret_cast<unsigned long long*>(ctr->slots);
+    for (int i = tid; i < kSlots * 8; i += kSmallBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
+    if (tid == 0) ctr->near_count = 0;
+    int32_t c[kSmallItems], st[kSmallItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+        const int i = tid * kSmallItems + k;
+        c[k] = 0;
+        st[k] = 0;
+        if (i < F) {
+            const int32_t u = q[i];
+            const uint32_t m = qmask[u];
+            const double* du = dist + static_cast<int64_t>(u) * ks;
+            double d = kHeavy ? 0.0 : HUGE_VAL;
+            for (uint32_t b = m; b; b &= b - 1) {
+                const double x = du[__ffs(b) - 1];
+                d = kHeavy ? fmax(d, x) : fmin(d, x);
+            }
+            const int32_t rb = rowptr[u], re = rowptr[u + 1];
+            int32_t lo = rb, hi = re;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (d + w[mid] < threshold) lo = mid + 1;
+                else hi = mid;
+            }
+            c[k] = kHeavy ? re - lo : lo - rb;
+            st[k] = kHeavy ? lo : rb;
+        }
+        sum += c[k];
+    }
+    int32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int32_t run = incl - sum, total = 0;
+    for (int v = 0; v < kSmallBT / 64; ++v) {
+        if (v < wave) run += s_wave[v];
+        total += s_wave[v];
+    }
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+        const int i = tid * kSmallItems + k;
+        if (i < F) {
+            scan[i] = run;
+            start[i] = st[k];
+            run += c[k];
+        }
+    }
+    if (tid == 0) scan[F] = total;
+}
+
 // The queue's masks are consumed: clear them for reuse two rounds later.
 __global__ void k_clear_mask(const int32_t* __restrict__ q, int32_t F, uint32_t* __restrict__ mask) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < F;
@@ -458,15 +530,26 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
         double threshold = delta;
         // Edge-balanced pass over queue q (F entries, masks qm, rows holding
         // at most E_max edges: the grid bound).
+        // The pass also resets the counters it fills.
         auto pass = [&](bool heavy, const int32_t* q, int32_t Fq, const uint32_t* qm, int64_t E_max) -> int {
             const unsigned vgrid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(Fq, 256), 65535)));
-            if (heavy)
-                hipLaunchKernelGGL(k_mss_rows<true>, dim3(vgrid), dim3(256), 0, s, q, Fq, qm, g->out.rowptr, L.w, dist,
-                                   kp, threshold, scan, qstart);
-            else
-                hipLaunchKernelGGL(k_mss_rows<false>, dim3(vgrid), dim3(256), 0, s, q, Fq, qm, g->out.rowptr, L.w, dist,
-                                   kp, threshold, scan, qstart);
-            GHET_CHECK(exclusive_scan_i32(scan, scan, Fq, aux, s));
+            if (Fq <= kSmallMax) {
+                if (heavy)
+                    hipLaunchKernelGGL(k_mss_rows_small<true>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, qm, g->out.rowptr,
+                                       L.w, dist, kp, threshold, scan, qstart, ctr);
+                else
+                    hipLaunchKernelGGL(k_mss_rows_small<false>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, qm, g->out.rowptr,
+                                       L.w, dist, kp, threshold, scan, qstart, ctr);
+            } else {
+                hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, -1);
+                if (heavy)
+                    hipLaunchKernelGGL(k_mss_rows<true>, dim3(vgrid), dim3(256), 0, s, q, Fq, qm, g->out.rowptr, L.w,
+                                       dist, kp, threshold, scan, qstart);
+                else
+                    hipLaunchKernelGGL(k_mss_rows<false>, dim3(vgrid), dim3(256), 0, s, q, Fq, qm, g->out.rowptr, L.w,
+                                       dist, kp, threshold, scan, qstart);
+                GHET_CHECK(exclusive_scan_i32(scan, scan, Fq, aux, s));
+            }
             const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
             if (heavy)
                 hipLaunchKernelGGL(k_mss_relax<true>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr,
@@ -486,7 +569,6 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
         };
         for (;;) {
             while (F > 0) {
-                hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, -1);
                 if (E > 0) {
                     GHET_CHECK(pass(false, cur, F, mcur, E));
                     F = hc->near_count;
@@ -505,7 +587,6 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
             }
             // heavy pass over the drained bucket (also clears its bucket masks)
             if (list_len > 0) {
-                hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, -1);
                 if (list_edges > 0) {
                     GHET_CHECK(pass(true, list, list_len, bmask, list_edges));
                     far_count = hc->far_count[sel];
