@@ -230,40 +230,60 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
             }
         }
         __syncthreads();
+        // The thread's kItems edges go through each stage together, so their
+        // loads (row distance, weight, target, then the target distances)
+        // are in flight at the same time instead of one edge after another.
+        int32_t u[kItems], pos[kItems];
+        bool ok[kItems];
+#pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const int32_t e = t0 + k * kBT + tid;
-            bool push_a = false, push_b = false;
-            int32_t v = 0;
-            if (e < t1) {
-                int32_t pos, u;
+            ok[k] = e < t1;
+            u[k] = 0;
+            pos[k] = 0;  // E > 0, so edge 0 exists: masked lanes load it
+            if (ok[k]) {
                 if (use_lds) {
-                    int j = upper_bound_i32(s_scan, span, e) - 1;
-                    pos = s_start[j] + (e - s_scan[j]);
-                    u = s_src[j];
+                    const int j = upper_bound_i32(s_scan, span, e) - 1;
+                    pos[k] = s_start[j] + (e - s_scan[j]);
+                    u[k] = s_src[j];
                 } else {
-                    int j = upper_bound_i32(scan + i0, span, e) - 1;
-                    u = q[i0 + j];
-                    pos = start[i0 + j] + (e - scan[i0 + j]);
+                    const int j = upper_bound_i32(scan + i0, span, e) - 1;
+                    u[k] = q[i0 + j];
+                    pos[k] = start[i0 + j] + (e - scan[i0 + j]);
                 }
-                const double nd = dist[u] + wts[pos];
-                if (kHeavy ? nd >= threshold : nd < threshold) {
-                    v = col[pos];
-                    if (nd < dist[v]) {
-                        const double old = atomic_min_nonneg(&dist[v], nd);
-                        if (nd < old) {
-                            if (kHeavy) {
-                                push_a = atomicExch(&stamp_far[v], far_gen) != far_gen;
-                            } else {
-                                const int32_t prev = atomicExch(&stamp_near[v], next_round);
-                                push_a = prev != next_round;
-                                push_b = prev < r0;
-                            }
-                        }
+            }
+        }
+        double nd[kItems];
+        int32_t v[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            nd[k] = dist[u[k]] + wts[pos[k]];
+            v[k] = col[pos[k]];
+        }
+        bool cand[kItems];
+        double dv[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            cand[k] = ok[k] && (kHeavy ? nd[k] >= threshold : nd[k] < threshold);
+            dv[k] = cand[k] ? dist[v[k]] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            bool push_a = false, push_b = false;
+            if (cand[k] && nd[k] < dv[k]) {
+                const double old = atomic_min_nonneg(&dist[v[k]], nd[k]);
+                if (nd[k] < old) {
+                    if (kHeavy) {
+                        push_a = atomicExch(&stamp_far[v[k]], far_gen) != far_gen;
+                    } else {
+                        const int32_t prev = atomicExch(&stamp_near[v[k]], next_round);
+                        push_a = prev != next_round;
+                        push_b = prev < r0;
                     }
                 }
             }
-            lq_push(s_qa, &s_na, push_a, v);
-            if (!kHeavy) lq_push(s_qb, &s_nb, push_b, v);
+            lq_push(s_qa, &s_na, push_a, v[k]);
+            if (!kHeavy) lq_push(s_qb, &s_nb, push_b, v[k]);
         }
         if (kHeavy) {
             lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->far_count[far_sel], far_list, kOut - kTile, nop);
