@@ -266,48 +266,60 @@ __global__ void __launch_bounds__(kBT) k_mss_relax(const int32_t* __restrict__ q
                 s_mask[j] = mask_cur[u];
             }
         __syncthreads();
+        // Each stage runs over the thread's kItems edges together, so their
+        // row, weight and target loads are in flight at the same time.
+        int32_t u[kItems], pos[kItems], v[kItems];
+        uint32_t msk[kItems];
+        double w[kItems];
+#pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const int32_t e = t0 + k * kBT + tid;
-            bool push_a = false, push_b = false;
-            int32_t v = 0;
+            u[k] = 0;
+            pos[k] = 0;  // E > 0, so edge 0 exists: masked lanes load it
+            msk[k] = 0;
             if (e < t1) {
-                int32_t pos, u;
-                uint32_t m;
                 if (use_lds) {
                     const int j = upper_bound_i32(s_scan, span, e) - 1;
-                    pos = s_start[j] + (e - s_scan[j]);
-                    u = s_src[j];
-                    m = s_mask[j];
+                    pos[k] = s_start[j] + (e - s_scan[j]);
+                    u[k] = s_src[j];
+                    msk[k] = s_mask[j];
                 } else {
                     const int j = upper_bound_i32(scan + i0, span, e) - 1;
-                    u = q[i0 + j];
-                    pos = start[i0 + j] + (e - scan[i0 + j]);
-                    m = mask_cur[u];
-                }
-                v = col[pos];
-                const double w = wts[pos];
-                const double* du = dist + static_cast<int64_t>(u) * ks;
-                double* dv = dist + static_cast<int64_t>(v) * ks;
-                uint32_t bits = 0;
-                for (uint32_t b = m; b; b &= b - 1) {
-                    const int s = __ffs(b) - 1;
-                    const double nd = du[s] + w;
-                    if ((kHeavy ? nd >= threshold : nd < threshold) && nd < dv[s]) {
-                        const double old = atomic_min_nonneg(&dv[s], nd);
-                        if (nd < old) bits |= 1u << s;
-                    }
-                }
-                if (bits) {
-                    if (kHeavy) {
-                        push_a = atomicOr(&fmask[v], bits) == 0u;
-                    } else {
-                        push_a = atomicOr(&mask_next[v], bits) == 0u;
-                        push_b = atomicOr(&bmask[v], bits) == 0u;
-                    }
+                    u[k] = q[i0 + j];
+                    pos[k] = start[i0 + j] + (e - scan[i0 + j]);
+                    msk[k] = mask_cur[u[k]];
                 }
             }
-            lq_push(s_qa, &s_na, push_a, v);
-            if (!kHeavy) lq_push(s_qb, &s_nb, push_b, v);
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            v[k] = col[pos[k]];
+            w[k] = wts[pos[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            bool push_a = false, push_b = false;
+            const double* du = dist + static_cast<int64_t>(u[k]) * ks;
+            double* dv = dist + static_cast<int64_t>(v[k]) * ks;
+            uint32_t bits = 0;
+            for (uint32_t b = msk[k]; b; b &= b - 1) {
+                const int s = __ffs(b) - 1;
+                const double nd = du[s] + w[k];
+                if ((kHeavy ? nd >= threshold : nd < threshold) && nd < dv[s]) {
+                    const double old = atomic_min_nonneg(&dv[s], nd);
+                    if (nd < old) bits |= 1u << s;
+                }
+            }
+            if (bits) {
+                if (kHeavy) {
+                    push_a = atomicOr(&fmask[v[k]], bits) == 0u;
+                } else {
+                    push_a = atomicOr(&mask_next[v[k]], bits) == 0u;
+                    push_b = atomicOr(&bmask[v[k]], bits) == 0u;
+                }
+            }
+            lq_push(s_qa, &s_na, push_a, v[k]);
+            if (!kHeavy) lq_push(s_qb, &s_nb, push_b, v[k]);
         }
         if (kHeavy) {
             lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->far_count[far_sel], far_list, kOut - kTile, nop);
