@@ -128,30 +128,42 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
             }
         }
         __syncthreads();
+        // Each stage runs over the thread's kItems edges together, so the
+        // target loads, then the visited-word loads, are in flight at once.
+        int32_t pos[kItems], v[kItems];
+        bool ok[kItems];
+#pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const int32_t e = t0 + k * kBT + tid;
-            bool disc = false;
-            int32_t v = 0;
-            if (e < t1) {
-                int32_t pos;
+            ok[k] = e < t1;
+            pos[k] = 0;  // E > 0, so edge 0 exists: masked lanes load it
+            if (ok[k]) {
                 if (use_lds) {
-                    int j = upper_bound_i32(s_scan, span, e) - 1;
-                    pos = s_start[j] + (e - s_scan[j]);
+                    const int j = upper_bound_i32(s_scan, span, e) - 1;
+                    pos[k] = s_start[j] + (e - s_scan[j]);
                 } else {
-                    int j = upper_bound_i32(scan + i0, span, e) - 1;
-                    pos = rowptr[q[i0 + j]] + (e - scan[i0 + j]);
-                }
-                v = col[pos];
-                const uint32_t bit = 1u << (v & 31);
-                if (!(visited[v >> 5] & bit)) {
-                    uint32_t old = atomicOr(&visited[v >> 5], bit);
-                    if (!(old & bit)) {
-                        depth[v] = next_level;
-                        disc = true;
-                    }
+                    const int j = upper_bound_i32(scan + i0, span, e) - 1;
+                    pos[k] = rowptr[q[i0 + j]] + (e - scan[i0 + j]);
                 }
             }
-            lq_push(s_out, &s_cnt, disc, v);
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) v[k] = col[pos[k]];
+        uint32_t vis[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) vis[k] = ok[k] ? visited[v[k] >> 5] : ~0u;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const uint32_t bit = 1u << (v[k] & 31);
+            bool disc = false;
+            if (!(vis[k] & bit)) {
+                const uint32_t old = atomicOr(&visited[v[k] >> 5], bit);
+                if (!(old & bit)) {
+                    depth[v[k]] = next_level;
+                    disc = true;
+                }
+            }
+            lq_push(s_out, &s_cnt, disc, v[k]);
         }
         lq_flush<kBT>(s_out, &s_cnt, &s_base, &ctr->qcount, next_q, kOut - kTile, on_item);
     }
@@ -314,22 +326,50 @@ __global__ void k_queue_to_bitmap(const int32_t* __restrict__ q, int32_t F, uint
     }
 }
 
-// Frontier bitmap -> queue, staged in LDS (one global atomic per flush).
-__global__ void __launch_bounds__(kBT) k_bitmap_to_queue(int32_t n, const uint32_t* __restrict__ bm,
+// Frontier bitmap -> queue: each thread takes 4 bitmap words (128 vertices),
+// the block scans the popcounts and reserves its range of the queue with one
+// global atomic.  One pass, no loop: words is a multiple of 8, so a thread's
+// 4 words are either all present or all past the end.
+__global__ void __launch_bounds__(kBT) k_bitmap_to_queue(int64_t words, int32_t n, const uint32_t* __restrict__ bm,
                                                          int32_t* __restrict__ q, int32_t* counter) {
-    __shared__ int32_t s_out[kOut];
-    __shared__ int32_t s_cnt, s_base;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();  // counter zeroed before any wave appends
-    auto nop = [](int32_t) {};
-    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < n; base += static_cast<int64_t>(gridDim.x) * kBT) {
-        const int64_t v = base + threadIdx.x;
-        const uint32_t word = bm[2 * (v >> 6) + (lane_id() >> 5)];
-        const bool set = v < n && ((word >> (lane_id() & 31)) & 1u);
-        lq_push(s_out, &s_cnt, set, static_cast<int32_t>(v));
-        lq_flush<kBT>(s_out, &s_cnt, &s_base, counter, q, kOut - kBT, nop);
+    __shared__ int32_t s_wave[kBT / 64];
+    __shared__ int32_t s_base;
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) * 4;
+    uint4 x = make_uint4(0u, 0u, 0u, 0u);
+    if (w0 < words) x = *reinterpret_cast<const uint4*>(bm + w0);
+    // bits of ids >= n (padding) never count
+    auto valid = [n](int64_t wi) -> uint32_t {
+        const int64_t lo = wi * 32;
+        return lo + 32 <= n ? ~0u : (lo >= n ? 0u : (1u << (n - lo)) - 1u);
+    };
+    x.x &= valid(w0);
+    x.y &= valid(w0 + 1);
+    x.z &= valid(w0 + 2);
+    x.w &= valid(w0 + 3);
+    const int c = __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+    int incl = c;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
     }
-    lq_flush<kBT>(s_out, &s_cnt, &s_base, counter, q, 0, nop);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kBT / 64; ++w) t += s_wave[w];
+        s_base = t > 0 ? atomicAdd(counter, t) : 0;
+    }
+    __syncthreads();
+    int off = s_base + incl - c;
+    for (int w = 0; w < wave; ++w) off += s_wave[w];
+    const uint32_t ws[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        for (uint32_t b = ws[k]; b; b &= b - 1) {
+            q[off++] = static_cast<int32_t>((w0 + k) * 32 + (__ffs(b) - 1));
+        }
+    }
 }
 
 // Parent = smallest u with an edge u->v and depth[u] == depth[v] - 1
@@ -458,7 +498,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         const int32_t next_level = level + 1;
         if (mode == GHET_MODE_PUSH && !have_q) {
             hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(512), 0, s, ctr);
-            hipLaunchKernelGGL(k_bitmap_to_queue, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, bma, qa, &ctr->qcount);
+            hipLaunchKernelGGL(k_bitmap_to_queue, dim3(static_cast<unsigned>(ceil_div(words, 4 * kBT))), dim3(kBT), 0, s,
+                               words, n, bma, qa, &ctr->qcount);
             have_q = true;
         }
         hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(512), 0, s, ctr);
