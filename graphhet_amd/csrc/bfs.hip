@@ -40,6 +40,9 @@ constexpr int kTile = kBT * kItems;  // edges per push tile
 constexpr int kOut = 2 * kTile;      // LDS output queue capacity
 constexpr int kHubDeg = 4096;
 constexpr int kMaxBlocks = 2048;     // persistent grid cap (8 blocks per CU)
+constexpr int kSmallBT = 1024;
+constexpr int kSmallItems = 16;
+constexpr int kSmallMax = kSmallBT * kSmallItems;  // frontier entries of a one-block push prep
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
     int lo = 0, hi = len;
@@ -51,12 +54,22 @@ __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_
     return lo;
 }
 
-__global__ void k_seed(int32_t* depth, uint32_t* visited, int32_t* queue, int32_t src) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        depth[src] = 0;
-        visited[src >> 5] |= 1u << (src & 31);
-        queue[0] = src;
+// Level-0 state in one pass: depth INT_MAX except the source (0), visited =
+// the no-in-edge mask (none without an in-CSR) plus the source, queue = {src}.
+__global__ void __launch_bounds__(kBT) k_bfs_init(int32_t n, int64_t words, const uint32_t* __restrict__ skip,
+                                                  int32_t src, int32_t* __restrict__ depth,
+                                                  uint32_t* __restrict__ visited, int32_t* __restrict__ queue) {
+    const int64_t len = n > words ? n : words;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; i < len;
+         i += static_cast<int64_t>(gridDim.x) * kBT) {
+        if (i < n) depth[i] = i == src ? 0 : kIntInf;
+        if (i < words) {
+            uint32_t w = skip ? skip[i] : 0u;
+            if (i == (src >> 5)) w |= 1u << (src & 31);
+            visited[i] = w;
+        }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) queue[0] = src;
 }
 
 __global__ void k_zero_ctr(BfsCtr* c) {
@@ -71,6 +84,62 @@ __global__ void k_frontier_deg(const int32_t* __restrict__ q, int32_t F, const i
         int32_t u = q[i];
         deg[i] = rowptr[u + 1] - rowptr[u];
     }
+}
+
+// Block 0 clears the counters the next level will use (the other buffer of
+// the pair; the host has already copied it), so that level needs no reset
+// launch.
+__device__ __forceinline__ void zero_next(BfsCtr* c) {
+    if (c && blockIdx.x == 0) {
+        unsigned long long* w = reinterpret_cast<unsigned long long*>(c);
+        for (int i = threadIdx.x; i < kCtrWords; i += blockDim.x) w[i] = 0;
+    }
+}
+
+// A push level with at most kSmallMax frontier entries, prepared in one
+// block: clears the level's counters and writes the exclusive scan of the
+// frontier's out-degrees to scan[0..F] (scan[F] = the level's edge count).
+// Replaces the reset, degree and three scan launches of a large level.
+__global__ void __launch_bounds__(kSmallBT) k_push_prep_small(const int32_t* __restrict__ q, int32_t F,
+                                                              const int32_t* __restrict__ rowptr,
+                                                              int32_t* __restrict__ scan, BfsCtr* ctr) {
+    __shared__ int32_t s_wave[kSmallBT / 64];
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    unsigned long long* cw = reinterpret_cast<unsigned long long*>(ctr);
+    for (int i = tid; i < kCtrWords; i += kSmallBT) cw[i] = 0;
+    int32_t d[kSmallItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+        const int i = tid * kSmallItems + k;
+        d[k] = 0;
+        if (i < F) {
+            const int32_t u = q[i];
+            d[k] = rowptr[u + 1] - rowptr[u];
+        }
+        sum += d[k];
+    }
+    int32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int32_t run = incl - sum, total = 0;
+    for (int w = 0; w < kSmallBT / 64; ++w) {
+        if (w < wave) run += s_wave[w];
+        total += s_wave[w];
+    }
+#pragma unroll
+    for (int k = 0; k < kSmallItems; ++k) {
+        const int i = tid * kSmallItems + k;
+        if (i < F) {
+            scan[i] = run;
+            run += d[k];
+        }
+    }
+    if (tid == 0) scan[F] = total;
 }
 
 // Block totals of (count, out-edges, in-edges) into the block's slot.
@@ -94,7 +163,7 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
                                                      const int32_t* __restrict__ col, uint32_t* visited,
                                                      int32_t* __restrict__ depth, int32_t next_level,
                                                      int32_t* __restrict__ next_q, BfsCtr* ctr,
-                                                     const int32_t* __restrict__ in_rowptr) {
+                                                     const int32_t* __restrict__ in_rowptr, BfsCtr* next_ctr) {
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
     __shared__ int32_t s_out[kOut];
@@ -102,6 +171,7 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
     __shared__ unsigned long long s_red[kBT / 64];
     const int tid = threadIdx.x;
     if (tid == 0) s_cnt = 0;
+    zero_next(next_ctr);
     __syncthreads();  // counter zeroed before any wave appends
     unsigned long long cnt = 0, od = 0, id = 0;
     auto on_item = [&](int32_t v) {
@@ -122,9 +192,20 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
         const int i0 = s_i0, span = s_span;
         const bool use_lds = span <= kTile;
         if (use_lds) {
-            for (int j = tid; j < span; j += kBT) {
-                s_scan[j] = scan[i0 + j];
-                s_start[j] = rowptr[q[i0 + j]];
+            // span <= kTile = kItems * kBT: one pass, the queue loads first
+            int32_t qv[kItems];
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                const int j = tid + k * kBT;
+                qv[k] = j < span ? q[i0 + j] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                const int j = tid + k * kBT;
+                if (j < span) {
+                    s_scan[j] = scan[i0 + j];
+                    s_start[j] = rowptr[qv[k]];
+                }
             }
         }
         __syncthreads();
@@ -195,9 +276,10 @@ __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restri
                                               const int32_t* __restrict__ in_col, const uint32_t* __restrict__ cur_bm,
                                               uint32_t* __restrict__ next_bm, uint32_t* __restrict__ visited,
                                               int32_t* __restrict__ depth, int32_t next_level, BfsCtr* ctr,
-                                              const int32_t* __restrict__ out_rowptr) {
+                                              const int32_t* __restrict__ out_rowptr, BfsCtr* next_ctr) {
     __shared__ unsigned long long s_red[kBT / 64];
     const int lane = lane_id();
+    zero_next(next_ctr);
     unsigned long long od = 0, id = 0, cnt = 0;
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < n; base += static_cast<int64_t>(gridDim.x) * kBT) {
         const int64_t v = base + threadIdx.x;
@@ -434,7 +516,7 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     const int64_t words = 8 * ceil_div(std::max(n, 1), 256);
     int32_t *depth, *qa, *qb, *scan, *aux, *hubq;
     uint32_t *visited, *bma, *bmb;
-    BfsCtr* ctr;
+    BfsCtr *ctrs, *ctr;
     void* p;
     GHET_CHECK(g->ws.get(WS_DEPTH, 4 * size_t(n), &p)); depth = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_VISITED, 4 * size_t(words), &p)); visited = static_cast<uint32_t*>(p);
@@ -445,7 +527,10 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_MISC, 4 * size_t(n) + 256, &p)); hubq = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(BfsCtr), &p)); ctr = static_cast<BfsCtr*>(p);
+    // two counter buffers: level L counts into ctrs[L & 1], and its main
+    // kernel clears the other one for level L + 1
+    GHET_CHECK(g->ws.get(WS_COUNTERS, 2 * sizeof(BfsCtr), &p)); ctrs = static_cast<BfsCtr*>(p);
+    ctr = ctrs;
 
     SchedState sched;
     sched_begin(&sched, schedule, GHET_NET_BFS, n, g->out.m, have_in ? g->in.m : g->out.m);
@@ -462,10 +547,9 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     }
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
-    GHET_CHECK(fill<int32_t>(depth, kIntInf, n, s));
-    if (have_in) GHET_HIP(hipMemcpyAsync(visited, g->bfs_skip, 4 * size_t(words), hipMemcpyDeviceToDevice, s));
-    else GHET_CHECK(fill<uint32_t>(visited, 0u, words, s));
-    hipLaunchKernelGGL(k_seed, dim3(1), dim3(64), 0, s, depth, visited, qa, src);
+    hipLaunchKernelGGL(k_bfs_init, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(std::max<int64_t>(n, words), kBT), 8192))),
+                       dim3(kBT), 0, s, n, words, have_in ? g->bfs_skip : nullptr, src, depth, visited, qa);
+    bool zeroed[2] = {false, false};  // ctrs[i] known to be all zero
 
     int32_t F = 1;
     int64_t m_f = g->h_out_rowptr[src + 1] - g->h_out_rowptr[src];
@@ -496,27 +580,41 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
             GHET_HIP(hipEventRecord(g->ev_k0, s));
         }
         const int32_t next_level = level + 1;
+        ctr = ctrs + (level & 1);
+        BfsCtr* next_ctr = ctrs + ((level + 1) & 1);
+        bool next_zeroed = false;
+        auto clear = [&]() {
+            if (!zeroed[level & 1]) hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(512), 0, s, ctr);
+            zeroed[level & 1] = true;
+        };
         if (mode == GHET_MODE_PUSH && !have_q) {
-            hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(512), 0, s, ctr);
+            clear();
             hipLaunchKernelGGL(k_bitmap_to_queue, dim3(static_cast<unsigned>(ceil_div(words, 4 * kBT))), dim3(kBT), 0, s,
                                words, n, bma, qa, &ctr->qcount);
+            zeroed[level & 1] = false;
             have_q = true;
         }
-        hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(512), 0, s, ctr);
         if (mode == GHET_MODE_PUSH) {
-            hipLaunchKernelGGL(k_frontier_deg, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s, qa, F,
-                               g->out.rowptr, scan);
-            GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
+            if (F <= kSmallMax) {
+                hipLaunchKernelGGL(k_push_prep_small, dim3(1), dim3(kSmallBT), 0, s, qa, F, g->out.rowptr, scan, ctr);
+            } else {
+                clear();
+                hipLaunchKernelGGL(k_frontier_deg, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s, qa,
+                                   F, g->out.rowptr, scan);
+                GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
+            }
             const int32_t E = static_cast<int32_t>(m_f);
             if (E > 0) {
                 unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), kMaxBlocks));
                 hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, qa, F, scan, E, g->out.rowptr,
-                                   g->out.col, visited, depth, next_level, qb, ctr, in_rp);
+                                   g->out.col, visited, depth, next_level, qb, ctr, in_rp, next_ctr);
+                next_zeroed = true;
             }
             std::swap(qa, qb);
             have_q = true;
             have_bm = false;
         } else {
+            clear();
             if (!have_bm) {
                 GHET_CHECK(fill<uint32_t>(bma, 0u, words, s));
                 hipLaunchKernelGGL(k_queue_to_bitmap, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s,
@@ -525,7 +623,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
             }
             if (mode == GHET_MODE_PULL) {
                 hipLaunchKernelGGL(k_pull, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->in.rowptr, g->in.col,
-                                   bma, bmb, visited, depth, next_level, ctr, g->out.rowptr);
+                                   bma, bmb, visited, depth, next_level, ctr, g->out.rowptr, next_ctr);
+                next_zeroed = true;
             } else {
                 GHET_CHECK(fill<uint32_t>(bmb, 0u, words, s));
                 hipLaunchKernelGGL(k_bitmap_topdown, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->out.rowptr,
@@ -537,6 +636,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
             have_bm = true;
             have_q = false;
         }
+        zeroed[level & 1] = false;
+        zeroed[(level + 1) & 1] = next_zeroed;
         GHET_HIP(hipGetLastError());
         GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
         if (tr) GHET_HIP(hipEventRecord(g->ev_k1, s));

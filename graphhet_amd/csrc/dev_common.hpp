@@ -161,10 +161,20 @@ __device__ __forceinline__ void lq_flush(T* buf, int* cnt, Count* s_base, Count*
     if (threadIdx.x == 0) *s_base = atomicAdd(gcount, static_cast<Count>(c));
     __syncthreads();
     const Count base = *s_base;
-    for (int j = threadIdx.x; j < c; j += BT) {
-        const T v = buf[j];
-        gqueue[base + j] = v;
-        on_item(v);
+    // four items per step, so on_item's loads for them overlap
+    for (int j0 = threadIdx.x; j0 < c; j0 += 4 * BT) {
+        T v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int j = j0 + k * BT;
+            v[k] = buf[j < c ? j : j0];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (j0 + k * BT < c) gqueue[base + j0 + k * BT] = v[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (j0 + k * BT < c) on_item(v[k]);
     }
     __syncthreads();
     if (threadIdx.x == 0) *cnt = 0;
