@@ -23,13 +23,15 @@
 namespace ghet {
 namespace {
 
-// Per-level counters.  qcount/hubs hand out queue positions (one atomic per
-// LDS flush / per hub); the sums are spread over kSlots slots:
+// Per-level counters.  qcount/hubs/alloc/conv hand out queue positions (one
+// atomic per LDS flush / hub / block); the sums are spread over kSlots slots:
 // v[0] vertices discovered, v[1] their out-edges, v[2] their in-edges.
 struct BfsCtr {
     int32_t qcount;
     int32_t hubs;
-    int32_t pad[14];
+    unsigned long long alloc;  // push prep: (entries << 32) | edges reserved so far
+    int32_t conv;              // bitmap -> queue conversion
+    int32_t pad[11];
     CtrSlot slots[kSlots];
 };
 constexpr int kCtrWords = sizeof(BfsCtr) / 8;
@@ -40,9 +42,7 @@ constexpr int kTile = kBT * kItems;  // edges per push tile
 constexpr int kOut = 2 * kTile;      // LDS output queue capacity
 constexpr int kHubDeg = 4096;
 constexpr int kMaxBlocks = 2048;     // persistent grid cap (8 blocks per CU)
-constexpr int kSmallBT = 1024;
-constexpr int kSmallItems = 16;
-constexpr int kSmallMax = kSmallBT * kSmallItems;  // frontier entries of a one-block push prep
+constexpr int kPrepItems = 4;        // queue entries per thread of k_push_prep
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
     int lo = 0, hi = len;
@@ -55,10 +55,16 @@ __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_
 }
 
 // Level-0 state in one pass: depth INT_MAX except the source (0), visited =
-// the no-in-edge mask (none without an in-CSR) plus the source, queue = {src}.
+// the no-in-edge mask (none without an in-CSR) plus the source, queue =
+// {src}, both counter buffers zero.
 __global__ void __launch_bounds__(kBT) k_bfs_init(int32_t n, int64_t words, const uint32_t* __restrict__ skip,
                                                   int32_t src, int32_t* __restrict__ depth,
-                                                  uint32_t* __restrict__ visited, int32_t* __restrict__ queue) {
+                                                  uint32_t* __restrict__ visited, int32_t* __restrict__ queue,
+                                                  BfsCtr* ctrs) {
+    if (blockIdx.x == 0) {  // both counter buffers
+        unsigned long long* w = reinterpret_cast<unsigned long long*>(ctrs);
+        for (int i = threadIdx.x; i < 2 * kCtrWords; i += kBT) w[i] = 0;
+    }
     const int64_t len = n > words ? n : words;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; i < len;
          i += static_cast<int64_t>(gridDim.x) * kBT) {
@@ -77,15 +83,6 @@ __global__ void k_zero_ctr(BfsCtr* c) {
     for (int i = threadIdx.x; i < kCtrWords; i += blockDim.x) w[i] = 0;
 }
 
-__global__ void k_frontier_deg(const int32_t* __restrict__ q, int32_t F, const int32_t* __restrict__ rowptr,
-                               int32_t* __restrict__ deg) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < F) {
-        int32_t u = q[i];
-        deg[i] = rowptr[u + 1] - rowptr[u];
-    }
-}
-
 // Block 0 clears the counters the next level will use (the other buffer of
 // the pair; the host has already copied it), so that level needs no reset
 // launch.
@@ -96,27 +93,36 @@ __device__ __forceinline__ void zero_next(BfsCtr* c) {
     }
 }
 
-// A push level with at most kSmallMax frontier entries, prepared in one
-// block: clears the level's counters and writes the exclusive scan of the
-// frontier's out-degrees to scan[0..F] (scan[F] = the level's edge count).
-// Replaces the reset, degree and three scan launches of a large level.
-__global__ void __launch_bounds__(kSmallBT) k_push_prep_small(const int32_t* __restrict__ q, int32_t F,
-                                                              const int32_t* __restrict__ rowptr,
-                                                              int32_t* __restrict__ scan, BfsCtr* ctr) {
-    __shared__ int32_t s_wave[kSmallBT / 64];
+// Push-level prep: the frontier's out-degree scan in one launch.  Each block
+// takes kPrepItems * kBT queue entries, scans their degrees and reserves a
+// range of entries AND of edges with one 64-bit atomic on ctr->alloc
+// ((entries << 32) | edges), then writes, at its entry range, scan[] (first
+// edge of the entry) and start[] (its row offset).  Blocks land in any order,
+// but entry and edge ranges are reserved together, so scan[] ascends; the
+// block that ends at F writes scan[F] = E.  The expansion reads start[]
+// instead of the queue.
+__global__ void __launch_bounds__(kBT) k_push_prep(const int32_t* __restrict__ q, int32_t F,
+                                                   const int32_t* __restrict__ rowptr, int32_t* __restrict__ scan,
+                                                   int32_t* __restrict__ start, BfsCtr* ctr) {
+    __shared__ int32_t s_wave[kBT / 64];
+    __shared__ unsigned long long s_base;
     const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    unsigned long long* cw = reinterpret_cast<unsigned long long*>(ctr);
-    for (int i = tid; i < kCtrWords; i += kSmallBT) cw[i] = 0;
-    int32_t d[kSmallItems];
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kBT * kPrepItems;
+    int32_t b[kPrepItems], d[kPrepItems];
     int32_t sum = 0;
+    // thread tid owns entries i0 + tid*kPrepItems + k: the valid entries of
+    // the block are its first cnt, in scan order
+    int32_t u[kPrepItems];
 #pragma unroll
-    for (int k = 0; k < kSmallItems; ++k) {
-        const int i = tid * kSmallItems + k;
-        d[k] = 0;
-        if (i < F) {
-            const int32_t u = q[i];
-            d[k] = rowptr[u + 1] - rowptr[u];
-        }
+    for (int k = 0; k < kPrepItems; ++k) {
+        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
+        u[k] = i < F ? q[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
+        b[k] = rowptr[u[k]];
+        d[k] = i < F ? rowptr[u[k] + 1] - b[k] : 0;
         sum += d[k];
     }
     int32_t incl = sum;
@@ -127,19 +133,26 @@ __global__ void __launch_bounds__(kSmallBT) k_push_prep_small(const int32_t* __r
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
     int32_t run = incl - sum, total = 0;
-    for (int w = 0; w < kSmallBT / 64; ++w) {
+    for (int w = 0; w < kBT / 64; ++w) {
         if (w < wave) run += s_wave[w];
         total += s_wave[w];
     }
+    const int32_t cnt = F - i0 < kBT * kPrepItems ? static_cast<int32_t>(F - i0) : kBT * kPrepItems;
+    if (tid == 0)
+        s_base = atomicAdd(&ctr->alloc, (static_cast<unsigned long long>(cnt) << 32) | static_cast<uint32_t>(total));
+    __syncthreads();
+    const int32_t ebase = static_cast<int32_t>(s_base >> 32), xbase = static_cast<int32_t>(s_base & 0xFFFFFFFFu);
 #pragma unroll
-    for (int k = 0; k < kSmallItems; ++k) {
-        const int i = tid * kSmallItems + k;
+    for (int k = 0; k < kPrepItems; ++k) {
+        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
         if (i < F) {
-            scan[i] = run;
+            const int32_t slot = ebase + tid * kPrepItems + k;
+            scan[slot] = xbase + run;
+            start[slot] = b[k];
             run += d[k];
         }
     }
-    if (tid == 0) scan[F] = total;
+    if (tid == 0 && ebase + cnt == F) scan[F] = xbase + total;
 }
 
 // Block totals of (count, out-edges, in-edges) into the block's slot.
@@ -155,10 +168,12 @@ __device__ __forceinline__ void block_report(unsigned long long cnt, unsigned lo
     }
 }
 
-// PUSH: persistent blocks over tiles of kTile consecutive frontier edges.
+// PUSH: persistent blocks over tiles of tile_edges (<= kTile) consecutive
+// frontier edges; entry j of the level covers edges [scan[j], scan[j+1]) of
+// the row starting at start[j].
 // Discoveries are staged in LDS and flushed to the next queue in bulk.
-__global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__ q, int32_t F,
-                                                     const int32_t* __restrict__ scan, int32_t E,
+__global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__ start, int32_t F,
+                                                     const int32_t* __restrict__ scan, int32_t E, int32_t tile_edges,
                                                      const int32_t* __restrict__ rowptr,
                                                      const int32_t* __restrict__ col, uint32_t* visited,
                                                      int32_t* __restrict__ depth, int32_t next_level,
@@ -179,9 +194,9 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
         od += rowptr[v + 1] - rowptr[v];
         if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
     };
-    for (int64_t tile = blockIdx.x; tile * kTile < E; tile += gridDim.x) {
-        const int32_t t0 = static_cast<int32_t>(tile * kTile);
-        const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
+    for (int64_t tile = blockIdx.x; tile * tile_edges < E; tile += gridDim.x) {
+        const int32_t t0 = static_cast<int32_t>(tile * tile_edges);
+        const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + tile_edges, E));
         if (tid == 0) {
             int i0 = upper_bound_i32(scan, F + 1, t0) - 1;
             int i1 = upper_bound_i32(scan, F + 1, t1 - 1) - 1;
@@ -192,20 +207,9 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
         const int i0 = s_i0, span = s_span;
         const bool use_lds = span <= kTile;
         if (use_lds) {
-            // span <= kTile = kItems * kBT: one pass, the queue loads first
-            int32_t qv[kItems];
-#pragma unroll
-            for (int k = 0; k < kItems; ++k) {
-                const int j = tid + k * kBT;
-                qv[k] = j < span ? q[i0 + j] : 0;
-            }
-#pragma unroll
-            for (int k = 0; k < kItems; ++k) {
-                const int j = tid + k * kBT;
-                if (j < span) {
-                    s_scan[j] = scan[i0 + j];
-                    s_start[j] = rowptr[qv[k]];
-                }
+            for (int j = tid; j < span; j += kBT) {
+                s_scan[j] = scan[i0 + j];
+                s_start[j] = start[i0 + j];
             }
         }
         __syncthreads();
@@ -224,7 +228,7 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
                     pos[k] = s_start[j] + (e - s_scan[j]);
                 } else {
                     const int j = upper_bound_i32(scan + i0, span, e) - 1;
-                    pos[k] = rowptr[q[i0 + j]] + (e - scan[i0 + j]);
+                    pos[k] = start[i0 + j] + (e - scan[i0 + j]);
                 }
             }
         }
@@ -514,7 +518,7 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
 
     // 8 words per 256 vertices: block iterations of 256 never index past the end
     const int64_t words = 8 * ceil_div(std::max(n, 1), 256);
-    int32_t *depth, *qa, *qb, *scan, *aux, *hubq;
+    int32_t *depth, *qa, *qb, *scan, *start, *hubq;
     uint32_t *visited, *bma, *bmb;
     BfsCtr *ctrs, *ctr;
     void* p;
@@ -525,7 +529,7 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     GHET_CHECK(g->ws.get(WS_QUEUE_A, 4 * size_t(n), &p)); qa = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_QUEUE_B, 4 * size_t(n), &p)); qb = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_BFS_START, 4 * size_t(n), &p)); start = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_MISC, 4 * size_t(n) + 256, &p)); hubq = static_cast<int32_t*>(p);
     // two counter buffers: level L counts into ctrs[L & 1], and its main
     // kernel clears the other one for level L + 1
@@ -548,8 +552,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_bfs_init, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(std::max<int64_t>(n, words), kBT), 8192))),
-                       dim3(kBT), 0, s, n, words, have_in ? g->bfs_skip : nullptr, src, depth, visited, qa);
-    bool zeroed[2] = {false, false};  // ctrs[i] known to be all zero
+                       dim3(kBT), 0, s, n, words, have_in ? g->bfs_skip : nullptr, src, depth, visited, qa, ctrs);
+    bool zeroed[2] = {true, true};  // ctrs[i] known to be all zero
 
     int32_t F = 1;
     int64_t m_f = g->h_out_rowptr[src + 1] - g->h_out_rowptr[src];
@@ -589,24 +593,23 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         };
         if (mode == GHET_MODE_PUSH && !have_q) {
             clear();
+            // counts in ctr->conv: qcount, alloc and the slots stay zero
             hipLaunchKernelGGL(k_bitmap_to_queue, dim3(static_cast<unsigned>(ceil_div(words, 4 * kBT))), dim3(kBT), 0, s,
-                               words, n, bma, qa, &ctr->qcount);
-            zeroed[level & 1] = false;
+                               words, n, bma, qa, &ctr->conv);
             have_q = true;
         }
         if (mode == GHET_MODE_PUSH) {
-            if (F <= kSmallMax) {
-                hipLaunchKernelGGL(k_push_prep_small, dim3(1), dim3(kSmallBT), 0, s, qa, F, g->out.rowptr, scan, ctr);
-            } else {
-                clear();
-                hipLaunchKernelGGL(k_frontier_deg, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s, qa,
-                                   F, g->out.rowptr, scan);
-                GHET_CHECK(exclusive_scan_i32(scan, scan, F, aux, s));
-            }
+            clear();
+            hipLaunchKernelGGL(k_push_prep, dim3(static_cast<unsigned>(ceil_div(F, kBT * kPrepItems))), dim3(kBT), 0, s,
+                               qa, F, g->out.rowptr, scan, start, ctr);
             const int32_t E = static_cast<int32_t>(m_f);
             if (E > 0) {
-                unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, kTile), kMaxBlocks));
-                hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, qa, F, scan, E, g->out.rowptr,
+                // about 512 tiles while the level is small, kTile-edge tiles
+                // over a persistent grid once it is large
+                const int32_t tile = static_cast<int32_t>(
+                    std::min<int64_t>(kTile, std::max<int64_t>(64, ceil_div(ceil_div(E, 512), 64) * 64)));
+                const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, tile), kMaxBlocks));
+                hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, start, F, scan, E, tile, g->out.rowptr,
                                    g->out.col, visited, depth, next_level, qb, ctr, in_rp, next_ctr);
                 next_zeroed = true;
             }
