@@ -43,7 +43,8 @@ struct SsspCtr {
     int32_t near_count;
     int32_t far_count[2];
     int32_t list_count;
-    int32_t pad[12];
+    unsigned long long alloc[2];  // k_row_prep reservations, alternating by pass
+    int32_t pad[8];
     CtrSlot slots[kSlots];
 };
 constexpr int kCtrWords = sizeof(SsspCtr) / 8;
@@ -85,33 +86,118 @@ __global__ void k_ctr_reset(SsspCtr* c, int other) {
     }
 }
 
-// Per queue entry u: the row segment a pass relaxes.  Rows are ordered by
+// Per queue entry u, the row segment a pass relaxes: rows are ordered by
 // weight, so fl(d[u] + w) is non-decreasing along the row and the edges with
 // d[u] + w < thr are a prefix (binary search).  Round: the prefix; heavy: the
-// rest.  cnt feeds the scan, start the relax kernel.
+// rest.  Two launch shapes prepare a pass: k_row_prep and k_row_split_small.
+//
+// k_row_prep, for queues of more than kSmallMax entries, in one launch: each
+// block takes kPrepItems * kBT queue entries (thread-major), splits their
+// rows (the binary searches of a thread's entries advance together), scans
+// the counts and
+// reserves a range of entries and of edges with one 64-bit atomic on alloc
+// ((entries << 32) | edges).  At its entry range it writes scan[] (first
+// edge), start[] (first position) and vert[] (the vertex): blocks land in any
+// order, but scan[] ascends, and the block that ends at F writes scan[F].
+// Block 0 also resets the pass's counters (as k_ctr_reset with no far list;
+// only the relax launch after this one uses them) and clears next_alloc, the
+// other reservation counter, for the next pass.
+constexpr int kPrepItems = 1;
+
 template <bool kHeavy>
-__global__ void k_row_split(const int32_t* __restrict__ q, int32_t F, const int32_t* __restrict__ rowptr,
-                            const double* __restrict__ w, const double* __restrict__ dist, double threshold,
-                            int32_t* __restrict__ cnt, int32_t* __restrict__ start) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < F) {
-        const int32_t u = q[i];
-        const int32_t b = rowptr[u], e = rowptr[u + 1];
-        const double du = dist[u];
-        int32_t lo = b, hi = e;
-        while (lo < hi) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (du + w[mid] < threshold) lo = mid + 1;
-            else hi = mid;
+__global__ void __launch_bounds__(kBT) k_row_prep(const int32_t* __restrict__ q, int32_t F,
+                                                  const int32_t* __restrict__ rowptr, const double* __restrict__ w,
+                                                  const double* __restrict__ dist, double threshold,
+                                                  int32_t* __restrict__ scan, int32_t* __restrict__ start,
+                                                  int32_t* __restrict__ vert, unsigned long long* alloc,
+                                                  unsigned long long* next_alloc, SsspCtr* ctr) {
+    __shared__ int32_t s_wave[kBT / 64];
+    __shared__ unsigned long long s_base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (blockIdx.x == 0) {
+        unsigned long long* words = reinterpret_cast<unsigned long long*>(ctr->slots);
+        for (int i = tid; i < kSlots * 8; i += kBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
+        if (tid == 0) {
+            ctr->near_count = 0;
+            *next_alloc = 0;
         }
-        cnt[i] = kHeavy ? e - lo : lo - b;
-        start[i] = kHeavy ? lo : b;
     }
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kBT * kPrepItems;
+    int32_t u[kPrepItems], lo[kPrepItems], hi[kPrepItems], b[kPrepItems], e[kPrepItems];
+    double du[kPrepItems];
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
+        u[k] = i < F ? q[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
+        b[k] = rowptr[u[k]];
+        e[k] = i < F ? rowptr[u[k] + 1] : b[k];
+        du[k] = dist[u[k]];
+        lo[k] = b[k];
+        hi[k] = e[k];
+    }
+    for (bool active = true; active;) {
+        active = false;
+        int32_t mid[kPrepItems];
+        double wm[kPrepItems];
+#pragma unroll
+        for (int k = 0; k < kPrepItems; ++k) {
+            mid[k] = (lo[k] + hi[k]) >> 1;
+            wm[k] = lo[k] < hi[k] ? w[mid[k]] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kPrepItems; ++k) {
+            if (lo[k] < hi[k]) {
+                if (du[k] + wm[k] < threshold) lo[k] = mid[k] + 1;
+                else hi[k] = mid[k];
+                active |= lo[k] < hi[k];
+            }
+        }
+    }
+    int32_t c[kPrepItems];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        c[k] = kHeavy ? e[k] - lo[k] : lo[k] - b[k];
+        sum += c[k];
+    }
+    int32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int32_t run = incl - sum, total = 0;
+    for (int v = 0; v < kBT / 64; ++v) {
+        if (v < wave) run += s_wave[v];
+        total += s_wave[v];
+    }
+    const int32_t cnt = F - i0 < kBT * kPrepItems ? static_cast<int32_t>(F - i0) : kBT * kPrepItems;
+    if (tid == 0)
+        s_base = atomicAdd(alloc, (static_cast<unsigned long long>(cnt) << 32) | static_cast<uint32_t>(total));
+    __syncthreads();
+    const int32_t ebase = static_cast<int32_t>(s_base >> 32), xbase = static_cast<int32_t>(s_base & 0xFFFFFFFFu);
+#pragma unroll
+    for (int k = 0; k < kPrepItems; ++k) {
+        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
+        if (i < F) {
+            const int32_t slot = ebase + tid * kPrepItems + k;
+            scan[slot] = xbase + run;
+            start[slot] = kHeavy ? lo[k] : b[k];
+            vert[slot] = u[k];
+            run += c[k];
+        }
+    }
+    if (tid == 0 && ebase + cnt == F) scan[F] = xbase + total;
 }
 
 // Small queues (at most kSmallMax entries): one 1024-thread block resets the
-// pass's counters (as k_ctr_reset with no far list), splits the rows (as
-// k_row_split) and scans the counts, in place of five launches.
+// pass's counters (as k_ctr_reset with no far list), splits the rows and
+// scans the counts in queue order.
 constexpr int kSmallBT = 1024;
 constexpr int kSmallItems = 4;
 constexpr int kSmallMax = kSmallBT * kSmallItems;
@@ -127,7 +213,11 @@ __global__ void __launch_bounds__(kSmallBT) k_row_split_small(const int32_t* __r
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     unsigned long long* words = reinterpret_cast<unsigned long long*>(ctr->slots);
     for (int i = tid; i < kSlots * 8; i += kSmallBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
-    if (tid == 0) ctr->near_count = 0;
+    if (tid == 0) {
+        ctr->near_count = 0;
+        ctr->alloc[0] = 0;  // whichever k_row_prep runs next starts from zero
+        ctr->alloc[1] = 0;
+    }
     int32_t c[kSmallItems], st[kSmallItems];
     int32_t sum = 0;
 #pragma unroll
@@ -520,7 +610,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     int trace_len = 0;
 
     double* dist;
-    int32_t *qa, *qb, *scan, *aux, *stamp_near, *stamp_far, *far[2], *list, *qstart;
+    int32_t *qa, *qb, *scan, *qvert, *stamp_near, *stamp_far, *far[2], *list, *qstart;
     SsspCtr* ctr;
     void* p;
     GHET_CHECK(g->ws.get(WS_DIST, 8 * size_t(n), &p)); dist = static_cast<double*>(p);
@@ -533,7 +623,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     GHET_CHECK(g->ws.get(WS_STAMP_NEAR, 4 * size_t(n), &p)); stamp_near = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_STAMP_FAR, 4 * size_t(n), &p)); stamp_far = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_SSSP_VERT, 4 * size_t(n), &p)); qvert = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(SsspCtr), &p)); ctr = static_cast<SsspCtr*>(p);
     const SsspCtr* hc = reinterpret_cast<const SsspCtr*>(g->pinned);
 
@@ -578,6 +668,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     // prefix (round) or heavy rest (heavy pass).
     const SsspLayout& L = g->sl;
     // The pass also resets the counters it fills.
+    int64_t passes = 0;
     auto launch_pass = [&](bool heavy, const int32_t* q, int32_t Fq, int64_t E_max, int32_t next_round) -> int {
         if (Fq <= kSmallMax) {
             if (heavy)
@@ -587,16 +678,18 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
                 hipLaunchKernelGGL(k_row_split_small<false>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, g->out.rowptr, L.w,
                                    dist, threshold, scan, qstart, ctr);
         } else {
-            hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, -1);
-            const unsigned gq = static_cast<unsigned>(ceil_div(Fq, 256));
+            const unsigned gq = static_cast<unsigned>(ceil_div(Fq, kBT * kPrepItems));
+            unsigned long long* al = ctr->alloc + (passes & 1);
+            unsigned long long* next_al = ctr->alloc + ((passes + 1) & 1);
             if (heavy)
-                hipLaunchKernelGGL(k_row_split<true>, dim3(gq), dim3(256), 0, s, q, Fq, g->out.rowptr, L.w, dist,
-                                   threshold, scan, qstart);
+                hipLaunchKernelGGL(k_row_prep<true>, dim3(gq), dim3(kBT), 0, s, q, Fq, g->out.rowptr, L.w, dist,
+                                   threshold, scan, qstart, qvert, al, next_al, ctr);
             else
-                hipLaunchKernelGGL(k_row_split<false>, dim3(gq), dim3(256), 0, s, q, Fq, g->out.rowptr, L.w, dist,
-                                   threshold, scan, qstart);
-            GHET_CHECK(exclusive_scan_i32(scan, scan, Fq, aux, s));
+                hipLaunchKernelGGL(k_row_prep<false>, dim3(gq), dim3(kBT), 0, s, q, Fq, g->out.rowptr, L.w, dist,
+                                   threshold, scan, qstart, qvert, al, next_al, ctr);
+            q = qvert;  // the relax reads the entries in their reserved order
         }
+        ++passes;
         const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
         if (heavy)
             hipLaunchKernelGGL(k_relax<true>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
