@@ -31,7 +31,8 @@ struct BfsCtr {
     int32_t hubs;
     unsigned long long alloc;  // push prep: (entries << 32) | edges reserved so far
     int32_t conv;              // bitmap -> queue conversion
-    int32_t pad[11];
+    unsigned int done;         // blocks finished (mailbox ticket)
+    int32_t pad[10];
     CtrSlot slots[kSlots];
 };
 constexpr int kCtrWords = sizeof(BfsCtr) / 8;
@@ -156,15 +157,38 @@ __global__ void __launch_bounds__(kBT) k_push_prep(const int32_t* __restrict__ q
 }
 
 // Block totals of (count, out-edges, in-edges) into the block's slot.
-__device__ __forceinline__ void block_report(unsigned long long cnt, unsigned long long od, unsigned long long id,
-                                             BfsCtr* ctr, unsigned long long* s_red) {
+// Returns (in thread 0) a token of the atomics' results for post_level.
+__device__ __forceinline__ unsigned long long block_report(unsigned long long cnt, unsigned long long od,
+                                                           unsigned long long id, BfsCtr* ctr,
+                                                           unsigned long long* s_red) {
     cnt = block_sum_u64<kBT>(cnt, s_red);
     od = block_sum_u64<kBT>(od, s_red);
     id = block_sum_u64<kBT>(id, s_red);
+    unsigned long long token = 0;
     if (threadIdx.x == 0) {
-        slot_add(ctr->slots, 0, cnt);
-        slot_add(ctr->slots, 1, od);
-        slot_add(ctr->slots, 2, id);
+        token ^= slot_add_ret(ctr->slots, 0, cnt);
+        token ^= slot_add_ret(ctr->slots, 1, od);
+        token ^= slot_add_ret(ctr->slots, 2, id);
+    }
+    return token;
+}
+
+// The level's last block posts (discovered, their out-edges, their in-edges)
+// to the host mailbox.  Block-uniform; mail == nullptr posts nothing.
+__device__ __forceinline__ void post_level(BfsCtr* ctr, HostMail* mail, unsigned long long seq,
+                                           unsigned long long token) {
+    if (!mail) return;
+    if (arrive_last(ctr->slots, &ctr->done, token) && threadIdx.x < 64) {  // wave 0: lane i reads slot i
+        static_assert(kSlots == 64, "one slot per lane");
+        const unsigned long long a = wave_sum(load_agent(&ctr->slots[threadIdx.x].v[0]));
+        const unsigned long long b = wave_sum(load_agent(&ctr->slots[threadIdx.x].v[1]));
+        const unsigned long long c = wave_sum(load_agent(&ctr->slots[threadIdx.x].v[2]));
+        if (threadIdx.x == 0) {
+            mail->v[0] = a;
+            mail->v[1] = b;
+            mail->v[2] = c;
+            mail_post(mail, seq);
+        }
     }
 }
 
@@ -178,7 +202,8 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
                                                      const int32_t* __restrict__ col, uint32_t* visited,
                                                      int32_t* __restrict__ depth, int32_t next_level,
                                                      int32_t* __restrict__ next_q, BfsCtr* ctr,
-                                                     const int32_t* __restrict__ in_rowptr, BfsCtr* next_ctr) {
+                                                     const int32_t* __restrict__ in_rowptr, BfsCtr* next_ctr,
+                                                     HostMail* mail, unsigned long long seq) {
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
     __shared__ int32_t s_out[kOut];
@@ -253,7 +278,7 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
         lq_flush<kBT>(s_out, &s_cnt, &s_base, &ctr->qcount, next_q, kOut - kTile, on_item);
     }
     lq_flush<kBT>(s_out, &s_cnt, &s_base, &ctr->qcount, next_q, 0, on_item);
-    block_report(cnt, od, id, ctr, s_red);
+    post_level(ctr, mail, seq, block_report(cnt, od, id, ctr, s_red));
 }
 
 // Bits of the vertices without in-edges (and of the padding past n): no BFS
@@ -280,7 +305,8 @@ __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restri
                                               const int32_t* __restrict__ in_col, const uint32_t* __restrict__ cur_bm,
                                               uint32_t* __restrict__ next_bm, uint32_t* __restrict__ visited,
                                               int32_t* __restrict__ depth, int32_t next_level, BfsCtr* ctr,
-                                              const int32_t* __restrict__ out_rowptr, BfsCtr* next_ctr) {
+                                              const int32_t* __restrict__ out_rowptr, BfsCtr* next_ctr,
+                                              HostMail* mail, unsigned long long seq) {
     __shared__ unsigned long long s_red[kBT / 64];
     const int lane = lane_id();
     zero_next(next_ctr);
@@ -315,7 +341,7 @@ __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restri
             id += in_rowptr[v + 1] - in_rowptr[v];
         }
     }
-    block_report(cnt, od, id, ctr, s_red);
+    post_level(ctr, mail, seq, block_report(cnt, od, id, ctr, s_red));
 }
 
 __device__ __forceinline__ bool visit(int32_t v, uint32_t* visited, uint32_t* next_bm, int32_t* depth,
@@ -384,7 +410,8 @@ __global__ void __launch_bounds__(kBT) k_hub_expand(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ col, uint32_t* next_bm,
                                                     uint32_t* visited, int32_t* __restrict__ depth,
                                                     int32_t next_level, BfsCtr* ctr, const int32_t* hub_q,
-                                                    const int32_t* __restrict__ in_rowptr) {
+                                                    const int32_t* __restrict__ in_rowptr, HostMail* mail,
+                                                    unsigned long long seq) {
     __shared__ unsigned long long s_red[kBT / 64];
     const int32_t H = ctr->hubs;
     unsigned long long od = 0, id = 0, cnt = 0;
@@ -401,7 +428,7 @@ __global__ void __launch_bounds__(kBT) k_hub_expand(const int32_t* __restrict__ 
             }
         }
     }
-    block_report(cnt, od, id, ctr, s_red);
+    post_level(ctr, mail, seq, block_report(cnt, od, id, ctr, s_red));
 }
 
 __global__ void k_queue_to_bitmap(const int32_t* __restrict__ q, int32_t F, uint32_t* bm) {
@@ -541,7 +568,7 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     ghet_level_trace* trace = (o && o->size >= sizeof(ghet_opts)) ? o->trace : nullptr;
     const int trace_cap = trace ? o->trace_cap : 0;
     int trace_len = 0;
-    const BfsCtr* hc = reinterpret_cast<const BfsCtr*>(g->pinned);
+    GHET_CHECK(mail_init(g));
 
     if (have_in && !g->bfs_skip) {
         GHET_HIP(hipMalloc(reinterpret_cast<void**>(&g->bfs_skip), 4 * size_t(words)));
@@ -587,6 +614,7 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         ctr = ctrs + (level & 1);
         BfsCtr* next_ctr = ctrs + ((level + 1) & 1);
         bool next_zeroed = false;
+        unsigned long long seq = 0;  // mailbox sequence of the level's last launch (0: none)
         auto clear = [&]() {
             if (!zeroed[level & 1]) hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(512), 0, s, ctr);
             zeroed[level & 1] = true;
@@ -609,8 +637,9 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
                 const int32_t tile = static_cast<int32_t>(
                     std::min<int64_t>(kTile, std::max<int64_t>(64, ceil_div(ceil_div(E, 512), 64) * 64)));
                 const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, tile), kMaxBlocks));
+                seq = mail_next(g);
                 hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, start, F, scan, E, tile, g->out.rowptr,
-                                   g->out.col, visited, depth, next_level, qb, ctr, in_rp, next_ctr);
+                                   g->out.col, visited, depth, next_level, qb, ctr, in_rp, next_ctr, g->mail_dev, seq);
                 next_zeroed = true;
             }
             std::swap(qa, qb);
@@ -625,15 +654,17 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
                 have_bm = true;
             }
             if (mode == GHET_MODE_PULL) {
+                seq = mail_next(g);
                 hipLaunchKernelGGL(k_pull, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->in.rowptr, g->in.col,
-                                   bma, bmb, visited, depth, next_level, ctr, g->out.rowptr, next_ctr);
+                                   bma, bmb, visited, depth, next_level, ctr, g->out.rowptr, next_ctr, g->mail_dev, seq);
                 next_zeroed = true;
             } else {
                 GHET_CHECK(fill<uint32_t>(bmb, 0u, words, s));
                 hipLaunchKernelGGL(k_bitmap_topdown, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->out.rowptr,
                                    g->out.col, bma, bmb, visited, depth, next_level, ctr, hubq, in_rp);
+                seq = mail_next(g);
                 hipLaunchKernelGGL(k_hub_expand, dim3(1024), dim3(kBT), 0, s, g->out.rowptr, g->out.col, bmb, visited,
-                                   depth, next_level, ctr, hubq, in_rp);
+                                   depth, next_level, ctr, hubq, in_rp, g->mail_dev, seq);
             }
             std::swap(bma, bmb);
             have_bm = true;
@@ -642,10 +673,18 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         zeroed[level & 1] = false;
         zeroed[(level + 1) & 1] = next_zeroed;
         GHET_HIP(hipGetLastError());
-        GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
         if (tr) GHET_HIP(hipEventRecord(g->ev_k1, s));
-        GHET_HIP(hipStreamSynchronize(s));
+        // the level's totals arrive in the mailbox; a push level with no
+        // edges discovers nothing
+        unsigned long long found = 0, found_out = 0, found_in = 0;
+        if (seq) {
+            GHET_CHECK(mail_wait(g, s, seq));
+            found = g->mail->v[0];
+            found_out = g->mail->v[1];
+            found_in = g->mail->v[2];
+        }
         if (tr) {
+            GHET_HIP(hipEventSynchronize(g->ev_k1));
             float ms = 0;
             GHET_HIP(hipEventElapsedTime(&ms, g->ev_k0, g->ev_k1));
             trace[trace_len].mode = mode;
@@ -654,9 +693,9 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         }
         prev_F = F;
         prev_mode = mode;
-        F = static_cast<int32_t>(slot_sum(hc->slots, 0));
-        m_f = static_cast<int64_t>(slot_sum(hc->slots, 1));
-        visited_in += static_cast<int64_t>(slot_sum(hc->slots, 2));
+        F = static_cast<int32_t>(found);
+        m_f = static_cast<int64_t>(found_out);
+        visited_in += static_cast<int64_t>(found_in);
         n_visited += F;
         ++level;
     }

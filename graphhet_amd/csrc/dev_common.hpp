@@ -110,11 +110,69 @@ __host__ __device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b) { ret
 // the host.
 constexpr int kSlots = 64;
 struct alignas(64) CtrSlot {
-    unsigned long long v[8];
+    unsigned long long v[8];  // v[7]: arrive_last's group count
 };
 
 __device__ __forceinline__ void slot_add(CtrSlot* slots, int field, unsigned long long x) {
     if (x) atomicAdd(&slots[blockIdx.x & (kSlots - 1)].v[field], x);
+}
+// slot_add that waits for the atomic and returns the slot's previous value
+// (0 when x is 0): a token for arrive_last.
+__device__ __forceinline__ unsigned long long slot_add_ret(CtrSlot* slots, int field, unsigned long long x) {
+    return x ? atomicAdd(&slots[blockIdx.x & (kSlots - 1)].v[field], x) : 0ULL;
+}
+
+// Counter mailbox.  In place of a device-to-host copy and a stream
+// synchronisation after each level or pass, the launch's last block writes
+// the totals the host needs, then a sequence number, to fine-grained host
+// memory that the host polls (about 6 us per round trip on MI355X against
+// about 14 us for copy + synchronise).
+struct alignas(128) HostMail {
+    unsigned long long seq;
+    unsigned long long v[15];
+};
+
+// Called by every block after its counter atomics (block-uniform).  True in
+// the one block that arrives last; its counter reads then see every block's
+// atomics.  No fences: a device-scope fence writes back the XCD's L2 (all of
+// the level's dirty data) in every block.  Ordering comes from returned
+// values instead: thread 0 passes `token`, a value computed from what its
+// counter atomics returned, so the ticket atomic is issued only after they
+// have completed (the test on it never fails in practice).  Arrival is
+// counted in two levels so that no address takes more than about
+// gridDim.x / kSlots atomics: block b first counts in slots[b % kSlots].v[7],
+// and the last block of each group counts in *done.  The counters must be
+// zero before the launch.
+__device__ __forceinline__ bool arrive_last(CtrSlot* slots, unsigned int* done, unsigned long long token) {
+    __shared__ int s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int G = gridDim.x, grp = blockIdx.x & (kSlots - 1);
+        const unsigned int groups = G < kSlots ? G : kSlots;
+        const unsigned long long in_grp = (G - grp + kSlots - 1) / kSlots;
+        int last = 0;
+        if (token != ~0ULL && atomicAdd(&slots[grp].v[7], 1ULL) == in_grp - 1)
+            last = atomicAdd(done, 1u) == groups - 1;
+        s_last = last;
+    }
+    __syncthreads();
+    return s_last != 0;
+}
+
+__device__ __forceinline__ unsigned long long load_agent(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long slot_total(const CtrSlot* slots, int field) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kSlots; ++i) t += load_agent(&slots[i].v[field]);
+    return t;
+}
+
+// After the values: the sequence number, a system-scope release store (the
+// values land first).
+__device__ __forceinline__ void mail_post(HostMail* m, unsigned long long seq) {
+    __hip_atomic_store(&m->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 inline unsigned long long slot_sum(const CtrSlot* host_slots, int field) {

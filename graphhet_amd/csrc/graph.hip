@@ -5,6 +5,8 @@
 // release_device_memory (GPU_csr.hpp:120-215): instead of eight separate
 // cudaMallocs, one arena holds every CSR array at 256-byte-aligned offsets,
 // and the copies are queued on the graph's stream with one final sync.
+#include <atomic>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 
@@ -122,6 +124,7 @@ void destroy_graph(ghet_graph* g) {
     if (g->bfs_skip) hipFree(g->bfs_skip);
     if (g->arena) hipFree(g->arena);
     if (g->pinned) hipHostFree(g->pinned);
+    if (g->mail) hipHostFree(g->mail);
     if (g->ev_begin) hipEventDestroy(g->ev_begin);
     if (g->ev_end) hipEventDestroy(g->ev_end);
     if (g->ev_k0) hipEventDestroy(g->ev_k0);
@@ -132,6 +135,39 @@ void destroy_graph(ghet_graph* g) {
 }
 
 }  // namespace
+
+int mail_init(ghet_graph* g) {
+    if (g->mail) return GHET_OK;
+    void* h = nullptr;
+    GHET_HIP(hipHostMalloc(&h, sizeof(HostMail), hipHostMallocCoherent | hipHostMallocMapped));
+    void* d = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+    if (e != hipSuccess) {
+        hipHostFree(h);
+        GHET_HIP(e);
+    }
+    std::memset(h, 0, sizeof(HostMail));
+    g->mail = static_cast<HostMail*>(h);
+    g->mail_dev = static_cast<HostMail*>(d);
+    g->mail_seq = 0;
+    return GHET_OK;
+}
+
+int mail_wait(ghet_graph* g, hipStream_t s, unsigned long long seq) {
+    const volatile unsigned long long* p = &g->mail->seq;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t k = 1;; ++k) {
+        if (*p == seq) break;
+        if ((k & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+            GHET_HIP(hipStreamSynchronize(s));
+            if (*p != seq) return set_error(GHET_ERR_HIP, "counter mailbox: sequence %llu not posted",
+                                            static_cast<unsigned long long>(seq));
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return GHET_OK;
+}
 
 // A directed graph owning an arena laid out as ghet_graph_create lays it out
 // (out-CSR and weights, in-CSR and weights, all-CSR; absent parts have size

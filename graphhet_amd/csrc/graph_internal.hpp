@@ -145,10 +145,22 @@ struct ghet_graph {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
     int32_t* pinned = nullptr;  // pinned host buffer (kPinnedBytes) for per-level counters
+    ghet::HostMail* mail = nullptr;      // fine-grained host mailbox (host address), made on first use
+    ghet::HostMail* mail_dev = nullptr;  // its device address
+    unsigned long long mail_seq = 0;     // last sequence number handed out
     std::vector<hipEvent_t> kevents;  // per-launch timing events, created on demand
 };
 
 namespace ghet {
+
+// Creates the graph's counter mailbox on first use.
+int mail_init(ghet_graph* g);
+// Next sequence number for a launch that posts to the mailbox.
+inline unsigned long long mail_next(ghet_graph* g) { return ++g->mail_seq; }
+// Waits until the mailbox holds sequence `seq` (polling; after 20 ms it
+// synchronises the stream, which reports a failed launch, and checks once
+// more).
+int mail_wait(ghet_graph* g, hipStream_t s, unsigned long long seq);
 
 // Resolves the stream to launch on (opts->stream or the graph's own stream).
 hipStream_t pick_stream(ghet_graph* g, const ghet_opts* opts);
