@@ -44,7 +44,8 @@ struct SsspCtr {
     int32_t far_count[2];
     int32_t list_count;
     unsigned long long alloc[2];  // k_row_prep reservations, alternating by pass
-    int32_t pad[8];
+    unsigned int done;            // blocks finished (mailbox ticket)
+    int32_t pad[7];
     CtrSlot slots[kSlots];
 };
 constexpr int kCtrWords = sizeof(SsspCtr) / 8;
@@ -82,7 +83,38 @@ __global__ void k_ctr_reset(SsspCtr* c, int other) {
     for (int i = threadIdx.x; i < kSlots * 8; i += blockDim.x) w[i] = (i % 8 == 1) ? kInfBits : 0ULL;
     if (threadIdx.x == 0) {
         c->near_count = 0;
+        c->done = 0;
         if (other >= 0) c->far_count[other] = 0;
+    }
+}
+
+// The pass's last block posts what the host loop reads: slot sums v0 (edges
+// of the next near queue), v2 (edges of new list entries), v3 (edges
+// scanned), the slot minimum v1 (far-list distance bits), and the near, far
+// and list counts.  Block-uniform.
+__device__ __forceinline__ void post_pass(SsspCtr* ctr, HostMail* mail, unsigned long long seq,
+                                          unsigned long long token) {
+    if (arrive_last(ctr->slots, &ctr->done, token) && threadIdx.x < 64) {  // wave 0: lane i reads slot i
+        static_assert(kSlots == 64, "one slot per lane");
+        const CtrSlot* sl = ctr->slots + threadIdx.x;
+        const unsigned long long a = wave_sum(load_agent(&sl->v[0]));
+        const unsigned long long mn = wave_min(load_agent(&sl->v[1]));
+        const unsigned long long c = wave_sum(load_agent(&sl->v[2]));
+        const unsigned long long d = wave_sum(load_agent(&sl->v[3]));
+        if (threadIdx.x == 0) {
+            auto ld32 = [](const int32_t* p) {
+                return static_cast<unsigned long long>(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            };
+            mail->v[0] = a;
+            mail->v[1] = mn;
+            mail->v[2] = c;
+            mail->v[3] = d;
+            mail->v[4] = ld32(&ctr->near_count);
+            mail->v[5] = ld32(&ctr->far_count[0]);
+            mail->v[6] = ld32(&ctr->far_count[1]);
+            mail->v[7] = ld32(&ctr->list_count);
+            mail_post(mail, seq);
+        }
     }
 }
 
@@ -119,6 +151,7 @@ __global__ void __launch_bounds__(kBT) k_row_prep(const int32_t* __restrict__ q,
         for (int i = tid; i < kSlots * 8; i += kBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
         if (tid == 0) {
             ctr->near_count = 0;
+            ctr->done = 0;
             *next_alloc = 0;
         }
     }
@@ -215,6 +248,7 @@ __global__ void __launch_bounds__(kSmallBT) k_row_split_small(const int32_t* __r
     for (int i = tid; i < kSlots * 8; i += kSmallBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
     if (tid == 0) {
         ctr->near_count = 0;
+        ctr->done = 0;
         ctr->alloc[0] = 0;  // whichever k_row_prep runs next starts from zero
         ctr->alloc[1] = 0;
     }
@@ -281,7 +315,7 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
                                                int32_t* stamp_near, int32_t next_round, int32_t r0,
                                                int32_t* __restrict__ next_q, int32_t* __restrict__ list,
                                                int32_t* stamp_far, int32_t far_gen, int32_t* far_list, int far_sel,
-                                               SsspCtr* ctr) {
+                                               SsspCtr* ctr, HostMail* mail, unsigned long long seq) {
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
     __shared__ int32_t s_src[kTile];
@@ -299,7 +333,8 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
     auto on_list = [&](int32_t v) { ld += rowptr[v + 1] - rowptr[v]; };
     auto nop = [](int32_t) {};
     const int32_t E = scan[F];
-    if (blockIdx.x == 0 && tid == 0) slot_add(ctr->slots, 3, static_cast<unsigned long long>(E));
+    unsigned long long token = 0;  // thread 0: results of its counter atomics (post_pass)
+    if (blockIdx.x == 0 && tid == 0) token ^= slot_add_ret(ctr->slots, 3, static_cast<unsigned long long>(E));
     for (int64_t tile = blockIdx.x; tile * kTile < E; tile += gridDim.x) {
         const int32_t t0 = static_cast<int32_t>(tile * kTile);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
@@ -390,10 +425,11 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
         od = block_sum_u64<kBT>(od, s_red);
         ld = block_sum_u64<kBT>(ld, s_red);
         if (tid == 0) {
-            slot_add(ctr->slots, 0, od);
-            slot_add(ctr->slots, 2, ld);
+            token ^= slot_add_ret(ctr->slots, 0, od);
+            token ^= slot_add_ret(ctr->slots, 2, ld);
         }
     }
+    post_pass(ctr, mail, seq, token);
 }
 
 // Splits far list `sel` at new_thr: entries below old_thr are stale (already
@@ -404,7 +440,8 @@ __global__ void __launch_bounds__(kBT) k_far_split(const int32_t* __restrict__ f
                                                    const double* __restrict__ dist, double old_thr, double new_thr,
                                                    int32_t* stamp_near, int32_t next_round, int32_t* stamp_far,
                                                    int32_t new_gen, int32_t* __restrict__ list, int32_t* far_out,
-                                                   int other, const int32_t* __restrict__ rowptr, SsspCtr* ctr) {
+                                                   int other, const int32_t* __restrict__ rowptr, SsspCtr* ctr,
+                                                   HostMail* mail, unsigned long long seq) {
     __shared__ int32_t s_near[kOut], s_far[kOut];
     __shared__ int32_t s_nc, s_fc, s_base;
     __shared__ unsigned long long s_red[kBT / 64];
@@ -447,11 +484,13 @@ __global__ void __launch_bounds__(kBT) k_far_split(const int32_t* __restrict__ f
     dmin = wave_min(dmin);
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = dmin;
     __syncthreads();
+    unsigned long long token = 0;
     if (threadIdx.x == 0) {
         for (int w = 0; w < kBT / 64; ++w) dmin = s_red[w] < dmin ? s_red[w] : dmin;
-        slot_add(ctr->slots, 0, od);
-        if (dmin != kInfBits) atomicMin(&ctr->slots[blockIdx.x & (kSlots - 1)].v[1], dmin);
+        token ^= slot_add_ret(ctr->slots, 0, od);
+        if (dmin != kInfBits) token ^= atomicMin(&ctr->slots[blockIdx.x & (kSlots - 1)].v[1], dmin);
     }
+    post_pass(ctr, mail, seq, token);
 }
 
 // Parent = smallest u != v with an edge u->v and d[u] + w(u,v) == d[v]
@@ -625,7 +664,8 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SSSP_VERT, 4 * size_t(n), &p)); qvert = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(SsspCtr), &p)); ctr = static_cast<SsspCtr*>(p);
-    const SsspCtr* hc = reinterpret_cast<const SsspCtr*>(g->pinned);
+    GHET_CHECK(mail_init(g));
+    const HostMail* hm = g->mail;  // totals of the last pass (mail_wait)
 
     GHET_CHECK(build_sssp_layout(g));
     SchedState sched;
@@ -691,17 +731,17 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         }
         ++passes;
         const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
+        const unsigned long long seq = mail_next(g);
         if (heavy)
             hipLaunchKernelGGL(k_relax<true>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
                                L.w, dist, threshold, stamp_near, next_round, r0, nxt, list, stamp_far, far_gen,
-                               far[sel], sel, ctr);
+                               far[sel], sel, ctr, g->mail_dev, seq);
         else
             hipLaunchKernelGGL(k_relax<false>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
                                L.w, dist, threshold, stamp_near, next_round, r0, nxt, list, stamp_far, far_gen,
-                               far[sel], sel, ctr);
+                               far[sel], sel, ctr, g->mail_dev, seq);
         GHET_HIP(hipGetLastError());
-        GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(SsspCtr), hipMemcpyDeviceToHost, s));
-        GHET_HIP(hipStreamSynchronize(s));
+        GHET_CHECK(mail_wait(g, s, seq));
         return GHET_OK;
     };
     int64_t rounds = 0, relaxed = 0;
@@ -712,11 +752,11 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
             const int32_t next_round = round + 1;
             if (E > 0) {
                 GHET_CHECK(launch_pass(false, cur, F, E, next_round));
-                relaxed += static_cast<int64_t>(slot_sum(hc->slots, 3));
-                F = hc->near_count;
-                E = static_cast<int64_t>(slot_sum(hc->slots, 0));
-                list_edges += static_cast<int64_t>(slot_sum(hc->slots, 2));
-                list_count = hc->list_count;
+                relaxed += static_cast<int64_t>(hm->v[3]);
+                F = static_cast<int32_t>(hm->v[4]);
+                E = static_cast<int64_t>(hm->v[0]);
+                list_edges += static_cast<int64_t>(hm->v[2]);
+                list_count = static_cast<int32_t>(hm->v[7]);
                 list_len = list_count - list_base;
             } else {
                 F = 0;
@@ -730,8 +770,8 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         // all land at or above the threshold
         if (list_edges > 0) {
             GHET_CHECK(launch_pass(true, list + list_base, list_len, list_edges, round));
-            relaxed += static_cast<int64_t>(slot_sum(hc->slots, 3));
-            far_count = hc->far_count[sel];
+            relaxed += static_cast<int64_t>(hm->v[3]);
+            far_count = static_cast<int32_t>(hm->v[5 + sel]);
         }
         if (trace && trace_len < trace_cap) {
             GHET_HIP(hipEventRecord(g->ev_k1, s));
@@ -764,18 +804,17 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
             hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, other);
             const int32_t next_round = round + 1;
             unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, kBT), 1024)));
+            const unsigned long long seq = mail_next(g);
             hipLaunchKernelGGL(k_far_split, dim3(grid), dim3(kBT), 0, s, far[sel], far_count, dist, old_thr, new_thr,
                                stamp_near, next_round, stamp_far, far_gen + 1, list, far[other], other, g->out.rowptr,
-                               ctr);
+                               ctr, g->mail_dev, seq);
             GHET_HIP(hipGetLastError());
-            GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(SsspCtr), hipMemcpyDeviceToHost, s));
-            GHET_HIP(hipStreamSynchronize(s));
-            list_count = hc->list_count;
+            GHET_CHECK(mail_wait(g, s, seq));
+            list_count = static_cast<int32_t>(hm->v[7]);
             F = list_count - list_base;
-            E = static_cast<int64_t>(slot_sum(hc->slots, 0));
-            far_count = hc->far_count[other];
-            unsigned long long mb = kInfBits;
-            for (int i = 0; i < kSlots; ++i) mb = std::min(mb, hc->slots[i].v[1]);
+            E = static_cast<int64_t>(hm->v[0]);
+            far_count = static_cast<int32_t>(hm->v[5 + other]);
+            const unsigned long long mb = hm->v[1];
             double far_min;
             std::memcpy(&far_min, &mb, sizeof(double));
             sel = other;
