@@ -99,6 +99,8 @@ def main():
                     help="largest small graph is 2^this vertices (the full form uses 18)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-heuristic section")
+    ap.add_argument("--step-schedule", choices=("heuristic", "learned"), default="heuristic",
+                    help="BFS schedule of the timed step (SSSP always uses the heuristic width)")
     ap.add_argument("--sched-file", default=os.path.join(ROOT, "graphhet_amd", "sched_net.txt"),
                     help="trained schedule networks (python -m graphhet_amd.learn)")
     ap.add_argument("--gen", choices=("device", "host"), default="device",
@@ -130,15 +132,14 @@ def main():
     log(f"R-MAT scale {args.scale}: n={n} m={m} on the GPU in {time.time() - t0:.1f}s "
         f"({args.gen} generation, {dg.device_bytes() / 1e9:.2f} GB)")
     src = int(np.argmax(o.degrees()))
-    # BFS traverses with the trained mode network when it loads (trained on
-    # R-MAT scales 18-22; this graph is held out); SSSP keeps the heuristic
-    # bucket width, because the width network trained on scales 18-22 does
-    # not transfer to scale 24 (the learned section reports it).  The
-    # heuristic ("plain") form is timed beside it.  Outputs are identical
-    # whichever schedule runs.
-    sched = G.SCHED_HEURISTIC
-    if not args.no_learned and G.sched_load(args.sched_file):
-        sched = G.SCHED_LEARNED
+    # The step's BFS uses the heuristic mode rule unless --step-schedule
+    # learned: on this graph the trained mode network (R-MAT scales 18-22,
+    # this graph held out) is within a few percent of the heuristic, and the
+    # learned section below compares the two on the same sources.  SSSP keeps
+    # the heuristic bucket width (the width network does not transfer from
+    # scales 18-22 to 24).  Outputs are identical whichever schedule runs.
+    have_net = not args.no_learned and G.sched_load(args.sched_file)
+    sched = G.SCHED_LEARNED if (args.step_schedule == "learned" and have_net) else G.SCHED_HEURISTIC
     sched_name = "learned BFS modes, heuristic SSSP width" if sched == G.SCHED_LEARNED else "heuristic"
 
     def step():
@@ -188,7 +189,7 @@ def main():
                 ms = statistics.median(x.device_ms for x in sts)
                 plain[alg] = {"ms": round(ms, 3), "gteps": round(gteps(sts[0].edges_traversed, ms), 3)}
     learned = None
-    if sched == G.SCHED_LEARNED:
+    if have_net:
         learned = learned_section(G, dg, o, src)
 
     other = {}
