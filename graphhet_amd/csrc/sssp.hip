@@ -124,7 +124,7 @@ __device__ __forceinline__ void post_pass(SsspCtr* ctr, HostMail* mail, unsigned
 // rest.  Two launch shapes prepare a pass: k_row_prep and k_row_split_small.
 //
 // k_row_prep, for queues of more than kSmallMax entries, in one launch: each
-// block takes kPrepItems * kBT queue entries (thread-major), splits their
+// block takes kPrepItems * kPrepBT queue entries (thread-major), splits their
 // rows (the binary searches of a thread's entries advance together), scans
 // the counts and
 // reserves a range of entries and of edges with one 64-bit atomic on alloc
@@ -134,28 +134,29 @@ __device__ __forceinline__ void post_pass(SsspCtr* ctr, HostMail* mail, unsigned
 // Block 0 also resets the pass's counters (as k_ctr_reset with no far list;
 // only the relax launch after this one uses them) and clears next_alloc, the
 // other reservation counter, for the next pass.
+constexpr int kPrepBT = 1024;  // entries per k_row_prep block: one reservation atomic each
 constexpr int kPrepItems = 1;
 
 template <bool kHeavy>
-__global__ void __launch_bounds__(kBT) k_row_prep(const int32_t* __restrict__ q, int32_t F,
+__global__ void __launch_bounds__(kPrepBT) k_row_prep(const int32_t* __restrict__ q, int32_t F,
                                                   const int32_t* __restrict__ rowptr, const double* __restrict__ w,
                                                   const double* __restrict__ dist, double threshold,
                                                   int32_t* __restrict__ scan, int32_t* __restrict__ start,
                                                   int32_t* __restrict__ vert, unsigned long long* alloc,
                                                   unsigned long long* next_alloc, SsspCtr* ctr) {
-    __shared__ int32_t s_wave[kBT / 64];
+    __shared__ int32_t s_wave[kPrepBT / 64];
     __shared__ unsigned long long s_base;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (blockIdx.x == 0) {
         unsigned long long* words = reinterpret_cast<unsigned long long*>(ctr->slots);
-        for (int i = tid; i < kSlots * 8; i += kBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
+        for (int i = tid; i < kSlots * 8; i += kPrepBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
         if (tid == 0) {
             ctr->near_count = 0;
             ctr->done = 0;
             *next_alloc = 0;
         }
     }
-    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kBT * kPrepItems;
+    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kPrepBT * kPrepItems;
     int32_t u[kPrepItems], lo[kPrepItems], hi[kPrepItems], b[kPrepItems], e[kPrepItems];
     double du[kPrepItems];
 #pragma unroll
@@ -205,11 +206,11 @@ __global__ void __launch_bounds__(kBT) k_row_prep(const int32_t* __restrict__ q,
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
     int32_t run = incl - sum, total = 0;
-    for (int v = 0; v < kBT / 64; ++v) {
+    for (int v = 0; v < kPrepBT / 64; ++v) {
         if (v < wave) run += s_wave[v];
         total += s_wave[v];
     }
-    const int32_t cnt = F - i0 < kBT * kPrepItems ? static_cast<int32_t>(F - i0) : kBT * kPrepItems;
+    const int32_t cnt = F - i0 < kPrepBT * kPrepItems ? static_cast<int32_t>(F - i0) : kPrepBT * kPrepItems;
     if (tid == 0)
         s_base = atomicAdd(alloc, (static_cast<unsigned long long>(cnt) << 32) | static_cast<uint32_t>(total));
     __syncthreads();
@@ -718,14 +719,14 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
                 hipLaunchKernelGGL(k_row_split_small<false>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, g->out.rowptr, L.w,
                                    dist, threshold, scan, qstart, ctr);
         } else {
-            const unsigned gq = static_cast<unsigned>(ceil_div(Fq, kBT * kPrepItems));
+            const unsigned gq = static_cast<unsigned>(ceil_div(Fq, kPrepBT * kPrepItems));
             unsigned long long* al = ctr->alloc + (passes & 1);
             unsigned long long* next_al = ctr->alloc + ((passes + 1) & 1);
             if (heavy)
-                hipLaunchKernelGGL(k_row_prep<true>, dim3(gq), dim3(kBT), 0, s, q, Fq, g->out.rowptr, L.w, dist,
+                hipLaunchKernelGGL(k_row_prep<true>, dim3(gq), dim3(kPrepBT), 0, s, q, Fq, g->out.rowptr, L.w, dist,
                                    threshold, scan, qstart, qvert, al, next_al, ctr);
             else
-                hipLaunchKernelGGL(k_row_prep<false>, dim3(gq), dim3(kBT), 0, s, q, Fq, g->out.rowptr, L.w, dist,
+                hipLaunchKernelGGL(k_row_prep<false>, dim3(gq), dim3(kPrepBT), 0, s, q, Fq, g->out.rowptr, L.w, dist,
                                    threshold, scan, qstart, qvert, al, next_al, ctr);
             q = qvert;  // the relax reads the entries in their reserved order
         }
