@@ -67,6 +67,29 @@ __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_
     return lo;
 }
 
+// End of the light prefix of a weight-ordered row [b, e): the first i with
+// du + w[i] >= thr (e if none).  Galloping from b (probes b, b+2, b+5,
+// b+9, ...: gaps of 1, 2, 4, ...) and then a binary search inside the last
+// gap.  Light prefixes are short next to their rows (about a bucket width
+// out of weights spread over (0, 1]), and the first probes share a cache
+// line, so most rows take one memory round trip instead of log2(degree).
+__device__ __forceinline__ int32_t light_end(const double* __restrict__ w, int32_t b, int32_t e, double du,
+                                             double thr) {
+    int32_t lo = b, probe = b, gap = 1;  // every i < lo is light
+    while (probe < e && du + w[probe] < thr) {
+        lo = probe + 1;
+        probe = lo + gap;
+        gap <<= 1;
+    }
+    int32_t hi = probe < e ? probe : e;  // w[hi] is heavy, or hi == e
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (du + w[mid] < thr) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __global__ void k_sssp_seed(double* dist, int32_t* list, int32_t* stamp_near, int32_t src, SsspCtr* ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         dist[src] = 0.0;
@@ -125,8 +148,7 @@ __device__ __forceinline__ void post_pass(SsspCtr* ctr, HostMail* mail, unsigned
 //
 // k_row_prep, for queues of more than kSmallMax entries, in one launch: each
 // block takes kPrepItems * kPrepBT queue entries (thread-major), splits their
-// rows (the binary searches of a thread's entries advance together), scans
-// the counts and
+// rows (light_end), scans the counts and
 // reserves a range of entries and of edges with one 64-bit atomic on alloc
 // ((entries << 32) | edges).  At its entry range it writes scan[] (first
 // edge), start[] (first position) and vert[] (the vertex): blocks land in any
@@ -157,8 +179,7 @@ __global__ void __launch_bounds__(kPrepBT) k_row_prep(const int32_t* __restrict_
         }
     }
     const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kPrepBT * kPrepItems;
-    int32_t u[kPrepItems], lo[kPrepItems], hi[kPrepItems], b[kPrepItems], e[kPrepItems];
-    double du[kPrepItems];
+    int32_t u[kPrepItems], lo[kPrepItems], b[kPrepItems], e[kPrepItems];
 #pragma unroll
     for (int k = 0; k < kPrepItems; ++k) {
         const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
@@ -169,27 +190,7 @@ __global__ void __launch_bounds__(kPrepBT) k_row_prep(const int32_t* __restrict_
         const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
         b[k] = rowptr[u[k]];
         e[k] = i < F ? rowptr[u[k] + 1] : b[k];
-        du[k] = dist[u[k]];
-        lo[k] = b[k];
-        hi[k] = e[k];
-    }
-    for (bool active = true; active;) {
-        active = false;
-        int32_t mid[kPrepItems];
-        double wm[kPrepItems];
-#pragma unroll
-        for (int k = 0; k < kPrepItems; ++k) {
-            mid[k] = (lo[k] + hi[k]) >> 1;
-            wm[k] = lo[k] < hi[k] ? w[mid[k]] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < kPrepItems; ++k) {
-            if (lo[k] < hi[k]) {
-                if (du[k] + wm[k] < threshold) lo[k] = mid[k] + 1;
-                else hi[k] = mid[k];
-                active |= lo[k] < hi[k];
-            }
-        }
+        lo[k] = light_end(w, b[k], e[k], dist[u[k]], threshold);
     }
     int32_t c[kPrepItems];
     int32_t sum = 0;
@@ -263,13 +264,7 @@ __global__ void __launch_bounds__(kSmallBT) k_row_split_small(const int32_t* __r
         if (i < F) {
             const int32_t u = q[i];
             const int32_t b = rowptr[u], e = rowptr[u + 1];
-            const double du = dist[u];
-            int32_t lo = b, hi = e;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi) >> 1;
-                if (du + w[mid] < threshold) lo = mid + 1;
-                else hi = mid;
-            }
+            const int32_t lo = light_end(w, b, e, dist[u], threshold);
             c[k] = kHeavy ? e - lo : lo - b;
             st[k] = kHeavy ? lo : b;
         }
