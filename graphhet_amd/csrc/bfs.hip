@@ -44,6 +44,7 @@ constexpr int kOut = 2 * kTile;      // LDS output queue capacity
 constexpr int kHubDeg = 4096;
 constexpr int kMaxBlocks = 2048;     // persistent grid cap (8 blocks per CU)
 constexpr int kPrepItems = 4;        // queue entries per thread of k_push_prep
+constexpr int kPullSets = 2;         // vertices a k_pull thread scans side by side
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
     int lo = 0, hi = len;
@@ -298,9 +299,13 @@ __global__ void __launch_bounds__(kBT) k_skip_mask(int32_t n, const int32_t* __r
     }
 }
 
-// PULL: bottom-up over the in-CSR.  Each block iteration covers 256
-// consecutive vertices; wave w owns 64 of them = bitmap words 2g, 2g+1, so the
-// next frontier and the visited bits are written without atomics.
+// PULL: bottom-up over the in-CSR.  Each block iteration covers kPullSets
+// sets of 256 consecutive vertices; in each set wave w owns 64 of them =
+// bitmap words 2g, 2g+1, so the next frontier and the visited bits are
+// written without atomics.  A thread scans the in-lists of its kPullSets
+// vertices side by side, one in-edge of each per step, so their loads are in
+// flight together without probing more edges than a one-vertex scan.
+template <int kPullSets>
 __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restrict__ in_rowptr,
                                               const int32_t* __restrict__ in_col, const uint32_t* __restrict__ cur_bm,
                                               uint32_t* __restrict__ next_bm, uint32_t* __restrict__ visited,
@@ -311,34 +316,58 @@ __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restri
     const int lane = lane_id();
     zero_next(next_ctr);
     unsigned long long od = 0, id = 0, cnt = 0;
-    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < n; base += static_cast<int64_t>(gridDim.x) * kBT) {
-        const int64_t v = base + threadIdx.x;
-        const int64_t g = v >> 6;
-        const uint32_t vis = visited[2 * g + (lane >> 5)];
-        bool found = false;
-        if (v < n && !((vis >> (lane & 31)) & 1u)) {
-            const int32_t b = in_rowptr[v], e = in_rowptr[v + 1];
-            for (int32_t i = b; i < e; ++i) {
-                int32_t u = in_col[i];
-                if ((cur_bm[u >> 5] >> (u & 31)) & 1u) {
-                    found = true;
-                    break;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT * kPullSets; base < n;
+         base += static_cast<int64_t>(gridDim.x) * kBT * kPullSets) {
+        int64_t v[kPullSets];
+        uint32_t vis[kPullSets];
+        int32_t i[kPullSets], e[kPullSets];
+        bool found[kPullSets];
+#pragma unroll
+        for (int p = 0; p < kPullSets; ++p) {
+            v[p] = base + p * kBT + threadIdx.x;
+            vis[p] = visited[2 * (v[p] >> 6) + (lane >> 5)];
+        }
+#pragma unroll
+        for (int p = 0; p < kPullSets; ++p) {
+            const bool need = v[p] < n && !((vis[p] >> (lane & 31)) & 1u);
+            i[p] = need ? in_rowptr[v[p]] : 0;
+            e[p] = need ? in_rowptr[v[p] + 1] : 0;
+            found[p] = false;
+        }
+        for (bool more = true; more;) {
+            int32_t u[kPullSets];
+            uint32_t wd[kPullSets];
+#pragma unroll
+            for (int p = 0; p < kPullSets; ++p) u[p] = i[p] < e[p] ? in_col[i[p]] : -1;
+#pragma unroll
+            for (int p = 0; p < kPullSets; ++p) wd[p] = u[p] >= 0 ? cur_bm[u[p] >> 5] : 0u;
+            more = false;
+#pragma unroll
+            for (int p = 0; p < kPullSets; ++p) {
+                if (u[p] >= 0) {
+                    found[p] = ((wd[p] >> (u[p] & 31)) & 1u) != 0;
+                    i[p] = found[p] ? e[p] : i[p] + 1;  // a found vertex stops scanning
+                    more |= i[p] < e[p];
                 }
             }
         }
-        const uint64_t mask = __ballot(found);
-        if (lane == 0) {
-            next_bm[2 * g] = static_cast<uint32_t>(mask);
-            visited[2 * g] = vis | static_cast<uint32_t>(mask);
-        } else if (lane == 32) {
-            next_bm[2 * g + 1] = static_cast<uint32_t>(mask >> 32);
-            visited[2 * g + 1] = vis | static_cast<uint32_t>(mask >> 32);
-        }
-        if (found) {
-            depth[v] = next_level;
-            ++cnt;
-            od += out_rowptr[v + 1] - out_rowptr[v];
-            id += in_rowptr[v + 1] - in_rowptr[v];
+#pragma unroll
+        for (int p = 0; p < kPullSets; ++p) {
+            const int64_t g = v[p] >> 6;
+            const uint64_t mask = __ballot(found[p]);
+            if (lane == 0) {
+                next_bm[2 * g] = static_cast<uint32_t>(mask);
+                visited[2 * g] = vis[p] | static_cast<uint32_t>(mask);
+            } else if (lane == 32) {
+                next_bm[2 * g + 1] = static_cast<uint32_t>(mask >> 32);
+                visited[2 * g + 1] = vis[p] | static_cast<uint32_t>(mask >> 32);
+            }
+            if (found[p]) {
+                depth[v[p]] = next_level;
+                ++cnt;
+                od += out_rowptr[v[p] + 1] - out_rowptr[v[p]];
+                id += in_rowptr[v[p] + 1] - in_rowptr[v[p]];
+            }
         }
     }
     post_level(ctr, mail, seq, block_report(cnt, od, id, ctr, s_red));
@@ -543,8 +572,9 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     const bool have_in = g->in.present();
     if (!have_in && (schedule == GHET_SCHED_PULL || schedule == GHET_SCHED_REFERENCE)) schedule = GHET_SCHED_PUSH;
 
-    // 8 words per 256 vertices: block iterations of 256 never index past the end
-    const int64_t words = 8 * ceil_div(std::max(n, 1), 256);
+    // 8 words per 256 vertices, padded to whole k_pull block iterations
+    // (kPullSets * 256 vertices), so no iteration indexes past the end
+    const int64_t words = 8 * kPullSets * ceil_div(std::max(n, 1), 256 * kPullSets);
     int32_t *depth, *qa, *qb, *scan, *start, *hubq;
     uint32_t *visited, *bma, *bmb;
     BfsCtr *ctrs, *ctr;
@@ -655,7 +685,7 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
             }
             if (mode == GHET_MODE_PULL) {
                 seq = mail_next(g);
-                hipLaunchKernelGGL(k_pull, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->in.rowptr, g->in.col,
+                hipLaunchKernelGGL(k_pull<kPullSets>, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->in.rowptr, g->in.col,
                                    bma, bmb, visited, depth, next_level, ctr, g->out.rowptr, next_ctr, g->mail_dev, seq);
                 next_zeroed = true;
             } else {
