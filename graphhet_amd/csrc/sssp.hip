@@ -56,6 +56,7 @@ constexpr int kItems = 4;
 constexpr int kTile = kBT * kItems;  // edges per relax tile
 constexpr int kOut = 2 * kTile;      // LDS capacity of each staged queue
 constexpr int kMaxBlocks = 2048;
+constexpr int kSplitItems = 4;  // far-list entries per k_far_split thread and step
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
     int lo = 0, hi = len;
@@ -449,30 +450,38 @@ __global__ void __launch_bounds__(kBT) k_far_split(const int32_t* __restrict__ f
     unsigned long long od = 0, dmin = kInfBits;
     auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
     auto nop = [](int32_t) {};
-    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < count;
-         base += static_cast<int64_t>(gridDim.x) * kBT) {
-        const int64_t i = base + threadIdx.x;
-        bool to_near = false, keep = false;
-        int32_t v = 0;
-        if (i < count) {
-            v = far_in[i];
-            const double d = dist[v];
-            if (d >= old_thr) {
-                if (d < new_thr) {
-                    stamp_near[v] = next_round;
+    // kSplitItems entries per thread per step: their vertex and distance
+    // loads are in flight together
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT * kSplitItems; base < count;
+         base += static_cast<int64_t>(gridDim.x) * kBT * kSplitItems) {
+        int32_t v[kSplitItems];
+        double d[kSplitItems];
+#pragma unroll
+        for (int k = 0; k < kSplitItems; ++k) {
+            const int64_t i = base + k * kBT + threadIdx.x;
+            v[k] = i < count ? far_in[i] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kSplitItems; ++k) d[k] = v[k] >= 0 ? dist[v[k]] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kSplitItems; ++k) {
+            bool to_near = false, keep = false;
+            if (v[k] >= 0 && d[k] >= old_thr) {
+                if (d[k] < new_thr) {
+                    stamp_near[v[k]] = next_round;
                     to_near = true;
                 } else {
-                    stamp_far[v] = new_gen;
+                    stamp_far[v[k]] = new_gen;
                     keep = true;
-                    const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(d));
+                    const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(d[k]));
                     dmin = bits < dmin ? bits : dmin;
                 }
             }
+            lq_push(s_near, &s_nc, to_near, v[k]);
+            lq_push(s_far, &s_fc, keep, v[k]);
         }
-        lq_push(s_near, &s_nc, to_near, v);
-        lq_push(s_far, &s_fc, keep, v);
-        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, kOut - kBT, on_near);
-        lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, kOut - kBT, nop);
+        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, kOut - kBT * kSplitItems, on_near);
+        lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, kOut - kBT * kSplitItems, nop);
     }
     lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, 0, on_near);
     lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, 0, nop);
@@ -799,7 +808,8 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
             const int other = 1 - sel;
             hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, other);
             const int32_t next_round = round + 1;
-            unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, kBT), 1024)));
+            unsigned grid = static_cast<unsigned>(
+                std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, kBT * kSplitItems), 1024)));
             const unsigned long long seq = mail_next(g);
             hipLaunchKernelGGL(k_far_split, dim3(grid), dim3(kBT), 0, s, far[sel], far_count, dist, old_thr, new_thr,
                                stamp_near, next_round, stamp_far, far_gen + 1, list, far[other], other, g->out.rowptr,
