@@ -58,11 +58,11 @@ __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_
 
 // Level-0 state in one pass: depth INT_MAX except the source (0), visited =
 // the no-in-edge mask (none without an in-CSR) plus the source, queue =
-// {src}, both counter buffers zero.
+// {src}, both counter buffers zero, and clear_bm (if given) zero.
 __global__ void __launch_bounds__(kBT) k_bfs_init(int32_t n, int64_t words, const uint32_t* __restrict__ skip,
                                                   int32_t src, int32_t* __restrict__ depth,
                                                   uint32_t* __restrict__ visited, int32_t* __restrict__ queue,
-                                                  BfsCtr* ctrs) {
+                                                  BfsCtr* ctrs, uint32_t* __restrict__ clear_bm) {
     if (blockIdx.x == 0) {  // both counter buffers
         unsigned long long* w = reinterpret_cast<unsigned long long*>(ctrs);
         for (int i = threadIdx.x; i < 2 * kCtrWords; i += kBT) w[i] = 0;
@@ -75,6 +75,7 @@ __global__ void __launch_bounds__(kBT) k_bfs_init(int32_t n, int64_t words, cons
             uint32_t w = skip ? skip[i] : 0u;
             if (i == (src >> 5)) w |= 1u << (src & 31);
             visited[i] = w;
+            if (clear_bm) clear_bm[i] = 0u;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) queue[0] = src;
@@ -195,7 +196,9 @@ __device__ __forceinline__ void post_level(BfsCtr* ctr, HostMail* mail, unsigned
 
 // PUSH: persistent blocks over tiles of tile_edges (<= kTile) consecutive
 // frontier edges; entry j of the level covers edges [scan[j], scan[j+1]) of
-// the row starting at start[j].
+// the row starting at start[j].  Discoveries go to the next queue and, when
+// next_bm is given (a cleared bitmap), to the next frontier bitmap too, so a
+// pull level after a large push level needs no queue-to-bitmap pass.
 // Discoveries are staged in LDS and flushed to the next queue in bulk.
 __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__ start, int32_t F,
                                                      const int32_t* __restrict__ scan, int32_t E, int32_t tile_edges,
@@ -204,7 +207,7 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
                                                      int32_t* __restrict__ depth, int32_t next_level,
                                                      int32_t* __restrict__ next_q, BfsCtr* ctr,
                                                      const int32_t* __restrict__ in_rowptr, BfsCtr* next_ctr,
-                                                     HostMail* mail, unsigned long long seq) {
+                                                     uint32_t* next_bm, HostMail* mail, unsigned long long seq) {
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
     __shared__ int32_t s_out[kOut];
@@ -272,6 +275,7 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
                 if (!(old & bit)) {
                     depth[v[k]] = next_level;
                     disc = true;
+                    if (next_bm) atomicOr(&next_bm[v[k] >> 5], bit);
                 }
             }
             lq_push(s_out, &s_cnt, disc, v[k]);
@@ -609,7 +613,9 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_bfs_init, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(std::max<int64_t>(n, words), kBT), 8192))),
-                       dim3(kBT), 0, s, n, words, have_in ? g->bfs_skip : nullptr, src, depth, visited, qa, ctrs);
+                       dim3(kBT), 0, s, n, words, have_in ? g->bfs_skip : nullptr, src, depth, visited, qa, ctrs,
+                       have_in ? bma : nullptr);
+    bool bma_clear = have_in;  // bma is all zero (until a level writes it)
     bool zeroed[2] = {true, true};  // ctrs[i] known to be all zero
 
     int32_t F = 1;
@@ -661,6 +667,10 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
             hipLaunchKernelGGL(k_push_prep, dim3(static_cast<unsigned>(ceil_div(F, kBT * kPrepItems))), dim3(kBT), 0, s,
                                qa, F, g->out.rowptr, scan, start, ctr);
             const int32_t E = static_cast<int32_t>(m_f);
+            // a level this large is usually followed by a pull level: write
+            // its frontier bitmap on the way
+            const bool want_bm = E > 0 && E >= n / 128 && have_in;
+            if (want_bm && !bma_clear) GHET_CHECK(fill<uint32_t>(bma, 0u, words, s));
             if (E > 0) {
                 // about 512 tiles while the level is small, kTile-edge tiles
                 // over a persistent grid once it is large
@@ -669,12 +679,13 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
                 const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, tile), kMaxBlocks));
                 seq = mail_next(g);
                 hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, start, F, scan, E, tile, g->out.rowptr,
-                                   g->out.col, visited, depth, next_level, qb, ctr, in_rp, next_ctr, g->mail_dev, seq);
+                                   g->out.col, visited, depth, next_level, qb, ctr, in_rp, next_ctr,
+                                   want_bm ? bma : nullptr, g->mail_dev, seq);
                 next_zeroed = true;
             }
             std::swap(qa, qb);
             have_q = true;
-            have_bm = false;
+            have_bm = want_bm;  // bma then holds the same frontier
         } else {
             clear();
             if (!have_bm) {
@@ -702,6 +713,7 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         }
         zeroed[level & 1] = false;
         zeroed[(level + 1) & 1] = next_zeroed;
+        bma_clear = false;
         GHET_HIP(hipGetLastError());
         if (tr) GHET_HIP(hipEventRecord(g->ev_k1, s));
         // the level's totals arrive in the mailbox; a push level with no
