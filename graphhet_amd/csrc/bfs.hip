@@ -303,6 +303,18 @@ __global__ void __launch_bounds__(kBT) k_skip_mask(int32_t n, const int32_t* __r
     }
 }
 
+// First in-neighbour of every vertex (-1 without in-edges): the first probe
+// of a pull scan then reads a coalesced array instead of one scattered in-CSR
+// line per vertex.
+__global__ void __launch_bounds__(kBT) k_first_in(int32_t n, const int32_t* __restrict__ in_rowptr,
+                                                  const int32_t* __restrict__ in_col, int32_t* __restrict__ first) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int32_t b = in_rowptr[v];
+        first[v] = in_rowptr[v + 1] > b ? in_col[b] : -1;
+    }
+}
+
 // PULL: bottom-up over the in-CSR.  Each block iteration covers kPullSets
 // sets of 256 consecutive vertices; in each set wave w owns 64 of them =
 // bitmap words 2g, 2g+1, so the next frontier and the visited bits are
@@ -311,7 +323,8 @@ __global__ void __launch_bounds__(kBT) k_skip_mask(int32_t n, const int32_t* __r
 // flight together without probing more edges than a one-vertex scan.
 template <int kPullSets>
 __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restrict__ in_rowptr,
-                                              const int32_t* __restrict__ in_col, const uint32_t* __restrict__ cur_bm,
+                                              const int32_t* __restrict__ in_col, const int32_t* __restrict__ first_in,
+                                              const uint32_t* __restrict__ cur_bm,
                                               uint32_t* __restrict__ next_bm, uint32_t* __restrict__ visited,
                                               int32_t* __restrict__ depth, int32_t next_level, BfsCtr* ctr,
                                               const int32_t* __restrict__ out_rowptr, BfsCtr* next_ctr,
@@ -331,12 +344,26 @@ __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restri
             v[p] = base + p * kBT + threadIdx.x;
             vis[p] = visited[2 * (v[p] >> 6) + (lane >> 5)];
         }
+        int32_t fst[kPullSets];
 #pragma unroll
         for (int p = 0; p < kPullSets; ++p) {
             const bool need = v[p] < n && !((vis[p] >> (lane & 31)) & 1u);
             i[p] = need ? in_rowptr[v[p]] : 0;
             e[p] = need ? in_rowptr[v[p] + 1] : 0;
+            fst[p] = need ? first_in[v[p]] : -1;
             found[p] = false;
+        }
+        {  // first probe: the coalesced first in-neighbour
+            uint32_t wd[kPullSets];
+#pragma unroll
+            for (int p = 0; p < kPullSets; ++p) wd[p] = fst[p] >= 0 ? cur_bm[fst[p] >> 5] : 0u;
+#pragma unroll
+            for (int p = 0; p < kPullSets; ++p) {
+                if (fst[p] >= 0) {
+                    found[p] = ((wd[p] >> (fst[p] & 31)) & 1u) != 0;
+                    i[p] = found[p] ? e[p] : i[p] + 1;
+                }
+            }
         }
         for (bool more = true; more;) {
             int32_t u[kPullSets];
@@ -604,6 +631,12 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     int trace_len = 0;
     GHET_CHECK(mail_init(g));
 
+    if (have_in && !g->bfs_first) {
+        GHET_HIP(hipMalloc(reinterpret_cast<void**>(&g->bfs_first), 4 * size_t(std::max(n, 1))));
+        hipLaunchKernelGGL(k_first_in, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(n, kBT), 8192))),
+                           dim3(kBT), 0, s, n, g->in.rowptr, g->in.col, g->bfs_first);
+        GHET_HIP(hipGetLastError());
+    }
     if (have_in && !g->bfs_skip) {
         GHET_HIP(hipMalloc(reinterpret_cast<void**>(&g->bfs_skip), 4 * size_t(words)));
         hipLaunchKernelGGL(k_skip_mask, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(words, kBT), 4096))),
@@ -697,7 +730,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
             if (mode == GHET_MODE_PULL) {
                 seq = mail_next(g);
                 hipLaunchKernelGGL(k_pull<kPullSets>, dim3(grid_for_vertices(n)), dim3(kBT), 0, s, n, g->in.rowptr, g->in.col,
-                                   bma, bmb, visited, depth, next_level, ctr, g->out.rowptr, next_ctr, g->mail_dev, seq);
+                                   g->bfs_first, bma, bmb, visited, depth, next_level, ctr, g->out.rowptr, next_ctr,
+                                   g->mail_dev, seq);
                 next_zeroed = true;
             } else {
                 GHET_CHECK(fill<uint32_t>(bmb, 0u, words, s));

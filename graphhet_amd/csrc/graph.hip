@@ -122,6 +122,7 @@ void destroy_graph(ghet_graph* g) {
     if (g->ms.mem) hipFree(g->ms.mem);
     if (g->sl.mem) hipFree(g->sl.mem);
     if (g->bfs_skip) hipFree(g->bfs_skip);
+    if (g->bfs_first) hipFree(g->bfs_first);
     if (g->arena) hipFree(g->arena);
     if (g->pinned) hipHostFree(g->pinned);
     if (g->mail) hipHostFree(g->mail);

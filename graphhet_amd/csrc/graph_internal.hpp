@@ -141,6 +141,7 @@ struct ghet_graph {
     ghet::SsspLayout sl;
     double mean_w = -1.0;  // mean out-edge weight (SSSP bucket width), reduced on first use
     uint32_t* bfs_skip = nullptr;  // BFS: bit set for vertices no pull can reach (no in-edges), built on first use
+    int32_t* bfs_first = nullptr;  // BFS: first in-neighbour of each vertex (-1 if none), built with bfs_skip
     ghet::Tuning tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
