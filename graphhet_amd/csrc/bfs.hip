@@ -785,8 +785,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
-    if (depth_out) GHET_HIP(hipMemcpyAsync(depth_out, depth, 4 * size_t(n), hipMemcpyDeviceToHost, s));
-    if (pre_out) GHET_HIP(hipMemcpyAsync(pre_out, g->ws.ptr[WS_PRE], 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    if (depth_out) GHET_CHECK(copy_out(g, depth_out, depth, 4 * size_t(n), s));
+    if (pre_out) GHET_CHECK(copy_out(g, pre_out, g->ws.ptr[WS_PRE], 4 * size_t(n), s));
     if (st) {
         CtrSlot* slots = reinterpret_cast<CtrSlot*>(ctr);
         GHET_CHECK(reached_stats_i32(depth, g->out.rowptr, n, slots, s));

@@ -457,7 +457,7 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
-    if (label_out) GHET_HIP(hipMemcpyAsync(label_out, la, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    if (label_out) GHET_CHECK(copy_out(g, label_out, la, 4 * size_t(n), s));
     GHET_HIP(hipStreamSynchronize(s));
     if (st) {
         double hot = 0;

@@ -126,6 +126,7 @@ void destroy_graph(ghet_graph* g) {
     if (g->arena) hipFree(g->arena);
     if (g->pinned) hipHostFree(g->pinned);
     if (g->mail) hipHostFree(g->mail);
+    if (g->stage) hipHostFree(g->stage);
     if (g->ev_begin) hipEventDestroy(g->ev_begin);
     if (g->ev_end) hipEventDestroy(g->ev_end);
     if (g->ev_k0) hipEventDestroy(g->ev_k0);
@@ -136,6 +137,30 @@ void destroy_graph(ghet_graph* g) {
 }
 
 }  // namespace
+
+int copy_out(ghet_graph* g, void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return GHET_OK;
+    if (bytes < (size_t(4) << 20)) {
+        GHET_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        GHET_HIP(hipStreamSynchronize(s));
+        return GHET_OK;
+    }
+    if (g->stage_bytes < bytes) {
+        GHET_HIP(hipStreamSynchronize(s));  // the old buffer may still be a copy target
+        if (g->stage) GHET_HIP(hipHostFree(g->stage));
+        g->stage = nullptr;
+        g->stage_bytes = 0;
+        GHET_HIP(hipHostMalloc(&g->stage, bytes, hipHostMallocDefault));
+        g->stage_bytes = bytes;
+    }
+    GHET_HIP(hipMemcpyAsync(g->stage, src, bytes, hipMemcpyDeviceToHost, s));
+    GHET_HIP(hipStreamSynchronize(s));
+    const char* from = static_cast<const char*>(g->stage);
+    char* to = static_cast<char*>(dst);
+    parallel_for(static_cast<int64_t>(bytes), std::min(hw_threads(0), 16),
+                 [&](int64_t b, int64_t e, int) { std::memcpy(to + b, from + b, static_cast<size_t>(e - b)); });
+    return GHET_OK;
+}
 
 int mail_init(ghet_graph* g) {
     if (g->mail) return GHET_OK;

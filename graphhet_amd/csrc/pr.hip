@@ -715,7 +715,7 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     hipLaunchKernelGGL(k_pr_unpermute, dim3(init_blocks), dim3(kBT), 0, s, rank, L.perm, n, scal, norm, out);
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
-    if (pr_out) GHET_HIP(hipMemcpyAsync(pr_out, out, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+    if (pr_out) GHET_CHECK(copy_out(g, pr_out, out, 8 * size_t(n), s));
     GHET_HIP(hipMemcpyAsync(g->pinned, scal, sizeof(PRScalars), hipMemcpyDeviceToHost, s));
     GHET_HIP(hipStreamSynchronize(s));
     if (st) {

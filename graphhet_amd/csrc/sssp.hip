@@ -850,8 +850,8 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
-    if (dist_out) GHET_HIP(hipMemcpyAsync(dist_out, dist, 8 * size_t(n), hipMemcpyDeviceToHost, s));
-    if (pre_out) GHET_HIP(hipMemcpyAsync(pre_out, g->ws.ptr[WS_PRE], 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    if (dist_out) GHET_CHECK(copy_out(g, dist_out, dist, 8 * size_t(n), s));
+    if (pre_out) GHET_CHECK(copy_out(g, pre_out, g->ws.ptr[WS_PRE], 4 * size_t(n), s));
     if (st) {
         GHET_CHECK(reached_stats_f64(dist, inf, g->out.rowptr, n, ctr->slots, s));
         GHET_HIP(hipMemcpyAsync(g->pinned, ctr->slots, sizeof(CtrSlot) * kSlots, hipMemcpyDeviceToHost, s));

@@ -216,7 +216,7 @@ int run_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* o, ghet_run_stats
     GHET_CHECK(enqueue_tc(g, cnt, g->tune.tc_ept, g->tune.tc_waves_per_cu, s));
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
-    if (count_out) GHET_HIP(hipMemcpyAsync(count_out, cnt, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+    if (count_out) GHET_CHECK(copy_out(g, count_out, cnt, 8 * size_t(n), s));
     GHET_HIP(hipStreamSynchronize(s));
     if (st) {
         float ms = 0;

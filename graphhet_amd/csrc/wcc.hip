@@ -161,7 +161,7 @@ int run_wcc(ghet_graph* g, int32_t* label_out, const ghet_opts* o, ghet_run_stat
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
-    if (label_out && n > 0) GHET_HIP(hipMemcpyAsync(label_out, parent, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    if (label_out && n > 0) GHET_CHECK(copy_out(g, label_out, parent, 4 * size_t(n), s));
     GHET_HIP(hipStreamSynchronize(s));
     if (st) {
         float ms = 0;
