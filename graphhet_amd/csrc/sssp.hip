@@ -54,7 +54,10 @@ constexpr unsigned long long kInfBits = 0x7FF0000000000000ULL;
 constexpr int kBT = 256;
 constexpr int kItems = 4;
 constexpr int kTile = kBT * kItems;  // edges per relax tile
-constexpr int kOut = 2 * kTile;      // LDS capacity of each staged queue
+// LDS capacity of each staged queue: a tile pushes at most kTile entries and
+// a flush empties it above kOut - kTile.  1.25 kTile (instead of 2 kTile)
+// lets 7 relax blocks share a CU instead of 5: s24 13.3 -> 12.3 ms.
+constexpr int kOut = kTile + kTile / 4;
 constexpr int kMaxBlocks = 2048;
 constexpr int kSplitItems = 4;  // far-list entries per k_far_split thread and step
 
