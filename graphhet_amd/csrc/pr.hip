@@ -166,8 +166,10 @@ __device__ __forceinline__ void pr_stage(const PRItem& it, bool valid, const int
 // finishes its vertex; any other tile row (the hot part of a vertex with cold
 // sources, or a cold segment) and every chunk store a partial sum for
 // k_pr_long.  Every thread keeps its sink mass in item order;
-// part[blockIdx.x] gets the block's total.
-__global__ void __launch_bounds__(kBT) k_pr_iter(
+// part[blockIdx.x] gets the block's total.  Launch bounds ask for 8 waves per
+// SIMD (64 VGPRs, no spills): 1691 -> 1632 us per iteration at s24 against
+// the 5 waves the default allocation (82 VGPRs) allowed.
+__global__ void __launch_bounds__(kBT, 8) k_pr_iter(
     const PRItem* __restrict__ items, int32_t num_items, const int32_t* __restrict__ rowptr,
     const int32_t* __restrict__ col, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ slot, int32_t n,
     double* __restrict__ long_part, const float* __restrict__ contrib, float* __restrict__ next,
