@@ -25,7 +25,8 @@ namespace {
 
 // Per-level counters.  qcount/hubs/alloc/conv hand out queue positions (one
 // atomic per LDS flush / hub / block); the sums are spread over kSlots slots:
-// v[0] vertices discovered, v[1] their out-edges, v[2] their in-edges.
+// v[0] vertices discovered, v[1] their out-edges, v[2] their in-edges
+// (v[7] and done count block arrivals for the mailbox post).
 struct BfsCtr {
     int32_t qcount;
     int32_t hubs;

@@ -36,7 +36,8 @@ namespace {
 // Queue positions (one atomic per LDS flush) plus spread sums:
 // slot v[0] out-edges of the next near queue, v[1] bits of the smallest kept
 // far distance (atomicMin on non-negative doubles' bit patterns), v[2]
-// out-edges of the vertices appended to the bucket list.  list_count is the
+// out-edges of the vertices appended to the bucket list, v[3] edges scanned
+// (v[7] and done count block arrivals for the mailbox post).  list_count is the
 // running length of the bucket lists (all buckets share one n-entry buffer:
 // every vertex is listed in exactly one bucket) and is never reset.
 struct SsspCtr {
