@@ -294,12 +294,14 @@ class HostGraph:
             pass
 
 
-def _opts(schedule=SCHED_HEURISTIC, sssp_delta=0.0, trace=None, flags=0) -> Opts:
+def _opts(schedule=SCHED_HEURISTIC, sssp_delta=0.0, trace=None, flags=0, stream=None) -> Opts:
+    """stream: a hipStream_t as an integer (e.g. torch.cuda.Stream().cuda_stream)
+    or None for the graph's own stream."""
     o = Opts()
     o.size = C.sizeof(Opts)
     o.schedule = schedule
     o.sssp_delta = sssp_delta
-    o.stream = None
+    o.stream = stream
     if trace is not None:
         o.trace = C.cast(trace, P(LevelTrace))
         o.trace_cap = len(trace)
@@ -404,11 +406,11 @@ class DeviceGraph:
             pass
 
     # ---------------------------------------------------------------- runs --
-    def bfs(self, src: int, schedule=SCHED_HEURISTIC, out=True, trace_cap=0):
+    def bfs(self, src: int, schedule=SCHED_HEURISTIC, out=True, trace_cap=0, stream=None):
         depth = np.empty(self.n, np.int32) if out else None
         st = RunStats()
         tr = (LevelTrace * trace_cap)() if trace_cap else None
-        _check(lib().ghet_bfs(self._h, src, _ptr(depth, C.c_int32), C.byref(_opts(schedule, trace=tr)),
+        _check(lib().ghet_bfs(self._h, src, _ptr(depth, C.c_int32), C.byref(_opts(schedule, trace=tr, stream=stream)),
                               C.byref(st)))
         if trace_cap:
             return depth, st, [tr[i] for i in range(st.trace_len)]
@@ -423,12 +425,12 @@ class DeviceGraph:
         return depth, pre, st
 
     def sssp(self, src: int, inf: float = DBL_MAX, delta: float = 0.0, schedule=SCHED_HEURISTIC, out=True,
-             trace_cap=0):
+             trace_cap=0, stream=None):
         dist = np.empty(self.n, np.float64) if out else None
         st = RunStats()
         tr = (LevelTrace * trace_cap)() if trace_cap else None
         _check(lib().ghet_sssp(self._h, src, inf, _ptr(dist, C.c_double),
-                               C.byref(_opts(schedule, sssp_delta=delta, trace=tr)), C.byref(st)))
+                               C.byref(_opts(schedule, sssp_delta=delta, trace=tr, stream=stream)), C.byref(st)))
         if trace_cap:
             return dist, st, [tr[i] for i in range(st.trace_len)]
         return dist, st

@@ -146,6 +146,29 @@ def test_medium_tc(medium):
     np.testing.assert_array_equal(c, O.tc(s.rowptr, s.col))
 
 
+def test_caller_stream(medium):
+    """BFS and SSSP on a stream the caller owns (opts.stream): level and pass
+    totals still reach the host through the mailbox, results through the
+    staging copy, and they equal the graph-stream results."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime libghet.so is linked against
+    stream = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+    try:
+        o, dg, src = medium["o"], medium["dg"], medium["src"]
+        d0, s0 = dg.bfs(src)
+        d1, s1 = dg.bfs(src, stream=stream.value)
+        np.testing.assert_array_equal(d0, d1)
+        assert s0.levels == s1.levels and s0.edges_traversed == s1.edges_traversed
+        x0, _ = dg.sssp(src)
+        x1, _ = dg.sssp(src, stream=stream.value)
+        np.testing.assert_array_equal(x0, x1)
+        np.testing.assert_array_equal(x1, O.sssp(o.rowptr, o.col, o.w, src))
+    finally:
+        assert hip.hipStreamSynchronize(stream) == 0
+        hip.hipStreamDestroy(stream)
+
+
 def test_results_are_deterministic(medium):
     dg = medium["dg"]
     a, _ = dg.pr(10)
