@@ -42,6 +42,7 @@ constexpr int kBT = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBT * kItems;  // edges per push tile
 constexpr int kOut = 2 * kTile;      // LDS output queue capacity
+static_assert(kOut >= kTile, "a push tile can stage kTile discoveries");
 constexpr int kHubDeg = 4096;
 constexpr int kMaxBlocks = 2048;     // persistent grid cap (8 blocks per CU)
 constexpr int kPrepItems = 4;        // queue entries per thread of k_push_prep

@@ -58,6 +58,8 @@ constexpr int kTile = kBT * kItems;
 // flush empties it above kOut - kTile): 1.25 tiles instead of 2 lets 3 relax
 // blocks share a CU instead of 2 (256 sources on s22: 483 -> 456 ms).
 constexpr int kOut = kTile + kTile / 4;
+// a relax tile pushes at most kTile entries per queue, a split step kBT
+static_assert(kOut >= kTile && kOut >= kBT, "staged queues smaller than one step's pushes");
 constexpr int kMaxBlocks = 2048;
 constexpr int kMaxK = 32;  // sources per pass (mask width)
 

@@ -61,6 +61,11 @@ constexpr int kTile = kBT * kItems;  // edges per relax tile
 constexpr int kOut = kTile + kTile / 4;
 constexpr int kMaxBlocks = 2048;
 constexpr int kSplitItems = 4;  // far-list entries per k_far_split thread and step
+// A staged queue must hold what one step can push on top of what a flush
+// leaves: a relax tile pushes at most kTile entries per queue (flush above
+// kOut - kTile), a far-split step kBT * kSplitItems (flush above
+// kOut - kBT * kSplitItems).
+static_assert(kOut >= kTile && kOut >= kBT * kSplitItems, "staged queues smaller than one step's pushes");
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
     int lo = 0, hi = len;
