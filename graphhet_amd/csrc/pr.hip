@@ -674,7 +674,10 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
         std::max<int64_t>(1, std::min<int64_t>(int64_t(std::max(occ, 1)) * std::max(cus, 1), L.num_items)));
     const int32_t init_blocks = static_cast<int32_t>(std::min<int64_t>(ceil_div(n, kBT), 4096));
     // k_pr_long's sink partials follow k_pr_iter's in the same buffer
-    const int32_t long_blocks = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(L.num_long_rows, kBT), 1024)));
+    // one row per thread: k_pr_long's loads (row, slots, out-degree) are a
+    // dependent chain, so every row gets its own thread (1024 blocks with a
+    // grid-stride loop took 70 us at scale 24)
+    const int32_t long_blocks = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(L.num_long_rows, kBT), 65535)));
     void* p;
     float *ca, *cb;
     double *rank, *out, *part, *long_part;
