@@ -113,8 +113,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        log("bench.py measures one GPU: this build has no multi-GPU path "
-            "(the sharding design is described in DESIGN.md); run with --gpus 1.")
+        log("bench.py measures one GPU: the library has no multi-GPU path "
+            "(graphs are not sharded across devices); run with --gpus 1.")
         sys.exit(2)
     if args.copy_probe_mb:
         copy_probe(args.copy_probe_mb)
