@@ -23,7 +23,11 @@ inline std::vector<int> bfs_pre(int* d_row_offsets_out, int* d_column_indices_ou
     return result;
 }
 
-// Cuda_BFS_pre (GPU_BFS_pre_csr.cuh:15,126-156).
+// Cuda_BFS_pre (GPU_BFS_pre_csr.cuh:15,126-156), including the reference's
+// host-side check (GPU_BFS_pre_csr.cuh:133-153): every parent chain is walked
+// back towards the source, and a link pre -> now that is not an out-edge of
+// pre in graph.OUTs prints "Not found!".  Parents are one level up, so a
+// chain ends after at most depth[v] links; the walk is bounded by V anyway.
 inline std::vector<std::pair<std::string, int>> Cuda_BFS_pre(graph_structure<double>& graph,
                                                              CSR_graph<double>& csr_graph, std::string src_v,
                                                              std::vector<int>& pre_v, int min_depth = 0,
@@ -42,5 +46,22 @@ inline std::vector<std::pair<std::string, int>> Cuda_BFS_pre(graph_structure<dou
                 depth[v] = INT_MAX;
                 pre_v[v] = -1;
             }
+    for (int v = 0; v < graph.V; ++v) {
+        int now = v, up = pre_v[v];
+        for (int steps = 0; up != -1 && steps < graph.V; ++steps) {
+            bool linked = false;
+            for (const auto& e : graph.OUTs[up])
+                if (e.first == now) {
+                    linked = true;
+                    break;
+                }
+            if (!linked) {
+                std::printf("Not found!\n");
+                break;
+            }
+            now = up;
+            up = pre_v[up];
+        }
+    }
     return graph.res_trans_id_val(depth);
 }

@@ -54,18 +54,6 @@ public:
     void load_graph();
     void save_to_CSV(std::vector<std::pair<std::string, std::string>>& res, std::string file_path);
 
-    size_t getTotalMemory() {
-        size_t total = sizeof(*this);
-        for (int i = 0; i < this->V; i++) {
-            total += this->OUTs[i].capacity() * sizeof(std::pair<int, weight_type>);
-            total += this->INs[i].capacity() * sizeof(std::pair<int, weight_type>);
-        }
-        total += this->invalid_vertex_id.size() * sizeof(int);
-        total += this->vertex_str_to_id.size() * (sizeof(std::string) + sizeof(int));
-        total += this->vertex_id_to_str.capacity() * sizeof(std::pair<std::string, bool>);
-        return total;
-    }
-
 private:
     static bool flag(const std::string& v) { return v != "false"; }
     void set_option(const std::vector<std::string>& key, const std::string& value);

@@ -11,6 +11,11 @@
 // Pinning (see tests/test_oracle.py):
 //   bfs   golden vectors from oracle/_ref; networkx shortest-path lengths;
 //         closed forms (path, star, binary tree, grid).
+//   bfs_parent, sssp_parent
+//         validity (pre[v] -> v is an out-edge one level up, or with
+//         d[pre] + w == d[v] in FP64) and minimality, both computed from the
+//         out-CSR by a scatter-min over edges; planted mistakes (largest
+//         parent, in/out lists swapped) must fail.
 //   sssp  golden vectors; scipy.sparse.csgraph.dijkstra; edge invariants.
 //   wcc   golden vectors; scipy connected_components(weak) + min-id closed form.
 //   pr    golden vectors (bit-exact, same operation order); directed cycle

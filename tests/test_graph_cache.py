@@ -12,7 +12,7 @@ import pytest
 
 import graphhet_amd as G
 
-BIN = os.path.join(os.path.dirname(G.LIB_PATH), "..", "bin")
+BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "bin")
 
 
 def _vv_bytes(rows, dtype):
@@ -121,7 +121,7 @@ def test_rmat_cached_generates_once_then_loads(tmp_path):
 
 def test_header_and_library_interoperate(tmp_path):
     """The C++ header's writer/reader and the library's cache agree
-    (graphhet_amd/drivers/cache_selftest.cpp)."""
+    (tests/drivers/cache_selftest.cpp)."""
     exe = os.path.join(BIN, "cache_selftest")
     if not os.path.exists(exe):
         pytest.skip("drivers not built")

@@ -24,6 +24,7 @@ import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "graphhet_amd", "bin")
+TEST_BIN = os.path.join(ROOT, "tests", "bin")
 LLONG_MAX = 9223372036854775807
 DBL_MAX = np.finfo(np.float64).max
 INT_MAX = 2**31 - 1
@@ -87,7 +88,7 @@ def write_dataset(d, name, directed=True, scale=9, seed=3):
 def test_loader_and_checkers(tmp_path, directed):
     _build_bins()
     names, src, n, m, has_unreached = write_dataset(str(tmp_path), "g", directed)
-    out = subprocess.run([os.path.join(BIN, "ldbc_selftest"), str(tmp_path), "g"], capture_output=True, text=True,
+    out = subprocess.run([os.path.join(TEST_BIN, "ldbc_selftest"), str(tmp_path), "g"], capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 0, out.stderr
     cfg = dict(kv.split("=") for kv in

@@ -4,6 +4,8 @@ routines (networkx / scipy), and closed forms chosen so that a plausible slip
 (off-by-one depth, a dropped dangling term, a wrong tie rule, in- vs
 out-lists swapped) fails at least one of them."""
 import math
+import os
+import sys
 
 import numpy as np
 import pytest
@@ -12,6 +14,9 @@ import scipy.sparse.csgraph as csg
 
 import graphhet_amd as G
 import oracle as O
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from conftest import golden_names, load_golden  # noqa: E402
 
 INT_INF = 2**31 - 1
 DBL_MAX = np.finfo(np.float64).max
@@ -220,3 +225,147 @@ def test_planted_mistakes_are_caught(golden):
     if not np.array_equal(O.cdlp(empty_rp, np.zeros(0, np.int32), orp, g["out_col"], 10), g["cdlp"]):
         return
     pytest.skip("graph symmetric enough that dropping in-lists is invisible")
+
+
+# ---------------------------------------------------------------- parents --
+# The parent rules are pinned from the out-CSR by a route that shares nothing
+# with the oracle's in-list scan: every out-edge (u, v) that satisfies the
+# level (or distance) condition proposes u to v, and a scatter-min keeps the
+# smallest proposal.  Graphs with many equal-length paths (unit or quantised
+# weights) make the "smallest" rule visible.
+
+def _edges(rowptr, col):
+    n = len(rowptr) - 1
+    return np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr)), np.asarray(col, np.int64)
+
+
+def _bfs_parent_by_out_edges(out_rowptr, out_col, depth):
+    n = len(out_rowptr) - 1
+    u, v = _edges(out_rowptr, out_col)
+    d = np.asarray(depth, np.int64)
+    ok = (d[u] != INT_INF) & (d[v] == d[u] + 1)
+    best = np.full(n, n, np.int64)
+    np.minimum.at(best, v[ok], u[ok])
+    return np.where(best == n, -1, best).astype(np.int32)
+
+
+def _sssp_parent_by_out_edges(out_rowptr, out_col, out_w, dist, src):
+    n = len(out_rowptr) - 1
+    u, v = _edges(out_rowptr, out_col)
+    d = np.asarray(dist, np.float64)
+    ok = (u != v) & (d[u] != DBL_MAX) & (d[u] + np.asarray(out_w, np.float64) == d[v]) & (v != src)
+    best = np.full(n, n, np.int64)
+    np.minimum.at(best, v[ok], u[ok])
+    return np.where(best == n, -1, best).astype(np.int32)
+
+
+def _check_bfs_parents(out_rowptr, out_col, depth, src, pre):
+    """(i) pre[v] -> v is an out-edge, (ii) depth[pre[v]] == depth[v] - 1,
+    (iii) no smaller in-neighbour qualifies; -1 exactly for src and unreached."""
+    n = len(out_rowptr) - 1
+    pre = np.asarray(pre, np.int64)
+    has = pre >= 0
+    reached = (np.asarray(depth) != INT_INF) & (np.arange(n) != src)
+    assert np.array_equal(has, reached), "parent present exactly for reached non-source vertices"
+    vs = np.nonzero(has)[0]
+    key_edges = set(zip(*(_edges(out_rowptr, out_col))))
+    assert all((int(pre[x]), int(x)) in key_edges for x in vs), "pre[v] -> v is not an out-edge"
+    assert np.all(np.asarray(depth)[pre[vs]] == np.asarray(depth)[vs] - 1), "parent not one level up"
+    np.testing.assert_array_equal(pre, _bfs_parent_by_out_edges(out_rowptr, out_col, depth))
+
+
+def _check_sssp_parents(out_rowptr, out_col, out_w, dist, src, pre):
+    n = len(out_rowptr) - 1
+    pre = np.asarray(pre, np.int64)
+    has = pre >= 0
+    reached = (np.asarray(dist) != DBL_MAX) & (np.arange(n) != src)
+    assert np.array_equal(has, reached), "parent present exactly for reached non-source vertices"
+    u, v = _edges(out_rowptr, out_col)
+    w_of = {(int(a), int(b)): float(x) for a, b, x in zip(u, v, out_w)}
+    for x in np.nonzero(has)[0]:
+        e = (int(pre[x]), int(x))
+        assert e in w_of, "pre[v] -> v is not an out-edge"
+        assert dist[e[0]] + w_of[e] == dist[x], "d[pre] + w != d[v] in FP64"
+    np.testing.assert_array_equal(pre, _sssp_parent_by_out_edges(out_rowptr, out_col, out_w, dist, src))
+
+
+def _parent_graphs():
+    """Golden graphs plus graphs with many ties: R-MAT with quantised
+    weights, ER with unit weights, a grid and K_n."""
+    out = []
+    for name in golden_names():
+        g = load_golden(name)
+        out.append((name, g["out_rowptr"], g["out_col"], g["out_w"], g["in_rowptr"], g["in_col"], g["in_w"],
+                    int(g["src"])))
+    for name, hg in (("rmat_s11_q", G.HostGraph.rmat(11, 8, seed=7)), ("er_2000", G.HostGraph.er(2000, 12000, seed=3))):
+        o = hg.view(G.VIEW_OUT)
+        q = 1.0 if name.startswith("er") else None
+        w = np.ones(o.m) if q else np.ceil(o.w * 3) / 4  # {0.25, 0.5, 0.75}
+        hq = G.HostGraph.from_csr(G.Csr(o.n, o.rowptr, o.col, w))
+        o2, i2 = hq.view(G.VIEW_OUT), hq.view(G.VIEW_IN)
+        out.append((name, o2.rowptr, o2.col, o2.w, i2.rowptr, i2.col, i2.w, int(np.argmax(o2.degrees()))))
+    R, Cc = 9, 13
+    e = []
+    for r in range(R):
+        for c in range(Cc):
+            x = r * Cc + c
+            if c + 1 < Cc:
+                e += [(x, x + 1), (x + 1, x)]
+            if r + 1 < R:
+                e += [(x, x + Cc), (x + Cc, x)]
+    hg = G.HostGraph.from_edges(R * Cc, [a for a, _ in e], [b for _, b in e])
+    o, i = hg.view(G.VIEW_OUT), hg.view(G.VIEW_IN)
+    out.append(("grid", o.rowptr, o.col, o.w, i.rowptr, i.col, i.w, (R // 2) * Cc + Cc // 2))
+    return out
+
+
+PARENT_GRAPHS = _parent_graphs()
+
+
+@pytest.mark.parametrize("case", PARENT_GRAPHS, ids=[c[0] for c in PARENT_GRAPHS])
+def test_bfs_parent_oracle_is_valid_and_minimal(case):
+    _, orp, ocol, ow, irp, icol, iw, src = case
+    d = O.bfs(orp, ocol, src)
+    _check_bfs_parents(orp, ocol, d, src, O.bfs_parent(irp, icol, d))
+
+
+@pytest.mark.parametrize("case", PARENT_GRAPHS, ids=[c[0] for c in PARENT_GRAPHS])
+def test_sssp_parent_oracle_is_valid_and_minimal(case):
+    _, orp, ocol, ow, irp, icol, iw, src = case
+    d = O.sssp(orp, ocol, ow, src)
+    _check_sssp_parents(orp, ocol, ow, d, src, O.sssp_parent(irp, icol, iw, d, src))
+
+
+def _reverse_rows(rowptr, col, w=None):
+    c = np.array(col, copy=True)
+    ww = None if w is None else np.array(w, copy=True)
+    for v in range(len(rowptr) - 1):
+        a, b = rowptr[v], rowptr[v + 1]
+        c[a:b] = c[a:b][::-1]
+        if ww is not None:
+            ww[a:b] = ww[a:b][::-1]
+    return c, ww
+
+
+def test_parent_checks_reject_planted_mistakes():
+    """Mistakes planted in the oracle's inputs on the CPU: the largest qualifying
+    in-neighbour (rows scanned in descending order) and in/out lists swapped.
+    Each must fail the checks on at least one graph with ties."""
+    caught = {"bfs_largest": 0, "bfs_swapped": 0, "sssp_largest": 0, "sssp_swapped": 0}
+    for _, orp, ocol, ow, irp, icol, iw, src in PARENT_GRAPHS:
+        d = O.bfs(orp, ocol, src)
+        dist = O.sssp(orp, ocol, ow, src)
+        rcol, rw = _reverse_rows(irp, icol, iw)
+        for key, pre in (("bfs_largest", O.bfs_parent(irp, rcol, d)), ("bfs_swapped", O.bfs_parent(orp, ocol, d))):
+            try:
+                _check_bfs_parents(orp, ocol, d, src, pre)
+            except AssertionError:
+                caught[key] += 1
+        for key, pre in (("sssp_largest", O.sssp_parent(irp, rcol, rw, dist, src)),
+                         ("sssp_swapped", O.sssp_parent(orp, ocol, ow, dist, src))):
+            try:
+                _check_sssp_parents(orp, ocol, ow, dist, src, pre)
+            except AssertionError:
+                caught[key] += 1
+    assert all(v > 0 for v in caught.values()), caught
+    print(caught)
