@@ -9,6 +9,7 @@ an ImportError-like GhetError when it is missing: there is no fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 from dataclasses import dataclass
@@ -78,7 +79,8 @@ EXPORTS = [
     "ghet_gen_rmat", "ghet_gen_er", "ghet_host_graph_from_edges", "ghet_host_graph_view", "ghet_host_graph_free",
     "ghet_host_graph_save", "ghet_host_graph_load", "ghet_host_graph_from_csr",
     "ghet_graph_create", "ghet_graph_wrap_device", "ghet_graph_destroy", "ghet_graph_device_ptrs",
-    "ghet_graph_device_bytes", "ghet_graph_gen_rmat", "ghet_graph_download",
+    "ghet_graph_device_bytes", "ghet_graph_gen_rmat", "ghet_graph_download", "ghet_shutdown",
+    "ghet_graph_set_workspace_cap",
     "ghet_dyn_create", "ghet_dyn_from_graph", "ghet_dyn_update", "ghet_dyn_edges", "ghet_dyn_snapshot",
     "ghet_dyn_destroy",
     "ghet_bfs", "ghet_bfs_pre", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_cdlp", "ghet_tc",
@@ -130,8 +132,10 @@ def lib() -> C.CDLL:
         "ghet_graph_wrap_device": (C.c_int, [C.c_int32, C.c_int64, vp, vp, vp, C.c_int64, vp, vp, C.c_int64, vp, vp,
                                              C.c_uint32, P(vp)]),
         "ghet_graph_destroy": (None, [vp]),
+        "ghet_shutdown": (C.c_int, []),
         "ghet_graph_device_ptrs": (C.c_int, [vp, P(DevicePtrs)]),
         "ghet_graph_device_bytes": (C.c_int64, [vp]),
+        "ghet_graph_set_workspace_cap": (C.c_int, [vp, C.c_int64]),
         "ghet_bfs": (C.c_int, [vp, C.c_int32, i32p, P(Opts), P(RunStats)]),
         "ghet_bfs_pre": (C.c_int, [vp, C.c_int32, i32p, i32p, P(Opts), P(RunStats)]),
         "ghet_sssp": (C.c_int, [vp, C.c_int32, C.c_double, f64p, P(Opts), P(RunStats)]),
@@ -160,6 +164,25 @@ def lib() -> C.CDLL:
         fn.argtypes = args
     _lib = L
     return L
+
+
+# Handles record the generation they were made in; shutdown() starts a new
+# one, after which older handles are never passed to the library again.
+_GEN = [0]
+
+
+def shutdown() -> None:
+    """Releases every graph, dynamic store and schedule network the library
+    holds on the GPU (ghet_shutdown).  Runs at interpreter exit, before the
+    HIP runtime and any attached profiler are torn down, so nothing is freed
+    after them.  Objects made before the call are closed; the library stays
+    usable."""
+    if _lib is not None:
+        _GEN[0] += 1
+        _lib.ghet_shutdown()
+
+
+atexit.register(shutdown)
 
 
 def _check(code: int, allow_fallback: bool = True) -> int:
@@ -262,15 +285,22 @@ class HostGraph:
         g.save(path)
         return g
 
+    def _borrow(self, ptr, count: int, ctype, dtype) -> np.ndarray:
+        """numpy array over library-owned memory that keeps this graph alive
+        for as long as the array (or any view of it) exists."""
+        buf = (ctype * count).from_address(C.cast(ptr, C.c_void_p).value)
+        buf._owner = self
+        return np.frombuffer(buf, dtype=dtype)
+
     def view(self, kind: int = VIEW_OUT, copy: bool = True) -> Csr:
         v = CsrView()
         _check(lib().ghet_host_graph_view(self._h, kind, C.byref(v)))
         n, m = v.n, v.m
-        rp = np.ctypeslib.as_array(v.rowptr, shape=(n + 1,)) if n >= 0 else np.zeros(1, np.int32)
-        col = np.ctypeslib.as_array(v.col, shape=(m,)) if m > 0 else np.zeros(0, np.int32)
+        rp = self._borrow(v.rowptr, n + 1, C.c_int32, np.int32) if n >= 0 else np.zeros(1, np.int32)
+        col = self._borrow(v.col, m, C.c_int32, np.int32) if m > 0 else np.zeros(0, np.int32)
         w = None
         if v.w and m > 0:
-            w = np.ctypeslib.as_array(v.w, shape=(m,))
+            w = self._borrow(v.w, m, C.c_double, np.float64)
         elif v.w and m == 0:
             w = np.zeros(0, np.float64)
         if copy:
@@ -346,6 +376,7 @@ class DeviceGraph:
                                        _ptr(irp, C.c_int32), _ptr(icol, C.c_int32), _ptr(iw, C.c_double),
                                        _ptr(arp, C.c_int32), _ptr(acol, C.c_int32), flags, C.byref(h)))
         self._h = h
+        self._gen = _GEN[0]
 
     @classmethod
     def from_host(cls, hg: HostGraph, sym: bool = False, weights: bool = True, with_all: bool = True):
@@ -367,6 +398,7 @@ class DeviceGraph:
         _check(lib().ghet_graph_gen_rmat(scale, edge_factor, a, b, c, seed, flags, C.byref(h)))
         self = cls.__new__(cls)
         self._h = h
+        self._gen = _GEN[0]
         self.n = 1 << scale
         return self
 
@@ -394,9 +426,15 @@ class DeviceGraph:
     def device_bytes(self) -> int:
         return lib().ghet_graph_device_bytes(self._h)
 
+    def set_workspace_cap(self, max_bytes: int) -> None:
+        """Caps the graph's workspace (0 removes the cap); calls that would
+        need more raise GhetError(ERR_NOMEM)."""
+        _check(lib().ghet_graph_set_workspace_cap(self._h, max_bytes))
+
     def close(self):
         if self._h and self._h.value:
-            lib().ghet_graph_destroy(self._h)
+            if getattr(self, "_gen", -1) == _GEN[0]:
+                lib().ghet_graph_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
@@ -510,6 +548,7 @@ class DynGraph:
 
     def __init__(self, handle, n: int):
         self._h = handle
+        self._gen = _GEN[0]
         self.n = n
 
     @classmethod
@@ -551,12 +590,14 @@ class DynGraph:
         _check(lib().ghet_dyn_snapshot(self._h, flags, C.byref(h)))
         g = DeviceGraph.__new__(DeviceGraph)
         g._h = h
+        g._gen = _GEN[0]
         g.n = self.n
         return g
 
     def close(self):
         if self._h and self._h.value:
-            lib().ghet_dyn_destroy(self._h)
+            if getattr(self, "_gen", -1) == _GEN[0]:
+                lib().ghet_dyn_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -589,8 +589,10 @@ int store_update(ghet_dyn* d, Store& st, const int32_t* a, const int32_t* b, con
     return GHET_OK;
 }
 
+// Frees a store made by dyn_begin; a pointer that is not live (already
+// destroyed, or released by ghet_shutdown) is ignored.
 void dyn_free(ghet_dyn* d) {
-    if (!d) return;
+    if (!d || !live_remove(LIVE_DYN, d)) return;
     if (d->s) hipStreamSynchronize(d->s);
     for (Store& st : d->dir)
         for (Buf* b : {&st.beg, &st.len, &st.cap, &st.col, &st.w}) b->release();
@@ -608,6 +610,7 @@ int dyn_begin(int32_t n, ghet_dyn** out) {
         return set_error(GHET_ERR_NODEVICE, "no GPU visible to this process");
     }
     auto* d = new ghet_dyn();
+    live_add(LIVE_DYN, d);
     d->n = n;
     hipError_t e = hipGetDevice(&d->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->s, hipStreamNonBlocking);
@@ -802,3 +805,5 @@ int ghet_dyn_snapshot(ghet_dyn* d, uint32_t flags, ghet_graph** out) {
 void ghet_dyn_destroy(ghet_dyn* d) { dyn_free(d); }
 
 }  // extern "C"
+
+void ghet::dyn_release(void* d) { dyn_free(static_cast<ghet_dyn*>(d)); }

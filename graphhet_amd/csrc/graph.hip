@@ -9,6 +9,8 @@
 #include <chrono>
 #include <cstring>
 #include <mutex>
+#include <unordered_set>
+#include <vector>
 
 #include "graph_internal.hpp"
 
@@ -16,9 +18,40 @@ namespace ghet {
 ghet_host_graph* host_graph_adopt(int32_t n, const int32_t* rowptr, const int32_t* col, const double* w,
                                   int threads);
 
+namespace {
+// Every graph and dynamic store handed out and not yet destroyed.  The set is
+// allocated once and never freed, so a destroy that runs while the process
+// exits still finds it.
+struct LiveSet {
+    std::mutex mu;
+    std::unordered_set<void*> items[2];
+};
+LiveSet& live() {
+    static LiveSet* s = new LiveSet;
+    return *s;
+}
+}  // namespace
+
+void live_add(int kind, void* p) {
+    std::lock_guard<std::mutex> l(live().mu);
+    live().items[kind].insert(p);
+}
+bool live_remove(int kind, void* p) {
+    std::lock_guard<std::mutex> l(live().mu);
+    return live().items[kind].erase(p) != 0;
+}
+std::vector<void*> live_take(int kind) {
+    std::lock_guard<std::mutex> l(live().mu);
+    return std::vector<void*>(live().items[kind].begin(), live().items[kind].end());
+}
+
 int Workspace::get(int slot, size_t want, void** out) {
     if (want == 0) want = 256;
     if (bytes[slot] < want) {
+        const size_t rounded = (want + 4095) & ~size_t(4095);
+        if (cap && static_cast<size_t>(total()) - bytes[slot] + rounded > cap)
+            return set_error(GHET_ERR_NOMEM, "workspace slot %d needs %zu bytes: the graph's workspace cap of %zu bytes "
+                             "would be exceeded (%lld held)", slot, rounded, cap, static_cast<long long>(total()));
         if (ptr[slot]) {
             hipFree(ptr[slot]);
             ptr[slot] = nullptr;
@@ -112,8 +145,16 @@ int upload(ghet_graph* g, void* dst, const void* src, size_t bytes) {
     return GHET_OK;
 }
 
+ghet_graph* new_graph() {
+    auto* g = new ghet_graph();
+    live_add(LIVE_GRAPH, g);
+    return g;
+}
+
+// Frees a graph made by new_graph(); a pointer that is not live (already
+// destroyed, or released by ghet_shutdown) is ignored.
 void destroy_graph(ghet_graph* g) {
-    if (!g) return;
+    if (!g || !live_remove(LIVE_GRAPH, g)) return;
     if (g->stream) hipStreamSynchronize(g->stream);
     g->ws.release();
     if (g->pr.mem) hipFree(g->pr.mem);
@@ -202,7 +243,7 @@ int mail_wait(ghet_graph* g, hipStream_t s, unsigned long long seq) {
 int graph_alloc_arena(int32_t n, int64_t m_out, bool want_in, int64_t m_in, bool want_all, int64_t m_all,
                       bool want_w, ghet_graph** out) {
     *out = nullptr;
-    auto* g = new ghet_graph();
+    auto* g = new_graph();
     g->n = n;
     g->directed = true;
     int st = init_runtime(g);
@@ -311,7 +352,7 @@ int ghet_graph_create(int32_t n, const int32_t* out_rowptr, const int32_t* out_c
         if (st != GHET_OK) return st;
     }
 
-    auto* g = new ghet_graph();
+    auto* g = new_graph();
     g->n = n;
     g->directed = directed;
     int st = init_runtime(g);
@@ -398,7 +439,7 @@ int ghet_graph_wrap_device(int32_t n, int64_t m_out, const int32_t* d_out_rowptr
     *out = nullptr;
     if (n < 0 || !d_out_rowptr || (m_out > 0 && !d_out_col))
         return set_error(GHET_ERR_INVALID, "wrap: out-CSR pointers missing");
-    auto* g = new ghet_graph();
+    auto* g = new_graph();
     g->n = n;
     g->owns_csr = false;
     g->directed = !(flags & GHET_GRAPH_UNDIRECTED);
@@ -438,6 +479,14 @@ int ghet_graph_wrap_device(int32_t n, int64_t m_out, const int32_t* d_out_rowptr
 
 void ghet_graph_destroy(ghet_graph* g) { destroy_graph(g); }
 
+int ghet_shutdown(void) {
+    clear_error();
+    for (void* d : live_take(LIVE_DYN)) dyn_release(d);
+    for (void* g : live_take(LIVE_GRAPH)) destroy_graph(static_cast<ghet_graph*>(g));
+    mlp_release();
+    return GHET_OK;
+}
+
 int ghet_graph_device_ptrs(const ghet_graph* g, ghet_device_ptrs* out) {
     clear_error();
     if (!g || !out) return set_error(GHET_ERR_INVALID, "NULL argument");
@@ -453,6 +502,13 @@ int ghet_graph_device_ptrs(const ghet_graph* g, ghet_device_ptrs* out) {
     out->in_w = g->in.w;
     out->all_rowptr = g->all.rowptr;
     out->all_col = g->all.col;
+    return GHET_OK;
+}
+
+int ghet_graph_set_workspace_cap(ghet_graph* g, int64_t max_bytes) {
+    clear_error();
+    if (!g || max_bytes < 0) return set_error(GHET_ERR_INVALID, "graph handle is NULL or cap < 0");
+    g->ws.cap = static_cast<size_t>(max_bytes);
     return GHET_OK;
 }
 

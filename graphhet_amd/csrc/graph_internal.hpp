@@ -37,7 +37,10 @@ enum WsSlot {
 struct Workspace {
     void* ptr[WS_NUM_SLOTS] = {};
     size_t bytes[WS_NUM_SLOTS] = {};
+    size_t cap = 0;  // 0: unlimited; else the most the slots may hold together
     // Returns a buffer of at least `want` bytes for `slot` (contents undefined).
+    // GHET_ERR_NOMEM when the device allocation fails or the slots would
+    // exceed `cap`; the slot keeps its old buffer in that case.
     int get(int slot, size_t want, void** out);
     void release();
     int64_t total() const {
@@ -155,6 +158,15 @@ struct ghet_graph {
 };
 
 namespace ghet {
+
+// Registry of live handles, so that ghet_shutdown can release everything the
+// library still holds before the HIP runtime is torn down at process exit.
+enum LiveKind { LIVE_GRAPH = 0, LIVE_DYN = 1 };
+void live_add(int kind, void* p);
+bool live_remove(int kind, void* p);  // false if p was not live
+std::vector<void*> live_take(int kind);
+void dyn_release(void* d);  // dyn.hip: frees a live dynamic store
+void mlp_release();         // mlp.hip: frees the schedule networks' device memory
 
 // Copies `bytes` of device results to caller memory and returns when they
 // are there.  Large results go through the graph's pinned staging buffer:

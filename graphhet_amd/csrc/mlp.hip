@@ -334,6 +334,11 @@ void net_forward_host(int kind, const float* x, float* y) {
     mlp_forward_host_t(g_nets[kind].hw.data(), x, y);
 }
 
+void mlp_release() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (Net& n : g_nets) net_free_device(n);
+}
+
 }  // namespace ghet
 
 using namespace ghet;

@@ -141,7 +141,19 @@ int ghet_graph_wrap_device(int32_t n,
                            int64_t m_in, const int32_t* d_in_rowptr, const int32_t* d_in_col,
                            int64_t m_all, const int32_t* d_all_rowptr, const int32_t* d_all_col,
                            uint32_t flags, ghet_graph** out);
+/* Frees the graph.  A handle that is not live (destroyed before, or released
+ * by ghet_shutdown) is ignored. */
 void ghet_graph_destroy(ghet_graph* g);
+/* Releases everything the library holds on the GPU: every live graph and
+ * dynamic store (their streams, events, pinned and fine-grained host buffers,
+ * layouts and workspaces) and the schedule networks' device memory.  Handles
+ * given out before become invalid; destroying them afterwards is a no-op.
+ * Call it before the process exits while the HIP runtime (and any profiler
+ * attached to it) is still up; the Python binding registers it with atexit.
+ * The library stays usable afterwards (schedule networks must be loaded
+ * again).  No reference counterpart: the reference frees in destructors
+ * (GPU_csr.hpp:167-215). */
+int ghet_shutdown(void);
 
 typedef struct ghet_device_ptrs {
     int32_t n;
@@ -151,6 +163,12 @@ typedef struct ghet_device_ptrs {
 } ghet_device_ptrs;
 /* Device addresses of the CSR arrays (the CSR_graph public pointer fields). */
 int ghet_graph_device_ptrs(const ghet_graph* g, ghet_device_ptrs* out);
+/* Caps the graph's grow-only workspace (the per-call scratch slots, not the
+ * CSR arena or the per-graph layouts) at max_bytes; 0 removes the cap.  A
+ * call that would need more returns GHET_ERR_NOMEM with the sizes in
+ * ghet_last_error and leaves the graph usable.  The reference has no cap: it
+ * prints a failed cudaMalloc and continues (GPU_csr.hpp:21-25). */
+int ghet_graph_set_workspace_cap(ghet_graph* g, int64_t max_bytes);
 /* Device bytes held by the graph (CSR + layouts + workspace). */
 int64_t ghet_graph_device_bytes(const ghet_graph* g);
 /* On-GPU R-MAT: the graph ghet_gen_rmat(scale, edge_factor, a, b, c, seed)

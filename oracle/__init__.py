@@ -58,6 +58,7 @@ def lib() -> C.CDLL:
             "oracle_pr_step": [I32, i32p, i32p, i32p, F64, I32, f64p, f64p],
             "oracle_cdlp": [I32, i32p, i32p, i32p, i32p, I32, I32, i32p],
             "oracle_tc": [I32, i32p, i32p, I32, I32, i64p],
+            "oracle_tc_forward": [I32, i32p, i32p, I32, i64p],
         })
     return _oracle
 
@@ -174,6 +175,16 @@ def tc(rowptr, col, v_begin: int = 0, v_end: int | None = None) -> np.ndarray:
     n = len(rp) - 1
     out = np.empty(n, np.int64)
     lib().oracle_tc(n, _p(rp, I32), _p(c, I32), v_begin, n if v_end is None else v_end, _p(out, C.c_int64))
+    return out
+
+
+def tc_forward(rowptr, col, threads: int = 0) -> np.ndarray:
+    """t(v), triangles through v, of a symmetric loop-free graph (forward
+    algorithm; 6 * t(v) is the reference's per-vertex convention)."""
+    rp, c = _a(rowptr, np.int32), _a(col, np.int32)
+    n = len(rp) - 1
+    out = np.empty(n, np.int64)
+    lib().oracle_tc_forward(n, _p(rp, I32), _p(c, I32), threads or min(16, hw_threads()), _p(out, C.c_int64))
     return out
 
 

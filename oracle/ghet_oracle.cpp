@@ -22,6 +22,9 @@
 //         closed form 1/n; mass conservation.
 //   cdlp  golden vectors; brute force on tiny graphs; K_n and isolated cases.
 //   tc    golden vectors; networkx.triangles x 6; K_n closed form.
+//   tc_forward
+//         networkx.triangles; K_n closed form; equality with oracle_tc (6 t)
+//         on R-MAT and Erdos-Renyi graphs.
 #include <algorithm>
 #include <atomic>
 #include <climits>
@@ -292,6 +295,80 @@ void oracle_tc(int32_t n, const int32_t* rowptr, const int32_t* col, int32_t v_b
             }
         }
     }
+}
+
+// An independent triangle counter for checking large symmetric loop-free
+// graphs: the forward algorithm over a degree-ordered orientation (Schank and
+// Wagner, "Finding, counting and listing all triangles in large graphs",
+// WEA 2005; the orientation helper of CPU_TC.hpp:9-29 is the same idea).
+// Vertices are ranked by (degree, id); u -> v when rank(u) < rank(v).  Every
+// triangle a < b < c (by rank) is found once, at u = a, v = b, w = c, by
+// probing N+(v) against a bitmap of N+(u), and adds 1 to t[a], t[b], t[c].
+// tri[v] receives t(v), the number of triangles through v (6 * t(v) is the
+// reference's convention, CPU_TC.hpp:31-60).  `threads` workers, each with a
+// private bitmap and count array.
+void oracle_tc_forward(int32_t n, const int32_t* rowptr, const int32_t* col, int32_t threads, int64_t* tri) {
+    threads = std::max(1, threads);
+    auto before = [&](int32_t a, int32_t b) {
+        const int32_t da = rowptr[a + 1] - rowptr[a], db = rowptr[b + 1] - rowptr[b];
+        return da < db || (da == db && a < b);
+    };
+    std::vector<int64_t> off(static_cast<size_t>(n) + 1, 0);
+    for (int32_t u = 0; u < n; ++u) {
+        int64_t k = 0;
+        for (int32_t e = rowptr[u]; e < rowptr[u + 1]; ++e) k += before(u, col[e]) ? 1 : 0;
+        off[u + 1] = off[u] + k;
+    }
+    std::vector<int32_t> fwd(static_cast<size_t>(off[n]));
+    run_chunks(threads, threads, [&](int q) {
+        const int32_t b = static_cast<int32_t>(static_cast<int64_t>(q) * n / threads);
+        const int32_t e = static_cast<int32_t>(static_cast<int64_t>(q + 1) * n / threads);
+        for (int32_t u = b; u < e; ++u) {
+            int64_t o = off[u];
+            for (int32_t x = rowptr[u]; x < rowptr[u + 1]; ++x)
+                if (before(u, col[x])) fwd[o++] = col[x];
+        }
+    });
+    std::vector<std::vector<int64_t>> part(static_cast<size_t>(threads));
+    std::atomic<int32_t> next(0);
+    const int32_t grain = 1024;
+    run_chunks(threads, threads, [&](int t) {
+        std::vector<int64_t>& cnt = part[t];
+        cnt.assign(static_cast<size_t>(n), 0);
+        std::vector<uint64_t> mark((static_cast<size_t>(n) + 63) / 64, 0);
+        for (int32_t b; (b = next.fetch_add(grain)) < n;) {
+            const int32_t e = std::min(n, b + grain);
+            for (int32_t u = b; u < e; ++u) {
+                const int64_t ub = off[u], ue = off[u + 1];
+                if (ue - ub < 2) continue;
+                for (int64_t i = ub; i < ue; ++i) mark[fwd[i] >> 6] |= 1ull << (fwd[i] & 63);
+                int64_t tu = 0;
+                for (int64_t i = ub; i < ue; ++i) {
+                    const int32_t v = fwd[i];
+                    int64_t tv = 0;
+                    for (int64_t j = off[v]; j < off[v + 1]; ++j) {
+                        const int32_t w = fwd[j];
+                        if (mark[w >> 6] >> (w & 63) & 1) {
+                            ++tv;
+                            ++cnt[w];
+                        }
+                    }
+                    cnt[v] += tv;
+                    tu += tv;
+                }
+                cnt[u] += tu;
+                for (int64_t i = ub; i < ue; ++i) mark[fwd[i] >> 6] = 0;
+            }
+        }
+    });
+    run_chunks(threads, threads, [&](int q) {
+        const int64_t b = static_cast<int64_t>(q) * n / threads, e = static_cast<int64_t>(q + 1) * n / threads;
+        for (int64_t v = b; v < e; ++v) {
+            int64_t s = 0;
+            for (const auto& p : part) s += p[v];
+            tri[v] = s;
+        }
+    });
 }
 
 }  // extern "C"

@@ -369,3 +369,40 @@ def test_parent_checks_reject_planted_mistakes():
                 caught[key] += 1
     assert all(v > 0 for v in caught.values()), caught
     print(caught)
+
+
+def test_tc_forward_against_networkx_and_restatement(golden):
+    """The forward-algorithm counter that checks TC at full size: networkx
+    per-vertex triangles on every golden graph's symmetrisation, and 6 t(v)
+    equal to the reference restatement's per-vertex counts."""
+    nx = pytest.importorskip("networkx")
+    rp, col = golden["sym_rowptr"], golden["sym_col"]
+    n = len(rp) - 1
+    U = nx.Graph()
+    U.add_nodes_from(range(n))
+    for u in range(n):
+        for e in range(rp[u], rp[u + 1]):
+            U.add_edge(u, int(col[e]))
+    tri = nx.triangles(U)
+    t = O.tc_forward(rp, col)
+    np.testing.assert_array_equal(t, [tri[v] for v in range(n)])
+    np.testing.assert_array_equal(6 * t, golden["tc_sym"])
+
+
+@pytest.mark.parametrize("make", [lambda: G.HostGraph.rmat(13, 16, seed=5), lambda: G.HostGraph.er(5000, 60000, seed=2)],
+                         ids=["rmat_s13", "er_5000"])
+def test_tc_forward_equals_restatement(make):
+    s = make().view(G.VIEW_SYM)
+    for threads in (1, 3):
+        np.testing.assert_array_equal(6 * O.tc_forward(s.rowptr, s.col, threads), O.tc(s.rowptr, s.col))
+
+
+def test_tc_forward_closed_forms():
+    for n in (3, 5, 9):  # K_n: C(n-1, 2) triangles through every vertex
+        e = [(i, j) for i in range(n) for j in range(n) if i != j]
+        s = G.HostGraph.from_edges(n, [a for a, _ in e], [b for _, b in e]).view(G.VIEW_SYM)
+        np.testing.assert_array_equal(O.tc_forward(s.rowptr, s.col), [math.comb(n - 1, 2)] * n)
+    # complete bipartite K_{4,5}: no triangles
+    e = [(i, 4 + j) for i in range(4) for j in range(5)]
+    s = G.HostGraph.from_edges(9, [a for a, _ in e], [b for _, b in e]).view(G.VIEW_SYM)
+    assert not O.tc_forward(s.rowptr, s.col).any()
