@@ -83,7 +83,7 @@ EXPORTS = [
     "ghet_graph_set_workspace_cap",
     "ghet_dyn_create", "ghet_dyn_from_graph", "ghet_dyn_update", "ghet_dyn_edges", "ghet_dyn_snapshot",
     "ghet_dyn_destroy",
-    "ghet_bfs", "ghet_bfs_pre", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_cdlp", "ghet_tc",
+    "ghet_bfs", "ghet_bfs_pre", "ghet_bfs_noalloc", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_cdlp", "ghet_tc",
     "ghet_bfs_multi", "ghet_sssp_multi", "ghet_wcc_many", "ghet_tc_many",
     "ghet_sched_load", "ghet_sched_save", "ghet_sched_clear", "ghet_sched_set_params", "ghet_sched_get_params",
     "ghet_mlp_forward", "ghet_mlp_forward_host", "ghet_mlp_train_step", "ghet_autotune",
@@ -138,6 +138,7 @@ def lib() -> C.CDLL:
         "ghet_graph_set_workspace_cap": (C.c_int, [vp, C.c_int64]),
         "ghet_bfs": (C.c_int, [vp, C.c_int32, i32p, P(Opts), P(RunStats)]),
         "ghet_bfs_pre": (C.c_int, [vp, C.c_int32, i32p, i32p, P(Opts), P(RunStats)]),
+        "ghet_bfs_noalloc": (C.c_int, [C.c_int32, C.c_int64, vp, vp, C.c_int32, vp, vp]),
         "ghet_sssp": (C.c_int, [vp, C.c_int32, C.c_double, f64p, P(Opts), P(RunStats)]),
         "ghet_sssp_pre": (C.c_int, [vp, C.c_int32, C.c_double, f64p, i32p, P(Opts), P(RunStats)]),
         "ghet_wcc": (C.c_int, [vp, i32p, P(Opts), P(RunStats)]),

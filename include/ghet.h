@@ -288,6 +288,22 @@ int ghet_cdlp(ghet_graph* g, int32_t iterations, int32_t* label_out,
  * Replaces Cuda_TC_PerNode / Cuda_TC (GPU_TC_csr.cuh:46-69). */
 int ghet_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* opts, ghet_run_stats* stats);
 
+/* BFS depths that allocate nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The caller passes a
+ * device out-CSR (d_rowptr: n+1 offsets, d_col: m targets) and a device
+ * output d_depth_out[n]; depths are those of ghet_bfs (0 at src, INT32_MAX
+ * unreached).  Frontier queues and counters live in a fixed device array of
+ * the library (32 MiB + 256 B, held by each GPU the library runs on until the
+ * process ends), so n is limited to 4,194,304 (2^22) vertices: larger graphs
+ * return GHET_ERR_UNSUPPORTED.  Runs on `stream` (a hipStream_t, NULL for the
+ * null stream) and returns when the depths are written.  Calls from one
+ * process are serialised by a lock, since they share the array; the array
+ * keeps the last call's contents, and each call clears what it needs.  The
+ * device arrays are not validated (as with the reference's raw cores). */
+int ghet_bfs_noalloc(int32_t n, int64_t m, const int32_t* d_rowptr, const int32_t* d_col, int32_t src,
+                     int32_t* d_depth_out, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched forms (no reference counterpart; results equal per-instance runs) */
 /* ------------------------------------------------------------------------ */
