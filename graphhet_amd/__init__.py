@@ -532,12 +532,14 @@ class DeviceGraph:
         _check(lib().ghet_autotune(self._h, cache_path.encode() if cache_path else None, C.byref(best)))
         return best.value
 
-    def sssp_multi(self, srcs, inf: float = DBL_MAX, delta: float = 0.0, out=True, per_pass: int = 0):
+    def sssp_multi(self, srcs, inf: float = DBL_MAX, delta: float = 0.0, out=True, per_pass: int = 0,
+                   phase_budget: int = 0):
         srcs = np.ascontiguousarray(srcs, dtype=np.int32)
         dist = np.empty((len(srcs), self.n), np.float64) if out else None
         st = RunStats()
         o = _opts(sssp_delta=delta)
         o.reserved[3] = per_pass
+        o.reserved[5] = phase_budget
         _check(lib().ghet_sssp_multi(self._h, len(srcs), _ptr(srcs, C.c_int32), inf, _ptr(dist, C.c_double),
                                      C.byref(o), C.byref(st)))
         return dist, st

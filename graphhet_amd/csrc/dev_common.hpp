@@ -193,6 +193,44 @@ __device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long x
     return t;
 }
 
+// Grid-wide barrier for cooperative launches, in place of the cooperative
+// groups one (about 60 us per barrier with 768 blocks on MI355X, measured).
+// Here thread 0 of each block fences once on each side and arrives in two
+// levels: kBarGroups group counters, then the last
+// arrival of each group bumps the top counter, which the waiters poll.
+// Counters only grow (barrier k completes when the top counter reaches
+// k * groups), so nothing is reset between barriers.  `bar` holds
+// kBarWords zeroed words before the launch; `epoch` is the caller's count of
+// barriers passed (thread 0's copy is the one used).
+constexpr int kBarGroups = 16;
+constexpr int kBarWords = (kBarGroups + 1) * 8;  // one 64-byte line per counter
+__device__ __forceinline__ void grid_barrier(unsigned long long* bar, unsigned long long& epoch) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int G = gridDim.x;
+        const unsigned int grp = blockIdx.x % kBarGroups;
+        const unsigned long long groups = G < kBarGroups ? G : kBarGroups;
+        const unsigned long long gsize = (G - grp + kBarGroups - 1) / kBarGroups;
+        ++epoch;
+        // The release fence writes back this XCD's L2 and waits for it before
+        // the arrival atomic issues; the group's last arrival bumps the top
+        // counter only after its own atomic has returned, so every block's
+        // writes are in memory before a waiter sees the barrier complete.
+        // One fence per side per block: about 13 us per barrier with 768
+        // blocks, against about 60 us for the cooperative-groups barrier.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned long long old =
+            __hip_atomic_fetch_add(bar + grp * 8, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == epoch * gsize)
+            __hip_atomic_fetch_add(bar + kBarGroups * 8, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long target = epoch * groups;
+        while (__hip_atomic_load(bar + kBarGroups * 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+            __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
 // LDS-staged output queue.  lq_push appends the lanes with pred (one LDS
 // atomic per wave); lq_flush (block-uniform) moves the staged items to a
 // global queue with ONE global atomic, calling on_item(v) for each item.

@@ -1,33 +1,37 @@
-// Batched SSSP: up to 32 sources per pass advanced together, one launch per
-// round for the whole batch (near-far buckets with a threshold shared by the
-// batch, light/heavy edge split as in the single-source form).
+// Batched SSSP: up to 32 sources per pass advanced together in ONE persistent
+// cooperative launch per pass (near-far buckets with a threshold shared by
+// the batch, light/heavy edge split as in the single-source form).  The whole
+// bucket loop runs on the device: rounds, heavy passes and far-list splits
+// are separated by grid barriers, and the host only seeds the pass and
+// collects its distances.
 //
 // No reference counterpart: the reference runs one csr_sssp per source
-// (GPU_SSSP_csr.cuh:253).  Every source converges to the same fixpoint as its
-// own run (the minimum over walks of the left-to-right rounded path sum, as
-// for a single source), so results are bit-identical to per-source runs
-// whatever the interleaving.
+// (GPU_SSSP_csr.cuh:253), with host-driven rounds (:287-307).  Every source
+// converges to the same fixpoint as its own run (the minimum over walks of
+// the left-to-right rounded path sum, as for a single source), so results
+// are bit-identical to per-source runs whatever the interleaving.
 //
 // Vertex queues with source masks.  A round's frontier is a queue of DISTINCT
 // vertices; vertex u carries a 32-bit mask of the sources whose distance at
-// u improved below the threshold.  One scan over the queue balances the
-// round's edges for all sources at once, and each edge (u -> v, w) is read
-// once and relaxed for every source in u's mask.  Distances are vertex-major
-// [n][K] (the K values of a vertex share cache lines).  Rows are ordered by
-// weight (the single-source layout), so a round reads only the prefix of u's
-// row that is light for the smallest masked distance, and relaxes per source
-// only the edges with d_s[u] + w < thr; an improvement ORs the source's bit
-// into v's next-round mask, whose 0 -> nonzero transition appends v to the
-// next queue.  It also ORs the bit into v's bucket mask, whose 0 -> nonzero
-// transition appends v to the bucket list.  When the near queue drains, one
-// heavy pass over the bucket list relaxes, per bucket-mask source, the edges
-// with d_s[u] + w >= thr (now final) and sends improvements to the far list
-// through far masks.  The far list is then split at the next threshold per
-// (vertex, source) bit; bits of entries already below the old threshold are
-// stale and dropped.
+// u improved below the threshold.  One scan over the queue
+// balances the round's edges for all sources at once, and each edge
+// (u -> v, w) is read once and relaxed for every source in u's mask.
+// Distances are vertex-major [n][K] (the K values of a vertex share cache
+// lines).  Rows are ordered by weight (the single-source layout), so a round
+// reads only the prefix of u's row that is light for the smallest masked
+// distance, and relaxes per source only the edges with d_s[u] + w < thr; an
+// improvement ORs the source's bit into v's next-round mask, whose
+// 0 -> nonzero transition appends v to the next queue.  It also ORs the bit
+// into v's bucket mask, whose 0 -> nonzero transition appends v to the
+// bucket list.  When the near queue drains, one heavy pass over the bucket
+// list relaxes, per bucket-mask source, the edges with d_s[u] + w >= thr
+// (now final) and sends improvements to the far list through far masks.  The
+// far list is then split at the next threshold per (vertex, source) bit; bits
+// of entries already below the old threshold are stale and dropped.
 //
 // The caller's [k][n] rows are produced by one tiled transpose per pass.
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 
 #include "graph_internal.hpp"
@@ -39,29 +43,34 @@ int build_sssp_layout(ghet_graph* g);
 
 namespace {
 
-// v[0] out-edges of the next near queue, v[1] bits of the smallest kept far
-// distance, v[2] out-edges appended to the bucket list, v[3] edges scanned.
-struct MssCtr {
-    int32_t near_count;
-    int32_t far_count[2];
-    int32_t list_count;
-    int32_t pad[12];
-    CtrSlot slots[kSlots];
+// One phase's counters, in three sets: phase P counts in set P % 3 and clears
+// set (P + 1) % 3, whose values every block read at the start of phase P - 1
+// (before that phase's grid barriers).  Queue lengths are carried by every
+// block in registers; a phase only counts its own appends.
+struct alignas(64) PhaseCtr {
+    int32_t near;              // vertices appended to the next near queue
+    int32_t list;              // vertices appended to the bucket list
+    int32_t far;               // vertices appended to a far list
+    int32_t stop;              // set by block 0: phase or time budget spent
+    int32_t tiles;             // relax tiles handed out beyond the first gridDim.x
+    int32_t pad32;
+    unsigned long long pad[5];
+    CtrSlot slots[kSlots];     // v[1] smallest kept far distance (bits), v[3] edges scanned
 };
-constexpr int kCtrWords = sizeof(MssCtr) / 8;
+static_assert(offsetof(PhaseCtr, slots) == 64, "phase_reset assumes 8 header words before the slots");
 constexpr unsigned long long kInfBits = 0x7FF0000000000000ULL;
 
 constexpr int kBT = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBT * kItems;
 // LDS capacity of each staged queue (a tile pushes at most kTile entries; a
-// flush empties it above kOut - kTile): 1.25 tiles instead of 2 lets 3 relax
-// blocks share a CU instead of 2 (256 sources on s22: 483 -> 456 ms).
+// flush empties it above kOut - kTile): 1.25 tiles instead of 2 lets 3
+// blocks share a CU instead of 2.
 constexpr int kOut = kTile + kTile / 4;
 // a relax tile pushes at most kTile entries per queue, a split step kBT
 static_assert(kOut >= kTile && kOut >= kBT, "staged queues smaller than one step's pushes");
-constexpr int kMaxBlocks = 2048;
-constexpr int kMaxK = 32;  // sources per pass (mask width)
+constexpr int kMaxK = 32;                        // sources per pass (mask width)
+constexpr int kMaxGrid = 4096;                   // blocks of the cooperative launch (btot entries)
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
     int lo = 0, hi = len;
@@ -73,202 +82,249 @@ __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_
     return lo;
 }
 
-// Seeds: q[j] = distinct source vertex with its sources qmask[j]: distance 0,
-// near and bucket masks, and the bucket list (= the first near queue).
-__global__ void k_mss_seed(const int32_t* __restrict__ q, const uint32_t* __restrict__ qmask, int32_t F, int32_t ks,
-                           double* dist, uint32_t* mask, uint32_t* bmask, int32_t* list, MssCtr* ctr) {
-    for (int j = threadIdx.x; j < F; j += blockDim.x) {
-        const int32_t v = q[j];
-        const uint32_t m = qmask[j];
-        mask[v] = m;
-        bmask[v] = m;
-        list[j] = v;
-        for (uint32_t b = m; b; b &= b - 1) dist[static_cast<int64_t>(v) * ks + (__ffs(b) - 1)] = 0.0;
-    }
-    if (threadIdx.x == 0) ctr->list_count = F;
+struct MssArgs {
+    int32_t n, ks;                     // vertices, sources in this pass
+    const int32_t* rowptr;             // out-CSR offsets
+    const int32_t* col;                // weight-ordered rows (targets)
+    const double* w;                   // weight-ordered rows (weights)
+    double* dist;                      // [n][ks]
+    uint32_t *mask_a, *mask_b;         // near masks of the current / next round
+    uint32_t *fmask, *bmask;           // far masks, bucket masks
+    int32_t *qa, *qb, *far0, *far1, *list;
+    int32_t *scan, *start, *vert;      // a pass's entries: first edge, first position, vertex
+    uint32_t* emask;                   // and source mask
+    int32_t* btot;                     // [gridDim.x] edges of each block's entry range
+    unsigned long long* bar;           // grid-barrier counters (kBarWords, zeroed before the launch)
+    PhaseCtr* ctr;                     // [3]
+    const int32_t* seed_v;             // distinct source vertices of the pass
+    const uint32_t* seed_m;            // and their source masks
+    int32_t seeds;
+    double inf, delta;
+    long long max_phases;              // budgets: the kernel stops (and reports it) past either
+    unsigned long long max_cycles;     // wall-clock ticks
+    long long* stats;                  // [0] rounds, [1] edges scanned, [2] stopped early
+};
+
+struct MssShared {
+    int32_t scan[kTile], start[kTile], src[kTile];
+    uint32_t mask[kTile];
+    int32_t qa[kOut], qb[kOut];        // staged appends (relax: near, list / far; split: list, far)
+    int32_t i0, span, na, nb, base;
+    int64_t next_tile;
+    int32_t wave[kBT / 64];
+    unsigned long long red[kBT / 64];
+    long long ctl[6];
+};
+
+// Block 0 clears a counter set (v[1] of every slot to +inf bits).
+__device__ __forceinline__ void phase_reset(PhaseCtr* c) {
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(c);
+    constexpr int kWords = sizeof(PhaseCtr) / 8;
+    for (int i = threadIdx.x; i < kWords; i += kBT) w[i] = (i >= 8 && (i - 8) % 8 == 1) ? kInfBits : 0ULL;
 }
 
-// Resets the near queue and the spread sums; also far list `other` (split)
-// and the bucket list (split: a new bucket starts).
-__global__ void k_mss_reset(MssCtr* c, int other) {
-    unsigned long long* w = reinterpret_cast<unsigned long long*>(c->slots);
-    for (int i = threadIdx.x; i < kSlots * 8; i += blockDim.x) w[i] = (i % 8 == 1) ? kInfBits : 0ULL;
-    if (threadIdx.x == 0) {
-        c->near_count = 0;
-        if (other >= 0) {
-            c->far_count[other] = 0;
-            c->list_count = 0;
+struct PhaseRes {
+    int32_t near, list, far, stop;
+    unsigned long long dmin, scanned;
+};
+
+// The finished phase's counts (block-uniform; call after a grid barrier).
+__device__ __forceinline__ PhaseRes phase_read(const PhaseCtr* c, MssShared& sh) {
+    if (threadIdx.x < 64) {  // wave 0: lane i reads slot i
+        const CtrSlot* sl = c->slots + threadIdx.x;
+        const unsigned long long mn = wave_min(load_agent(&sl->v[1]));
+        const unsigned long long sc = wave_sum(load_agent(&sl->v[3]));
+        if (threadIdx.x == 0) {
+            auto ld = [](const int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+            sh.ctl[0] = ld(&c->near);
+            sh.ctl[1] = ld(&c->list);
+            sh.ctl[2] = ld(&c->far);
+            sh.ctl[3] = ld(&c->stop);
+            sh.ctl[4] = static_cast<long long>(mn);
+            sh.ctl[5] = static_cast<long long>(sc);
         }
     }
+    __syncthreads();
+    PhaseRes r;
+    r.near = static_cast<int32_t>(sh.ctl[0]);
+    r.list = static_cast<int32_t>(sh.ctl[1]);
+    r.far = static_cast<int32_t>(sh.ctl[2]);
+    r.stop = static_cast<int32_t>(sh.ctl[3]);
+    r.dmin = static_cast<unsigned long long>(sh.ctl[4]);
+    r.scanned = static_cast<unsigned long long>(sh.ctl[5]);
+    __syncthreads();
+    return r;
 }
 
-// Per queue entry u with source mask m: the row segment a pass reads.  Round:
-// the prefix with min_s d_s[u] + w < thr (it holds every source's light
-// edges); heavy: the rest from the prefix end of max_s d_s[u] (it holds every
-// source's heavy edges).
+// Row segments of the F queue entries q[] (masks qm[], consumed and cleared
+// here): round, the prefix with min_s d_s[u] + w < thr, which holds every
+// masked source's light edges; heavy, the rest from the prefix end of
+// max_s d_s[u], which holds every masked source's heavy edges.  Entry i's
+// edge count goes to scan[i], its first position to start[i], its vertex and
+// mask to vert[i] and emask[i].  Block b takes the contiguous entries
+// [F b / G, F (b + 1) / G), one per thread at a time, and writes their total
+// to btot[b]; mss_scan turns the counts into offsets after a grid barrier.
+// Queues of at most kBT entries are done by block 0 alone, scan included
+// (returns true there: no barrier and no mss_scan needed).
+__device__ __forceinline__ int64_t range_lo(int32_t F, int64_t b) { return static_cast<int64_t>(F) * b / gridDim.x; }
+
 template <bool kHeavy>
-__global__ void k_mss_rows(const int32_t* __restrict__ q, int32_t F, const uint32_t* __restrict__ qmask,
-                           const int32_t* __restrict__ rowptr, const double* __restrict__ w,
-                           const double* __restrict__ dist, int32_t ks, double threshold, int32_t* __restrict__ cnt,
-                           int32_t* __restrict__ start) {
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < F;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int32_t u = q[i];
-        const uint32_t m = qmask[u];
-        const double* du = dist + static_cast<int64_t>(u) * ks;
-        double d = kHeavy ? 0.0 : HUGE_VAL;
-        for (uint32_t b = m; b; b &= b - 1) {
-            const double x = du[__ffs(b) - 1];
-            d = kHeavy ? fmax(d, x) : fmin(d, x);
-        }
-        const int32_t rb = rowptr[u], re = rowptr[u + 1];
-        int32_t lo = rb, hi = re;
-        while (lo < hi) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (d + w[mid] < threshold) lo = mid + 1;
-            else hi = mid;
-        }
-        cnt[i] = kHeavy ? re - lo : lo - rb;
-        start[i] = kHeavy ? lo : rb;
-    }
-}
-
-// Small queues (at most kSmallMax entries): one 1024-thread block resets the
-// pass's counters (as k_mss_reset with no far list), computes the row
-// segments (as k_mss_rows) and scans the counts.
-constexpr int kSmallBT = 1024;
-constexpr int kSmallItems = 4;
-constexpr int kSmallMax = kSmallBT * kSmallItems;
-
-template <bool kHeavy>
-__global__ void __launch_bounds__(kSmallBT) k_mss_rows_small(const int32_t* __restrict__ q, int32_t F,
-                                                             const uint32_t* __restrict__ qmask,
-                                                             const int32_t* __restrict__ rowptr,
-                                                             const double* __restrict__ w,
-                                                             const double* __restrict__ dist, int32_t ks,
-                                                             double threshold, int32_t* __restrict__ scan,
-                                                             int32_t* __restrict__ start, MssCtr* ctr) {
-    __shared__ int32_t s_wave[kSmallBT / 64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    unsigned long long* words = reinterpret_cast<unsigned long long*>(ctr->slots);
-    for (int i = tid; i < kSlots * 8; i += kSmallBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
-    if (tid == 0) ctr->near_count = 0;
-    int32_t c[kSmallItems], st[kSmallItems];
-    int32_t sum = 0;
+__device__ __forceinline__ int32_t row_segment(const MssArgs& a, int32_t u, uint32_t m, double thr, int32_t* st) {
+    const double* du = a.dist + static_cast<int64_t>(u) * a.ks;
+    double d = kHeavy ? 0.0 : HUGE_VAL;
+    for (uint32_t rem = m; rem;) {  // four loads in flight at a time
+        double x[4];
 #pragma unroll
-    for (int k = 0; k < kSmallItems; ++k) {
-        const int i = tid * kSmallItems + k;
-        c[k] = 0;
-        st[k] = 0;
-        if (i < F) {
-            const int32_t u = q[i];
-            const uint32_t m = qmask[u];
-            const double* du = dist + static_cast<int64_t>(u) * ks;
-            double d = kHeavy ? 0.0 : HUGE_VAL;
-            for (uint32_t b = m; b; b &= b - 1) {
-                const double x = du[__ffs(b) - 1];
-                d = kHeavy ? fmax(d, x) : fmin(d, x);
-            }
-            const int32_t rb = rowptr[u], re = rowptr[u + 1];
-            int32_t lo = rb, hi = re;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi) >> 1;
-                if (d + w[mid] < threshold) lo = mid + 1;
-                else hi = mid;
-            }
-            c[k] = kHeavy ? re - lo : lo - rb;
-            st[k] = kHeavy ? lo : rb;
+        for (int j = 0; j < 4; ++j) {
+            const int sx = __ffs(rem) - 1;
+            rem &= rem - 1;
+            x[j] = sx >= 0 ? du[sx] : (kHeavy ? 0.0 : HUGE_VAL);
         }
-        sum += c[k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d = kHeavy ? fmax(d, x[j]) : fmin(d, x[j]);
     }
-    int32_t incl = sum;
+    const int32_t rb = a.rowptr[u], re = a.rowptr[u + 1];
+    int32_t lo = rb, hi = re;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (d + a.w[mid] < thr) lo = mid + 1;
+        else hi = mid;
+    }
+    *st = kHeavy ? lo : rb;
+    return kHeavy ? re - lo : lo - rb;
+}
+
+// Exclusive scan of one value per thread over the block; *total gets the sum.
+__device__ __forceinline__ int32_t block_excl_scan(int32_t x, int32_t* total, MssShared& sh) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t incl = x;
     for (int off = 1; off < 64; off <<= 1) {
         const int32_t y = __shfl_up(incl, off);
         if (lane >= off) incl += y;
     }
-    if (lane == 63) s_wave[wave] = incl;
+    if (lane == 63) sh.wave[wave] = incl;
     __syncthreads();
-    int32_t run = incl - sum, total = 0;
-    for (int v = 0; v < kSmallBT / 64; ++v) {
-        if (v < wave) run += s_wave[v];
-        total += s_wave[v];
+    int32_t run = incl - x, t = 0;
+    for (int v = 0; v < kBT / 64; ++v) {
+        if (v < wave) run += sh.wave[v];
+        t += sh.wave[v];
     }
-#pragma unroll
-    for (int k = 0; k < kSmallItems; ++k) {
-        const int i = tid * kSmallItems + k;
-        if (i < F) {
-            scan[i] = run;
-            start[i] = st[k];
-            run += c[k];
-        }
-    }
-    if (tid == 0) scan[F] = total;
+    __syncthreads();  // sh.wave is reused by the next call
+    *total = t;
+    return run;
 }
 
-// The queue's masks are consumed: clear them for reuse two rounds later.
-__global__ void k_clear_mask(const int32_t* __restrict__ q, int32_t F, uint32_t* __restrict__ mask) {
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < F;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        mask[q[i]] = 0u;
-}
-
-// One pass, persistent blocks over 2048-edge tiles; entry j of queue q covers
-// edges [start[j], start[j] + scan[j+1] - scan[j]) of the weight-ordered rows
-// and scan[F] is the pass's edge count.
-// Round (kHeavy false): per source in mask_cur[u], relaxes the edges with
-// d_s[u] + w < thr; improvements go to v's next-round mask (0 -> nonzero
-// appends v to next_q) and its bucket mask (0 -> nonzero appends v to list).
-// Heavy (kHeavy true): q is the drained bucket's list and mask_cur the
-// bucket masks; relaxes the edges with d_s[u] + w >= thr, improvements go to
-// v's far mask (0 -> nonzero appends v to the far list).
 template <bool kHeavy>
-__global__ void __launch_bounds__(kBT) k_mss_relax(const int32_t* __restrict__ q, int32_t F,
-                                                   const int32_t* __restrict__ scan,
-                                                   const int32_t* __restrict__ start,
-                                                   const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                                   const double* __restrict__ wts, int32_t ks, double* dist,
-                                                   double threshold, const uint32_t* __restrict__ mask_cur,
-                                                   uint32_t* mask_next, uint32_t* bmask, uint32_t* fmask,
-                                                   int32_t* __restrict__ next_q, int32_t* __restrict__ list,
-                                                   int32_t* far_list, int far_sel, MssCtr* ctr) {
-    __shared__ int32_t s_scan[kTile];
-    __shared__ int32_t s_start[kTile];
-    __shared__ int32_t s_src[kTile];
-    __shared__ uint32_t s_mask[kTile];
-    __shared__ int32_t s_qa[kOut], s_qb[kOut];  // round: near, list; heavy: far
-    __shared__ int32_t s_i0, s_span, s_na, s_nb, s_base;
-    __shared__ unsigned long long s_red[kBT / 64];
+__device__ bool mss_count(const MssArgs& a, const int32_t* __restrict__ q, int32_t F, uint32_t* qm, double thr,
+                          MssShared& sh) {
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        s_na = 0;
-        s_nb = 0;
+    if (F <= kBT) {
+        if (blockIdx.x == 0) {
+            int32_t cnt = 0, st = 0, u = 0;
+            uint32_t m = 0;
+            if (tid < F) {
+                u = q[tid];
+                m = qm[u];
+                qm[u] = 0u;
+                cnt = row_segment<kHeavy>(a, u, m, thr, &st);
+            }
+            int32_t total;
+            const int32_t off = block_excl_scan(cnt, &total, sh);
+            if (tid < F) {
+                a.scan[tid] = off;
+                a.start[tid] = st;
+                a.vert[tid] = u;
+                a.emask[tid] = m;
+            }
+            if (tid == 0) a.scan[F] = total;
+        }
+        return true;
     }
-    __syncthreads();  // counters zeroed before any wave appends
-    unsigned long long od = 0, ld = 0;
-    auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
-    auto on_list = [&](int32_t v) { ld += rowptr[v + 1] - rowptr[v]; };
+    const int64_t b0 = range_lo(F, blockIdx.x), b1 = range_lo(F, blockIdx.x + 1);
+    int32_t sum = 0;
+    for (int64_t i = b0 + tid; i < b1; i += kBT) {
+        const int32_t u = q[i];
+        const uint32_t m = qm[u];
+        qm[u] = 0u;
+        int32_t st;
+        const int32_t cnt = row_segment<kHeavy>(a, u, m, thr, &st);
+        a.scan[i] = cnt;
+        a.start[i] = st;
+        a.vert[i] = u;
+        a.emask[i] = m;
+        sum += cnt;
+    }
+    int32_t total;
+    (void)block_excl_scan(sum, &total, sh);
+    if (tid == 0) a.btot[blockIdx.x] = total;
+    return false;
+}
+
+// After a grid barrier: block b adds the totals of blocks before it and
+// scans its range's counts in place; the last block writes scan[F].
+__device__ void mss_scan(const MssArgs& a, int32_t F, MssShared& sh) {
+    const int tid = threadIdx.x;
+    int32_t part = 0;
+    for (int j = tid; j < static_cast<int>(blockIdx.x); j += kBT) part += a.btot[j];
+    int32_t carry;
+    (void)block_excl_scan(part, &carry, sh);
+    const int64_t b0 = range_lo(F, blockIdx.x), b1 = range_lo(F, blockIdx.x + 1);
+    for (int64_t base = b0; base < b1; base += kBT) {
+        const int64_t i = base + tid;
+        const int32_t c = i < b1 ? a.scan[i] : 0;
+        int32_t t;
+        const int32_t off = block_excl_scan(c, &t, sh);
+        if (i < b1) a.scan[i] = carry + off;
+        carry += t;
+    }
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) a.scan[F] = carry;
+}
+
+// One pass over the prepared entries, in kTile-edge tiles; entry j covers
+// edges [start[j], start[j] + scan[j+1] - scan[j]) of the weight-ordered
+// rows.  Round (kHeavy false): per source in the entry's mask, relaxes the
+// edges with d_s[u] + w < thr; an improvement ORs the source's bit into v's
+// next-round mask (0 -> nonzero appends v to next_q) and its bucket mask
+// (0 -> nonzero appends v to list_out).  Heavy: relaxes the edges with
+// d_s[u] + w >= thr; improvements go to v's far mask (0 -> nonzero appends v
+// to far_out).
+template <bool kHeavy>
+__device__ void mss_relax(const MssArgs& a, int32_t F, double thr, uint32_t* mask_next, int32_t* next_q,
+                          int32_t* list_out, int32_t* far_out, PhaseCtr* c, MssShared& sh) {
+    const int tid = threadIdx.x;
+    const int32_t E = __hip_atomic_load(a.scan + F, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (E == 0) return;  // block-uniform
+    if (tid == 0) {
+        sh.na = 0;
+        sh.nb = 0;
+    }
+    __syncthreads();
     auto nop = [](int32_t) {};
-    const int32_t E = scan[F];
-    if (blockIdx.x == 0 && tid == 0) slot_add(ctr->slots, 3, static_cast<unsigned long long>(E));
-    for (int64_t tile = blockIdx.x; tile * kTile < E; tile += gridDim.x) {
+    if (blockIdx.x == 0 && tid == 0) slot_add(c->slots, 3, static_cast<unsigned long long>(E));
+    // Tiles are handed out dynamically: block b starts with tile b, then
+    // takes the next unclaimed one (one atomic per tile), so blocks that
+    // draw cheap tiles take more of them.
+    const int64_t ntiles = ceil_div(E, kTile);
+    for (int64_t tile = blockIdx.x; tile < ntiles;) {
         const int32_t t0 = static_cast<int32_t>(tile * kTile);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
         if (tid == 0) {
-            const int i0 = upper_bound_i32(scan, F + 1, t0) - 1;
-            const int i1 = upper_bound_i32(scan, F + 1, t1 - 1) - 1;
-            s_i0 = i0;
-            s_span = i1 - i0 + 1;
+            const int i0 = upper_bound_i32(a.scan, F + 1, t0) - 1;
+            const int i1 = upper_bound_i32(a.scan, F + 1, t1 - 1) - 1;
+            sh.i0 = i0;
+            sh.span = i1 - i0 + 1;
+            sh.next_tile = static_cast<int64_t>(gridDim.x) + atomicAdd(&c->tiles, 1);
         }
         __syncthreads();
-        const int i0 = s_i0, span = s_span;
+        const int i0 = sh.i0, span = sh.span;
+        const int64_t next_tile = sh.next_tile;
         const bool use_lds = span <= kTile;
         if (use_lds)
             for (int j = tid; j < span; j += kBT) {
-                const int32_t u = q[i0 + j];
-                s_scan[j] = scan[i0 + j];
-                s_start[j] = start[i0 + j];
-                s_src[j] = u;
-                s_mask[j] = mask_cur[u];
+                sh.scan[j] = a.scan[i0 + j];
+                sh.start[j] = a.start[i0 + j];
+                sh.src[j] = a.vert[i0 + j];
+                sh.mask[j] = a.emask[i0 + j];
             }
         __syncthreads();
         // Each stage runs over the thread's kItems edges together, so their
@@ -284,91 +340,98 @@ __global__ void __launch_bounds__(kBT) k_mss_relax(const int32_t* __restrict__ q
             msk[k] = 0;
             if (e < t1) {
                 if (use_lds) {
-                    const int j = upper_bound_i32(s_scan, span, e) - 1;
-                    pos[k] = s_start[j] + (e - s_scan[j]);
-                    u[k] = s_src[j];
-                    msk[k] = s_mask[j];
+                    const int j = upper_bound_i32(sh.scan, span, e) - 1;
+                    pos[k] = sh.start[j] + (e - sh.scan[j]);
+                    u[k] = sh.src[j];
+                    msk[k] = sh.mask[j];
                 } else {
-                    const int j = upper_bound_i32(scan + i0, span, e) - 1;
-                    u[k] = q[i0 + j];
-                    pos[k] = start[i0 + j] + (e - scan[i0 + j]);
-                    msk[k] = mask_cur[u[k]];
+                    const int j = upper_bound_i32(a.scan + i0, span, e) - 1;
+                    u[k] = a.vert[i0 + j];
+                    pos[k] = a.start[i0 + j] + (e - a.scan[i0 + j]);
+                    msk[k] = a.emask[i0 + j];
                 }
             }
         }
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
-            v[k] = col[pos[k]];
-            w[k] = wts[pos[k]];
+            v[k] = a.col[pos[k]];
+            w[k] = a.w[pos[k]];
         }
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             bool push_a = false, push_b = false;
-            const double* du = dist + static_cast<int64_t>(u[k]) * ks;
-            double* dv = dist + static_cast<int64_t>(v[k]) * ks;
+            const double* du = a.dist + static_cast<int64_t>(u[k]) * a.ks;
+            double* dv = a.dist + static_cast<int64_t>(v[k]) * a.ks;
+            // the masked sources four at a time: their distance loads, then
+            // their atomics, are in flight together
             uint32_t bits = 0;
-            for (uint32_t b = msk[k]; b; b &= b - 1) {
-                const int s = __ffs(b) - 1;
-                const double nd = du[s] + w[k];
-                if ((kHeavy ? nd >= threshold : nd < threshold) && nd < dv[s]) {
-                    const double old = atomic_min_nonneg(&dv[s], nd);
-                    if (nd < old) bits |= 1u << s;
+            for (uint32_t rem = msk[k]; rem;) {
+                int sx[4];
+                double nd[4], cur[4], old[4];
+                bool c[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    sx[j] = __ffs(rem) - 1;  // -1 once the mask is spent
+                    rem &= rem - 1;
                 }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) nd[j] = sx[j] >= 0 ? du[sx[j]] + w[k] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c[j] = sx[j] >= 0 && (kHeavy ? nd[j] >= thr : nd[j] < thr);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cur[j] = c[j] ? dv[sx[j]] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    c[j] = c[j] && nd[j] < cur[j];
+                    old[j] = c[j] ? atomic_min_nonneg(&dv[sx[j]], nd[j]) : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (c[j] && nd[j] < old[j]) bits |= 1u << sx[j];
             }
             if (bits) {
                 if (kHeavy) {
-                    push_a = atomicOr(&fmask[v[k]], bits) == 0u;
+                    push_a = atomicOr(&a.fmask[v[k]], bits) == 0u;
                 } else {
                     push_a = atomicOr(&mask_next[v[k]], bits) == 0u;
-                    push_b = atomicOr(&bmask[v[k]], bits) == 0u;
+                    push_b = atomicOr(&a.bmask[v[k]], bits) == 0u;
                 }
             }
-            lq_push(s_qa, &s_na, push_a, v[k]);
-            if (!kHeavy) lq_push(s_qb, &s_nb, push_b, v[k]);
+            lq_push(sh.qa, &sh.na, push_a, v[k]);
+            if (!kHeavy) lq_push(sh.qb, &sh.nb, push_b, v[k]);
         }
         if (kHeavy) {
-            lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->far_count[far_sel], far_list, kOut - kTile, nop);
+            lq_flush<kBT>(sh.qa, &sh.na, &sh.base, &c->far, far_out, kOut - kTile, nop);
         } else {
-            lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->near_count, next_q, kOut - kTile, on_near);
-            lq_flush<kBT>(s_qb, &s_nb, &s_base, &ctr->list_count, list, kOut - kTile, on_list);
+            lq_flush<kBT>(sh.qa, &sh.na, &sh.base, &c->near, next_q, kOut - kTile, nop);
+            lq_flush<kBT>(sh.qb, &sh.nb, &sh.base, &c->list, list_out, kOut - kTile, nop);
         }
+        tile = next_tile;  // lq_flush's barriers order this read before sh.next_tile's next write
     }
     if (kHeavy) {
-        lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->far_count[far_sel], far_list, 0, nop);
+        lq_flush<kBT>(sh.qa, &sh.na, &sh.base, &c->far, far_out, 0, nop);
     } else {
-        lq_flush<kBT>(s_qa, &s_na, &s_base, &ctr->near_count, next_q, 0, on_near);
-        lq_flush<kBT>(s_qb, &s_nb, &s_base, &ctr->list_count, list, 0, on_list);
-        od = block_sum_u64<kBT>(od, s_red);
-        ld = block_sum_u64<kBT>(ld, s_red);
-        if (tid == 0) {
-            slot_add(ctr->slots, 0, od);
-            slot_add(ctr->slots, 2, ld);
-        }
+        lq_flush<kBT>(sh.qa, &sh.na, &sh.base, &c->near, next_q, 0, nop);
+        lq_flush<kBT>(sh.qb, &sh.nb, &sh.base, &c->list, list_out, 0, nop);
     }
 }
 
-// Splits far list `far_in` at new_thr: per (vertex, source) bit, entries below
+// Splits the far list at new_thr: per (vertex, source) bit, entries below
 // old_thr are stale (settled in an earlier bucket) and dropped, entries below
 // new_thr start the next bucket, the rest stay far.  A vertex with bits in
 // the new bucket is appended to the bucket list, which is also the bucket's
 // first near queue (its near and bucket masks get those bits; both masks are
-// all-zero here and far-list entries are distinct).
-__global__ void __launch_bounds__(kBT) k_mss_split(const int32_t* __restrict__ far_in, int32_t count,
-                                                   const double* __restrict__ dist, int32_t ks, double old_thr,
-                                                   double new_thr, uint32_t* fmask, uint32_t* mask_near,
-                                                   uint32_t* bmask, int32_t* __restrict__ list, int32_t* far_out,
-                                                   int other, const int32_t* __restrict__ rowptr, MssCtr* ctr) {
-    __shared__ int32_t s_near[kOut], s_far[kOut];
-    __shared__ int32_t s_nc, s_fc, s_base;
-    __shared__ unsigned long long s_red[kBT / 64];
+// all-zero here and far-list entries are distinct).  The smallest kept far
+// distance goes to the slots' v[1].
+__device__ void mss_split(const MssArgs& a, const int32_t* __restrict__ far_in, int32_t count, double old_thr,
+                          double new_thr, uint32_t* mask_near, int32_t* far_out, PhaseCtr* c, MssShared& sh) {
     if (threadIdx.x == 0) {
-        s_nc = 0;
-        s_fc = 0;
+        sh.na = 0;
+        sh.nb = 0;
     }
-    __syncthreads();  // counters zeroed before any wave appends
-    unsigned long long od = 0, dmin = kInfBits;
-    auto on_near = [&](int32_t v) { od += rowptr[v + 1] - rowptr[v]; };
+    __syncthreads();
     auto nop = [](int32_t) {};
+    unsigned long long dmin = kInfBits;
     for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBT; base < count;
          base += static_cast<int64_t>(gridDim.x) * kBT) {
         const int64_t i = base + threadIdx.x;
@@ -376,8 +439,8 @@ __global__ void __launch_bounds__(kBT) k_mss_split(const int32_t* __restrict__ f
         int32_t v = 0;
         if (i < count) {
             v = far_in[i];
-            const uint32_t m = fmask[v];
-            const double* dv = dist + static_cast<int64_t>(v) * ks;
+            const uint32_t m = a.fmask[v];
+            const double* dv = a.dist + static_cast<int64_t>(v) * a.ks;
             uint32_t nb = 0, kb = 0;
             for (uint32_t b = m; b; b &= b - 1) {
                 const int s = __ffs(b) - 1;
@@ -391,29 +454,174 @@ __global__ void __launch_bounds__(kBT) k_mss_split(const int32_t* __restrict__ f
                     }
                 }
             }
-            fmask[v] = kb;
+            a.fmask[v] = kb;
             keep = kb != 0u;
             if (nb) {
                 mask_near[v] = nb;
-                bmask[v] = nb;
+                a.bmask[v] = nb;
                 to_near = true;
             }
         }
-        lq_push(s_near, &s_nc, to_near, v);
-        lq_push(s_far, &s_fc, keep, v);
-        lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, kOut - kBT, on_near);
-        lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, kOut - kBT, nop);
+        lq_push(sh.qa, &sh.na, to_near, v);
+        lq_push(sh.qb, &sh.nb, keep, v);
+        lq_flush<kBT>(sh.qa, &sh.na, &sh.base, &c->list, a.list, kOut - kBT, nop);
+        lq_flush<kBT>(sh.qb, &sh.nb, &sh.base, &c->far, far_out, kOut - kBT, nop);
     }
-    lq_flush<kBT>(s_near, &s_nc, &s_base, &ctr->list_count, list, 0, on_near);
-    lq_flush<kBT>(s_far, &s_fc, &s_base, &ctr->far_count[other], far_out, 0, nop);
-    od = block_sum_u64<kBT>(od, s_red);
+    lq_flush<kBT>(sh.qa, &sh.na, &sh.base, &c->list, a.list, 0, nop);
+    lq_flush<kBT>(sh.qb, &sh.nb, &sh.base, &c->far, far_out, 0, nop);
     dmin = wave_min(dmin);
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = dmin;
+    if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = dmin;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 0; w < kBT / 64; ++w) dmin = s_red[w] < dmin ? s_red[w] : dmin;
-        slot_add(ctr->slots, 0, od);
-        if (dmin != kInfBits) atomicMin(&ctr->slots[blockIdx.x & (kSlots - 1)].v[1], dmin);
+        for (int w = 0; w < kBT / 64; ++w) dmin = sh.red[w] < dmin ? sh.red[w] : dmin;
+        if (dmin != kInfBits) atomicMin(&c->slots[blockIdx.x & (kSlots - 1)].v[1], dmin);
+    }
+    __syncthreads();
+}
+
+// One pass of up to 32 sources in one cooperative launch: the whole bucket
+// loop (light rounds, the heavy pass, far-list splits) runs on the device,
+// with grid barriers between the steps.  Every block reads the same counts
+// after each barrier, so all take the same branches and meet at every
+// barrier.  A phase or wall-clock budget, checked by block 0 and published
+// through the phase's counters, ends the loop uniformly (stats[2] = 1).
+__global__ void __launch_bounds__(kBT) k_mss_run(MssArgs a) {
+    __shared__ MssShared sh;
+    unsigned long long epoch = 0;  // grid barriers passed
+    const int tid = threadIdx.x;
+    const int64_t gthreads = static_cast<int64_t>(gridDim.x) * kBT;
+    const int64_t gtid = static_cast<int64_t>(blockIdx.x) * kBT + tid;
+    const int64_t kn = static_cast<int64_t>(a.n) * a.ks;
+    for (int64_t i = gtid; i < kn; i += gthreads) a.dist[i] = a.inf;
+    for (int64_t i = gtid; i < a.n; i += gthreads) {
+        a.mask_a[i] = 0u;
+        a.mask_b[i] = 0u;
+        a.fmask[i] = 0u;
+        a.bmask[i] = 0u;
+    }
+    if (blockIdx.x == 0) {
+        phase_reset(a.ctr);
+        phase_reset(a.ctr + 1);
+    }
+    grid_barrier(a.bar, epoch);
+    if (blockIdx.x == 0)
+        for (int j = tid; j < a.seeds; j += kBT) {
+            const int32_t v = a.seed_v[j];
+            const uint32_t m = a.seed_m[j];
+            a.mask_a[v] = m;
+            a.bmask[v] = m;
+            a.list[j] = v;
+            for (uint32_t b = m; b; b &= b - 1) a.dist[static_cast<int64_t>(v) * a.ks + (__ffs(b) - 1)] = 0.0;
+        }
+    grid_barrier(a.bar, epoch);
+
+    uint32_t *mcur = a.mask_a, *mnext = a.mask_b;
+    const int32_t* cur = a.list;  // a bucket's first near queue is its list
+    int32_t* nxt = a.qa;
+    int32_t* far[2] = {a.far0, a.far1};
+    int32_t F = a.seeds, list_count = a.seeds, far_count[2] = {0, 0};
+    int sel = 0;
+    double thr = a.delta;
+    long long P = 0, rounds = 0;
+    unsigned long long scanned = 0;
+    bool stopped = false;
+    const unsigned long long t0 = wall_clock64();
+    // Start of a phase: block 0 clears the set after this phase's and flags
+    // a spent budget in this phase's set.
+    auto begin_phase = [&]() -> PhaseCtr* {
+        PhaseCtr* c = a.ctr + (P % 3);
+        if (blockIdx.x == 0) {
+            phase_reset(a.ctr + ((P + 1) % 3));
+            if (tid == 0 && (P + 1 >= a.max_phases || wall_clock64() - t0 > a.max_cycles)) c->stop = 1;
+        }
+        return c;
+    };
+    for (;;) {
+        // light rounds of the current bucket
+        while (F > 0) {
+            PhaseCtr* c = begin_phase();
+            if (!mss_count<false>(a, cur, F, mcur, thr, sh)) {
+                grid_barrier(a.bar, epoch);
+                mss_scan(a, F, sh);
+            }
+            grid_barrier(a.bar, epoch);
+            mss_relax<false>(a, F, thr, mnext, nxt, a.list + list_count, nullptr, c, sh);
+            grid_barrier(a.bar, epoch);
+            const PhaseRes r = phase_read(c, sh);
+            ++P;
+            ++rounds;
+            scanned += r.scanned;
+            F = r.near;
+            list_count += r.list;
+            cur = nxt;
+            nxt = (cur == a.qa) ? a.qb : a.qa;
+            uint32_t* t = mcur;
+            mcur = mnext;
+            mnext = t;
+            if (r.stop) {
+                stopped = true;
+                break;
+            }
+        }
+        if (stopped) break;
+        // heavy pass over the drained bucket: its distances are final
+        if (list_count > 0) {
+            PhaseCtr* c = begin_phase();
+            if (!mss_count<true>(a, a.list, list_count, a.bmask, thr, sh)) {
+                grid_barrier(a.bar, epoch);
+                mss_scan(a, list_count, sh);
+            }
+            grid_barrier(a.bar, epoch);
+            mss_relax<true>(a, list_count, thr, nullptr, nullptr, nullptr, far[sel] + far_count[sel], c, sh);
+            grid_barrier(a.bar, epoch);
+            const PhaseRes r = phase_read(c, sh);
+            ++P;
+            scanned += r.scanned;
+            far_count[sel] += r.far;
+            if (r.stop) {
+                stopped = true;
+                break;
+            }
+        }
+        if (far_count[sel] == 0) break;
+        // split the far list at the next threshold into the next bucket's
+        // list (its near masks go to mcur, all-zero after the last round)
+        double old_thr = thr;
+        double new_thr = fmax(thr + a.delta, nextafter(thr, HUGE_VAL));
+        for (;;) {
+            const int other = 1 - sel;
+            PhaseCtr* c = begin_phase();
+            mss_split(a, far[sel], far_count[sel], old_thr, new_thr, mcur, far[other], c, sh);
+            grid_barrier(a.bar, epoch);
+            const PhaseRes r = phase_read(c, sh);
+            ++P;
+            ++rounds;
+            F = r.list;
+            list_count = r.list;
+            far_count[other] = r.far;
+            far_count[sel] = 0;
+            sel = other;
+            if (r.stop) {
+                stopped = true;
+                break;
+            }
+            if (F > 0 || far_count[sel] == 0) break;
+            // nothing in the next bucket: jump to the smallest far distance
+            // (strictly above it, so the split always makes progress)
+            const double far_min = __longlong_as_double(static_cast<long long>(r.dmin));
+            old_thr = new_thr;
+            new_thr = fmax(far_min + a.delta, nextafter(far_min, HUGE_VAL));
+        }
+        if (stopped) break;
+        thr = new_thr;
+        cur = a.list;
+        nxt = a.qa;
+        if (F == 0 && far_count[sel] == 0) break;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        a.stats[0] = rounds;
+        a.stats[1] = static_cast<long long>(scanned);
+        a.stats[2] = stopped ? 1 : 0;
     }
 }
 
@@ -476,46 +684,68 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     const int32_t K = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>({kcap, int64_t(kMaxK), int64_t(k)})));
     const int64_t KN = int64_t(K) * n;
     void* p;
-    double* dist;
-    uint32_t *mask_a, *mask_b, *fmask, *bmask, *d_qmask;
-    int32_t *qa, *qb, *far[2], *list, *qstart, *scan, *aux, *d_q;
-    MssCtr* ctr;
     GHET_CHECK(build_sssp_layout(g));
     const SsspLayout& L = g->sl;
-    GHET_CHECK(g->ws.get(WS_MULTI_A, 8 * size_t(KN), &p)); dist = static_cast<double*>(p);
+    MssArgs a;
+    a.n = n;
+    a.rowptr = g->out.rowptr;
+    a.col = L.col;
+    a.w = L.w;
+    a.inf = inf;
+    a.delta = delta;
+    GHET_CHECK(g->ws.get(WS_MULTI_A, 8 * size_t(KN), &p)); a.dist = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_MULTI_B, 16 * size_t(n), &p));
-    mask_a = static_cast<uint32_t*>(p);
-    mask_b = mask_a + n;
-    fmask = mask_b + n;
-    bmask = fmask + n;
-    GHET_CHECK(g->ws.get(WS_MULTI_C, 24 * size_t(n), &p));
-    qa = static_cast<int32_t*>(p);
-    qb = qa + n;
-    far[0] = qb + n;
-    far[1] = far[0] + n;
-    list = far[1] + n;
-    qstart = list + n;
-    GHET_CHECK(g->ws.get(WS_MULTI_D, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(MssCtr), &p)); ctr = static_cast<MssCtr*>(p);
-    GHET_CHECK(g->ws.get(WS_MISC, 8 * size_t(kMaxK) + 256, &p));
-    d_q = static_cast<int32_t*>(p);
-    d_qmask = reinterpret_cast<uint32_t*>(d_q + kMaxK);
+    a.mask_a = static_cast<uint32_t*>(p);
+    a.mask_b = a.mask_a + n;
+    a.fmask = a.mask_b + n;
+    a.bmask = a.fmask + n;
+    GHET_CHECK(g->ws.get(WS_MULTI_C, 32 * size_t(n), &p));
+    a.qa = static_cast<int32_t*>(p);
+    a.qb = a.qa + n;
+    a.far0 = a.qb + n;
+    a.far1 = a.far0 + n;
+    a.list = a.far1 + n;
+    a.start = a.list + n;
+    a.vert = a.start + n;
+    a.emask = reinterpret_cast<uint32_t*>(a.vert + n);
+    GHET_CHECK(g->ws.get(WS_MULTI_D, 4 * (size_t(n) + 1), &p)); a.scan = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_COUNTERS, 3 * sizeof(PhaseCtr) + 64, &p));
+    a.ctr = static_cast<PhaseCtr*>(p);
+    a.stats = reinterpret_cast<long long*>(a.ctr + 3);
+    GHET_CHECK(g->ws.get(WS_MISC, 8 * size_t(kMaxK) + 4 * size_t(kMaxGrid) + 256, &p));
+    int32_t* d_q = static_cast<int32_t*>(p);
+    uint32_t* d_qmask = reinterpret_cast<uint32_t*>(d_q + kMaxK);
+    a.seed_v = d_q;
+    a.seed_m = d_qmask;
+    a.btot = d_q + 2 * kMaxK;
+    GHET_CHECK(g->ws.get(WS_BAR, 8 * size_t(kBarWords), &p));
+    a.bar = static_cast<unsigned long long*>(p);
     double* tr = nullptr;
     if (dist_out) {
         GHET_CHECK(g->ws.get(WS_PR_OUT, 8 * size_t(KN), &p));
         tr = static_cast<double*>(p);
     }
-    const MssCtr* hc = reinterpret_cast<const MssCtr*>(g->pinned);
+    // Grid: every block resident at once (the launch is cooperative).
+    int occ = 0, cus = 0, clock_khz = 0;
+    GHET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_mss_run), kBT, 0));
+    GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    GHET_HIP(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, g->device));
+    if (occ < 1) return set_error(GHET_ERR_HIP, "batched SSSP kernel cannot be resident (occupancy 0)");
+    const unsigned grid = static_cast<unsigned>(std::min(occ * cus, kMaxGrid));
+    // Budgets that end a pass that does not converge (a logic error, never a
+    // valid input): phases, and two minutes of device time.  reserved[5]
+    // overrides the phase budget (tests force a small one).
+    a.max_phases = (have_opts && o->reserved[5] > 0) ? o->reserved[5] : 64 * (static_cast<long long>(n) + 1024);
+    a.max_cycles = static_cast<unsigned long long>(std::max(clock_khz, 1)) * 1000ULL * 120ULL;
+    const long long* hstats = reinterpret_cast<const long long*>(g->pinned);
+    const CtrSlot* hslots = reinterpret_cast<const CtrSlot*>(reinterpret_cast<const char*>(g->pinned) + 64);
+    static_assert(64 + sizeof(CtrSlot) * kSlots <= kPinnedBytes, "pinned buffer");
 
     int64_t traversed = 0, rounds = 0, scanned = 0;
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     for (int32_t p0 = 0; p0 < k; p0 += K) {
         const int32_t kp = std::min(K, k - p0);
         const int64_t kn = int64_t(kp) * n;
-        GHET_CHECK(fill<double>(dist, inf, kn, s));
-        GHET_CHECK(fill<uint32_t>(mask_a, 0u, 4 * int64_t(n), s));
-        GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(ctr), 0, kCtrWords, s));
         // the pass's distinct source vertices and their source masks
         std::vector<int32_t> hq;
         std::vector<uint32_t> hm;
@@ -529,140 +759,37 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
             }
             hm[j] |= 1u << i;
         }
-        int32_t F = static_cast<int32_t>(hq.size());
-        int64_t E = 0;
-        for (int32_t v : hq) E += g->h_out_rowptr[v + 1] - g->h_out_rowptr[v];
-        int32_t list_len = F;
-        int64_t list_edges = E;
         // pageable sources: hipMemcpyAsync stages them before returning
         GHET_HIP(hipMemcpyAsync(d_q, hq.data(), 4 * hq.size(), hipMemcpyHostToDevice, s));
         GHET_HIP(hipMemcpyAsync(d_qmask, hm.data(), 4 * hm.size(), hipMemcpyHostToDevice, s));
-        uint32_t *mcur = mask_a, *mnext = mask_b;
-        hipLaunchKernelGGL(k_mss_seed, dim3(1), dim3(64), 0, s, d_q, d_qmask, F, kp, dist, mcur, bmask, list, ctr);
-        // a bucket's first near queue is its list; later rounds alternate qa/qb
-        const int32_t* cur = list;
-        int32_t* nxt = qa;
-        int sel = 0;
-        int64_t far_count = 0;
-        double threshold = delta;
-        // Edge-balanced pass over queue q (F entries, masks qm, rows holding
-        // at most E_max edges: the grid bound).
-        // The pass also resets the counters it fills.
-        auto pass = [&](bool heavy, const int32_t* q, int32_t Fq, const uint32_t* qm, int64_t E_max) -> int {
-            const unsigned vgrid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(Fq, 256), 65535)));
-            if (Fq <= kSmallMax) {
-                if (heavy)
-                    hipLaunchKernelGGL(k_mss_rows_small<true>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, qm, g->out.rowptr,
-                                       L.w, dist, kp, threshold, scan, qstart, ctr);
-                else
-                    hipLaunchKernelGGL(k_mss_rows_small<false>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, qm, g->out.rowptr,
-                                       L.w, dist, kp, threshold, scan, qstart, ctr);
-            } else {
-                hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, -1);
-                if (heavy)
-                    hipLaunchKernelGGL(k_mss_rows<true>, dim3(vgrid), dim3(256), 0, s, q, Fq, qm, g->out.rowptr, L.w,
-                                       dist, kp, threshold, scan, qstart);
-                else
-                    hipLaunchKernelGGL(k_mss_rows<false>, dim3(vgrid), dim3(256), 0, s, q, Fq, qm, g->out.rowptr, L.w,
-                                       dist, kp, threshold, scan, qstart);
-                GHET_CHECK(exclusive_scan_i32(scan, scan, Fq, aux, s));
-            }
-            const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
-            if (heavy)
-                hipLaunchKernelGGL(k_mss_relax<true>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr,
-                                   L.col, L.w, kp, dist, threshold, qm, mnext, bmask, fmask, nxt, list, far[sel], sel,
-                                   ctr);
-            else
-                hipLaunchKernelGGL(k_mss_relax<false>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr,
-                                   L.col, L.w, kp, dist, threshold, qm, mnext, bmask, fmask, nxt, list, far[sel], sel,
-                                   ctr);
-            // the pass's masks are consumed: clear them for reuse
-            hipLaunchKernelGGL(k_clear_mask, dim3(vgrid), dim3(256), 0, s, q, Fq, const_cast<uint32_t*>(qm));
-            GHET_HIP(hipGetLastError());
-            GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(MssCtr), hipMemcpyDeviceToHost, s));
-            GHET_HIP(hipStreamSynchronize(s));
-            scanned += static_cast<int64_t>(slot_sum(hc->slots, 3));
-            return GHET_OK;
-        };
-        for (;;) {
-            while (F > 0) {
-                if (E > 0) {
-                    GHET_CHECK(pass(false, cur, F, mcur, E));
-                    F = hc->near_count;
-                    E = static_cast<int64_t>(slot_sum(hc->slots, 0));
-                    list_edges += static_cast<int64_t>(slot_sum(hc->slots, 2));
-                    list_len = hc->list_count;
-                } else {
-                    hipLaunchKernelGGL(k_clear_mask, dim3(static_cast<unsigned>(ceil_div(F, 256))), dim3(256), 0, s,
-                                       cur, F, mcur);
-                    F = 0;
-                }
-                cur = nxt;
-                nxt = (nxt == qa) ? qb : qa;
-                std::swap(mcur, mnext);
-                ++rounds;
-            }
-            // heavy pass over the drained bucket (also clears its bucket masks)
-            if (list_len > 0) {
-                if (list_edges > 0) {
-                    GHET_CHECK(pass(true, list, list_len, bmask, list_edges));
-                    far_count = hc->far_count[sel];
-                } else {
-                    hipLaunchKernelGGL(k_clear_mask, dim3(static_cast<unsigned>(ceil_div(list_len, 256))), dim3(256),
-                                       0, s, list, list_len, bmask);
-                }
-            }
-            if (far_count == 0) break;
-            // split the far list at the next threshold into a new bucket list
-            // (its near masks go to mcur, all-zero after the last round)
-            double old_thr = threshold;
-            double new_thr = std::max(threshold + delta, std::nextafter(threshold, HUGE_VAL));
-            for (;;) {
-                const int other = 1 - sel;
-                hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, other);
-                const unsigned grid =
-                    static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(far_count, kBT), 2048)));
-                hipLaunchKernelGGL(k_mss_split, dim3(grid), dim3(kBT), 0, s, far[sel], static_cast<int32_t>(far_count),
-                                   dist, kp, old_thr, new_thr, fmask, mcur, bmask, list, far[other], other,
-                                   g->out.rowptr, ctr);
-                GHET_HIP(hipGetLastError());
-                GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(MssCtr), hipMemcpyDeviceToHost, s));
-                GHET_HIP(hipStreamSynchronize(s));
-                F = hc->list_count;
-                E = static_cast<int64_t>(slot_sum(hc->slots, 0));
-                far_count = hc->far_count[other];
-                unsigned long long mb = kInfBits;
-                for (int i = 0; i < kSlots; ++i) mb = std::min(mb, hc->slots[i].v[1]);
-                double far_min;
-                std::memcpy(&far_min, &mb, sizeof(double));
-                sel = other;
-                ++rounds;
-                if (F > 0 || far_count == 0) break;
-                // nothing in the next bucket: jump to the smallest far distance
-                old_thr = new_thr;
-                new_thr = std::max(far_min + delta, std::nextafter(far_min, HUGE_VAL));
-            }
-            threshold = new_thr;
-            list_len = F;
-            list_edges = E;
-            cur = list;
-            nxt = qa;
-            if (F == 0 && far_count == 0) break;
-        }
+        a.ks = kp;
+        a.seeds = static_cast<int32_t>(hq.size());
+        GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(a.bar), 0, kBarWords, s));
+        void* args[] = {&a};
+        GHET_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_mss_run), dim3(grid), dim3(kBT), args, 0, s));
+        GHET_HIP(hipMemcpyAsync(g->pinned, a.stats, 3 * sizeof(long long), hipMemcpyDeviceToHost, s));
+        GHET_HIP(hipStreamSynchronize(s));
+        if (hstats[2])
+            return set_error(GHET_ERR_HIP, "batched SSSP: pass at source %d stopped after %lld rounds (phase or time "
+                             "budget spent); results are invalid", p0, static_cast<long long>(hstats[0]));
+        rounds += hstats[0];
+        scanned += hstats[1];
         if (st) {
-            hipLaunchKernelGGL(k_mss_reset, dim3(1), dim3(512), 0, s, ctr, -1);
-            hipLaunchKernelGGL(k_mss_reached, dim3(2048), dim3(256), 0, s, dist, kn, kp, inf, g->out.rowptr, ctr->slots);
-            GHET_HIP(hipMemcpyAsync(g->pinned, ctr, sizeof(MssCtr), hipMemcpyDeviceToHost, s));
+            GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(a.ctr->slots), 0, sizeof(CtrSlot) / 8 * kSlots, s));
+            hipLaunchKernelGGL(k_mss_reached, dim3(2048), dim3(256), 0, s, a.dist, kn, kp, inf, g->out.rowptr,
+                               a.ctr->slots);
+            GHET_HIP(hipMemcpyAsync(reinterpret_cast<char*>(g->pinned) + 64, a.ctr->slots, sizeof(CtrSlot) * kSlots,
+                                    hipMemcpyDeviceToHost, s));
         }
         if (dist_out) {
             hipLaunchKernelGGL(k_mss_transpose,
                                dim3(static_cast<unsigned>(ceil_div(n, 64)), static_cast<unsigned>(ceil_div(kp, 64))),
-                               dim3(64, 4), 0, s, dist, n, kp, tr);
+                               dim3(64, 4), 0, s, a.dist, n, kp, tr);
             GHET_HIP(hipGetLastError());
             GHET_HIP(hipMemcpyAsync(dist_out + int64_t(p0) * n, tr, 8 * size_t(kn), hipMemcpyDeviceToHost, s));
         }
         GHET_HIP(hipStreamSynchronize(s));
-        if (st) traversed += static_cast<int64_t>(slot_sum(hc->slots, 2));
+        if (st) traversed += static_cast<int64_t>(slot_sum(hslots, 2));
     }
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipStreamSynchronize(s));
@@ -671,6 +798,7 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
         GHET_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
         st->device_ms = ms;
         st->levels = static_cast<int32_t>(rounds);
+        st->hot_kernel_launches = static_cast<int32_t>(ceil_div(k, K));  // one cooperative launch per pass
         st->edges_traversed = traversed;
         st->extra[0] = delta;
         st->extra[1] = K;

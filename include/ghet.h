@@ -227,6 +227,9 @@ typedef struct ghet_opts {
      * reserved[4]     : PR source windows: 0 = automatic, else
      *                   log2(hot sources) | log2(window) << 8 (the layout
      *                   is rebuilt when this changes)
+     * reserved[5]     : batched SSSP phase budget per pass (0 = automatic,
+     *                   64 (n + 1024)); a pass that spends it, or two
+     *                   minutes, stops and the call returns GHET_ERR_HIP
      * other entries must be 0                                             */
     int32_t reserved[6];
 } ghet_opts;
@@ -311,7 +314,8 @@ int ghet_bfs_noalloc(int32_t n, int64_t m, const int32_t* d_rowptr, const int32_
 /* k BFS sources in one persistent launch.  depth_out: k*n, row i = source i. */
 int ghet_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_out,
                    const ghet_opts* opts, ghet_run_stats* stats);
-/* k SSSP sources in one persistent launch.  dist_out: k*n. */
+/* k SSSP sources, up to 32 per pass, one persistent cooperative launch per
+ * pass (the bucket loop runs on the device).  dist_out: k*n. */
 int ghet_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, double* dist_out,
                     const ghet_opts* opts, ghet_run_stats* stats);
 

@@ -1,5 +1,6 @@
-"""Batched forms: many BFS/SSSP sources in one launch and many small graphs
-in one captured launch.  Each must equal the per-instance results bit for
+"""Batched forms: many BFS/SSSP sources in persistent cooperative launches
+(BFS: 256 sources per launch; SSSP: 32 per launch, the whole bucket loop on
+the device) and many small graphs in one captured launch.  Each must equal the per-instance results bit for
 bit (and the per-instance results equal the oracle, see test_gpu_parity)."""
 import numpy as np
 import pytest
@@ -60,6 +61,61 @@ def test_sssp_multi_matches_single(rmat14):
     np.testing.assert_array_equal(dist[3], O.sssp(o.rowptr, o.col, o.w, int(srcs[3])))
     dist7, _ = dg.sssp_multi(srcs, per_pass=7)  # pass width changes nothing
     np.testing.assert_array_equal(dist7, dist)
+
+
+@pytest.mark.parametrize("delta", [1e-4, 0.05, 10.0])
+def test_sssp_multi_bucket_widths(rmat14, delta):
+    """Many narrow buckets (far-list jumps), a few, and one bucket for all:
+    the device-side bucket loop gives the per-source distances every time."""
+    hg, o, i, dg = rmat14
+    srcs = sources(o, 40, 5)
+    dist, _ = dg.sssp_multi(srcs, delta=delta)
+    for j in range(0, len(srcs), 7):
+        ref, _ = dg.sssp(int(srcs[j]))
+        np.testing.assert_array_equal(dist[j], ref, err_msg=f"delta {delta} source #{j}")
+
+
+def test_sssp_multi_long_paths_and_ties():
+    """A 3000-vertex path (one vertex per round: thousands of grid-barrier
+    phases in one launch) and a grid with unit weights (many equal sums)."""
+    n = 3000
+    hg = G.HostGraph.from_edges(n, list(range(n - 1)), list(range(1, n)), np.full(n - 1, 0.5))
+    dg = G.DeviceGraph.from_host(hg)
+    o = hg.view(G.VIEW_OUT)
+    srcs = np.array([0, 10, 1500, 2999, 0], np.int32)
+    dist, st = dg.sssp_multi(srcs, delta=0.25)
+    for j, s_ in enumerate(srcs):
+        np.testing.assert_array_equal(dist[j], O.sssp(o.rowptr, o.col, o.w, int(s_)))
+    R, Cc = 40, 50
+    e = []
+    for r in range(R):
+        for c in range(Cc):
+            x = r * Cc + c
+            if c + 1 < Cc:
+                e += [(x, x + 1), (x + 1, x)]
+            if r + 1 < R:
+                e += [(x, x + Cc), (x + Cc, x)]
+    hg = G.HostGraph.from_edges(R * Cc, [a for a, _ in e], [b for _, b in e], np.ones(len(e)))
+    dg = G.DeviceGraph.from_host(hg)
+    o = hg.view(G.VIEW_OUT)
+    srcs = np.arange(0, R * Cc, 61, dtype=np.int32)
+    dist, _ = dg.sssp_multi(srcs)
+    for j, s_ in enumerate(srcs):
+        np.testing.assert_array_equal(dist[j], O.sssp(o.rowptr, o.col, o.w, int(s_)))
+
+
+def test_sssp_multi_phase_budget_stops_cleanly(rmat14):
+    """A pass that spends its phase budget (forced tiny from the host) stops
+    on the device for every block at once and the call reports it; the next
+    call runs normally."""
+    hg, o, i, dg = rmat14
+    srcs = sources(o, 8, 9)
+    with pytest.raises(G.GhetError) as e:
+        dg.sssp_multi(srcs, phase_budget=3)
+    assert e.value.code == G.ERR_HIP and "budget" in str(e.value)
+    dist, _ = dg.sssp_multi(srcs)
+    ref, _ = dg.sssp(int(srcs[0]))
+    np.testing.assert_array_equal(dist[0], ref)
 
 
 def small_graphs(count, seed, sym):
