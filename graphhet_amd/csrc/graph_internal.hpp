@@ -112,6 +112,7 @@ struct TCLayout {
     DevCSR dag;               // oriented by (degree, id)
     int32_t* active = nullptr;  // vertices with >= 2 out-neighbours in the CSR used
     int32_t num_active = 0;
+    int64_t isect_bytes = 0;    // sum over DAG (or CSR) edges (u, v) of 4 (d+(u) + d+(v))
     void* mem = nullptr;
 };
 

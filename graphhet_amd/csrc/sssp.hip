@@ -30,6 +30,8 @@
 #include "graph_internal.hpp"
 #include "sched.hpp"
 
+#include <rocprim/rocprim.hpp>
+
 namespace ghet {
 namespace {
 
@@ -558,42 +560,58 @@ __global__ void k_pre_finish(int32_t* pre, int32_t n) {
 
 }  // namespace
 
-// Orders each out-row by (weight, target) into g->sl.  Built once per graph on
-// the first SSSP; the host sorts the rows in parallel (per-row index sort).
+// Sort keys of the weights: the bit patterns of non-negative doubles order as
+// unsigned integers (-0.0 is mapped to +0.0).
+__global__ void k_weight_keys(const double* __restrict__ w, int64_t m, unsigned long long* __restrict__ key) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double x = w[i];
+        key[i] = x == 0.0 ? 0ULL : static_cast<unsigned long long>(__double_as_longlong(x));
+    }
+}
+
+// Orders each out-row by weight into g->sl, built once per graph on the first
+// SSSP: a stable segmented radix sort (rocPRIM) of (weight bits, target) with
+// one segment per row, so equal weights keep the row's target order
+// (ascending in a canonical CSR).  The sorted keys are the weights.
 int build_sssp_layout(ghet_graph* g) {
     SsspLayout& L = g->sl;
     if (L.built) return GHET_OK;
     const int64_t m = g->out.m;
     const int32_t n = g->n;
+    if (m > 0xFFFFFFFFLL) return set_error(GHET_ERR_UNSUPPORTED, "SSSP layout: %lld edges", static_cast<long long>(m));
     const size_t col_bytes = (4 * size_t(m) + 255) & ~size_t(255);
     GHET_HIP(hipMalloc(&L.mem, col_bytes + 8 * size_t(std::max<int64_t>(m, 1))));
     L.col = static_cast<int32_t*>(L.mem);
     L.w = reinterpret_cast<double*>(static_cast<char*>(L.mem) + col_bytes);
     if (m > 0) {
-        std::vector<int32_t> hcol(m);
-        std::vector<double> hw(m);
-        GHET_HIP(hipMemcpy(hcol.data(), g->out.col, 4 * size_t(m), hipMemcpyDeviceToHost));
-        GHET_HIP(hipMemcpy(hw.data(), g->out.w, 8 * size_t(m), hipMemcpyDeviceToHost));
-        const int32_t* rp = g->h_out_rowptr.data();
-        std::vector<int32_t> scol(m);
-        std::vector<double> sw(m);
-        parallel_for(n, hw_threads(0), [&](int64_t b, int64_t e, int) {
-            std::vector<int32_t> idx;
-            for (int64_t r = b; r < e; ++r) {
-                const int32_t r0 = rp[r], len = rp[r + 1] - rp[r];
-                idx.resize(len);
-                for (int32_t k = 0; k < len; ++k) idx[k] = r0 + k;
-                std::sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
-                    return hw[x] < hw[y] || (hw[x] == hw[y] && hcol[x] < hcol[y]);
-                });
-                for (int32_t k = 0; k < len; ++k) {
-                    scol[r0 + k] = hcol[idx[k]];
-                    sw[r0 + k] = hw[idx[k]];
-                }
-            }
-        });
-        GHET_HIP(hipMemcpy(L.col, scol.data(), 4 * size_t(m), hipMemcpyHostToDevice));
-        GHET_HIP(hipMemcpy(L.w, sw.data(), 8 * size_t(m), hipMemcpyHostToDevice));
+        hipStream_t s = g->stream;
+        unsigned long long* keys = nullptr;
+        void* tmp = nullptr;
+        size_t tb = 0;
+        auto sort = [&](void* t) {
+            return rocprim::segmented_radix_sort_pairs(
+                t, tb, keys, reinterpret_cast<unsigned long long*>(L.w), g->out.col, L.col, static_cast<unsigned>(m),
+                static_cast<unsigned>(n), g->out.rowptr, g->out.rowptr + 1, 0, 64, s);
+        };
+        hipError_t e = hipMalloc(&keys, 8 * size_t(m));
+        if (e == hipSuccess) {
+            const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(m, 256), 65536));
+            hipLaunchKernelGGL(k_weight_keys, dim3(grid), dim3(256), 0, s, g->out.w, m, keys);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = sort(nullptr);
+        if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tb, 256));
+        if (e == hipSuccess) e = sort(tmp);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (tmp) hipFree(tmp);
+        if (keys) hipFree(keys);
+        if (e != hipSuccess) {
+            hipFree(L.mem);
+            L = SsspLayout{};
+            (void)hipGetLastError();
+            return set_error(GHET_ERR_HIP, "SSSP layout sort: %s", hipGetErrorString(e));
+        }
     }
     L.built = true;
     return GHET_OK;

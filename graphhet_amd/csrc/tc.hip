@@ -130,6 +130,22 @@ __global__ void __launch_bounds__(kBT) k_tc_count(const int32_t* __restrict__ ro
     }
 }
 
+// Intersection bytes of the oriented CSR: over its edges (u, v), 4 bytes per
+// entry of both lists, 4 (d+(u) + d+(v)); per-block sums into spread slots.
+__global__ void __launch_bounds__(kBT) k_isect_bytes(const int32_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ col, int32_t n, CtrSlot* slots) {
+    __shared__ unsigned long long s_red[kBT / 64];
+    unsigned long long x = 0;
+    for (int64_t u = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; u < n;
+         u += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int32_t b = rowptr[u], e = rowptr[u + 1];
+        const unsigned long long du = static_cast<unsigned long long>(e - b);
+        for (int32_t i = b; i < e; ++i) x += du + static_cast<unsigned long long>(rowptr[col[i] + 1] - rowptr[col[i]]);
+    }
+    x = block_sum_u64<kBT>(x, s_red);
+    if (threadIdx.x == 0) slot_add(slots, 0, 4 * x);
+}
+
 __global__ void k_scale_i64(unsigned long long* c, int32_t n, unsigned long long f) {
     for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
          v += static_cast<int64_t>(gridDim.x) * blockDim.x)
@@ -171,6 +187,20 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
         GHET_HIP(hipGetLastError());
         GHET_HIP(hipStreamSynchronize(s));
         L.dag = {n, m_dag, drp, dcol, nullptr};
+    }
+    {
+        // the reported intersection-bytes figure of the CSR the count reads
+        const DevCSR& X = L.symmetric ? L.dag : g->out;
+        GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(CtrSlot) * kSlots, &p));
+        CtrSlot* slots = static_cast<CtrSlot*>(p);
+        GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(slots), 0, sizeof(CtrSlot) / 8 * kSlots, s));
+        if (n > 0)
+            hipLaunchKernelGGL(k_isect_bytes, dim3(grid_cap(std::min<int64_t>(ceil_div(n, kBT), 4096))), dim3(kBT), 0, s,
+                               X.rowptr, X.col, n, slots);
+        GHET_HIP(hipGetLastError());
+        GHET_HIP(hipMemcpyAsync(g->pinned, slots, sizeof(CtrSlot) * kSlots, hipMemcpyDeviceToHost, s));
+        GHET_HIP(hipStreamSynchronize(s));
+        L.isect_bytes = static_cast<int64_t>(slot_sum(reinterpret_cast<const CtrSlot*>(g->pinned), 0));
     }
     L.built = true;
     return GHET_OK;
@@ -227,6 +257,7 @@ int run_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* o, ghet_run_stats
         st->levels = 1;
         st->extra[0] = L.symmetric ? 1.0 : 0.0;
         st->extra[1] = static_cast<double>(X.m);
+        st->extra[2] = static_cast<double>(L.isect_bytes);  // 4 (d+(u) + d+(v)) over the counted edges
     }
     return GHET_OK;
 }

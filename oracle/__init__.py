@@ -59,7 +59,9 @@ def lib() -> C.CDLL:
             "oracle_cdlp": [I32, i32p, i32p, i32p, i32p, I32, I32, i32p],
             "oracle_tc": [I32, i32p, i32p, I32, I32, i64p],
             "oracle_tc_forward": [I32, i32p, i32p, I32, i64p],
+            "oracle_tc_vertices": [I32, i32p, i32p, i32p, I32, i64p],
         })
+        _oracle.oracle_tc_vertices.restype = C.c_int64
     return _oracle
 
 
@@ -83,7 +85,8 @@ def ref() -> C.CDLL:
         _ref.ref_graph_free.restype = None
         _ref.ref_graph_free.argtypes = [C.c_void_p]
         for name, args in (("ref_graph_bfs", [C.c_void_p, I32, i32p]), ("ref_graph_sssp", [C.c_void_p, I32, f64p]),
-                           ("ref_graph_pr", [C.c_void_p, F64, I32, f64p])):
+                           ("ref_graph_pr", [C.c_void_p, F64, I32, f64p]), ("ref_graph_wcc", [C.c_void_p, i32p]),
+                           ("ref_graph_cdlp", [C.c_void_p, I32, i32p])):
             fn = getattr(_ref, name)
             fn.restype = C.c_double
             fn.argtypes = args
@@ -178,6 +181,15 @@ def tc(rowptr, col, v_begin: int = 0, v_end: int | None = None) -> np.ndarray:
     return out
 
 
+def tc_vertices(rowptr, col, verts):
+    """oracle_tc with its outer loop over `verts` only: (counts, merge steps)."""
+    rp, c, vs = _a(rowptr, np.int32), _a(col, np.int32), _a(verts, np.int32)
+    n = len(rp) - 1
+    out = np.empty(n, np.int64)
+    steps = lib().oracle_tc_vertices(n, _p(rp, I32), _p(c, I32), _p(vs, I32), len(vs), _p(out, C.c_int64))
+    return out, int(steps)
+
+
 def tc_forward(rowptr, col, threads: int = 0) -> np.ndarray:
     """t(v), triangles through v, of a symmetric loop-free graph (forward
     algorithm; 6 * t(v) is the reference's per-vertex convention)."""
@@ -257,6 +269,14 @@ class RefGraph:
     def pr(self, damping, iters):
         out = np.empty(self.n, np.float64)
         return ref().ref_graph_pr(self._h, damping, iters, _p(out, F64)), out
+
+    def wcc(self):
+        out = np.empty(self.n, np.int32)
+        return ref().ref_graph_wcc(self._h, _p(out, I32)), out
+
+    def cdlp(self, iters):
+        out = np.empty(self.n, np.int32)
+        return ref().ref_graph_cdlp(self._h, iters, _p(out, I32)), out
 
     def close(self):
         if self._h:

@@ -297,6 +297,40 @@ void oracle_tc(int32_t n, const int32_t* rowptr, const int32_t* col, int32_t v_b
     }
 }
 
+// oracle_tc's outer loop over a list of vertices instead of a range (a
+// seeded vertex sample for timing the CPU baseline); returns the number of
+// merge steps taken, the unit the sample's rate is measured in.
+int64_t oracle_tc_vertices(int32_t n, const int32_t* rowptr, const int32_t* col, const int32_t* verts, int32_t k,
+                           int64_t* cnt) {
+    for (int32_t i = 0; i < n; ++i) cnt[i] = 0;
+    int64_t steps = 0;
+    for (int32_t x = 0; x < k; ++x) {
+        const int32_t u = verts[x];
+        const int32_t ub = rowptr[u], ue = rowptr[u + 1];
+        for (int32_t e = ub; e < ue; ++e) {
+            const int32_t v = col[e];
+            int32_t i = ub, j = rowptr[v];
+            const int32_t je = rowptr[v + 1];
+            while (i < ue && j < je) {
+                ++steps;
+                const int32_t a = col[i], b = col[j];
+                if (a == b) {
+                    ++cnt[u];
+                    ++cnt[v];
+                    ++cnt[a];
+                    ++i;
+                    ++j;
+                } else if (a < b) {
+                    ++i;
+                } else {
+                    ++j;
+                }
+            }
+        }
+    }
+    return steps;
+}
+
 // An independent triangle counter for checking large symmetric loop-free
 // graphs: the forward algorithm over a degree-ordered orientation (Schank and
 // Wagner, "Finding, counting and listing all triangles in large graphs",

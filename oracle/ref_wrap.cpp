@@ -108,6 +108,24 @@ double ref_graph_pr(void* h, double damp, int32_t iters, double* rank) {
     return s;
 }
 
+double ref_graph_wcc(void* h, int32_t* comp) {
+    auto& g = *static_cast<graph_structure<double>*>(h);
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> c = CPU_connected_components(g.OUTs, g.INs);
+    double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (comp) for (size_t i = 0; i < c.size(); ++i) comp[i] = c[i];
+    return s;
+}
+
+double ref_graph_cdlp(void* h, int32_t iters, int32_t* label) {
+    auto& g = *static_cast<graph_structure<double>*>(h);
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::string> r = CDLP(g, iters);
+    double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (label) for (size_t i = 0; i < r.size(); ++i) label[i] = std::stoi(r[i]);
+    return s;
+}
+
 void ref_tc(int32_t n, const int32_t* rp, const int32_t* col, int64_t* cnt) {
     auto g = make_graph(n, rp, col, nullptr, nullptr, nullptr, nullptr);
     std::vector<int> c = Tri_Counting(g.OUTs);

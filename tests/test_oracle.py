@@ -406,3 +406,21 @@ def test_tc_forward_closed_forms():
     e = [(i, 4 + j) for i in range(4) for j in range(5)]
     s = G.HostGraph.from_edges(9, [a for a, _ in e], [b for _, b in e]).view(G.VIEW_SYM)
     assert not O.tc_forward(s.rowptr, s.col).any()
+
+
+def test_tc_vertex_sample_is_the_restatement_over_those_vertices(golden):
+    """The CPU baseline's TC sample: oracle_tc's outer loop over a vertex list.
+    All vertices reproduce the full count; a split into two lists sums to it;
+    the merge steps equal the closed form sum over u, v in N(u) of the merge
+    length (at most deg u + deg v, exactly the steps until one list ends)."""
+    rp, col = golden["sym_rowptr"], golden["sym_col"]
+    n = len(rp) - 1
+    full = O.tc(rp, col)
+    allv, steps = O.tc_vertices(rp, col, np.arange(n))
+    np.testing.assert_array_equal(allv, full)
+    a, sa = O.tc_vertices(rp, col, np.arange(0, n, 2))
+    b, sb = O.tc_vertices(rp, col, np.arange(1, n, 2))
+    np.testing.assert_array_equal(a + b, full)
+    assert sa + sb == steps
+    deg = np.diff(rp)
+    assert steps <= int(sum(deg[u] * deg[u] + deg[col[rp[u]:rp[u + 1]]].sum() for u in range(n)))
