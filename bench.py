@@ -198,10 +198,22 @@ def main():
         bytes_it = pr_iter_bytes(n, m)
         pr_gbs = bytes_it / (k_ms * 1e-3) / 1e9
         traffic, traffic_note = measured_traffic(f"rmat-s{args.scale}-ef{int(args.edge_factor)}", args.seed)
+        # after the timed region: the phases timed apart (two launches per
+        # iteration), and the per-edge gather pattern's own rate over the
+        # same layout
+        ph = dg.pr(args.pr_iters, 0.85, out=False, phases=True)[1]
+        hot_ms = ph.extra[2] / max(ph.hot_kernel_launches, 1)
+        split_ms = ph.hot_kernel_ms / max(ph.hot_kernel_launches, 1)
+        ceil = dg.pr_gather_ceiling(5)
         roofline = {"bound": "hbm", "achieved": round(pr_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(pr_gbs / PEAK_HBM_GBS, 4), "traffic": traffic, "traffic_note": traffic_note,
                     "kernel": "k_pr_iter+k_pr_long", "kernel_avg_us": round(k_ms * 1e3, 2),
-                    "bytes_per_launch": bytes_it}
+                    "bytes_per_launch": bytes_it,
+                    "phases_us": {"hot": round(hot_ms * 1e3, 2), "cold_and_finish": round((split_ms - hot_ms) * 1e3, 2),
+                                  "note": "timed as two launches per iteration, after the timed region"},
+                    "gather_ceiling_us": {"hot": round(ceil["hot_ms"] * 1e3, 2), "cold": round(ceil["cold_ms"] * 1e3, 2),
+                                          "hot_edges": ceil["hot_edges"], "cold_edges": ceil["cold_edges"]},
+                    "frac_of_gather_ceiling": round((ceil["hot_ms"] + ceil["cold_ms"]) / k_ms, 4) if k_ms > 0 else None}
 
     plain = {}
     if sched != G.SCHED_HEURISTIC:

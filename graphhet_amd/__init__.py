@@ -83,7 +83,8 @@ EXPORTS = [
     "ghet_graph_set_workspace_cap",
     "ghet_dyn_create", "ghet_dyn_from_graph", "ghet_dyn_update", "ghet_dyn_edges", "ghet_dyn_snapshot",
     "ghet_dyn_destroy",
-    "ghet_bfs", "ghet_bfs_pre", "ghet_bfs_noalloc", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_cdlp", "ghet_tc",
+    "ghet_bfs", "ghet_bfs_pre", "ghet_bfs_noalloc", "ghet_sssp", "ghet_sssp_pre", "ghet_wcc", "ghet_pr", "ghet_pr_gather_ceiling",
+    "ghet_cdlp", "ghet_tc",
     "ghet_bfs_multi", "ghet_sssp_multi", "ghet_wcc_many", "ghet_tc_many",
     "ghet_sched_load", "ghet_sched_save", "ghet_sched_clear", "ghet_sched_set_params", "ghet_sched_get_params",
     "ghet_mlp_forward", "ghet_mlp_forward_host", "ghet_mlp_train_step", "ghet_autotune",
@@ -143,6 +144,7 @@ def lib() -> C.CDLL:
         "ghet_sssp_pre": (C.c_int, [vp, C.c_int32, C.c_double, f64p, i32p, P(Opts), P(RunStats)]),
         "ghet_wcc": (C.c_int, [vp, i32p, P(Opts), P(RunStats)]),
         "ghet_pr": (C.c_int, [vp, C.c_int32, C.c_double, f64p, P(Opts), P(RunStats)]),
+        "ghet_pr_gather_ceiling": (C.c_int, [vp, C.c_int32, P(Opts), f64p]),
         "ghet_cdlp": (C.c_int, [vp, C.c_int32, i32p, P(Opts), P(RunStats)]),
         "ghet_tc": (C.c_int, [vp, i64p, P(Opts), P(RunStats)]),
         "ghet_bfs_multi": (C.c_int, [vp, C.c_int32, i32p, i32p, P(Opts), P(RunStats)]),
@@ -488,16 +490,33 @@ class DeviceGraph:
         _check(lib().ghet_wcc(self._h, _ptr(lab, C.c_int32), C.byref(_opts()), C.byref(st)))
         return lab, st
 
-    def pr(self, iters: int = 10, damping: float = 0.85, normalise: bool = True, out=True, split=None):
-        """split = (log2 hot sources, log2 window) forces the source-window
-        layout (None: automatic)."""
+    def pr(self, iters: int = 10, damping: float = 0.85, normalise: bool = True, out=True, split=None,
+           phases=False):
+        """split = (log2 hot sources, log2 window[, blocked]) forces the
+        source-window layout, with the source-blocked hot phase when blocked
+        is true and per-edge gathers otherwise (None: automatic).  phases:
+        run the hot and cold gathers as separate launches and report the hot
+        phase's ms in stats.extra[2]."""
         r = np.empty(self.n, np.float64) if out else None
         st = RunStats()
-        o = _opts(flags=0 if normalise else 1)
+        o = _opts(flags=(0 if normalise else 1) | (2 if phases else 0))
         if split is not None:
-            o.reserved[4] = int(split[0]) | (int(split[1]) << 8)
+            blocked = len(split) > 2 and bool(split[2])
+            o.reserved[4] = int(split[0]) | (int(split[1]) << 8) | ((2 if blocked else 1) << 16)
         _check(lib().ghet_pr(self._h, iters, damping, _ptr(r, C.c_double), C.byref(o), C.byref(st)))
         return r, st
+
+    def pr_gather_ceiling(self, reps: int = 5, split=None):
+        """Median ms of a pure-gather kernel (sum of contrib[col[e]], no rows)
+        over the PR layout's hot edges and its cold edges, and the two edge
+        counts: {"hot_ms", "cold_ms", "hot_edges", "cold_edges"}."""
+        o = _opts()
+        if split is not None:
+            blocked = len(split) > 2 and bool(split[2])
+            o.reserved[4] = int(split[0]) | (int(split[1]) << 8) | ((2 if blocked else 1) << 16)
+        r = np.zeros(4, np.float64)
+        _check(lib().ghet_pr_gather_ceiling(self._h, reps, C.byref(o), _ptr(r, C.c_double)))
+        return {"hot_ms": float(r[0]), "cold_ms": float(r[1]), "hot_edges": int(r[2]), "cold_edges": int(r[3])}
 
     def cdlp(self, iters: int = 10, out=True):
         lab = np.empty(self.n, np.int32) if out else None

@@ -158,6 +158,7 @@ void destroy_graph(ghet_graph* g) {
     if (g->stream) hipStreamSynchronize(g->stream);
     g->ws.release();
     if (g->pr.mem) hipFree(g->pr.mem);
+    if (g->pr.hot_mem) hipFree(g->pr.hot_mem);
     if (g->cdlp.mem) hipFree(g->cdlp.mem);
     if (g->tc.mem) hipFree(g->tc.mem);
     if (g->ms.mem) hipFree(g->ms.mem);
@@ -551,6 +552,11 @@ int ghet_pr(ghet_graph* g, int32_t iterations, double damping, double* pr_out, c
             ghet_run_stats* stats) {
     GHET_ENTER(g);
     return run_pr(g, iterations, damping, pr_out, opts, stats);
+}
+int ghet_pr_gather_ceiling(ghet_graph* g, int32_t reps, const ghet_opts* opts, double* out4) {
+    ghet_run_stats* stats = nullptr;
+    GHET_ENTER(g);
+    return run_pr_gather_ceiling(g, reps, opts, out4);
 }
 int ghet_cdlp(ghet_graph* g, int32_t iterations, int32_t* label_out, const ghet_opts* opts, ghet_run_stats* stats) {
     GHET_ENTER(g);

@@ -60,6 +60,8 @@ struct PRLayout {
     int32_t num_slots = 0;        // partial sums written per iteration
     int32_t num_segments = 0;     // cold (row, source window) segments
     int32_t num_items = 0;        // hot items, then one group per cold window
+    int32_t num_hot_items = 0;    // items [0, num_hot_items) cover the hot parts
+    int64_t num_hot_edges = 0;    // col[0, num_hot_edges) are the hot parts, the rest cold segments
     void* items = nullptr;        // [num_items] 32-byte work items (pr.hip)
     void* long_rows = nullptr;    // [num_long_rows] int4 {row, first slot, slots, 0}
     int32_t* outdeg = nullptr;    // [n] out-degree of relabeled vertex r (sinks have 0)
@@ -68,6 +70,16 @@ struct PRLayout {
     int32_t* slot = nullptr;      // [n+S] partial slot of a tile row / segment, -1: finish in place
     int32_t* col = nullptr;       // [m] relabeled sources: hot parts row-major, then cold segments
     void* mem = nullptr;
+    // Source-blocked hot phase (blocked == true): the hot edges regrouped
+    // into slices, one per (row tile, source block), each sorted by row; a
+    // word packs (row in tile << 14 | source in block).
+    bool blocked = false;
+    int32_t hot_tiles = 0;        // row tiles
+    int32_t hot_blocks = 0;       // source blocks
+    uint32_t* hot_words = nullptr;  // [hot edges]
+    int32_t* hot_soff = nullptr;    // [hot_tiles * hot_blocks + 1] slice offsets
+    int32_t* hot_tile_row = nullptr;  // [hot_tiles + 1] first row of each tile
+    void* hot_mem = nullptr;
 };
 
 // Degree buckets over the all-CSR for CDLP (vertex lists per class).
@@ -195,6 +207,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
              ghet_run_stats* st);
 int run_wcc(ghet_graph* g, int32_t* label_out, const ghet_opts* o, ghet_run_stats* st);
 int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const ghet_opts* o, ghet_run_stats* st);
+int run_pr_gather_ceiling(ghet_graph* g, int32_t reps, const ghet_opts* o, double* out4);
 int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* o, ghet_run_stats* st);
 int run_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* o, ghet_run_stats* st);
 int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_out, const ghet_opts* o,

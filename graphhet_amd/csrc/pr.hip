@@ -41,6 +41,8 @@
 
 #include "graph_internal.hpp"
 
+#include <rocprim/rocprim.hpp>
+
 namespace ghet {
 namespace {
 
@@ -50,6 +52,20 @@ constexpr int kTileRows = 512;
 constexpr int kStage = kTileNnz / kBT;    // staged values per thread
 constexpr int kRowsPT = kTileRows / kBT;  // tile rows per thread
 constexpr int kWaveRow = 32;            // rows with more staged values than this use a wave
+// Source-blocked hot phase (k_pr_hot)
+constexpr int kHotShift = 14;
+constexpr int kHotB = 1 << kHotShift;   // hot sources per LDS block (64 KB of FP32)
+constexpr int kHotT = 8192;             // rows per tile (two FP32 accumulators each: 64 KB)
+constexpr int kHotBT = 1024;            // threads per block (one block per CU)
+constexpr int kHotK = 8;                // consecutive slice words per thread and chunk
+constexpr int kHotFillMin = 2048;       // slices with fewer words gather from global memory
+constexpr int kHotChunk = kHotBT * kHotK;
+// Row sums are accumulated as unsigned fixed point with 60 fraction bits: a
+// row's sum of contributions is at most the total rank (1), integer adds are
+// exact and order-free (so the sums are deterministic), and the resolution
+// 2^-60 is far below the FP32 rounding of the contributions themselves.
+constexpr double kHotFix = 1152921504606846976.0;  // 2^60
+static_assert(kHotT <= (1 << (32 - kHotShift)), "row in tile must fit beside the source in a word");
 
 struct PRScalars {
     double sink;   // S of the current ranks
@@ -100,9 +116,11 @@ __global__ void __launch_bounds__(kBT) k_pr_init(const int32_t* __restrict__ out
 }
 
 // Sums part[0..count) in index order (strided lanes, then fixed tree) and
-// stores it in *out.
-__global__ void __launch_bounds__(1024) k_pr_reduce(const double* __restrict__ part, int32_t count, double* out) {
+// stores it in *out; also zeroes *zero when given.
+__global__ void __launch_bounds__(1024) k_pr_reduce(const double* __restrict__ part, int32_t count, double* out,
+                                                    int* zero) {
     __shared__ double s[1024];
+    if (zero && threadIdx.x == 0) *zero = 0;  // k_pr_hot's tile counter, for the next launch
     double x = 0;
     for (int i = threadIdx.x; i < count; i += 1024) x += part[i];
     s[threadIdx.x] = x;
@@ -324,6 +342,187 @@ __global__ void __launch_bounds__(kBT) k_pr_long(const int4* __restrict__ long_r
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// Source-blocked hot phase: one block (1024 threads, one per CU) per row
+// tile (at most kHotT rows) at a time, tiles handed out by an atomic counter
+// in row order (the heavy tiles first).  For each source block b the block
+// copies contrib[b kHotB, (b+1) kHotB) into LDS with coalesced loads, then
+// walks the slice (tile, b), whose words are sorted by row: every gather of
+// a hot source is an LDS read instead of a random global load (the per-edge
+// gather phase is bound by L2 requests).  Each thread takes kHotK
+// consecutive words, sums each run of equal rows in FP64 and adds it to the
+// row's LDS accumulator as a 64-bit fixed-point integer (ds_add_u64): integer
+// addition is exact and commutative, so the result does not depend on which
+// thread adds first.  After the last block the tile's rows are finished as
+// k_pr_iter finishes them (or write the partial of a row with cold sources
+// to its first slot).  Sink mass goes to part[tile].
+constexpr int kHotFill = kHotB / 4 / kHotBT;  // float4 loads per thread to fill one source block
+static_assert(kHotFill * 4 * kHotBT == kHotB, "fill split");
+
+// Block b's contributions (sources [b kHotB, min(hot, (b+1) kHotB))) into
+// registers, float4 i of thread t at t + i kHotBT; past `hot`: zeros.
+__device__ __forceinline__ void hot_fill_load(const float* __restrict__ contrib, int32_t hot, int32_t b,
+                                              float4* f) {
+    const int32_t src0 = b << kHotShift;
+    const int32_t cnt = hot - src0 < kHotB ? hot - src0 : kHotB;
+#pragma unroll
+    for (int i = 0; i < kHotFill; ++i) {
+        const int32_t q = static_cast<int32_t>(threadIdx.x) + i * kHotBT;  // float4 index
+        if (4 * q + 4 <= cnt) {
+            f[i] = reinterpret_cast<const float4*>(contrib + src0)[q];
+        } else {
+            f[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (4 * q + 0 < cnt) f[i].x = contrib[src0 + 4 * q + 0];
+            if (4 * q + 1 < cnt) f[i].y = contrib[src0 + 4 * q + 1];
+            if (4 * q + 2 < cnt) f[i].z = contrib[src0 + 4 * q + 2];
+        }
+    }
+}
+
+__device__ __forceinline__ void hot_fill_store(float* s_src, const float4* f) {
+#pragma unroll
+    for (int i = 0; i < kHotFill; ++i)
+        reinterpret_cast<float4*>(s_src)[static_cast<int32_t>(threadIdx.x) + i * kHotBT] = f[i];
+}
+
+// The kHotK words of this thread in the chunk starting at p (zeros past end).
+__device__ __forceinline__ void hot_words_load(const uint32_t* __restrict__ words, int32_t p, int32_t end,
+                                               uint32_t* wn) {
+    const int32_t my0 = p + static_cast<int32_t>(threadIdx.x) * kHotK;
+#pragma unroll
+    for (int k = 0; k < kHotK; ++k) wn[k] = my0 + k < end ? words[my0 + k] : 0u;
+}
+
+__device__ __forceinline__ unsigned long long hot_fix(double x) {
+    return static_cast<unsigned long long>(__double2ll_rn(x * kHotFix));
+}
+
+__global__ void __launch_bounds__(kHotBT) k_pr_hot(
+    const uint32_t* __restrict__ words, const int32_t* __restrict__ soff, const int32_t* __restrict__ tile_row,
+    int32_t num_tiles, int32_t nb, int32_t hot, const int32_t* __restrict__ outdeg, const int32_t* __restrict__ slot,
+    double* __restrict__ long_part, const float* __restrict__ contrib, float* __restrict__ next,
+    const PRScalars* __restrict__ scal, double nd, double damping, double* __restrict__ part,
+    double* __restrict__ rank_out, int last, int* tile_ctr) {
+    __shared__ float s_src[kHotB];
+    __shared__ unsigned long long s_acc[kHotT];
+    __shared__ double s_red[kHotBT / 64];
+    __shared__ int32_t s_tile;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double base = scal->sink * damping / nd + (1.0 - damping) / nd;
+    for (;;) {
+        __syncthreads();  // s_tile, s_red and the accumulators are free again
+        if (tid == 0) s_tile = atomicAdd(tile_ctr, 1);
+        __syncthreads();
+        const int32_t tile = s_tile;
+        if (tile >= num_tiles) break;
+        const int32_t row0 = tile_row[tile], rows = tile_row[tile + 1] - row0;  // rows <= kHotT
+        for (int r = tid; r < rows; r += kHotBT) s_acc[r] = 0ULL;
+        // Slices with at least kHotFillMin words read their sources from an
+        // LDS copy of the block; smaller ones (the light rows' tiles) gather
+        // from global memory, where one request per edge is cheaper than
+        // filling 64 KB.  The next chunk's words are always in flight (the
+        // tile's slices are contiguous, so the next chunk starts where this
+        // one ends).
+        const int32_t* ts = soff + int64_t(tile) * nb;
+        const int32_t t_end = ts[nb];
+        uint32_t wn[kHotK];
+        hot_words_load(words, ts[0], t_end, wn);
+        for (int32_t b = 0; b < nb; ++b) {
+            const int32_t e0 = ts[b], e1 = ts[b + 1];
+            const bool use_lds = e1 - e0 >= kHotFillMin;  // block-uniform
+            const int32_t src0 = b << kHotShift;
+            if (use_lds) {
+                float4 f[kHotFill];
+                hot_fill_load(contrib, hot, b, f);
+                __syncthreads();  // the previous readers of s_src are done
+                hot_fill_store(s_src, f);
+                __syncthreads();
+            }
+            for (int32_t c0 = e0; c0 < e1; c0 += kHotChunk) {
+                const int32_t c1 = e1 - c0 < kHotChunk ? e1 : c0 + kHotChunk;
+                const int32_t my0 = c0 + tid * kHotK;
+                const int32_t nvalid = c1 - my0 < 0 ? 0 : (c1 - my0 > kHotK ? kHotK : c1 - my0);
+                uint32_t w[kHotK];
+#pragma unroll
+                for (int k = 0; k < kHotK; ++k) w[k] = wn[k];
+                hot_words_load(words, c1, t_end, wn);  // the next chunk starts at c1
+                float v[kHotK];
+                if (use_lds) {
+#pragma unroll
+                    for (int k = 0; k < kHotK; ++k) v[k] = s_src[w[k] & (kHotB - 1)];
+                } else {
+#pragma unroll
+                    for (int k = 0; k < kHotK; ++k)
+                        v[k] = k < nvalid ? contrib[src0 + static_cast<int32_t>(w[k] & (kHotB - 1))] : 0.f;
+                }
+                // runs of equal rows (the words are sorted by row): each run's
+                // FP64 sum goes to the row in fixed point, one atomic per run
+                if (nvalid > 0) {
+                    int32_t row = static_cast<int32_t>(w[0] >> kHotShift);
+                    double run = static_cast<double>(v[0]);
+#pragma unroll
+                    for (int k = 1; k < kHotK; ++k) {
+                        if (k < nvalid) {
+                            const int32_t r = static_cast<int32_t>(w[k] >> kHotShift);
+                            if (r != row) {
+                                atomicAdd(&s_acc[row], hot_fix(run));
+                                row = r;
+                                run = 0.0;
+                            }
+                            run += static_cast<double>(v[k]);
+                        }
+                    }
+                    atomicAdd(&s_acc[row], hot_fix(run));
+                }
+            }
+        }
+        __syncthreads();
+        double sink = 0;  // per tile, whichever block takes it: part[tile] is deterministic
+        for (int lr = tid; lr < rows; lr += kHotBT) {
+            const int32_t row = row0 + lr;
+            const double sum = static_cast<double>(s_acc[lr]) * (1.0 / kHotFix);
+            const int32_t sl = slot[row];
+            if (sl >= 0) {
+                long_part[sl] = sum;
+            } else {
+                const double pr = base + damping * sum;
+                const int32_t d = outdeg[row];
+                if (d > 0) next[row] = static_cast<float>(pr / d);
+                else sink += pr;
+                if (last) rank_out[row] = pr;
+            }
+        }
+        sink = wave_sum(sink);
+        if (lane == 0) s_red[wave] = sink;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0;
+            for (int i = 0; i < kHotBT / 64; ++i) t += s_red[i];
+            part[tile] = t;
+        }
+    }
+}
+
+// Pure gathers over a range of the layout's sources: each thread sums
+// contrib[col[e]] over its edges with kGatherUnroll independent loads in
+// flight and each wave stores one value.  No rows, no compensation, no LDS
+// staging: the rate of this kernel is the ceiling of the per-edge gather
+// pattern itself, against which the iteration's phases are measured.
+constexpr int kGatherUnroll = 8;
+__global__ void __launch_bounds__(kBT) k_gather_probe(const int32_t* __restrict__ col, int64_t e0, int64_t e1,
+                                                      const float* __restrict__ contrib, float* __restrict__ out) {
+    float acc = 0.f;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBT * kGatherUnroll;
+    for (int64_t b = e0 + static_cast<int64_t>(blockIdx.x) * kBT * kGatherUnroll + threadIdx.x; b < e1; b += stride) {
+        int32_t idx[kGatherUnroll];
+#pragma unroll
+        for (int j = 0; j < kGatherUnroll; ++j) idx[j] = b + j * kBT < e1 ? col[b + j * kBT] : -1;
+#pragma unroll
+        for (int j = 0; j < kGatherUnroll; ++j) acc += idx[j] >= 0 ? contrib[idx[j]] : 0.f;
+    }
+    acc = wave_sum(acc);
+    if (lane_id() == 0) out[(static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6] = acc;
+}
+
 __global__ void __launch_bounds__(kBT) k_pr_sum(const double* __restrict__ rank, int32_t n, double* __restrict__ part) {
     __shared__ double s_red[kBT / 64];
     double x = 0;
@@ -370,28 +569,156 @@ __global__ void __launch_bounds__(kBT) k_permute_rows(const int32_t* __restrict_
 // Measured per iteration on R-MAT scale 24 (k_pr_iter + k_pr_long): no split
 // 1839 us; hot 2^19 with windows 2^20 1754, 2^21 1719, 2^23 1697; hot 2^18
 // or 2^20 are slower.
-void pr_split(int32_t n, int32_t code, int32_t* hot, int32_t* window) {
+// code bits 16-17 choose the hot phase: 2 = source-blocked (k_pr_hot),
+// anything else = per-edge gathers.  Code 0 keeps the gathers: on R-MAT
+// s24 ef16 the source-blocked phase measured 1.26 ms against 1.09 ms for the
+// gathers (it trades L2 requests for LDS fills, FP64 runs and 64-bit LDS
+// atomics, and is instruction-bound), so it is available only on request.
+void pr_split(int32_t n, int32_t code, int32_t* hot, int32_t* window, bool* blocked) {
     if (code > 0) {
         *hot = static_cast<int32_t>(std::min<int64_t>(int64_t(1) << std::min(code & 0xFF, 30), n));
         *window = static_cast<int32_t>(int64_t(1) << std::min(std::max((code >> 8) & 0xFF, 1), 30));
+        *blocked = ((code >> 16) & 3) == 2;
     } else if (n > (1 << 21)) {
         *hot = 1 << 19;
         *window = 1 << 23;
+        *blocked = false;
     } else {
         *hot = n;
         *window = 1 << 20;
+        *blocked = false;
     }
+}
+
+// Slice keys of the hot edges: tile * blocks + source block, and the packed
+// word (row in tile << kHotShift | source in block).  One wave per row.
+__global__ void __launch_bounds__(kBT) k_hot_keys(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                  int32_t n, const int32_t* __restrict__ tile_row, int32_t tiles,
+                                                  int32_t nb, uint32_t* __restrict__ key, uint32_t* __restrict__ word) {
+    const int lane = lane_id();
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    for (int64_t r = wave_global; r < n; r += wave_stride) {
+        int32_t lo = 0, hi = tiles;  // the tile t with tile_row[t] <= r < tile_row[t + 1]
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (tile_row[mid] <= r) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t tile_key = static_cast<uint32_t>(lo) * static_cast<uint32_t>(nb);
+        const uint32_t lr = static_cast<uint32_t>(r - tile_row[lo]) << kHotShift;
+        for (int32_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += 64) {
+            const uint32_t src = static_cast<uint32_t>(col[e]);
+            key[e] = tile_key + (src >> kHotShift);
+            word[e] = lr | (src & (kHotB - 1));
+        }
+    }
+}
+
+// soff[k] = first position of slice k in the sorted keys (empty slices get
+// the position of the next non-empty one); soff[num_keys] = count.
+__global__ void k_hot_soff(const uint32_t* __restrict__ key, int64_t count, uint32_t num_keys,
+                           int32_t* __restrict__ soff) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e <= count;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t prev = e > 0 ? static_cast<int64_t>(key[e - 1]) : -1;
+        const int64_t cur = e < count ? static_cast<int64_t>(key[e]) : static_cast<int64_t>(num_keys);
+        for (int64_t k = prev + 1; k <= cur; ++k) soff[k] = static_cast<int32_t>(e);
+    }
+}
+
+// Regroups the hot parts (rows [0, n) of the layout's rowptr / col, sources
+// ascending in each row) into (row tile, source block) slices with one
+// stable radix sort by slice key: inside a slice the rows stay in order and
+// each row's sources ascending.
+//
+// Tiles are cut by rows (at most kHotT) and by hot edges (at most about
+// 1/512 of them), so a few hub rows make a tile of their own and the atomic
+// tile counter balances the blocks.
+int build_hot_slices(PRLayout& L, int32_t n, int64_t num_hot, const std::vector<int32_t>& hot_len, hipStream_t s) {
+    std::vector<int32_t> tile_row;
+    {
+        const int64_t target = std::max<int64_t>(1, num_hot / 512);
+        int64_t rows = 0, edges = 0;
+        for (int32_t r = 0; r < n; ++r) {
+            if (rows == 0 || rows == kHotT || edges + hot_len[r] > target) {
+                tile_row.push_back(r);
+                rows = 0;
+                edges = 0;
+            }
+            ++rows;
+            edges += hot_len[r];
+        }
+        tile_row.push_back(n);
+    }
+    L.hot_tiles = static_cast<int32_t>(tile_row.size() - 1);
+    L.hot_blocks = static_cast<int32_t>(std::max<int64_t>(1, ceil_div(L.hot, kHotB)));
+    const int64_t num_keys = int64_t(L.hot_tiles) * L.hot_blocks;
+    if (num_keys >= (int64_t(1) << 31) || num_hot >= (int64_t(1) << 31))
+        return set_error(GHET_ERR_UNSUPPORTED, "PageRank hot slices: too many");
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t words_b = al(4 * size_t(std::max<int64_t>(num_hot, 1))), soff_b = al(4 * size_t(num_keys + 1));
+    GHET_HIP(hipMalloc(&L.hot_mem, words_b + soff_b + al(4 * tile_row.size())));
+    L.hot_words = static_cast<uint32_t*>(L.hot_mem);
+    L.hot_soff = reinterpret_cast<int32_t*>(static_cast<char*>(L.hot_mem) + words_b);
+    L.hot_tile_row = reinterpret_cast<int32_t*>(static_cast<char*>(L.hot_mem) + words_b + soff_b);
+    GHET_HIP(hipMemcpyAsync(L.hot_tile_row, tile_row.data(), 4 * tile_row.size(), hipMemcpyHostToDevice, s));
+    unsigned bits = 1;
+    while ((int64_t(1) << bits) < num_keys) ++bits;
+    uint32_t *ka = nullptr, *kb = nullptr, *wa = nullptr;
+    const uint32_t* sorted_keys = nullptr;
+    void* tmp = nullptr;
+    size_t tb = 0;
+    hipError_t e = hipSuccess;
+    if (num_hot > 0) {
+        e = hipMalloc(&ka, 4 * size_t(num_hot));
+        if (e == hipSuccess) e = hipMalloc(&kb, 4 * size_t(num_hot));
+        if (e == hipSuccess) e = hipMalloc(&wa, 4 * size_t(num_hot));
+        if (e == hipSuccess) {
+            const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBT / 64), 65535)));
+            hipLaunchKernelGGL(k_hot_keys, dim3(grid), dim3(kBT), 0, s, L.rowptr, L.col, n, L.hot_tile_row, L.hot_tiles,
+                               L.hot_blocks, ka, wa);
+            e = hipGetLastError();
+        }
+        rocprim::double_buffer<uint32_t> kd(ka, kb);
+        rocprim::double_buffer<uint32_t> vd(wa, L.hot_words);
+        if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, tb, kd, vd, static_cast<unsigned>(num_hot), 0, bits, s);
+        if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tb, 256));
+        if (e == hipSuccess) e = rocprim::radix_sort_pairs(tmp, tb, kd, vd, static_cast<unsigned>(num_hot), 0, bits, s);
+        if (e == hipSuccess && vd.current() != L.hot_words)
+            e = hipMemcpyAsync(L.hot_words, vd.current(), 4 * size_t(num_hot), hipMemcpyDeviceToDevice, s);
+        sorted_keys = kd.current();
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_hot_soff, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(num_hot + 1, 256), 65535))),
+                               dim3(256), 0, s, sorted_keys, num_hot, static_cast<uint32_t>(num_keys), L.hot_soff);
+            e = hipGetLastError();
+        }
+    } else {
+        hipLaunchKernelGGL(k_hot_soff, dim3(1), dim3(256), 0, s, static_cast<const uint32_t*>(nullptr), int64_t(0),
+                           static_cast<uint32_t>(num_keys), L.hot_soff);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(wa), tmp})
+        if (q) hipFree(q);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GHET_ERR_HIP, "PageRank hot slices: %s", hipGetErrorString(e));
+    }
+    return GHET_OK;
 }
 
 int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     PRLayout& L = g->pr;
     const int32_t n = g->n;
     int32_t H = 0, W = 0;
-    pr_split(n, split_code, &H, &W);
-    if (L.built && L.hot == H && L.window == W) return GHET_OK;
-    if (L.mem) {
+    bool blocked = false;
+    pr_split(n, split_code, &H, &W, &blocked);
+    if (L.built && L.hot == H && L.window == W && L.blocked == blocked) return GHET_OK;
+    if (L.mem || L.hot_mem) {
         GHET_HIP(hipStreamSynchronize(s));
-        GHET_HIP(hipFree(L.mem));
+        if (L.mem) GHET_HIP(hipFree(L.mem));
+        if (L.hot_mem) GHET_HIP(hipFree(L.hot_mem));
         L = PRLayout{};
     }
     const int64_t m = g->in.m;
@@ -476,8 +803,9 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
                 }
                 prev = nb;
             }
-            if (cold_slots > 0) nslots[r] = nslots_of(hl) + cold_slots;
-            else nslots[r] = hl > kTileNnz ? nslots_of(hl) : 0;
+            // source-blocked: the hot part is one partial (k_pr_hot's row sum)
+            if (cold_slots > 0) nslots[r] = (blocked ? 1 : nslots_of(hl)) + cold_slots;
+            else nslots[r] = (!blocked && hl > kTileNnz) ? nslots_of(hl) : 0;
         }
     });
     // window-major numbering: segment and cold-edge bases per (window, chunk)
@@ -521,7 +849,7 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
             std::copy(hcol.begin() + rp[r], hcol.begin() + rp[r] + hl, col_all.begin() + rp_all[r]);
             const bool partial = nslots[r] > 0;
             slot_all[r] = partial ? slot_base[r] : -1;
-            int32_t next_slot = slot_base[r] + (partial ? nslots_of(hl) : 0);
+            int32_t next_slot = slot_base[r] + (partial ? (blocked ? 1 : nslots_of(hl)) : 0);
             int32_t prev = rp[r] + hl;
             for (int32_t w = 0; w < NW; ++w) {
                 const int32_t nb = bound(static_cast<int32_t>(r), w + 1);
@@ -580,7 +908,8 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
             items.push_back(it);
         }
     };
-    emit(0, n);
+    if (!blocked) emit(0, n);  // source-blocked: k_pr_hot covers the hot parts
+    const int32_t num_hot_items = static_cast<int32_t>(items.size());
     std::vector<int64_t> wseg(static_cast<size_t>(NW) + 1, 0);
     for (int32_t w = 0; w < NW; ++w) wseg[w + 1] = wseg[w] + [&] {
         int64_t c = 0;
@@ -619,6 +948,8 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     L.num_slots = num_slots;
     L.num_segments = static_cast<int32_t>(S);
     L.num_items = static_cast<int32_t>(items.size());
+    L.num_hot_items = num_hot_items;
+    L.num_hot_edges = num_hot;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t rows_all = static_cast<size_t>(n + S);
     const size_t bytes = al(sizeof(PRItem) * items.size()) + al(sizeof(int4) * combine.size()) +
@@ -650,6 +981,8 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     GHET_HIP(hipStreamSynchronize(s));
     L.hot = H;
     L.window = W;
+    L.blocked = blocked;
+    if (blocked) GHET_CHECK(build_hot_slices(L, n, num_hot, hot_len, s));
     L.built = true;
     return GHET_OK;
 }
@@ -666,18 +999,34 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     const int32_t split_code = (o && o->size >= sizeof(ghet_opts)) ? o->reserved[4] : 0;
     GHET_CHECK(build_pr_layout(g, s, split_code));
     PRLayout& L = g->pr;
-    // persistent grid: as many blocks as stay resident, at most one per item
-    int occ = 0, cus = 0;
+    // One persistent k_pr_iter launch per iteration walks all items (hot
+    // parts, then the cold segments) with as many blocks as stay resident,
+    // and k_pr_long finishes the rows with partials.  The source-blocked
+    // phase (k_pr_hot) runs before it over the hot parts.  With opts
+    // reserved[0] bit 1 the gathers run as two launches, hot items then
+    // cold items, with a HIP event between them: the phases are timed
+    // separately (the split costs about 4 % at scale 24: the hot launch's
+    // tail no longer overlaps the cold items).
+    int occ = 0, occ_hot = 0, cus = 0;
     GHET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_pr_iter), kBT, 0));
+    GHET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_hot, reinterpret_cast<const void*>(k_pr_hot), kHotBT, 0));
     GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
-    const int32_t blocks = static_cast<int32_t>(
-        std::max<int64_t>(1, std::min<int64_t>(int64_t(std::max(occ, 1)) * std::max(cus, 1), L.num_items)));
+    auto resident = [&](int o, int64_t work) {
+        return static_cast<int32_t>(std::max<int64_t>(0, std::min<int64_t>(int64_t(std::max(o, 1)) * std::max(cus, 1), work)));
+    };
+    const bool phases = (o && o->size >= sizeof(ghet_opts) && (o->reserved[0] & 2)) || L.blocked;
+    const int32_t hot_items = phases ? L.num_hot_items : 0, cold_items = L.num_items - hot_items;
+    const int32_t hot_blocks = L.blocked ? std::max(1, resident(occ_hot, L.hot_tiles)) : resident(occ, hot_items);
+    const int32_t cold_blocks = resident(occ, cold_items);
     const int32_t init_blocks = static_cast<int32_t>(std::min<int64_t>(ceil_div(n, kBT), 4096));
-    // k_pr_long's sink partials follow k_pr_iter's in the same buffer
     // one row per thread: k_pr_long's loads (row, slots, out-degree) are a
     // dependent chain, so every row gets its own thread (1024 blocks with a
     // grid-stride loop took 70 us at scale 24)
     const int32_t long_blocks = static_cast<int32_t>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(L.num_long_rows, kBT), 65535)));
+    // sink partials: the hot phase's (one per tile for k_pr_hot, one per
+    // block for k_pr_iter), the cold phase's, then k_pr_long's
+    const int32_t hot_parts = L.blocked ? L.hot_tiles : hot_blocks;
+    const int32_t parts = hot_parts + cold_blocks + long_blocks;
     void* p;
     float *ca, *cb;
     double *rank, *out, *part, *long_part;
@@ -686,36 +1035,49 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     GHET_CHECK(g->ws.get(WS_LABEL_A, 4 * size_t(n), &p)); cb = static_cast<float*>(p);
     GHET_CHECK(g->ws.get(WS_PR_RANK, 8 * size_t(n), &p)); rank = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_PR_OUT, 8 * size_t(n), &p)); out = static_cast<double*>(p);
-    GHET_CHECK(g->ws.get(WS_PR_PART, 8 * size_t(std::max(blocks + long_blocks, init_blocks) + 1), &p)); part = static_cast<double*>(p);
+    GHET_CHECK(g->ws.get(WS_PR_PART, 8 * size_t(std::max(parts, init_blocks) + 1), &p)); part = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_PR_LONGPART, 8 * size_t(L.num_slots + 1), &p)); long_part = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_COUNTERS, 256, &p)); scal = static_cast<PRScalars*>(p);
+    int* tile_ctr = reinterpret_cast<int*>(static_cast<char*>(p) + 128);
     const double inv_n = 1.0 / n;
     const bool timing = st != nullptr;
-    double hot_ms = 0;
+    double iter_ms = 0, hot_ms = 0;
     hipEvent_t* kev = nullptr;
-    if (timing) GHET_CHECK(kernel_events(g, 2 * iters, &kev));
+    if (timing) GHET_CHECK(kernel_events(g, 3 * iters, &kev));
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_pr_init, dim3(init_blocks), dim3(kBT), 0, s, L.outdeg, n, inv_n, ca, part);
-    hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, init_blocks, &scal->sink);
+    hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, init_blocks, &scal->sink, tile_ctr);
     if (iters == 0) GHET_CHECK(fill<double>(rank, inv_n, n, s));
     for (int32_t it = 0; it < iters; ++it) {
         const int last = (it == iters - 1);
-        if (timing) GHET_HIP(hipEventRecord(kev[2 * it], s));
-        hipLaunchKernelGGL(k_pr_iter, dim3(blocks), dim3(kBT), 0, s, static_cast<const PRItem*>(L.items), L.num_items,
-                           L.rowptr, L.col, L.outdeg, L.slot, n, long_part, ca, cb, scal, static_cast<double>(n),
-                           damping, part, rank, last);
+        if (timing) GHET_HIP(hipEventRecord(kev[3 * it], s));
+        if (L.blocked)
+            hipLaunchKernelGGL(k_pr_hot, dim3(hot_blocks), dim3(kHotBT), 0, s, L.hot_words, L.hot_soff,
+                               L.hot_tile_row, L.hot_tiles, L.hot_blocks, L.hot, L.outdeg, L.slot, long_part, ca, cb,
+                               scal, static_cast<double>(n), damping, part, rank, last, tile_ctr);
+        else if (hot_blocks > 0)
+            hipLaunchKernelGGL(k_pr_iter, dim3(hot_blocks), dim3(kBT), 0, s, static_cast<const PRItem*>(L.items),
+                               hot_items, L.rowptr, L.col, L.outdeg, L.slot, n, long_part, ca, cb, scal,
+                               static_cast<double>(n), damping, part, rank, last);
+        if (timing) GHET_HIP(hipEventRecord(kev[3 * it + 1], s));
+        if (cold_blocks > 0)
+            hipLaunchKernelGGL(k_pr_iter, dim3(cold_blocks), dim3(kBT), 0, s,
+                               static_cast<const PRItem*>(L.items) + hot_items, cold_items, L.rowptr, L.col, L.outdeg,
+                               L.slot, n, long_part, ca, cb, scal, static_cast<double>(n), damping, part + hot_parts,
+                               rank, last);
         hipLaunchKernelGGL(k_pr_long, dim3(long_blocks), dim3(kBT), 0, s, static_cast<const int4*>(L.long_rows),
                            L.num_long_rows, long_part, L.outdeg, cb, scal, static_cast<double>(n), damping,
-                           part + blocks, rank, last);
-        if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
-        hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, blocks + long_blocks, &scal->sink);
+                           part + hot_parts + cold_blocks, rank, last);
+        if (timing) GHET_HIP(hipEventRecord(kev[3 * it + 2], s));
+        hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, parts, &scal->sink, tile_ctr);
         std::swap(ca, cb);
     }
     const int norm = (normalise && iters > 0) ? 1 : 0;
     if (norm) {
         hipLaunchKernelGGL(k_pr_sum, dim3(init_blocks), dim3(kBT), 0, s, rank, n, part);
-        hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, init_blocks, &scal->total);
+        hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, init_blocks, &scal->total,
+                           static_cast<int*>(nullptr));
     }
     hipLaunchKernelGGL(k_pr_unpermute, dim3(init_blocks), dim3(kBT), 0, s, rank, L.perm, n, scal, norm, out);
     GHET_HIP(hipEventRecord(g->ev_end, s));
@@ -725,22 +1087,79 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     GHET_HIP(hipStreamSynchronize(s));
     if (st) {
         for (int32_t it = 0; it < iters; ++it) {
-            float kms = 0;
-            GHET_HIP(hipEventElapsedTime(&kms, kev[2 * it], kev[2 * it + 1]));
-            hot_ms += kms;
+            float kms = 0, hms = 0;
+            GHET_HIP(hipEventElapsedTime(&kms, kev[3 * it], kev[3 * it + 2]));
+            GHET_HIP(hipEventElapsedTime(&hms, kev[3 * it], kev[3 * it + 1]));
+            iter_ms += kms;
+            hot_ms += hms;
         }
         float ms = 0;
         GHET_HIP(hipEventElapsedTime(&ms, g->ev_begin, g->ev_end));
         st->device_ms = ms;
-        st->hot_kernel_ms = hot_ms;
+        st->hot_kernel_ms = iter_ms;  // the iteration kernels (both phases and k_pr_long)
         st->hot_kernel_launches = iters;
         st->levels = iters;
         const PRScalars* hs = reinterpret_cast<const PRScalars*>(g->pinned);
         st->extra[0] = hs->sink;
         st->extra[1] = hs->total;
-        st->extra[2] = L.num_tiles;
-        st->extra[3] = L.num_long_chunks;
+        st->extra[2] = phases ? hot_ms : 0;  // of which the hot phase (timed when the phases run apart)
+        st->extra[3] = L.blocked ? 1 : 0;  // hot phase: 1 source-blocked, 0 per-edge gathers
     }
+    return GHET_OK;
+}
+
+// Times k_gather_probe over the hot edges and over the cold edges of the
+// PageRank layout (built as ghet_pr would build it for these options), each
+// `reps` times; out4 = {median hot ms, median cold ms, hot edges, cold edges}.
+int run_pr_gather_ceiling(ghet_graph* g, int32_t reps, const ghet_opts* o, double* out4) {
+    if (!out4) return set_error(GHET_ERR_INVALID, "out4 is NULL");
+    if (reps < 1 || reps > 1000) return set_error(GHET_ERR_INVALID, "reps = %d not in [1, 1000]", reps);
+    if (!g->in.present() || g->h_in_rowptr.empty()) return set_error(GHET_ERR_INVALID, "PageRank needs the in-CSR");
+    const int32_t n = g->n;
+    for (int i = 0; i < 4; ++i) out4[i] = 0;
+    if (n == 0) return GHET_OK;
+    hipStream_t s = pick_stream(g, o);
+    const int32_t split_code = (o && o->size >= sizeof(ghet_opts)) ? o->reserved[4] : 0;
+    GHET_CHECK(build_pr_layout(g, s, split_code));
+    const PRLayout& L = g->pr;
+    int occ = 0, cus = 0;
+    GHET_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_gather_probe), kBT, 0));
+    GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    const int32_t grid = std::max(1, std::max(occ, 1) * std::max(cus, 1));
+    void* p;
+    float* contrib;
+    GHET_CHECK(g->ws.get(WS_PR_CONTRIB, 4 * size_t(n), &p)); contrib = static_cast<float*>(p);
+    GHET_CHECK(g->ws.get(WS_MISC, 4 * size_t(grid) * (kBT / 64), &p));
+    float* out = static_cast<float*>(p);
+    GHET_CHECK(fill<float>(contrib, 1.0f / static_cast<float>(n), n, s));
+    const int64_t m = g->in.m, hot_e = L.num_hot_edges;
+    hipEvent_t* kev = nullptr;
+    GHET_CHECK(kernel_events(g, 2 * reps + 1, &kev));
+    std::vector<double> ms[2];
+    for (int phase = 0; phase < 2; ++phase) {
+        const int64_t e0 = phase == 0 ? 0 : hot_e, e1 = phase == 0 ? hot_e : m;
+        if (e1 <= e0) continue;
+        GHET_HIP(hipEventRecord(kev[0], s));
+        for (int32_t r = 0; r < reps; ++r) {
+            hipLaunchKernelGGL(k_gather_probe, dim3(grid), dim3(kBT), 0, s, L.col, e0, e1, contrib, out);
+            GHET_HIP(hipEventRecord(kev[1 + r], s));
+        }
+        GHET_HIP(hipGetLastError());
+        GHET_HIP(hipStreamSynchronize(s));
+        for (int32_t r = 0; r < reps; ++r) {
+            float t = 0;
+            GHET_HIP(hipEventElapsedTime(&t, kev[r], kev[1 + r]));
+            ms[phase].push_back(t);
+        }
+    }
+    for (int phase = 0; phase < 2; ++phase) {
+        std::vector<double>& v = ms[phase];
+        if (v.empty()) continue;
+        std::sort(v.begin(), v.end());
+        out4[phase] = v[v.size() / 2];
+    }
+    out4[2] = static_cast<double>(hot_e);
+    out4[3] = static_cast<double>(m - hot_e);
     return GHET_OK;
 }
 

@@ -220,13 +220,17 @@ typedef struct ghet_opts {
     int32_t trace_cap;      /* capacity of `trace`                                */
     int32_t block_threads;  /* 0 = tuned default; else 64..1024 multiple of 64    */
     /* reserved[0] bit0: PR without the FP64 normalisation (reference form)
+     *             bit1: PR runs its hot and cold gathers as two launches and
+     *                   reports the hot phase's ms in stats extra[2]
      * reserved[1] bit0: many-graph calls skip the autotuner
      * reserved[2]     : batched BFS pushes while frontier out-edges < m / this
      *                   (0 = default 16)
      * reserved[3]     : batched SSSP sources per pass (0 = 32, at most 32)
      * reserved[4]     : PR source windows: 0 = automatic, else
-     *                   log2(hot sources) | log2(window) << 8 (the layout
-     *                   is rebuilt when this changes)
+     *                   log2(hot sources) | log2(window) << 8, plus 2 << 16
+     *                   for the source-blocked hot phase (LDS copies of
+     *                   the hot sources instead of per-edge gathers); the
+     *                   layout is rebuilt when this changes
      * reserved[5]     : batched SSSP phase budget per pass (0 = automatic,
      *                   64 (n + 1024)); a pass that spends it, or two
      *                   minutes, stops and the call returns GHET_ERR_HIP
@@ -279,6 +283,13 @@ int ghet_wcc(ghet_graph* g, int32_t* label_out, const ghet_opts* opts, ghet_run_
  * semantics of PageRank (CPU_PageRank.hpp:12-56). */
 int ghet_pr(ghet_graph* g, int32_t iterations, double damping, double* pr_out,
             const ghet_opts* opts, ghet_run_stats* stats);
+/* The gather ceiling of ghet_pr's layout: times a kernel that only sums
+ * contrib[col[e]] (FP32, eight loads in flight per thread, no rows) over the
+ * hot edges and over the cold edges, `reps` times each.  out4 = {median hot
+ * ms, median cold ms, hot edges, cold edges}.  No counterpart in the
+ * reference: a measurement aid for callers comparing ghet_pr's phases (its
+ * stats: extra[2] hot-phase ms) with the access pattern's own rate. */
+int ghet_pr_gather_ceiling(ghet_graph* g, int32_t reps, const ghet_opts* opts, double* out4);
 /* Synchronous label propagation over in+out neighbours with multiplicity,
  * ties to the smallest label, isolated vertices keep their label.
  * Replaces CDLP_GPU / Cuda_CDLP (GPU_CDLP_csr.cuh:75-210); semantics of

@@ -223,11 +223,15 @@ def test_hub_heavy_graph():
     np.testing.assert_array_equal(G.DeviceGraph(s, undirected=True, weights=False).tc()[0], O.tc(s.rowptr, s.col))
 
 
-@pytest.mark.parametrize("split", [(0, 1), (4, 6), (9, 10), (12, 12)])
+@pytest.mark.parametrize("split", [(0, 1), (4, 6), (9, 10), (12, 12), (0, 1, True), (4, 6, True), (9, 10, True),
+                                   (12, 12, True), (15, 13, True), (20, 20, True)])
 def test_pr_source_windows_match_oracle(medium, split):
     """Forced hot/window splits of the relabeled sources (log2 hot sources,
     log2 window): hot parts and cold segments become partial slots summed by
-    k_pr_long; every split must meet the same tolerance as the default."""
+    k_pr_long; every split must meet the same tolerance as the default.
+    With blocked=True the hot parts go through the source-blocked phase
+    (k_pr_hot): several 16K-source blocks, no cold sources at all (20, 20),
+    hot blocks cut short by the split (12, 12)."""
     o, i, dg = medium["o"], medium["i"], medium["dg"]
     ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 20)
     r, st = dg.pr(20, 0.85, split=split)
@@ -250,7 +254,34 @@ def test_pr_source_windows_with_long_segments():
     o, i = hg.view(G.VIEW_OUT), hg.view(G.VIEW_IN)
     dg = G.DeviceGraph(o, i)
     ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 10)
-    for split in ((2, 13), (6, 12), (0, 14)):
+    for split in ((2, 13), (6, 12), (0, 14), (2, 13, True), (14, 14, True), (20, 20, True)):
         r, _ = dg.pr(10, 0.85, split=split)
         err = np.abs(r - ref)
         assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (split, err.sum(), err.max())
+
+
+@pytest.mark.parametrize("split", [None, (9, 10), (12, 12, True)])
+def test_pr_phase_timing_keeps_results(medium, split):
+    """Hot and cold gathers as two launches (phase timing): same tolerance
+    against the oracle, deterministic, and the hot phase is timed."""
+    o, i, dg = medium["o"], medium["i"], medium["dg"]
+    ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 20)
+    r, st = dg.pr(20, 0.85, split=split, phases=True)
+    err = np.abs(r - ref)
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (split, err.sum(), err.max())
+    r2, _ = dg.pr(20, 0.85, split=split, phases=True)
+    np.testing.assert_array_equal(r, r2)
+    assert 0 < st.extra[2] <= st.hot_kernel_ms
+
+
+def test_pr_gather_ceiling_covers_every_edge(medium):
+    """The gather probe walks the hot and the cold edges of the layout:
+    together every in-edge once; an unsplit layout has no cold edges."""
+    i, dg = medium["i"], medium["dg"]
+    c = dg.pr_gather_ceiling(3, split=(9, 10))
+    assert c["hot_edges"] + c["cold_edges"] == i.m and c["cold_edges"] > 0
+    assert c["hot_ms"] > 0 and c["cold_ms"] > 0
+    c = dg.pr_gather_ceiling(3)
+    assert c["hot_edges"] == i.m and c["cold_edges"] == 0 and c["cold_ms"] == 0
+    with pytest.raises(G.GhetError):
+        dg.pr_gather_ceiling(0)
