@@ -159,6 +159,7 @@ void destroy_graph(ghet_graph* g) {
     g->ws.release();
     if (g->pr.mem) hipFree(g->pr.mem);
     if (g->pr.hot_mem) hipFree(g->pr.hot_mem);
+    if (g->pr.items_mem) hipFree(g->pr.items_mem);
     if (g->cdlp.mem) hipFree(g->cdlp.mem);
     if (g->tc.mem) hipFree(g->tc.mem);
     if (g->ms.mem) hipFree(g->ms.mem);

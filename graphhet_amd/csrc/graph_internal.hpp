@@ -62,7 +62,8 @@ struct PRLayout {
     int32_t num_items = 0;        // hot items, then one group per cold window
     int32_t num_hot_items = 0;    // items [0, num_hot_items) cover the hot parts
     int64_t num_hot_edges = 0;    // col[0, num_hot_edges) are the hot parts, the rest cold segments
-    void* items = nullptr;        // [num_items] 32-byte work items (pr.hip)
+    void* items = nullptr;        // [num_items] 32-byte work items (pr.hip), in items_mem
+    void* items_mem = nullptr;
     void* long_rows = nullptr;    // [num_long_rows] int4 {row, first slot, slots, 0}
     int32_t* outdeg = nullptr;    // [n] out-degree of relabeled vertex r (sinks have 0)
     int32_t* perm = nullptr;      // [n] relabeled id -> caller's id (out-degree descending)

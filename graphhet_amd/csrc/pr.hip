@@ -37,6 +37,7 @@
 // k_pr_long then finishes the rows with partials (slots summed in order), and
 // a one-block kernel sums the sink partials in index order into S.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "graph_internal.hpp"
@@ -708,6 +709,214 @@ int build_hot_slices(PRLayout& L, int32_t n, int64_t num_hot, const std::vector<
     return GHET_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Layout construction on the device.  The relabeled in-CSR is permuted and
+// every row's sources sorted by a segmented radix sort; hot lengths, cold
+// segments, slots and the combined offsets then come from flags and scans
+// over edges, segments and rows.  Only the tile packing (a greedy walk over
+// the combined offsets) runs on the host.
+
+__device__ __forceinline__ int32_t pr_nslots(int32_t len) {
+    return len > kTileNnz ? static_cast<int32_t>((len + kTileNnz - 1) / kTileNnz) : 1;
+}
+
+// Class of relabeled source u: 0 hot (u < hot), else 1 + its window.
+__device__ __forceinline__ uint32_t pr_class(int32_t u, int32_t hot, int wshift) {
+    return u < hot ? 0u : 1u + (static_cast<uint32_t>(u - hot) >> wshift);
+}
+
+// Out-degree order: key ~outdeg (ascending = out-degree descending), value v;
+// a stable radix sort then keeps equal degrees in id order.
+__global__ void __launch_bounds__(kBT) k_pr_deg_keys(const int32_t* __restrict__ out_rp, int32_t n,
+                                                     uint32_t* __restrict__ key, int32_t* __restrict__ val) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; v < n; v += static_cast<int64_t>(gridDim.x) * kBT) {
+        key[v] = ~static_cast<uint32_t>(out_rp[v + 1] - out_rp[v]);
+        val[v] = static_cast<int32_t>(v);
+    }
+}
+
+// inv[perm[r]] = r; in- and out-degree of relabeled row r.
+__global__ void __launch_bounds__(kBT) k_pr_relabel(const int32_t* __restrict__ perm, int32_t n,
+                                                    const int32_t* __restrict__ in_rp, const int32_t* __restrict__ out_rp,
+                                                    int32_t* __restrict__ inv, int32_t* __restrict__ indeg,
+                                                    int32_t* __restrict__ outdeg) {
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; r < n; r += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int32_t v = perm[r];
+        inv[v] = static_cast<int32_t>(r);
+        indeg[r] = in_rp[v + 1] - in_rp[v];
+        outdeg[r] = out_rp[v + 1] - out_rp[v];
+    }
+}
+
+// Per row: hot length (its sources are ascending, so the first source at
+// or above `hot` ends the hot part), and a mark at the row's first edge.
+__global__ void __launch_bounds__(kBT) k_pr_hot_len(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                    int32_t n, int32_t hot, int32_t* __restrict__ hot_len,
+                                                    uint8_t* __restrict__ row_start) {
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; r < n; r += static_cast<int64_t>(gridDim.x) * kBT) {
+        int32_t lo = rp[r], hi = rp[r + 1];
+        if (lo < hi) row_start[lo] = 1;
+        const int32_t b = lo;
+        while (lo < hi) {
+            const int32_t mid = lo + ((hi - lo) >> 1);
+            if (col[mid] < hot) lo = mid + 1;
+            else hi = mid;
+        }
+        hot_len[r] = lo - b;
+    }
+}
+
+// Edge e opens a cold segment when its source is cold and it starts its row
+// or follows a source of another class.
+__global__ void __launch_bounds__(kBT) k_pr_seg_flags(const int32_t* __restrict__ col, const uint8_t* __restrict__ row_start,
+                                                      int64_t m, int32_t hot, int wshift, int32_t* __restrict__ flag) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; e < m; e += static_cast<int64_t>(gridDim.x) * kBT) {
+        const uint32_t c = pr_class(col[e], hot, wshift);
+        flag[e] = (c > 0 && (row_start[e] || pr_class(col[e - (e > 0)], hot, wshift) != c || e == 0)) ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kBT) k_pr_seg_start(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
+                                                      int64_t m, int32_t* __restrict__ seg_start) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; e < m; e += static_cast<int64_t>(gridDim.x) * kBT)
+        if (flag[e]) seg_start[pos[e]] = static_cast<int32_t>(e);
+}
+
+// Per segment q (row-major order): its row, length, window key and slot
+// count; the histogram of windows.
+__global__ void __launch_bounds__(kBT) k_pr_seg_info(const int32_t* __restrict__ seg_start, int32_t S, int64_t m,
+                                                     const int32_t* __restrict__ rp, int32_t n,
+                                                     const int32_t* __restrict__ col, int32_t hot, int wshift,
+                                                     int32_t* __restrict__ seg_row, int32_t* __restrict__ seg_len,
+                                                     uint32_t* __restrict__ seg_key, int32_t* __restrict__ seg_ns,
+                                                     int32_t* __restrict__ win_count) {
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; q < S; q += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int32_t e = seg_start[q];
+        int32_t lo = 0, hi = n;  // the row r with rp[r] <= e < rp[r + 1]: last r with rp[r] <= e
+        while (hi - lo > 1) {
+            const int32_t mid = lo + ((hi - lo) >> 1);
+            if (rp[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        const int64_t next = q + 1 < S ? seg_start[q + 1] : m;
+        const int32_t end = static_cast<int32_t>(next < rp[lo + 1] ? next : rp[lo + 1]);
+        const uint32_t key = pr_class(col[e], hot, wshift) - 1;
+        seg_row[q] = lo;
+        seg_len[q] = end - e;
+        seg_key[q] = key;
+        seg_ns[q] = pr_nslots(end - e);
+        atomicAdd(win_count + key, 1);
+    }
+}
+
+// Per row: partial slots (hot part and cold segments), and the offsets'
+// inputs for the rows finished by k_pr_long.
+__global__ void __launch_bounds__(kBT) k_pr_row_slots(const int32_t* __restrict__ rp, const int32_t* __restrict__ pos,
+                                                      const int32_t* __restrict__ sn, const int32_t* __restrict__ hot_len,
+                                                      int32_t n, int blocked, int32_t* __restrict__ nslots,
+                                                      int32_t* __restrict__ partial) {
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; r < n; r += static_cast<int64_t>(gridDim.x) * kBT) {
+        const int32_t cold = sn[pos[rp[r + 1]]] - sn[pos[rp[r]]];
+        const int32_t hl = hot_len[r];
+        const int32_t k = cold > 0 ? (blocked ? 1 : pr_nslots(hl)) + cold : ((!blocked && hl > kTileNnz) ? pr_nslots(hl) : 0);
+        nslots[r] = k;
+        partial[r] = k > 0 ? 1 : 0;
+    }
+}
+
+// Hot parts: slot of each row, its row of k_pr_long, and its hot edges
+// copied to the combined column array (one wave per row).
+__global__ void __launch_bounds__(kBT) k_pr_fill_hot(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ hot_len, const int32_t* __restrict__ rp_all,
+                                                     const int32_t* __restrict__ nslots, const int32_t* __restrict__ slot_base,
+                                                     const int32_t* __restrict__ lpos, int32_t n,
+                                                     int32_t* __restrict__ slot_all, int4* __restrict__ long_rows,
+                                                     int32_t* __restrict__ col_all) {
+    const int lane = lane_id();
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    for (int64_t r = wave_global; r < n; r += wave_stride) {
+        const int32_t k = nslots[r], sb = slot_base[r];
+        if (lane == 0) {
+            slot_all[r] = k > 0 ? sb : -1;
+            if (k > 0) long_rows[lpos[r]] = make_int4(static_cast<int32_t>(r), sb, k, 0);
+        }
+        const int32_t b = rp[r], d = hot_len[r], o = rp_all[r];
+        for (int32_t i = lane; i < d; i += 64) col_all[o + i] = col[b + i];
+    }
+}
+
+// Cold segments in window-major order: segment sid is row-major segment
+// order[sid]; its offset, slot (after the row's hot slots and the slots of
+// the row's earlier segments) and edges.  Segments up to kSegLight edges
+// are copied lane-serially, longer ones by the whole wave.
+constexpr int kSegLight = 16;
+__global__ void __launch_bounds__(kBT) k_pr_fill_cold(
+    const int32_t* __restrict__ order, int32_t S, const int32_t* __restrict__ seg_start,
+    const int32_t* __restrict__ seg_row, const int32_t* __restrict__ coff, const int32_t* __restrict__ sn,
+    const int32_t* __restrict__ pos, const int32_t* __restrict__ rp, const int32_t* __restrict__ hot_len,
+    const int32_t* __restrict__ slot_base, const int32_t* __restrict__ col, int32_t n, int64_t num_hot, int blocked,
+    int32_t* __restrict__ rp_all, int32_t* __restrict__ slot_all, int32_t* __restrict__ col_all) {
+    const int lane = lane_id();
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    for (int64_t g0 = wave_global * 64; g0 < S; g0 += wave_stride * 64) {
+        const int64_t sid = g0 + lane;
+        int32_t b = 0, len = 0, o = 0;
+        if (sid < S) {
+            const int32_t q = order[sid], r = seg_row[q];
+            b = seg_start[q];
+            o = static_cast<int32_t>(num_hot + coff[sid]);
+            len = coff[sid + 1] - coff[sid];
+            const int32_t hs = blocked ? 1 : pr_nslots(hot_len[r]);
+            rp_all[n + sid] = o;
+            slot_all[n + sid] = slot_base[r] + hs + sn[q] - sn[pos[rp[r]]];
+        }
+        if (len <= kSegLight)
+            for (int32_t i = 0; i < len; ++i) col_all[o + i] = col[b + i];
+        uint64_t heavy = __ballot(len > kSegLight);
+        while (heavy) {
+            const int srcl = __ffsll(static_cast<unsigned long long>(heavy)) - 1;
+            heavy &= heavy - 1;
+            const int32_t hb = __shfl(b, srcl), hl = __shfl(len, srcl), ho = __shfl(o, srcl);
+            for (int32_t i = lane; i < hl; i += 64) col_all[ho + i] = col[hb + i];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBT) k_pr_gather_len(const int32_t* __restrict__ order, const int32_t* __restrict__ seg_len,
+                                                       int32_t S, int32_t* __restrict__ out) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; i < S; i += static_cast<int64_t>(gridDim.x) * kBT)
+        out[i] = seg_len[order[i]];
+}
+
+__global__ void __launch_bounds__(kBT) k_pr_iota(int32_t* __restrict__ out, int32_t count) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; i < count; i += static_cast<int64_t>(gridDim.x) * kBT)
+        out[i] = static_cast<int32_t>(i);
+}
+
+// Owns the temporary device buffers of one layout build.
+struct DevTemp {
+    std::vector<void*> ptrs;
+    hipError_t err = hipSuccess;
+    template <typename T>
+    T* get(size_t count) {
+        void* p = nullptr;
+        if (err == hipSuccess) err = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256));
+        if (err != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~DevTemp() {
+        if (!ptrs.empty()) (void)hipDeviceSynchronize();  // no launch may still read them
+        for (void* p : ptrs) hipFree(p);
+    }
+};
+
+unsigned pr_grid(int64_t threads) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(threads, kBT), 65536)));
+}
+
 int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     PRLayout& L = g->pr;
     const int32_t n = g->n;
@@ -715,166 +924,191 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     bool blocked = false;
     pr_split(n, split_code, &H, &W, &blocked);
     if (L.built && L.hot == H && L.window == W && L.blocked == blocked) return GHET_OK;
-    if (L.mem || L.hot_mem) {
+    if (L.mem || L.hot_mem || L.items_mem) {
         GHET_HIP(hipStreamSynchronize(s));
         if (L.mem) GHET_HIP(hipFree(L.mem));
         if (L.hot_mem) GHET_HIP(hipFree(L.hot_mem));
+        if (L.items_mem) GHET_HIP(hipFree(L.items_mem));
         L = PRLayout{};
     }
     const int64_t m = g->in.m;
-    const std::vector<int32_t>& irp = g->h_in_rowptr;
-    const std::vector<int32_t>& orp = g->h_out_rowptr;
-    const int T = hw_threads(0);
-    // Counting sort of vertices by out-degree, descending, stable by id.
-    std::vector<int32_t> od(static_cast<size_t>(n)), perm(static_cast<size_t>(n)), inv(static_cast<size_t>(n));
-    int32_t maxd = 0;
-    for (int32_t v = 0; v < n; ++v) {
-        od[v] = orp[v + 1] - orp[v];
-        maxd = std::max(maxd, od[v]);
-    }
-    std::vector<int64_t> start(static_cast<size_t>(maxd) + 2, 0);
-    for (int32_t v = 0; v < n; ++v) ++start[maxd - od[v] + 1];
-    for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
-    for (int32_t v = 0; v < n; ++v) perm[start[maxd - od[v]]++] = v;
-    for (int32_t r = 0; r < n; ++r) inv[perm[r]] = r;
-    std::vector<int32_t> rp(static_cast<size_t>(n) + 1), odp(static_cast<size_t>(n));
-    rp[0] = 0;
-    for (int32_t r = 0; r < n; ++r) {
-        const int32_t v = perm[r];
-        rp[r + 1] = rp[r] + (irp[v + 1] - irp[v]);
-        odp[r] = od[v];
-    }
-    // Relabeled in-lists (device permute), then every row's sources ascending
-    // on the host: hot sources first, then the cold windows in order, each a
-    // contiguous run; neighbouring lanes of a tile share cache lines.
-    int32_t* d_scratch = nullptr;
-    std::vector<int32_t> hcol(static_cast<size_t>(m));
-    if (m > 0) {
-        GHET_HIP(hipMalloc(&d_scratch, 4 * (3 * size_t(n) + 1 + size_t(m))));
-        int32_t* d_perm = d_scratch;
-        int32_t* d_inv = d_perm + n;
-        int32_t* d_rp = d_inv + n;
-        int32_t* d_col = d_rp + n + 1;
-        hipError_t e = hipMemcpyAsync(d_perm, perm.data(), 4 * size_t(n), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(d_inv, inv.data(), 4 * size_t(n), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(d_rp, rp.data(), 4 * (size_t(n) + 1), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) {
-            const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBT / 64), 65535)));
-            hipLaunchKernelGGL(k_permute_rows, dim3(grid), dim3(kBT), 0, s, d_perm, d_inv, g->in.rowptr, g->in.col, d_rp,
-                               d_col, n);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess) e = hipMemcpyAsync(hcol.data(), d_col, 4 * size_t(m), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        hipFree(d_scratch);
-        GHET_HIP(e);
-        parallel_for(n, T, [&](int64_t b, int64_t e2, int) {
-            for (int64_t r = b; r < e2; ++r) std::sort(hcol.begin() + rp[r], hcol.begin() + rp[r + 1]);
-        });
-    }
+    if (m >= INT32_MAX) return set_error(GHET_ERR_UNSUPPORTED, "PageRank layout: %lld edges", static_cast<long long>(m));
+    int wshift = 0;
+    while ((int64_t(1) << wshift) < W) ++wshift;
     const int32_t NW = n > H ? static_cast<int32_t>(ceil_div(n - H, W)) : 0;
-    // boundary k of row r: first edge with source >= H + k*W (k = 0..NW)
-    auto bound = [&](int32_t r, int32_t k) -> int32_t {
-        const int64_t key = int64_t(H) + int64_t(k) * W;
-        if (key >= n) return rp[r + 1];
-        return static_cast<int32_t>(std::lower_bound(hcol.begin() + rp[r], hcol.begin() + rp[r + 1],
-                                                     static_cast<int32_t>(key)) - hcol.begin());
+    const unsigned gn = pr_grid(n), gm = pr_grid(m), gw = pr_grid(int64_t(n) * 64);
+    DevTemp t;
+    auto hip_ok = [&](hipError_t e) {
+        if (t.err == hipSuccess) t.err = e;
+        return t.err == hipSuccess;
     };
-    auto nslots_of = [](int32_t len) -> int32_t {
-        return len > kTileNnz ? static_cast<int32_t>(ceil_div(len, kTileNnz)) : 1;
+    auto fail = [&](const char* what) {
+        (void)hipGetLastError();
+        return set_error(GHET_ERR_HIP, "PageRank layout (%s): %s", what, hipGetErrorString(t.err));
     };
-    // Pass 1 (thread chunks of rows): hot length, slots, and per window the
-    // number of segments and cold edges of the chunk.
-    std::vector<int32_t> hot_len(static_cast<size_t>(n)), nslots(static_cast<size_t>(n));
-    std::vector<int64_t> cnt_seg(size_t(T) * std::max(NW, 1), 0), cnt_edge(size_t(T) * std::max(NW, 1), 0);
-    parallel_for(n, T, [&](int64_t b, int64_t e, int t) {
-        for (int64_t r = b; r < e; ++r) {
-            int32_t prev = bound(static_cast<int32_t>(r), 0);
-            const int32_t hl = prev - rp[r];
-            hot_len[r] = hl;
-            int32_t cold_slots = 0;
-            for (int32_t w = 0; w < NW; ++w) {
-                const int32_t nb = bound(static_cast<int32_t>(r), w + 1);
-                const int32_t len = nb - prev;
-                if (len > 0) {
-                    cold_slots += nslots_of(len);
-                    ++cnt_seg[size_t(t) * NW + w];
-                    cnt_edge[size_t(t) * NW + w] += len;
-                }
-                prev = nb;
-            }
-            // source-blocked: the hot part is one partial (k_pr_hot's row sum)
-            if (cold_slots > 0) nslots[r] = (blocked ? 1 : nslots_of(hl)) + cold_slots;
-            else nslots[r] = (!blocked && hl > kTileNnz) ? nslots_of(hl) : 0;
-        }
-    });
-    // window-major numbering: segment and cold-edge bases per (window, chunk)
-    const int64_t num_hot = [&] {
-        int64_t t = 0;
-        for (int32_t r = 0; r < n; ++r) t += hot_len[r];
-        return t;
-    }();
-    std::vector<int64_t> seg_base(size_t(T) * std::max(NW, 1)), edge_base(size_t(T) * std::max(NW, 1));
-    int64_t S = 0, ce = 0;
-    for (int32_t w = 0; w < NW; ++w)
-        for (int t = 0; t < T; ++t) {
-            seg_base[size_t(t) * NW + w] = S;
-            edge_base[size_t(t) * NW + w] = ce;
-            S += cnt_seg[size_t(t) * NW + w];
-            ce += cnt_edge[size_t(t) * NW + w];
-        }
-    if (n + S >= INT32_MAX) return set_error(GHET_ERR_UNSUPPORTED, "PageRank layout: too many segments");
-    std::vector<int32_t> slot_base(static_cast<size_t>(n) + 1, 0);
-    for (int32_t r = 0; r < n; ++r) slot_base[r + 1] = slot_base[r] + nslots[r];
-    const int32_t num_slots = slot_base[n];
-    // Pass 2: hot parts row-major, cold segments window-major; offsets,
-    // slots and the combined column array.
-    std::vector<int32_t> rp_all(static_cast<size_t>(n + S) + 1), slot_all(static_cast<size_t>(n + S));
-    std::vector<int32_t> col_all(static_cast<size_t>(m));
-    {
-        int64_t acc = 0;
-        for (int32_t r = 0; r < n; ++r) {
-            rp_all[r] = static_cast<int32_t>(acc);
-            acc += hot_len[r];
-        }
-        rp_all[n] = static_cast<int32_t>(acc);
+    const int64_t scan_max = std::max<int64_t>(m, n) + 1;
+    int32_t* aux = t.get<int32_t>(scan_aux_ints(scan_max));
+    auto scan = [&](const int32_t* in, int32_t* out, int64_t count) {
+        if (t.err == hipSuccess && exclusive_scan_i32(in, out, count, aux, s) != GHET_OK) t.err = hipErrorUnknown;
+    };
+    void* sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    auto sort_space = [&](size_t need) {  // grows one scratch buffer for the rocPRIM sorts
+        if (t.err != hipSuccess || need <= sort_bytes) return;
+        sort_tmp = t.get<char>(need);
+        sort_bytes = need;
+    };
+
+    // 1. Relabeling by out-degree, descending, stable by id.
+    uint32_t* ka = t.get<uint32_t>(n);
+    uint32_t* kb = t.get<uint32_t>(n);
+    int32_t* va = t.get<int32_t>(n);
+    int32_t* vb = t.get<int32_t>(n);
+    int32_t* inv = t.get<int32_t>(n);
+    int32_t* indeg = t.get<int32_t>(n);
+    int32_t* outdeg = t.get<int32_t>(n);
+    int32_t* rp = t.get<int32_t>(size_t(n) + 1);
+    if (t.err != hipSuccess) return fail("alloc");
+    hipLaunchKernelGGL(k_pr_deg_keys, dim3(gn), dim3(kBT), 0, s, g->out.rowptr, n, ka, va);
+    rocprim::double_buffer<uint32_t> kd(ka, kb);
+    rocprim::double_buffer<int32_t> vd(va, vb);
+    size_t need = 0;
+    hip_ok(rocprim::radix_sort_pairs(nullptr, need, kd, vd, static_cast<unsigned>(n), 0, 32, s));
+    sort_space(need);
+    hip_ok(rocprim::radix_sort_pairs(sort_tmp, need, kd, vd, static_cast<unsigned>(n), 0, 32, s));
+    int32_t* perm = vd.current();
+    hipLaunchKernelGGL(k_pr_relabel, dim3(gn), dim3(kBT), 0, s, perm, n, g->in.rowptr, g->out.rowptr, inv, indeg, outdeg);
+    hip_ok(hipGetLastError());
+    scan(indeg, rp, n);
+    // 2. Relabeled in-lists, then every row's sources ascending.
+    int32_t* pcol = t.get<int32_t>(size_t(m));
+    int32_t* col = t.get<int32_t>(size_t(m));
+    if (t.err != hipSuccess) return fail("relabel");
+    hipLaunchKernelGGL(k_permute_rows, dim3(gw), dim3(kBT), 0, s, perm, inv, g->in.rowptr, g->in.col, rp, pcol, n);
+    unsigned bits = 1;
+    while ((int64_t(1) << bits) < n) ++bits;
+    if (m > 0) {
+        hip_ok(rocprim::segmented_radix_sort_keys(nullptr, need, pcol, col, static_cast<unsigned>(m), static_cast<unsigned>(n),
+                                                  rp, rp + 1, 0, bits, s));
+        sort_space(need);
+        hip_ok(rocprim::segmented_radix_sort_keys(sort_tmp, need, pcol, col, static_cast<unsigned>(m),
+                                                  static_cast<unsigned>(n), rp, rp + 1, 0, bits, s));
     }
-    parallel_for(n, T, [&](int64_t b, int64_t e, int t) {
-        std::vector<int64_t> sc(seg_base.begin() + size_t(t) * std::max(NW, 1),
-                                seg_base.begin() + size_t(t) * std::max(NW, 1) + std::max(NW, 1));
-        std::vector<int64_t> ec(edge_base.begin() + size_t(t) * std::max(NW, 1),
-                                edge_base.begin() + size_t(t) * std::max(NW, 1) + std::max(NW, 1));
-        for (int64_t r = b; r < e; ++r) {
-            const int32_t hl = hot_len[r];
-            std::copy(hcol.begin() + rp[r], hcol.begin() + rp[r] + hl, col_all.begin() + rp_all[r]);
-            const bool partial = nslots[r] > 0;
-            slot_all[r] = partial ? slot_base[r] : -1;
-            int32_t next_slot = slot_base[r] + (partial ? (blocked ? 1 : nslots_of(hl)) : 0);
-            int32_t prev = rp[r] + hl;
-            for (int32_t w = 0; w < NW; ++w) {
-                const int32_t nb = bound(static_cast<int32_t>(r), w + 1);
-                const int32_t len = nb - prev;
-                if (len > 0) {
-                    const int64_t sid = sc[w]++;
-                    const int64_t off = num_hot + ec[w];
-                    ec[w] += len;
-                    std::copy(hcol.begin() + prev, hcol.begin() + nb, col_all.begin() + off);
-                    rp_all[n + sid] = static_cast<int32_t>(off);
-                    slot_all[n + sid] = next_slot;
-                    next_slot += nslots_of(len);
-                }
-                prev = nb;
-            }
-        }
-    });
-    rp_all[n + S] = static_cast<int32_t>(m);
-    hcol.clear();
-    hcol.shrink_to_fit();
-    // Work items over the combined row space [0, n+S): chunks of long rows
-    // or segments (more than kTileNnz edges) write their slots; tiles pack
-    // consecutive short ones.  Hot items first, then one group per window.
+    // 3. Hot lengths and cold segments (row-major numbering q).
+    int32_t* hot_len = t.get<int32_t>(n);
+    uint8_t* row_start = t.get<uint8_t>(size_t(m));
+    int32_t* flag = t.get<int32_t>(size_t(m));
+    int32_t* pos = t.get<int32_t>(size_t(m) + 1);
+    int32_t* win_count = t.get<int32_t>(size_t(std::max(NW, 1)));
+    if (t.err != hipSuccess) return fail("segments");
+    hip_ok(hipMemsetAsync(row_start, 0, std::max<size_t>(size_t(m), 1), s));
+    hip_ok(hipMemsetAsync(win_count, 0, 4 * size_t(std::max(NW, 1)), s));
+    hipLaunchKernelGGL(k_pr_hot_len, dim3(gn), dim3(kBT), 0, s, rp, col, n, H, hot_len, row_start);
+    hipLaunchKernelGGL(k_pr_seg_flags, dim3(gm), dim3(kBT), 0, s, col, row_start, m, H, wshift, flag);
+    hip_ok(hipGetLastError());
+    scan(flag, pos, m);
+    int32_t S = 0;
+    hip_ok(hipMemcpyAsync(&S, pos + m, 4, hipMemcpyDeviceToHost, s));
+    hip_ok(hipStreamSynchronize(s));
+    if (t.err != hipSuccess) return fail("segments");
+    if (int64_t(n) + S >= INT32_MAX) return set_error(GHET_ERR_UNSUPPORTED, "PageRank layout: too many segments");
+    int32_t* seg_start = t.get<int32_t>(S);
+    int32_t* seg_row = t.get<int32_t>(S);
+    int32_t* seg_len = t.get<int32_t>(S);
+    int32_t* seg_ns = t.get<int32_t>(S);
+    int32_t* sn = t.get<int32_t>(size_t(S) + 1);
+    uint32_t* sk_a = t.get<uint32_t>(S);
+    uint32_t* sk_b = t.get<uint32_t>(S);
+    int32_t* so_a = t.get<int32_t>(S);
+    int32_t* so_b = t.get<int32_t>(S);
+    int32_t* clen = t.get<int32_t>(S);
+    int32_t* coff = t.get<int32_t>(size_t(S) + 1);
+    if (t.err != hipSuccess) return fail("segments");
+    const unsigned gs = pr_grid(S);
+    hipLaunchKernelGGL(k_pr_seg_start, dim3(gm), dim3(kBT), 0, s, flag, pos, m, seg_start);
+    hipLaunchKernelGGL(k_pr_seg_info, dim3(gs), dim3(kBT), 0, s, seg_start, S, m, rp, n, col, H, wshift, seg_row, seg_len,
+                       sk_a, seg_ns, win_count);
+    hip_ok(hipGetLastError());
+    scan(seg_ns, sn, S);
+    // 4. Slots per row and their offsets; the rows k_pr_long finishes.
+    int32_t* nslots = t.get<int32_t>(n);
+    int32_t* slot_base = t.get<int32_t>(size_t(n) + 1);
+    int32_t* partial = t.get<int32_t>(n);
+    int32_t* lpos = t.get<int32_t>(size_t(n) + 1);
+    if (t.err != hipSuccess) return fail("slots");
+    hipLaunchKernelGGL(k_pr_row_slots, dim3(gn), dim3(kBT), 0, s, rp, pos, sn, hot_len, n, blocked ? 1 : 0, nslots, partial);
+    hip_ok(hipGetLastError());
+    scan(nslots, slot_base, n);
+    scan(partial, lpos, n);
+    int32_t counts[2] = {0, 0};  // slots, rows with partials
+    hip_ok(hipMemcpyAsync(counts, slot_base + n, 4, hipMemcpyDeviceToHost, s));
+    hip_ok(hipMemcpyAsync(counts + 1, lpos + n, 4, hipMemcpyDeviceToHost, s));
+    // 5. Window-major segment order: a stable radix sort of the window keys.
+    rocprim::double_buffer<uint32_t> skd(sk_a, sk_b);
+    rocprim::double_buffer<int32_t> sod(so_a, so_b);
+    hipLaunchKernelGGL(k_pr_iota, dim3(gs), dim3(kBT), 0, s, so_a, S);
+    if (NW > 1 && S > 0) {
+        unsigned wb = 1;
+        while ((int64_t(1) << wb) < NW) ++wb;
+        hip_ok(rocprim::radix_sort_pairs(nullptr, need, skd, sod, static_cast<unsigned>(S), 0, wb, s));
+        sort_space(need);
+        hip_ok(rocprim::radix_sort_pairs(sort_tmp, need, skd, sod, static_cast<unsigned>(S), 0, wb, s));
+    }
+    const int32_t* order = sod.current();
+    hipLaunchKernelGGL(k_pr_gather_len, dim3(gs), dim3(kBT), 0, s, order, seg_len, S, clen);
+    hip_ok(hipGetLastError());
+    scan(clen, coff, S);
+    std::vector<int32_t> wcount(static_cast<size_t>(std::max(NW, 1)));
+    hip_ok(hipMemcpyAsync(wcount.data(), win_count, 4 * wcount.size(), hipMemcpyDeviceToHost, s));
+    hip_ok(hipStreamSynchronize(s));
+    if (t.err != hipSuccess) return fail("slots");
+    const int32_t num_slots = counts[0], num_long_rows = counts[1];
+    // 6. The layout arrays and the combined offsets, slots and columns.
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t rows_all = static_cast<size_t>(n) + S;
+    const size_t bytes = al(sizeof(int4) * size_t(num_long_rows)) + 2 * al(4 * size_t(n)) + al(4 * (rows_all + 1)) +
+                         al(4 * std::max<size_t>(rows_all, 1)) + al(4 * size_t(m)) + 256;
+    GHET_HIP(hipMalloc(&L.mem, bytes));
+    char* p = static_cast<char*>(L.mem);
+    L.long_rows = p;
+    p += al(sizeof(int4) * size_t(num_long_rows));
+    auto take = [&](size_t cnt) { int32_t* q = reinterpret_cast<int32_t*>(p); p += al(4 * cnt); return q; };
+    L.outdeg = take(size_t(n));
+    L.perm = take(size_t(n));
+    L.rowptr = take(rows_all + 1);
+    L.slot = take(std::max<size_t>(rows_all, 1));
+    L.col = take(size_t(m));
+    hip_ok(hipMemcpyAsync(L.outdeg, outdeg, 4 * size_t(n), hipMemcpyDeviceToDevice, s));
+    hip_ok(hipMemcpyAsync(L.perm, perm, 4 * size_t(n), hipMemcpyDeviceToDevice, s));
+    scan(hot_len, L.rowptr, n);  // hot parts' offsets; L.rowptr[n] = hot edges
+    int32_t num_hot32 = 0;
+    hip_ok(hipMemcpyAsync(&num_hot32, L.rowptr + n, 4, hipMemcpyDeviceToHost, s));
+    hip_ok(hipStreamSynchronize(s));
+    if (t.err != hipSuccess) return fail("fill");
+    const int64_t num_hot = num_hot32;
+    hipLaunchKernelGGL(k_pr_fill_hot, dim3(gw), dim3(kBT), 0, s, rp, col, hot_len, L.rowptr, nslots, slot_base, lpos, n,
+                       L.slot, static_cast<int4*>(L.long_rows), L.col);
+    hipLaunchKernelGGL(k_pr_fill_cold, dim3(pr_grid(int64_t(ceil_div(S, 64)) * 64)), dim3(kBT), 0, s, order, S, seg_start,
+                       seg_row, coff, sn, pos, rp, hot_len, slot_base, col, n, num_hot, blocked ? 1 : 0, L.rowptr, L.slot,
+                       L.col);
+    hip_ok(hipGetLastError());
+    if (t.err != hipSuccess) return fail("fill");
+    GHET_CHECK(fill<int32_t>(L.rowptr + rows_all, static_cast<int32_t>(m), 1, s));
+    // 7. Work items over the combined row space [0, n+S) on the host: chunks
+    // of long rows or segments (more than kTileNnz edges) write their slots;
+    // tiles pack consecutive short ones.  Hot items first, then one group per
+    // window.
+    std::vector<int32_t> rp_all(rows_all + 1), slot_all(std::max<size_t>(rows_all, 1));
+    hip_ok(hipMemcpyAsync(rp_all.data(), L.rowptr, 4 * (rows_all + 1), hipMemcpyDeviceToHost, s));
+    hip_ok(hipMemcpyAsync(slot_all.data(), L.slot, 4 * rows_all, hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> hot_len_h;
+    if (blocked) {
+        hot_len_h.resize(static_cast<size_t>(n));
+        hip_ok(hipMemcpyAsync(hot_len_h.data(), hot_len, 4 * size_t(n), hipMemcpyDeviceToHost, s));
+    }
+    hip_ok(hipStreamSynchronize(s));
+    if (t.err != hipSuccess) return fail("download");
     std::vector<PRItem> items;
-    std::vector<int4> combine;
     auto emit = [&](int32_t lo, int32_t hi) {  // rows [lo, hi) of the combined space
         for (int32_t r = lo; r < hi;) {
             const int32_t d = rp_all[r + 1] - rp_all[r];
@@ -910,79 +1144,54 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     };
     if (!blocked) emit(0, n);  // source-blocked: k_pr_hot covers the hot parts
     const int32_t num_hot_items = static_cast<int32_t>(items.size());
-    std::vector<int64_t> wseg(static_cast<size_t>(NW) + 1, 0);
-    for (int32_t w = 0; w < NW; ++w) wseg[w + 1] = wseg[w] + [&] {
-        int64_t c = 0;
-        for (int t = 0; t < T; ++t) c += cnt_seg[size_t(t) * NW + w];
-        return c;
-    }();
-    for (int32_t w = 0; w < NW; ++w) emit(static_cast<int32_t>(n + wseg[w]), static_cast<int32_t>(n + wseg[w + 1]));
+    {
+        int64_t w0 = n;
+        for (int32_t w = 0; w < NW; ++w) {
+            emit(static_cast<int32_t>(w0), static_cast<int32_t>(w0 + wcount[w]));
+            w0 += wcount[w];
+        }
+        if (w0 != int64_t(n) + S) return set_error(GHET_ERR_INVALID, "PageRank layout: window counts do not cover the segments");
+    }
     int32_t chunks = 0, tiles = 0;
     for (const PRItem& it : items) (it.row_end < 0 ? chunks : tiles) += 1;
-    for (int32_t r = 0; r < n; ++r)
-        if (nslots[r] > 0) combine.push_back(make_int4(r, slot_base[r], nslots[r], 0));
-    // Every index a launch will use, checked once here: item edge ranges and
-    // rows, slots of chunks, tile rows and combine entries.
+    // Every index a launch will use, checked once here: offsets and slots of
+    // the combined rows, item edge ranges, rows and chunk slots.
     {
-        const int64_t rows_all = n + S;
-        bool ok = rp_all[0] == 0 && rp_all[rows_all] == m;
-        for (int64_t r = 0; ok && r < rows_all; ++r)
-            ok = rp_all[r] <= rp_all[r + 1] && slot_all[r] >= -1 && slot_all[r] < num_slots &&
-                 (r < n || slot_all[r] >= 0);
+        const int64_t ra = static_cast<int64_t>(rows_all);
+        std::atomic<bool> ok{rp_all[0] == 0 && rp_all[ra] == m && num_hot >= 0 && num_hot <= m};
+        parallel_for(ra, hw_threads(0), [&](int64_t b, int64_t e, int) {
+            bool good = true;
+            for (int64_t r = b; good && r < e; ++r)
+                good = rp_all[r] <= rp_all[r + 1] && slot_all[r] >= -1 && slot_all[r] < num_slots &&
+                       (r < n || slot_all[r] >= 0);
+            if (!good) ok = false;
+        });
         for (const PRItem& it : items) {
             if (!ok) break;
-            ok = it.e0 >= 0 && it.e0 <= it.e1 && it.e1 <= m && it.row >= 0 && it.row < rows_all &&
+            ok = it.e0 >= 0 && it.e0 <= it.e1 && it.e1 <= m && it.row >= 0 && it.row < ra &&
                  (it.row_end < 0 ? (it.chunk >= 0 && it.chunk < num_slots && it.e1 - it.e0 <= kTileNnz)
-                                 : (it.row_end > it.row && it.row_end <= rows_all && it.row_end - it.row <= kTileRows &&
+                                 : (it.row_end > it.row && it.row_end <= ra && it.row_end - it.row <= kTileRows &&
                                     it.e0 == rp_all[it.row] && it.e1 == rp_all[it.row_end] && it.e1 - it.e0 <= kTileNnz));
-        }
-        for (const int4& c : combine) {
-            if (!ok) break;
-            ok = c.x >= 0 && c.x < n && c.y >= 0 && c.z > 0 && int64_t(c.y) + c.z <= num_slots;
         }
         if (!ok) return set_error(GHET_ERR_INVALID, "PageRank layout failed its bounds check");
     }
+    GHET_HIP(hipMalloc(&L.items_mem, std::max<size_t>(sizeof(PRItem) * items.size(), 256)));
+    L.items = L.items_mem;
+    if (!items.empty())
+        GHET_HIP(hipMemcpyAsync(L.items, items.data(), sizeof(PRItem) * items.size(), hipMemcpyHostToDevice, s));
+    GHET_HIP(hipStreamSynchronize(s));
     L.num_tiles = tiles;
     L.num_long_chunks = chunks;
-    L.num_long_rows = static_cast<int32_t>(combine.size());
+    L.num_long_rows = num_long_rows;
     L.num_slots = num_slots;
-    L.num_segments = static_cast<int32_t>(S);
+    L.num_segments = S;
     L.num_items = static_cast<int32_t>(items.size());
     L.num_hot_items = num_hot_items;
     L.num_hot_edges = num_hot;
-    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t rows_all = static_cast<size_t>(n + S);
-    const size_t bytes = al(sizeof(PRItem) * items.size()) + al(sizeof(int4) * combine.size()) +
-                         2 * al(4 * size_t(n)) + al(4 * (rows_all + 1)) + al(4 * std::max<size_t>(rows_all, 1)) +
-                         al(4 * size_t(m)) + 256;
-    GHET_HIP(hipMalloc(&L.mem, bytes));
-    char* p = static_cast<char*>(L.mem);
-    L.items = p;
-    p += al(sizeof(PRItem) * items.size());
-    L.long_rows = p;
-    p += al(sizeof(int4) * combine.size());
-    auto take = [&](size_t cnt) { int32_t* q = reinterpret_cast<int32_t*>(p); p += al(4 * cnt); return q; };
-    L.outdeg = take(size_t(n));
-    L.perm = take(size_t(n));
-    L.rowptr = take(rows_all + 1);
-    L.slot = take(std::max<size_t>(rows_all, 1));
-    L.col = take(size_t(m));
-    auto up = [&](void* dst, const void* src, size_t b) -> int {
-        if (b) GHET_HIP(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, s));
-        return GHET_OK;
-    };
-    GHET_CHECK(up(L.items, items.data(), sizeof(PRItem) * items.size()));
-    GHET_CHECK(up(L.long_rows, combine.data(), sizeof(int4) * combine.size()));
-    GHET_CHECK(up(L.outdeg, odp.data(), 4 * size_t(n)));
-    GHET_CHECK(up(L.perm, perm.data(), 4 * size_t(n)));
-    GHET_CHECK(up(L.rowptr, rp_all.data(), 4 * (rows_all + 1)));
-    GHET_CHECK(up(L.slot, slot_all.data(), 4 * rows_all));
-    GHET_CHECK(up(L.col, col_all.data(), 4 * size_t(m)));
-    GHET_HIP(hipStreamSynchronize(s));
     L.hot = H;
     L.window = W;
     L.blocked = blocked;
-    if (blocked) GHET_CHECK(build_hot_slices(L, n, num_hot, hot_len, s));
+    if (blocked) GHET_CHECK(build_hot_slices(L, n, num_hot, hot_len_h, s));
     L.built = true;
     return GHET_OK;
 }
