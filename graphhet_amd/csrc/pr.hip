@@ -783,13 +783,12 @@ __global__ void __launch_bounds__(kBT) k_pr_seg_start(const int32_t* __restrict_
 }
 
 // Per segment q (row-major order): its row, length, window key and slot
-// count; the histogram of windows.
+// count.
 __global__ void __launch_bounds__(kBT) k_pr_seg_info(const int32_t* __restrict__ seg_start, int32_t S, int64_t m,
                                                      const int32_t* __restrict__ rp, int32_t n,
                                                      const int32_t* __restrict__ col, int32_t hot, int wshift,
                                                      int32_t* __restrict__ seg_row, int32_t* __restrict__ seg_len,
-                                                     uint32_t* __restrict__ seg_key, int32_t* __restrict__ seg_ns,
-                                                     int32_t* __restrict__ win_count) {
+                                                     uint32_t* __restrict__ seg_key, int32_t* __restrict__ seg_ns) {
     for (int64_t q = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; q < S; q += static_cast<int64_t>(gridDim.x) * kBT) {
         const int32_t e = seg_start[q];
         int32_t lo = 0, hi = n;  // the row r with rp[r] <= e < rp[r + 1]: last r with rp[r] <= e
@@ -805,7 +804,6 @@ __global__ void __launch_bounds__(kBT) k_pr_seg_info(const int32_t* __restrict__
         seg_len[q] = end - e;
         seg_key[q] = key;
         seg_ns[q] = pr_nslots(end - e);
-        atomicAdd(win_count + key, 1);
     }
 }
 
@@ -999,10 +997,9 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     uint8_t* row_start = t.get<uint8_t>(size_t(m));
     int32_t* flag = t.get<int32_t>(size_t(m));
     int32_t* pos = t.get<int32_t>(size_t(m) + 1);
-    int32_t* win_count = t.get<int32_t>(size_t(std::max(NW, 1)));
+    int32_t* win_start = t.get<int32_t>(size_t(NW) + 1);
     if (t.err != hipSuccess) return fail("segments");
     hip_ok(hipMemsetAsync(row_start, 0, std::max<size_t>(size_t(m), 1), s));
-    hip_ok(hipMemsetAsync(win_count, 0, 4 * size_t(std::max(NW, 1)), s));
     hipLaunchKernelGGL(k_pr_hot_len, dim3(gn), dim3(kBT), 0, s, rp, col, n, H, hot_len, row_start);
     hipLaunchKernelGGL(k_pr_seg_flags, dim3(gm), dim3(kBT), 0, s, col, row_start, m, H, wshift, flag);
     hip_ok(hipGetLastError());
@@ -1027,7 +1024,7 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     const unsigned gs = pr_grid(S);
     hipLaunchKernelGGL(k_pr_seg_start, dim3(gm), dim3(kBT), 0, s, flag, pos, m, seg_start);
     hipLaunchKernelGGL(k_pr_seg_info, dim3(gs), dim3(kBT), 0, s, seg_start, S, m, rp, n, col, H, wshift, seg_row, seg_len,
-                       sk_a, seg_ns, win_count);
+                       sk_a, seg_ns);
     hip_ok(hipGetLastError());
     scan(seg_ns, sn, S);
     // 4. Slots per row and their offsets; the rows k_pr_long finishes.
@@ -1055,11 +1052,14 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
         hip_ok(rocprim::radix_sort_pairs(sort_tmp, need, skd, sod, static_cast<unsigned>(S), 0, wb, s));
     }
     const int32_t* order = sod.current();
+    // first segment of each window in that order (win_start[NW] = S)
+    hipLaunchKernelGGL(k_hot_soff, dim3(pr_grid(int64_t(S) + 1)), dim3(kBT), 0, s, skd.current(), int64_t(S),
+                       static_cast<uint32_t>(NW), win_start);
     hipLaunchKernelGGL(k_pr_gather_len, dim3(gs), dim3(kBT), 0, s, order, seg_len, S, clen);
     hip_ok(hipGetLastError());
     scan(clen, coff, S);
-    std::vector<int32_t> wcount(static_cast<size_t>(std::max(NW, 1)));
-    hip_ok(hipMemcpyAsync(wcount.data(), win_count, 4 * wcount.size(), hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> wstart(static_cast<size_t>(NW) + 1);
+    hip_ok(hipMemcpyAsync(wstart.data(), win_start, 4 * wstart.size(), hipMemcpyDeviceToHost, s));
     hip_ok(hipStreamSynchronize(s));
     if (t.err != hipSuccess) return fail("slots");
     const int32_t num_slots = counts[0], num_long_rows = counts[1];
@@ -1144,13 +1144,10 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     };
     if (!blocked) emit(0, n);  // source-blocked: k_pr_hot covers the hot parts
     const int32_t num_hot_items = static_cast<int32_t>(items.size());
-    {
-        int64_t w0 = n;
-        for (int32_t w = 0; w < NW; ++w) {
-            emit(static_cast<int32_t>(w0), static_cast<int32_t>(w0 + wcount[w]));
-            w0 += wcount[w];
-        }
-        if (w0 != int64_t(n) + S) return set_error(GHET_ERR_INVALID, "PageRank layout: window counts do not cover the segments");
+    if (wstart[0] != 0 || wstart[NW] != S) return set_error(GHET_ERR_INVALID, "PageRank layout: window starts do not cover the segments");
+    for (int32_t w = 0; w < NW; ++w) {
+        if (wstart[w] > wstart[w + 1]) return set_error(GHET_ERR_INVALID, "PageRank layout: window starts out of order");
+        emit(n + wstart[w], n + wstart[w + 1]);
     }
     int32_t chunks = 0, tiles = 0;
     for (const PRItem& it : items) (it.row_end < 0 ? chunks : tiles) += 1;
