@@ -123,8 +123,10 @@ struct TCLayout {
     bool built = false;
     bool symmetric = false;   // out-CSR symmetric and loop-free
     DevCSR dag;               // oriented by (degree, id)
-    int32_t* active = nullptr;  // vertices with >= 2 out-neighbours in the CSR used
-    int32_t num_active = 0;
+    int32_t* heavy = nullptr;   // rows of the counted CSR longer than the per-wave table (block kernel)
+    int32_t num_heavy = 0;
+    int* task_ctr = nullptr;    // k_tc_wave's row-task counter (zeroed before each count)
+    void* heavy_mem = nullptr;
     int64_t isect_bytes = 0;    // sum over DAG (or CSR) edges (u, v) of 4 (d+(u) + d+(v))
     void* mem = nullptr;
 };
@@ -132,7 +134,7 @@ struct TCLayout {
 // Launch shapes the autotuner may change (results never depend on them).
 struct Tuning {
     bool tuned = false;
-    int tc_ept = 8;               // TC: edges per wave task
+    int tc_ept = 8;               // TC: rows per wave task
     int tc_waves_per_cu = 32;     // TC: grid size in waves per CU
     int wcc_light = 16;           // WCC: rows up to this length are linked lane-serially
     int wcc_waves_per_cu = 64;    // WCC: grid size in waves per CU

@@ -1,5 +1,5 @@
-// Triangle counting on gfx950: sorted-adjacency intersection, one wave per
-// edge, binary search of the shorter list's elements in the longer list.
+// Triangle counting on gfx950: per vertex u, its list N(u) goes into a hash
+// table in LDS, and the lists N(v) of its neighbours stream past it.
 //
 // Reference: Cuda_TC_PerNode (GPU_TC_csr.cuh:46-69) runs one block per vertex
 // u and one thread per v in N(u), merging N(u) and N(v) and adding 1 to u, v
@@ -13,13 +13,36 @@
 //            from lower to higher (degree, id), count each triangle once as
 //            (u, v, w) with u->v, u->w, v->w, add 1 to each, and report 6x.
 // Counts are int64 (the reference's int32 can wrap at scale-24 hubs).
+//
+// Counting kernels.  For each u, every w of N(v) (v in N(u)) is looked up in
+// u's table; a hit adds 1 to the hit slot's LDS counter (w's count) and to
+// v's count.  When u is done, its table's counters go to the global counts
+// with one 64-bit atomic per neighbour that was hit, and u's total with one:
+// device-scope atomics per vertex and neighbour instead of per triangle
+// corner (those go to the fabric: the XCDs' L2s are not coherent).
+//   k_tc_wave   rows with up to kWaveMax entries: one wave per row, a table
+//               of at most 2 kWaveMax slots per wave, rows handed out in
+//               tasks of `vpt` by an atomic counter;
+//   k_tc_block  longer rows, longest first: one 1024-thread block per row,
+//               a table of up to kBlockSlots slots, filled with
+//               kBlockSlots / 2 entries of N(u) at a time (v's counts then
+//               go to global memory directly, once per v and chunk).
+// Tables are open-addressed (multiplicative hash, linear probing) and at most
+// half full, so every probe ends at the key or at an empty slot.  The lists
+// are those of a canonical CSR (no duplicate entries in a row).
 #include "graph_internal.hpp"
+
+#include <rocprim/rocprim.hpp>
 
 namespace ghet {
 namespace {
 
 constexpr int kBT = 256;
-constexpr int kEdgesPerTask = 8;
+constexpr int kWaveMax = 256;                // rows up to this length: k_tc_wave
+constexpr int kWaveSlots = 2 * kWaveMax;     // table slots per wave
+constexpr int kBlockBT = 1024;               // k_tc_block threads
+constexpr int kBlockSlots = 8192;            // k_tc_block table slots (keys + counters: 64 KB, two blocks per CU)
+constexpr int kTcLight = 16;                 // lists N(v) up to this length are probed by one lane
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int64_t len, int32_t key) {
     int64_t lo = 0, hi = len;
@@ -86,48 +109,211 @@ __global__ void __launch_bounds__(kBT) k_dag(const int32_t* __restrict__ rowptr,
     }
 }
 
-// One wave per kEdgesPerTask consecutive edges (u, v) of X; for each, the
-// common out-neighbours w of u and v: cnt[w] += 1, cnt[u] += c, cnt[v] += c.
-__global__ void __launch_bounds__(kBT) k_tc_count(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                                  int32_t n, int64_t m, int ept, unsigned long long* cnt) {
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t tc_slot0(int32_t key, int shift) {
+    return (static_cast<uint32_t>(key) * 2654435761u) >> shift;
+}
+
+__device__ __forceinline__ void tc_insert(int32_t* keys, uint32_t mask, int shift, int32_t key) {
+    uint32_t s = tc_slot0(key, shift);
+    while (atomicCAS(&keys[s], -1, key) != -1) s = (s + 1) & mask;
+}
+
+__device__ __forceinline__ int tc_find(const int32_t* keys, uint32_t mask, int shift, int32_t key) {
+    uint32_t s = tc_slot0(key, shift);
+    for (;;) {
+        const int32_t k = keys[s];
+        if (k == key) return static_cast<int>(s);
+        if (k < 0) return -1;
+        s = (s + 1) & mask;
+    }
+}
+
+// One v per lane (dv = 0: none).  Every w of N(v) found in the table adds 1
+// to its slot's counter; returns the lane's own v's number of hits.  Lists
+// up to kTcLight entries are probed by their lane (loads issued first),
+// longer ones by the whole wave.
+__device__ __forceinline__ uint32_t tc_probe(int32_t vb, int32_t dv, const int32_t* __restrict__ col,
+                                             const int32_t* keys, uint32_t* cnts, uint32_t mask, int shift) {
     const int lane = lane_id();
-    const int64_t tasks = ceil_div(m, ept);
-    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
-    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
-    for (int64_t t = wave_global; t < tasks; t += wave_stride) {
-        const int64_t e0 = t * ept, e1 = min(e0 + ept, m);
-        int32_t u = upper_bound_i32(rowptr, static_cast<int64_t>(n) + 1, static_cast<int32_t>(e0)) - 1;
-        int32_t ub = rowptr[u], ue = rowptr[u + 1];
-        unsigned long long cu = 0;
-        for (int64_t e = e0; e < e1; ++e) {
-            while (ue <= e) {
-                if (lane == 0 && cu) atomicAdd(&cnt[u], cu);
-                cu = 0;
-                ++u;
-                ub = ue;
-                ue = rowptr[u + 1];
-            }
-            const int32_t v = col[e];
-            const int32_t vb = rowptr[v], ve = rowptr[v + 1];
-            const bool u_small = (ue - ub) <= (ve - vb);
-            const int32_t* small = col + (u_small ? ub : vb);
-            const int32_t ns = u_small ? ue - ub : ve - vb;
-            const int32_t* large = col + (u_small ? vb : ub);
-            const int32_t nl = u_small ? ve - vb : ue - ub;
-            unsigned long long c = 0;
-            for (int32_t i = lane; i < ns; i += 64) {
-                const int32_t w = small[i];
-                if (contains(large, nl, w)) {
+    uint32_t c = 0;
+    if (dv <= kTcLight) {
+        int32_t w[kTcLight];
+#pragma unroll
+        for (int i = 0; i < kTcLight; ++i) w[i] = i < dv ? col[vb + i] : -1;
+#pragma unroll
+        for (int i = 0; i < kTcLight; ++i) {
+            if (i < dv) {
+                const int sl = tc_find(keys, mask, shift, w[i]);
+                if (sl >= 0) {
+                    atomicAdd(&cnts[sl], 1u);
                     ++c;
-                    atomicAdd(&cnt[w], 1ULL);
                 }
             }
-            c = wave_sum(c);
-            if (lane == 0 && c) atomicAdd(&cnt[v], c);
-            cu += c;
         }
-        if (lane == 0 && cu) atomicAdd(&cnt[u], cu);
     }
+    uint64_t heavy = __ballot(dv > kTcLight);
+    while (heavy) {
+        const int src = __ffsll(static_cast<unsigned long long>(heavy)) - 1;
+        heavy &= heavy - 1;
+        const int32_t hb = __shfl(vb, src), hd = __shfl(dv, src);
+        uint32_t hc = 0;
+        for (int32_t i0 = 0; i0 < hd; i0 += 4 * 64) {  // four loads in flight per lane
+            int32_t w[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int32_t i = i0 + k * 64 + lane;
+                w[k] = i < hd ? col[hb + i] : -1;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (w[k] >= 0) {
+                    const int sl = tc_find(keys, mask, shift, w[k]);
+                    if (sl >= 0) {
+                        atomicAdd(&cnts[sl], 1u);
+                        ++hc;
+                    }
+                }
+            }
+        }
+        hc = wave_sum(hc);
+        if (lane == src) c = hc;
+    }
+    return c;
+}
+
+// Rows with min_d <= |N(u)| <= kWaveMax, one wave each; tasks of vpt rows
+// from *task_ctr (zero at launch).
+__global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                 int32_t n, int vpt, int min_d, int* task_ctr,
+                                                 unsigned long long* cnt) {
+    __shared__ int32_t s_keys[kBT / 64][kWaveSlots];
+    __shared__ uint32_t s_cnts[kBT / 64][kWaveSlots];
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    int32_t* keys = s_keys[wave];
+    uint32_t* cnts = s_cnts[wave];
+    const int32_t tasks = static_cast<int32_t>(ceil_div(n, vpt));
+    for (;;) {
+        int32_t t = 0;
+        if (lane == 0) t = atomicAdd(task_ctr, 1);
+        t = __shfl(t, 0);
+        if (t >= tasks) break;
+        const int32_t u1 = static_cast<int32_t>(std::min<int64_t>(int64_t(t + 1) * vpt, n));
+        for (int32_t u = t * vpt; u < u1; ++u) {
+            const int32_t b = rowptr[u], d = rowptr[u + 1] - b;
+            if (d < min_d || d > kWaveMax) continue;
+            int tbits = 6;
+            while ((1 << tbits) < 2 * d) ++tbits;
+            const uint32_t T = 1u << tbits, mask = T - 1;
+            const int shift = 32 - tbits;
+            for (uint32_t sl = lane; sl < T; sl += 64) {
+                keys[sl] = -1;
+                cnts[sl] = 0;
+            }
+            wave_lds_sync();
+            for (int32_t i = lane; i < d; i += 64) tc_insert(keys, mask, shift, col[b + i]);
+            wave_lds_sync();
+            uint32_t tot = 0;
+            for (int32_t i0 = 0; i0 < d; i0 += 64) {
+                const int32_t i = i0 + lane;
+                int32_t v = -1, vb = 0, dv = 0;
+                if (i < d) {
+                    v = col[b + i];
+                    vb = rowptr[v];
+                    dv = rowptr[v + 1] - vb;
+                }
+                const uint32_t c = tc_probe(vb, dv, col, keys, cnts, mask, shift);
+                if (c) {
+                    atomicAdd(&cnts[tc_find(keys, mask, shift, v)], c);  // v is in N(u)
+                    tot += c;
+                }
+            }
+            wave_lds_sync();
+            const unsigned long long total = wave_sum(static_cast<unsigned long long>(tot));
+            if (lane == 0 && total) atomicAdd(&cnt[u], total);
+            for (uint32_t sl = lane; sl < T; sl += 64) {
+                const uint32_t k = cnts[sl];
+                if (k) atomicAdd(&cnt[keys[sl]], static_cast<unsigned long long>(k));
+            }
+            wave_lds_sync();  // the next row clears the table
+        }
+    }
+}
+
+// Rows longer than kWaveMax (the list `heavy`), one block each.
+__global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ col,
+                                                       const int32_t* __restrict__ heavy, int32_t num_heavy,
+                                                       unsigned long long* cnt) {
+    __shared__ int32_t keys[kBlockSlots];
+    __shared__ uint32_t cnts[kBlockSlots];
+    __shared__ unsigned long long s_red[kBlockBT / 64];
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    for (int32_t h = blockIdx.x; h < num_heavy; h += gridDim.x) {
+        const int32_t u = heavy[h], b = rowptr[u], d = rowptr[u + 1] - b;
+        int tbits = 6;
+        while ((1 << tbits) < 2 * d && (1 << tbits) < kBlockSlots) ++tbits;
+        const uint32_t T = 1u << tbits, mask = T - 1;
+        const int shift = 32 - tbits;
+        const int32_t cap = static_cast<int32_t>(T / 2);
+        unsigned long long tot = 0;
+        for (int32_t c0 = 0; c0 < d; c0 += cap) {
+            const int32_t c1 = d - c0 < cap ? d : c0 + cap;
+            for (uint32_t sl = tid; sl < T; sl += kBlockBT) {
+                keys[sl] = -1;
+                cnts[sl] = 0;
+            }
+            __syncthreads();
+            for (int32_t i = c0 + tid; i < c1; i += kBlockBT) tc_insert(keys, mask, shift, col[b + i]);
+            __syncthreads();
+            for (int32_t i0 = wave * 64; i0 < d; i0 += kBlockBT) {
+                const int32_t i = i0 + lane;
+                int32_t v = -1, vb = 0, dv = 0;
+                if (i < d) {
+                    v = col[b + i];
+                    vb = rowptr[v];
+                    dv = rowptr[v + 1] - vb;
+                }
+                const uint32_t c = tc_probe(vb, dv, col, keys, cnts, mask, shift);
+                if (c) {
+                    atomicAdd(&cnt[v], static_cast<unsigned long long>(c));
+                    tot += c;
+                }
+            }
+            __syncthreads();
+            for (uint32_t sl = tid; sl < T; sl += kBlockBT) {
+                const uint32_t k = cnts[sl];
+                if (k) atomicAdd(&cnt[keys[sl]], static_cast<unsigned long long>(k));
+            }
+            __syncthreads();
+        }
+        tot = block_sum_u64<kBlockBT>(tot, s_red);
+        if (tid == 0 && tot) atomicAdd(&cnt[u], tot);
+    }
+}
+
+// Rows longer than kWaveMax: flags, then their ids in order.
+__global__ void k_tc_heavy_flags(const int32_t* __restrict__ rowptr, int32_t n, int32_t* __restrict__ flag) {
+    for (int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < n;
+         u += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        flag[u] = rowptr[u + 1] - rowptr[u] > kWaveMax ? 1 : 0;
+}
+__global__ void k_tc_heavy_keys(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ heavy, int32_t h,
+                                uint32_t* __restrict__ key) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        key[i] = ~static_cast<uint32_t>(rowptr[heavy[i] + 1] - rowptr[heavy[i]]);
+}
+__global__ void k_tc_heavy_fill(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int32_t n,
+                                int32_t* __restrict__ heavy) {
+    for (int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < n;
+         u += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        if (flag[u]) heavy[pos[u]] = static_cast<int32_t>(u);
 }
 
 // Intersection bytes of the oriented CSR: over its edges (u, v), 4 bytes per
@@ -189,6 +375,54 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
         L.dag = {n, m_dag, drp, dcol, nullptr};
     }
     {
+        // rows for the block kernel, and the wave kernel's task counter
+        const DevCSR& X = L.symmetric ? L.dag : g->out;
+        int32_t *flag, *pos, *aux;
+        GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); flag = static_cast<int32_t*>(p);
+        GHET_CHECK(g->ws.get(WS_TC_COUNT, 4 * (size_t(n) + 1), &p)); pos = static_cast<int32_t*>(p);
+        GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
+        const unsigned gn = grid_cap(std::min<int64_t>(ceil_div(n, 256), 4096));
+        hipLaunchKernelGGL(k_tc_heavy_flags, dim3(gn), dim3(256), 0, s, X.rowptr, n, flag);
+        GHET_CHECK(exclusive_scan_i32(flag, pos, n, aux, s));
+        GHET_HIP(hipMemcpyAsync(g->pinned, pos + n, 4, hipMemcpyDeviceToHost, s));
+        GHET_HIP(hipStreamSynchronize(s));
+        L.num_heavy = reinterpret_cast<const int32_t*>(g->pinned)[0];
+        GHET_HIP(hipMalloc(&L.heavy_mem, 256 + 4 * size_t(std::max(L.num_heavy, 1))));
+        L.task_ctr = static_cast<int*>(L.heavy_mem);
+        L.heavy = reinterpret_cast<int32_t*>(static_cast<char*>(L.heavy_mem) + 256);
+        hipLaunchKernelGGL(k_tc_heavy_fill, dim3(gn), dim3(256), 0, s, flag, pos, n, L.heavy);
+        GHET_HIP(hipGetLastError());
+        if (L.num_heavy > 1) {
+            // longest rows first: the block kernel's last blocks are the short ones
+            const int32_t h = L.num_heavy;
+            uint32_t *ka = nullptr, *kb = nullptr;
+            int32_t* vb = nullptr;
+            void* tmp = nullptr;
+            size_t tb = 0;
+            hipError_t e = hipMalloc(&ka, 4 * size_t(h));
+            if (e == hipSuccess) e = hipMalloc(&kb, 4 * size_t(h));
+            if (e == hipSuccess) e = hipMalloc(&vb, 4 * size_t(h));
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_tc_heavy_keys, dim3(grid_cap(ceil_div(h, 256))), dim3(256), 0, s, X.rowptr, L.heavy, h, ka);
+                e = hipGetLastError();
+            }
+            rocprim::double_buffer<uint32_t> kd(ka, kb);
+            rocprim::double_buffer<int32_t> vd(L.heavy, vb);
+            if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, tb, kd, vd, static_cast<unsigned>(h), 0, 32, s);
+            if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tb, 256));
+            if (e == hipSuccess) e = rocprim::radix_sort_pairs(tmp, tb, kd, vd, static_cast<unsigned>(h), 0, 32, s);
+            if (e == hipSuccess && vd.current() != L.heavy)
+                e = hipMemcpyAsync(L.heavy, vd.current(), 4 * size_t(h), hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(vb), tmp})
+                if (q) hipFree(q);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                return set_error(GHET_ERR_HIP, "TC heavy rows: %s", hipGetErrorString(e));
+            }
+        }
+    }
+    {
         // the reported intersection-bytes figure of the CSR the count reads
         const DevCSR& X = L.symmetric ? L.dag : g->out;
         GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(CtrSlot) * kSlots, &p));
@@ -211,18 +445,24 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
 int tc_prepare(ghet_graph* g, hipStream_t s) { return build_tc_layout(g, s); }
 
 // Enqueues the counting (fill, count, 6x scale) with no host synchronisation,
-// so it can be captured into a hipGraph.  ept: edges per wave task;
-// waves_per_cu: grid size in resident waves per CU.
-int enqueue_tc(ghet_graph* g, unsigned long long* cnt, int ept, int waves_per_cu, hipStream_t s) {
+// so it can be captured into a hipGraph.  vpt: rows per wave task of
+// k_tc_wave; waves_per_cu: its grid size in resident waves per CU.
+int enqueue_tc(ghet_graph* g, unsigned long long* cnt, int vpt, int waves_per_cu, hipStream_t s) {
     const int32_t n = g->n;
     const TCLayout& L = g->tc;
     const DevCSR& X = L.symmetric ? L.dag : g->out;
     GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(cnt), 0ULL, n, s));
     if (X.m > 0) {
-        const int64_t tasks = ceil_div(X.m, ept);
-        const int64_t cap = int64_t(256) * waves_per_cu / (kBT / 64);
-        hipLaunchKernelGGL(k_tc_count, dim3(grid_cap(std::min<int64_t>(ceil_div(tasks, kBT / 64), cap))), dim3(kBT), 0,
-                           s, X.rowptr, X.col, n, X.m, ept, cnt);
+        int cus = 0;
+        GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+        if (L.num_heavy > 0)
+            hipLaunchKernelGGL(k_tc_block, dim3(static_cast<unsigned>(std::min(L.num_heavy, 65535))), dim3(kBlockBT), 0, s,
+                               X.rowptr, X.col, L.heavy, L.num_heavy, cnt);
+        GHET_CHECK(fill<int32_t>(L.task_ctr, 0, 1, s));
+        const int64_t tasks = ceil_div(n, std::max(vpt, 1));
+        const int64_t cap = int64_t(std::max(cus, 1)) * waves_per_cu / (kBT / 64);
+        hipLaunchKernelGGL(k_tc_wave, dim3(grid_cap(std::min<int64_t>(ceil_div(tasks, kBT / 64), cap))), dim3(kBT), 0,
+                           s, X.rowptr, X.col, n, std::max(vpt, 1), L.symmetric ? 2 : 1, L.task_ctr, cnt);
     }
     if (L.symmetric)
         hipLaunchKernelGGL(k_scale_i64, dim3(grid_cap(std::min<int64_t>(ceil_div(n, 256), 8192))), dim3(256), 0, s, cnt,

@@ -146,6 +146,43 @@ def test_medium_tc(medium):
     np.testing.assert_array_equal(c, O.tc(s.rowptr, s.col))
 
 
+def test_medium_tc_directed(medium):
+    """The reference loop on the directed out-CSR (no DAG): rows above the
+    per-wave table go to the block kernel."""
+    o = medium["o"]
+    c, st = medium["dg"].tc()
+    assert st.extra[0] == 0.0
+    np.testing.assert_array_equal(c, O.tc(o.rowptr, o.col))
+
+
+def test_tc_clique_block_rows():
+    """K_600: every DAG row of rank r has 599 - r entries, so rows above 256
+    entries take the block kernel; each vertex is in C(599, 2) triangles."""
+    k = 600
+    src = [u for u in range(k) for v in range(k) if u != v]
+    dst = [v for u in range(k) for v in range(k) if u != v]
+    hg = G.HostGraph.from_edges(k, src, dst)
+    s = hg.view(G.VIEW_SYM)
+    c, st = G.DeviceGraph(s, undirected=True, weights=False).tc()
+    assert st.extra[0] == 1.0
+    np.testing.assert_array_equal(c, np.full(k, 6 * (k - 1) * (k - 2) // 2, np.int64))
+
+
+def test_tc_directed_hub_in_chunks():
+    """A directed hub with 20,000 out-neighbours (more than one block table
+    of 8,192 entries: three chunks), a path among them, self-loops on some
+    vertices and a second hub: the reference loop, against the oracle."""
+    n = 20003
+    rng = np.random.default_rng(5)
+    src = [0] * (n - 1) + list(range(1, n - 1)) + [7, 9, 11, 20001] + [1] * 9000
+    dst = list(range(1, n)) + list(range(2, n)) + [7, 9, 11, 20001] + rng.choice(np.arange(2, n), 9000, replace=False).tolist()
+    hg = G.HostGraph.from_edges(n, src, dst)
+    o = hg.view(G.VIEW_OUT)
+    c, st = G.DeviceGraph(o, weights=False).tc()
+    assert st.extra[0] == 0.0
+    np.testing.assert_array_equal(c, O.tc(o.rowptr, o.col))
+
+
 def test_caller_stream(medium):
     """BFS and SSSP on a stream the caller owns (opts.stream): level and pass
     totals still reach the host through the mailbox, results through the
