@@ -162,7 +162,8 @@ void destroy_graph(ghet_graph* g) {
     if (g->pr.items_mem) hipFree(g->pr.items_mem);
     if (g->cdlp.mem) hipFree(g->cdlp.mem);
     if (g->tc.mem) hipFree(g->tc.mem);
-    if (g->tc.heavy_mem) hipFree(g->tc.heavy_mem);
+    if (g->tc.in_mem) hipFree(g->tc.in_mem);
+    if (g->tc.task_mem) hipFree(g->tc.task_mem);
     if (g->ms.mem) hipFree(g->ms.mem);
     if (g->sl.mem) hipFree(g->sl.mem);
     if (g->bfs_skip) hipFree(g->bfs_skip);

@@ -123,10 +123,14 @@ struct TCLayout {
     bool built = false;
     bool symmetric = false;   // out-CSR symmetric and loop-free
     DevCSR dag;               // oriented by (degree, id)
-    int32_t* heavy = nullptr;   // rows of the counted CSR longer than the per-wave table (block kernel)
-    int32_t num_heavy = 0;
-    int* task_ctr = nullptr;    // k_tc_wave's row-task counter (zeroed before each count)
-    void* heavy_mem = nullptr;
+    int32_t* in_rowptr = nullptr;  // transpose of the counted CSR (DAG or out-CSR): in-lists
+    int32_t* in_col = nullptr;
+    int* task_ctr = nullptr;       // k_tc_wave's task counter (zeroed before each count)
+    void* in_mem = nullptr;
+    void* wave_tasks = nullptr;    // {vertex, first in-list entry} per task (tc.hip)
+    void* block_tasks = nullptr;
+    int32_t num_wave_tasks = 0, num_block_tasks = 0;
+    void* task_mem = nullptr;
     int64_t isect_bytes = 0;    // sum over DAG (or CSR) edges (u, v) of 4 (d+(u) + d+(v))
     void* mem = nullptr;
 };

@@ -14,19 +14,26 @@
 //            (u, v, w) with u->v, u->w, v->w, add 1 to each, and report 6x.
 // Counts are int64 (the reference's int32 can wrap at scale-24 hubs).
 //
-// Counting kernels.  For each u, every w of N(v) (v in N(u)) is looked up in
-// u's table; a hit adds 1 to the hit slot's LDS counter (w's count) and to
-// v's count.  When u is done, its table's counters go to the global counts
-// with one 64-bit atomic per neighbour that was hit, and u's total with one:
-// device-scope atomics per vertex and neighbour instead of per triangle
-// corner (those go to the fabric: the XCDs' L2s are not coherent).
-//   k_tc_wave   rows with up to kWaveMax entries: one wave per row, a table
-//               of at most 2 kWaveMax slots per wave, rows handed out in
-//               tasks of `vpt` by an atomic counter;
-//   k_tc_block  longer rows, longest first: one 1024-thread block per row,
-//               a table of up to kBlockSlots slots, filled with
-//               kBlockSlots / 2 entries of N(u) at a time (v's counts then
-//               go to global memory directly, once per v and chunk).
+// Counting kernels.  Every (u, v, w) counted is found from v: v's list N(v)
+// goes into a hash table in LDS, and for every u whose list holds v (the
+// transpose's in-list of v) N(u) streams past the table.  A hit on w adds 1
+// to w's slot counter, to u's count and to v's.  Streaming N(u) once per
+// edge u -> v costs sum over u of |N(u)|^2 entries, against sum over v of
+// |in(v)| |N(v)| for streaming N(v) past u's table: on the degree-ordered
+// DAG of R-MAT the first is about 1.8 times smaller (2.5e9 against 4.4e9
+// entries at scale 20).  When v's task is done, the slot counters go to the
+// global counts with one 64-bit atomic per neighbour that was hit, v's total
+// with one, and each u's with one: device-scope atomics per vertex and edge
+// instead of per triangle corner (they go to the fabric: the XCDs' L2s are
+// not coherent).
+//   tasks       v and a run of its in-list (kInWave or kInBlock entries), so
+//               a hub with a long in-list is shared by many waves;
+//   k_tc_wave   tasks of v with up to kWaveMax entries: one wave each, a
+//               table of at most 2 kWaveMax slots per wave, tasks taken
+//               `vpt` at a time from an atomic counter;
+//   k_tc_block  tasks of longer v: one 1024-thread block each, a table of up
+//               to kBlockSlots slots filled kBlockSlots / 2 entries of N(v)
+//               at a time.
 // Tables are open-addressed (multiplicative hash, linear probing) and at most
 // half full, so every probe ends at the key or at an empty slot.  The lists
 // are those of a canonical CSR (no duplicate entries in a row).
@@ -42,7 +49,9 @@ constexpr int kWaveMax = 256;                // rows up to this length: k_tc_wav
 constexpr int kWaveSlots = 2 * kWaveMax;     // table slots per wave
 constexpr int kBlockBT = 1024;               // k_tc_block threads
 constexpr int kBlockSlots = 8192;            // k_tc_block table slots (keys + counters: 64 KB, two blocks per CU)
-constexpr int kTcLight = 16;                 // lists N(v) up to this length are probed by one lane
+constexpr int kTcLight = 16;                 // streamed lists up to this length are probed by one lane
+constexpr int kInWave = 256;                 // in-list entries per wave task
+constexpr int kInBlock = 4096;               // in-list entries per block task
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int64_t len, int32_t key) {
     int64_t lo = 0, hi = len;
@@ -187,26 +196,36 @@ __device__ __forceinline__ uint32_t tc_probe(int32_t vb, int32_t dv, const int32
     return c;
 }
 
-// Rows with min_d <= |N(u)| <= kWaveMax, one wave each; tasks of vpt rows
-// from *task_ctr (zero at launch).
+// A task is a vertex v and a run of at most kInWave (wave) or kInBlock
+// (block) entries of its in-list, from `first`; tasks are taken `vpt` at a
+// time from *task_ctr (zero at launch).
+struct alignas(8) TcTask {
+    int32_t v, first;
+};
+
+// Tasks of vertices with |N(v)| <= kWaveMax: one wave each.  The wave
+// builds v's table, then every u of the task's in-list run streams N(u)
+// past it: c hits add c to u (one atomic per u with hits) and to v.
 __global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                                 int32_t n, int vpt, int min_d, int* task_ctr,
-                                                 unsigned long long* cnt) {
+                                                 const int32_t* __restrict__ in_rowptr,
+                                                 const int32_t* __restrict__ in_col, const TcTask* __restrict__ tasks,
+                                                 int32_t num_tasks, int vpt, int* task_ctr, unsigned long long* cnt) {
     __shared__ int32_t s_keys[kBT / 64][kWaveSlots];
     __shared__ uint32_t s_cnts[kBT / 64][kWaveSlots];
     const int lane = lane_id(), wave = threadIdx.x >> 6;
     int32_t* keys = s_keys[wave];
     uint32_t* cnts = s_cnts[wave];
-    const int32_t tasks = static_cast<int32_t>(ceil_div(n, vpt));
     for (;;) {
-        int32_t t = 0;
-        if (lane == 0) t = atomicAdd(task_ctr, 1);
-        t = __shfl(t, 0);
-        if (t >= tasks) break;
-        const int32_t u1 = static_cast<int32_t>(std::min<int64_t>(int64_t(t + 1) * vpt, n));
-        for (int32_t u = t * vpt; u < u1; ++u) {
-            const int32_t b = rowptr[u], d = rowptr[u + 1] - b;
-            if (d < min_d || d > kWaveMax) continue;
+        int32_t t0 = 0;
+        if (lane == 0) t0 = atomicAdd(task_ctr, vpt);
+        t0 = __shfl(t0, 0);
+        if (t0 >= num_tasks) break;
+        const int32_t t1 = num_tasks - t0 < vpt ? num_tasks : t0 + vpt;
+        for (int32_t t = t0; t < t1; ++t) {
+            const TcTask task = tasks[t];
+            const int32_t v = task.v, b = rowptr[v], d = rowptr[v + 1] - b;
+            const int32_t j0 = in_rowptr[v] + task.first;
+            const int32_t j1 = min(in_rowptr[v + 1], j0 + kInWave);
             int tbits = 6;
             while ((1 << tbits) < 2 * d) ++tbits;
             const uint32_t T = 1u << tbits, mask = T - 1;
@@ -219,43 +238,50 @@ __global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ row
             for (int32_t i = lane; i < d; i += 64) tc_insert(keys, mask, shift, col[b + i]);
             wave_lds_sync();
             uint32_t tot = 0;
-            for (int32_t i0 = 0; i0 < d; i0 += 64) {
-                const int32_t i = i0 + lane;
-                int32_t v = -1, vb = 0, dv = 0;
-                if (i < d) {
-                    v = col[b + i];
-                    vb = rowptr[v];
-                    dv = rowptr[v + 1] - vb;
+            for (int32_t jb = j0; jb < j1; jb += 64) {
+                const int32_t j = jb + lane;
+                int32_t u = -1, ub = 0, du = 0;
+                if (j < j1) {
+                    u = in_col[j];
+                    ub = rowptr[u];
+                    du = rowptr[u + 1] - ub;
                 }
-                const uint32_t c = tc_probe(vb, dv, col, keys, cnts, mask, shift);
+                const uint32_t c = tc_probe(ub, du, col, keys, cnts, mask, shift);
                 if (c) {
-                    atomicAdd(&cnts[tc_find(keys, mask, shift, v)], c);  // v is in N(u)
+                    atomicAdd(&cnt[u], static_cast<unsigned long long>(c));
                     tot += c;
                 }
             }
             wave_lds_sync();
             const unsigned long long total = wave_sum(static_cast<unsigned long long>(tot));
-            if (lane == 0 && total) atomicAdd(&cnt[u], total);
+            if (lane == 0 && total) atomicAdd(&cnt[v], total);
             for (uint32_t sl = lane; sl < T; sl += 64) {
                 const uint32_t k = cnts[sl];
                 if (k) atomicAdd(&cnt[keys[sl]], static_cast<unsigned long long>(k));
             }
-            wave_lds_sync();  // the next row clears the table
+            wave_lds_sync();  // the next task clears the table
         }
     }
 }
 
-// Rows longer than kWaveMax (the list `heavy`), one block each.
+// Tasks of vertices with |N(v)| > kWaveMax: one block each; v's table is
+// filled kBlockSlots / 2 entries at a time, and the task's in-list run
+// streams past each fill.
 __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restrict__ rowptr,
-                                                       const int32_t* __restrict__ col,
-                                                       const int32_t* __restrict__ heavy, int32_t num_heavy,
-                                                       unsigned long long* cnt) {
+                                                          const int32_t* __restrict__ col,
+                                                          const int32_t* __restrict__ in_rowptr,
+                                                          const int32_t* __restrict__ in_col,
+                                                          const TcTask* __restrict__ tasks, int32_t num_tasks,
+                                                          unsigned long long* cnt) {
     __shared__ int32_t keys[kBlockSlots];
     __shared__ uint32_t cnts[kBlockSlots];
     __shared__ unsigned long long s_red[kBlockBT / 64];
     const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    for (int32_t h = blockIdx.x; h < num_heavy; h += gridDim.x) {
-        const int32_t u = heavy[h], b = rowptr[u], d = rowptr[u + 1] - b;
+    for (int32_t t = blockIdx.x; t < num_tasks; t += gridDim.x) {
+        const TcTask task = tasks[t];
+        const int32_t v = task.v, b = rowptr[v], d = rowptr[v + 1] - b;
+        const int32_t j0 = in_rowptr[v] + task.first;
+        const int32_t j1 = min(in_rowptr[v + 1], j0 + kInBlock);
         int tbits = 6;
         while ((1 << tbits) < 2 * d && (1 << tbits) < kBlockSlots) ++tbits;
         const uint32_t T = 1u << tbits, mask = T - 1;
@@ -271,17 +297,17 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
             __syncthreads();
             for (int32_t i = c0 + tid; i < c1; i += kBlockBT) tc_insert(keys, mask, shift, col[b + i]);
             __syncthreads();
-            for (int32_t i0 = wave * 64; i0 < d; i0 += kBlockBT) {
-                const int32_t i = i0 + lane;
-                int32_t v = -1, vb = 0, dv = 0;
-                if (i < d) {
-                    v = col[b + i];
-                    vb = rowptr[v];
-                    dv = rowptr[v + 1] - vb;
+            for (int32_t jb = j0 + wave * 64; jb < j1; jb += kBlockBT) {
+                const int32_t j = jb + lane;
+                int32_t u = -1, ub = 0, du = 0;
+                if (j < j1) {
+                    u = in_col[j];
+                    ub = rowptr[u];
+                    du = rowptr[u + 1] - ub;
                 }
-                const uint32_t c = tc_probe(vb, dv, col, keys, cnts, mask, shift);
+                const uint32_t c = tc_probe(ub, du, col, keys, cnts, mask, shift);
                 if (c) {
-                    atomicAdd(&cnt[v], static_cast<unsigned long long>(c));
+                    atomicAdd(&cnt[u], static_cast<unsigned long long>(c));
                     tot += c;
                 }
             }
@@ -293,27 +319,52 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
             __syncthreads();
         }
         tot = block_sum_u64<kBlockBT>(tot, s_red);
-        if (tid == 0 && tot) atomicAdd(&cnt[u], tot);
+        if (tid == 0 && tot) atomicAdd(&cnt[v], tot);
     }
 }
 
-// Rows longer than kWaveMax: flags, then their ids in order.
-__global__ void k_tc_heavy_flags(const int32_t* __restrict__ rowptr, int32_t n, int32_t* __restrict__ flag) {
-    for (int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < n;
-         u += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        flag[u] = rowptr[u + 1] - rowptr[u] > kWaveMax ? 1 : 0;
+// Transpose of the counted CSR: the source of every edge, in row order.
+__global__ void __launch_bounds__(kBT) k_tc_sources(const int32_t* __restrict__ rowptr, int32_t n,
+                                                    int32_t* __restrict__ src) {
+    const int lane = lane_id();
+    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
+    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
+    for (int64_t u = wave_global; u < n; u += wave_stride)
+        for (int32_t e = rowptr[u] + lane; e < rowptr[u + 1]; e += 64) src[e] = static_cast<int32_t>(u);
 }
-__global__ void k_tc_heavy_keys(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ heavy, int32_t h,
-                                uint32_t* __restrict__ key) {
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        key[i] = ~static_cast<uint32_t>(rowptr[heavy[i] + 1] - rowptr[heavy[i]]);
+
+// off[k] = first position of key k in the sorted keys; off[num_keys] = count.
+__global__ void k_tc_key_offsets(const int32_t* __restrict__ key, int64_t count, int32_t num_keys,
+                                 int32_t* __restrict__ off) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e <= count;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t prev = e > 0 ? static_cast<int64_t>(key[e - 1]) : -1;
+        const int64_t cur = e < count ? static_cast<int64_t>(key[e]) : static_cast<int64_t>(num_keys);
+        for (int64_t k = prev + 1; k <= cur; ++k) off[k] = static_cast<int32_t>(e);
+    }
 }
-__global__ void k_tc_heavy_fill(const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int32_t n,
-                                int32_t* __restrict__ heavy) {
-    for (int64_t u = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < n;
-         u += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        if (flag[u]) heavy[pos[u]] = static_cast<int32_t>(u);
+
+// Tasks per vertex: none unless v has out- and in-entries (a triangle needs
+// w in N(v) and u with v in N(u)); then its in-list in runs of kInWave
+// (table fits a wave) or kInBlock.
+__global__ void k_tc_task_count(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ in_rowptr, int32_t n,
+                                int32_t* __restrict__ nw, int32_t* __restrict__ nb) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t d = rowptr[v + 1] - rowptr[v], di = in_rowptr[v + 1] - in_rowptr[v];
+        const bool any = d > 0 && di > 0;
+        nw[v] = any && d <= kWaveMax ? (di + kInWave - 1) / kInWave : 0;
+        nb[v] = any && d > kWaveMax ? (di + kInBlock - 1) / kInBlock : 0;
+    }
+}
+__global__ void k_tc_task_fill(const int32_t* __restrict__ nw, const int32_t* __restrict__ ow,
+                               const int32_t* __restrict__ nb, const int32_t* __restrict__ ob, int32_t n,
+                               TcTask* __restrict__ wt, TcTask* __restrict__ bt) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        for (int32_t k = 0; k < nw[v]; ++k) wt[ow[v] + k] = TcTask{static_cast<int32_t>(v), k * kInWave};
+        for (int32_t k = 0; k < nb[v]; ++k) bt[ob[v] + k] = TcTask{static_cast<int32_t>(v), k * kInBlock};
+    }
 }
 
 // Intersection bytes of the oriented CSR: over its edges (u, v), 4 bytes per
@@ -375,51 +426,82 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
         L.dag = {n, m_dag, drp, dcol, nullptr};
     }
     {
-        // rows for the block kernel, and the wave kernel's task counter
+        // transpose of the counted CSR (its in-lists, sources ascending) and
+        // the task lists of both kernels
         const DevCSR& X = L.symmetric ? L.dag : g->out;
-        int32_t *flag, *pos, *aux;
-        GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); flag = static_cast<int32_t*>(p);
-        GHET_CHECK(g->ws.get(WS_TC_COUNT, 4 * (size_t(n) + 1), &p)); pos = static_cast<int32_t*>(p);
-        GHET_CHECK(g->ws.get(WS_SCAN_AUX, 4 * size_t(scan_aux_ints(n)), &p)); aux = static_cast<int32_t*>(p);
-        const unsigned gn = grid_cap(std::min<int64_t>(ceil_div(n, 256), 4096));
-        hipLaunchKernelGGL(k_tc_heavy_flags, dim3(gn), dim3(256), 0, s, X.rowptr, n, flag);
-        GHET_CHECK(exclusive_scan_i32(flag, pos, n, aux, s));
-        GHET_HIP(hipMemcpyAsync(g->pinned, pos + n, 4, hipMemcpyDeviceToHost, s));
-        GHET_HIP(hipStreamSynchronize(s));
-        L.num_heavy = reinterpret_cast<const int32_t*>(g->pinned)[0];
-        GHET_HIP(hipMalloc(&L.heavy_mem, 256 + 4 * size_t(std::max(L.num_heavy, 1))));
-        L.task_ctr = static_cast<int*>(L.heavy_mem);
-        L.heavy = reinterpret_cast<int32_t*>(static_cast<char*>(L.heavy_mem) + 256);
-        hipLaunchKernelGGL(k_tc_heavy_fill, dim3(gn), dim3(256), 0, s, flag, pos, n, L.heavy);
-        GHET_HIP(hipGetLastError());
-        if (L.num_heavy > 1) {
-            // longest rows first: the block kernel's last blocks are the short ones
-            const int32_t h = L.num_heavy;
-            uint32_t *ka = nullptr, *kb = nullptr;
-            int32_t* vb = nullptr;
-            void* tmp = nullptr;
-            size_t tb = 0;
-            hipError_t e = hipMalloc(&ka, 4 * size_t(h));
-            if (e == hipSuccess) e = hipMalloc(&kb, 4 * size_t(h));
-            if (e == hipSuccess) e = hipMalloc(&vb, 4 * size_t(h));
-            if (e == hipSuccess) {
-                hipLaunchKernelGGL(k_tc_heavy_keys, dim3(grid_cap(ceil_div(h, 256))), dim3(256), 0, s, X.rowptr, L.heavy, h, ka);
-                e = hipGetLastError();
+        const int64_t mx = X.m;
+        if (mx >= INT32_MAX) return set_error(GHET_ERR_UNSUPPORTED, "TC: %lld edges", static_cast<long long>(mx));
+        int32_t *ka = nullptr, *kb = nullptr, *va = nullptr, *vb = nullptr, *nw = nullptr, *nbk = nullptr, *ow = nullptr,
+                *ob = nullptr, *aux = nullptr;
+        void* tmp = nullptr;
+        size_t tb = 0;
+        hipError_t e = hipSuccess;
+        auto get = [&](int32_t** q, size_t cnt) {
+            if (e == hipSuccess) e = hipMalloc(q, std::max<size_t>(4 * cnt, 256));
+        };
+        get(&ka, size_t(mx));
+        get(&kb, size_t(mx));
+        get(&va, size_t(mx));
+        get(&vb, size_t(mx));
+        get(&nw, size_t(n));
+        get(&nbk, size_t(n));
+        get(&ow, size_t(n) + 1);
+        get(&ob, size_t(n) + 1);
+        get(&aux, size_t(scan_aux_ints(n)));
+        const size_t in_b = ((4 * (size_t(n) + 1) + 255) & ~size_t(255)) + ((4 * size_t(mx) + 255) & ~size_t(255));
+        if (e == hipSuccess) e = hipMalloc(&L.in_mem, in_b + 256);
+        if (e == hipSuccess) {
+            L.in_rowptr = static_cast<int32_t*>(L.in_mem);
+            L.in_col = reinterpret_cast<int32_t*>(static_cast<char*>(L.in_mem) + ((4 * (size_t(n) + 1) + 255) & ~size_t(255)));
+            L.task_ctr = reinterpret_cast<int*>(static_cast<char*>(L.in_mem) + in_b);
+            if (mx > 0) {
+                hipLaunchKernelGGL(k_tc_sources, dim3(grid_cap(ceil_div(n, kBT / 64))), dim3(kBT), 0, s, X.rowptr, n, va);
+                e = hipMemcpyAsync(ka, X.col, 4 * size_t(mx), hipMemcpyDeviceToDevice, s);
             }
-            rocprim::double_buffer<uint32_t> kd(ka, kb);
-            rocprim::double_buffer<int32_t> vd(L.heavy, vb);
-            if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, tb, kd, vd, static_cast<unsigned>(h), 0, 32, s);
-            if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tb, 256));
-            if (e == hipSuccess) e = rocprim::radix_sort_pairs(tmp, tb, kd, vd, static_cast<unsigned>(h), 0, 32, s);
-            if (e == hipSuccess && vd.current() != L.heavy)
-                e = hipMemcpyAsync(L.heavy, vd.current(), 4 * size_t(h), hipMemcpyDeviceToDevice, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(vb), tmp})
-                if (q) hipFree(q);
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
-                return set_error(GHET_ERR_HIP, "TC heavy rows: %s", hipGetErrorString(e));
-            }
+        }
+        unsigned bits = 1;
+        while ((int64_t(1) << bits) < n) ++bits;
+        rocprim::double_buffer<int32_t> kd(ka, kb), vd(va, vb);
+        if (e == hipSuccess && mx > 0) e = rocprim::radix_sort_pairs(nullptr, tb, kd, vd, static_cast<unsigned>(mx), 0, bits, s);
+        if (e == hipSuccess && mx > 0) e = hipMalloc(&tmp, std::max<size_t>(tb, 256));
+        if (e == hipSuccess && mx > 0) e = rocprim::radix_sort_pairs(tmp, tb, kd, vd, static_cast<unsigned>(mx), 0, bits, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_tc_key_offsets, dim3(grid_cap(std::min<int64_t>(ceil_div(mx + 1, 256), 65535))), dim3(256), 0,
+                               s, kd.current(), mx, n, L.in_rowptr);
+            if (mx > 0) e = hipMemcpyAsync(L.in_col, vd.current(), 4 * size_t(mx), hipMemcpyDeviceToDevice, s);
+        }
+        if (e == hipSuccess) {
+            const unsigned gn = grid_cap(std::min<int64_t>(ceil_div(n, 256), 4096));
+            hipLaunchKernelGGL(k_tc_task_count, dim3(gn), dim3(256), 0, s, X.rowptr, L.in_rowptr, n, nw, nbk);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && exclusive_scan_i32(nw, ow, n, aux, s) != GHET_OK) e = hipErrorUnknown;
+        if (e == hipSuccess && exclusive_scan_i32(nbk, ob, n, aux, s) != GHET_OK) e = hipErrorUnknown;
+        int32_t totals[2] = {0, 0};
+        if (e == hipSuccess) e = hipMemcpyAsync(totals, ow + n, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(totals + 1, ob + n, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess) {
+            L.num_wave_tasks = totals[0];
+            L.num_block_tasks = totals[1];
+            e = hipMalloc(&L.task_mem, sizeof(TcTask) * (size_t(totals[0]) + size_t(totals[1])) + 256);
+        }
+        if (e == hipSuccess) {
+            L.wave_tasks = L.task_mem;
+            L.block_tasks = static_cast<TcTask*>(L.task_mem) + totals[0];
+            const unsigned gn = grid_cap(std::min<int64_t>(ceil_div(n, 256), 4096));
+            hipLaunchKernelGGL(k_tc_task_fill, dim3(gn), dim3(256), 0, s, nw, ow, nbk, ob, n,
+                               static_cast<TcTask*>(L.wave_tasks), static_cast<TcTask*>(L.block_tasks));
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(va), static_cast<void*>(vb),
+                        static_cast<void*>(nw), static_cast<void*>(nbk), static_cast<void*>(ow), static_cast<void*>(ob),
+                        static_cast<void*>(aux), tmp})
+            if (q) hipFree(q);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(GHET_ERR_HIP, "TC layout: %s", hipGetErrorString(e));
         }
     }
     {
@@ -445,8 +527,8 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
 int tc_prepare(ghet_graph* g, hipStream_t s) { return build_tc_layout(g, s); }
 
 // Enqueues the counting (fill, count, 6x scale) with no host synchronisation,
-// so it can be captured into a hipGraph.  vpt: rows per wave task of
-// k_tc_wave; waves_per_cu: its grid size in resident waves per CU.
+// so it can be captured into a hipGraph.  vpt: tasks per grab of k_tc_wave;
+// waves_per_cu: its grid size in resident waves per CU.
 int enqueue_tc(ghet_graph* g, unsigned long long* cnt, int vpt, int waves_per_cu, hipStream_t s) {
     const int32_t n = g->n;
     const TCLayout& L = g->tc;
@@ -455,14 +537,18 @@ int enqueue_tc(ghet_graph* g, unsigned long long* cnt, int vpt, int waves_per_cu
     if (X.m > 0) {
         int cus = 0;
         GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
-        if (L.num_heavy > 0)
-            hipLaunchKernelGGL(k_tc_block, dim3(static_cast<unsigned>(std::min(L.num_heavy, 65535))), dim3(kBlockBT), 0, s,
-                               X.rowptr, X.col, L.heavy, L.num_heavy, cnt);
-        GHET_CHECK(fill<int32_t>(L.task_ctr, 0, 1, s));
-        const int64_t tasks = ceil_div(n, std::max(vpt, 1));
-        const int64_t cap = int64_t(std::max(cus, 1)) * waves_per_cu / (kBT / 64);
-        hipLaunchKernelGGL(k_tc_wave, dim3(grid_cap(std::min<int64_t>(ceil_div(tasks, kBT / 64), cap))), dim3(kBT), 0,
-                           s, X.rowptr, X.col, n, std::max(vpt, 1), L.symmetric ? 2 : 1, L.task_ctr, cnt);
+        if (L.num_block_tasks > 0)
+            hipLaunchKernelGGL(k_tc_block, dim3(static_cast<unsigned>(std::min(L.num_block_tasks, 65535))), dim3(kBlockBT),
+                               0, s, X.rowptr, X.col, L.in_rowptr, L.in_col, static_cast<const TcTask*>(L.block_tasks),
+                               L.num_block_tasks, cnt);
+        if (L.num_wave_tasks > 0) {
+            GHET_CHECK(fill<int32_t>(L.task_ctr, 0, 1, s));
+            const int64_t grabs = ceil_div(L.num_wave_tasks, std::max(vpt, 1));
+            const int64_t cap = int64_t(std::max(cus, 1)) * waves_per_cu / (kBT / 64);
+            hipLaunchKernelGGL(k_tc_wave, dim3(grid_cap(std::min<int64_t>(ceil_div(grabs, kBT / 64), cap))), dim3(kBT), 0,
+                               s, X.rowptr, X.col, L.in_rowptr, L.in_col, static_cast<const TcTask*>(L.wave_tasks),
+                               L.num_wave_tasks, std::max(vpt, 1), L.task_ctr, cnt);
+        }
     }
     if (L.symmetric)
         hipLaunchKernelGGL(k_scale_i64, dim3(grid_cap(std::min<int64_t>(ceil_div(n, 256), 8192))), dim3(256), 0, s, cnt,
