@@ -49,7 +49,6 @@ constexpr int kWaveMax = 256;                // rows up to this length: k_tc_wav
 constexpr int kWaveSlots = 2 * kWaveMax;     // table slots per wave
 constexpr int kBlockBT = 1024;               // k_tc_block threads
 constexpr int kBlockSlots = 8192;            // k_tc_block table slots (keys + counters: 64 KB, two blocks per CU)
-constexpr int kTcLight = 16;                 // streamed lists up to this length are probed by one lane
 constexpr int kInWave = 256;                 // in-list entries per wave task
 constexpr int kInBlock = 4096;               // in-list entries per block task
 
@@ -143,56 +142,66 @@ __device__ __forceinline__ int tc_find(const int32_t* keys, uint32_t mask, int s
     }
 }
 
-// One v per lane (dv = 0: none).  Every w of N(v) found in the table adds 1
-// to its slot's counter; returns the lane's own v's number of hits.  Lists
-// up to kTcLight entries are probed by their lane (loads issued first),
-// longer ones by the whole wave.
-__device__ __forceinline__ uint32_t tc_probe(int32_t vb, int32_t dv, const int32_t* __restrict__ col,
-                                             const int32_t* keys, uint32_t* cnts, uint32_t mask, int shift) {
+// Per-wave LDS of tc_stream: list starts and offsets of the batch, and
+// each list's hits.
+struct TcBatch {
+    int32_t pre[65];   // exclusive prefix of the list lengths; pre[64] = total
+    int32_t base[64];  // first entry of each list in col
+    uint32_t hits[64];
+};
+
+// One list per lane (len 0: none), concatenated: the wave walks entries
+// k = 0 .. total-1 of the concatenation four per lane at a time, so
+// neighbouring lanes load neighbouring entries of a list (coalesced) and
+// four loads are in flight per lane.  Each entry found in the table adds 1
+// to its slot's counter and to its list's hits; returns the lane's own
+// list's hits.
+__device__ __forceinline__ uint32_t tc_stream(int32_t lb, int32_t len, const int32_t* __restrict__ col,
+                                              const int32_t* keys, uint32_t* cnts, uint32_t mask, int shift,
+                                              TcBatch& bt) {
     const int lane = lane_id();
-    uint32_t c = 0;
-    if (dv <= kTcLight) {
-        int32_t w[kTcLight];
+    int32_t incl = len;
 #pragma unroll
-        for (int i = 0; i < kTcLight; ++i) w[i] = i < dv ? col[vb + i] : -1;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    const int32_t total = __shfl(incl, 63);
+    if (total == 0) return 0;
+    bt.pre[lane] = incl - len;
+    bt.base[lane] = lb;
+    bt.hits[lane] = 0;
+    if (lane == 0) bt.pre[64] = total;
+    wave_lds_sync();
+    int owner = 0;  // list of entry k: the last i with pre[i] <= k (k only grows)
+    for (int32_t k0 = 0; k0 < total; k0 += 4 * 64) {
+        int32_t w[4];
+        int own[4];
 #pragma unroll
-        for (int i = 0; i < kTcLight; ++i) {
-            if (i < dv) {
-                const int sl = tc_find(keys, mask, shift, w[i]);
+        for (int j = 0; j < 4; ++j) {
+            const int32_t k = k0 + j * 64 + lane;
+            w[j] = -1;
+            own[j] = 0;
+            if (k < total) {
+                while (bt.pre[owner + 1] <= k) ++owner;
+                own[j] = owner;
+                w[j] = col[bt.base[owner] + (k - bt.pre[owner])];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (w[j] >= 0) {
+                const int sl = tc_find(keys, mask, shift, w[j]);
                 if (sl >= 0) {
                     atomicAdd(&cnts[sl], 1u);
-                    ++c;
+                    atomicAdd(&bt.hits[own[j]], 1u);
                 }
             }
         }
     }
-    uint64_t heavy = __ballot(dv > kTcLight);
-    while (heavy) {
-        const int src = __ffsll(static_cast<unsigned long long>(heavy)) - 1;
-        heavy &= heavy - 1;
-        const int32_t hb = __shfl(vb, src), hd = __shfl(dv, src);
-        uint32_t hc = 0;
-        for (int32_t i0 = 0; i0 < hd; i0 += 4 * 64) {  // four loads in flight per lane
-            int32_t w[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int32_t i = i0 + k * 64 + lane;
-                w[k] = i < hd ? col[hb + i] : -1;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (w[k] >= 0) {
-                    const int sl = tc_find(keys, mask, shift, w[k]);
-                    if (sl >= 0) {
-                        atomicAdd(&cnts[sl], 1u);
-                        ++hc;
-                    }
-                }
-            }
-        }
-        hc = wave_sum(hc);
-        if (lane == src) c = hc;
-    }
+    wave_lds_sync();
+    const uint32_t c = bt.hits[lane];
+    wave_lds_sync();  // the next batch rewrites the arrays
     return c;
 }
 
@@ -212,6 +221,7 @@ __global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ row
                                                  int32_t num_tasks, int vpt, int* task_ctr, unsigned long long* cnt) {
     __shared__ int32_t s_keys[kBT / 64][kWaveSlots];
     __shared__ uint32_t s_cnts[kBT / 64][kWaveSlots];
+    __shared__ TcBatch s_batch[kBT / 64];
     const int lane = lane_id(), wave = threadIdx.x >> 6;
     int32_t* keys = s_keys[wave];
     uint32_t* cnts = s_cnts[wave];
@@ -246,7 +256,7 @@ __global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ row
                     ub = rowptr[u];
                     du = rowptr[u + 1] - ub;
                 }
-                const uint32_t c = tc_probe(ub, du, col, keys, cnts, mask, shift);
+                const uint32_t c = tc_stream(ub, du, col, keys, cnts, mask, shift, s_batch[wave]);
                 if (c) {
                     atomicAdd(&cnt[u], static_cast<unsigned long long>(c));
                     tot += c;
@@ -275,6 +285,7 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
                                                           unsigned long long* cnt) {
     __shared__ int32_t keys[kBlockSlots];
     __shared__ uint32_t cnts[kBlockSlots];
+    __shared__ TcBatch s_batch[kBlockBT / 64];
     __shared__ unsigned long long s_red[kBlockBT / 64];
     const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     for (int32_t t = blockIdx.x; t < num_tasks; t += gridDim.x) {
@@ -305,7 +316,7 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
                     ub = rowptr[u];
                     du = rowptr[u + 1] - ub;
                 }
-                const uint32_t c = tc_probe(ub, du, col, keys, cnts, mask, shift);
+                const uint32_t c = tc_stream(ub, du, col, keys, cnts, mask, shift, s_batch[wave]);
                 if (c) {
                     atomicAdd(&cnt[u], static_cast<unsigned long long>(c));
                     tot += c;
