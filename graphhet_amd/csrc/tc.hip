@@ -34,9 +34,10 @@
 //   k_tc_block  tasks of longer v: one 1024-thread block each, a table of up
 //               to kBlockSlots slots filled kBlockSlots / 2 entries of N(v)
 //               at a time.
-// Tables are open-addressed (multiplicative hash, linear probing) and at most
-// half full, so every probe ends at the key or at an empty slot.  The lists
-// are those of a canonical CSR (no duplicate entries in a row).
+// Tables are 4-slot buckets (multiplicative hash, linear probing over
+// buckets), at most half full, so every probe ends at the key or at a bucket
+// with a free slot.  The lists are those of a canonical CSR (no duplicate
+// entries in a row).
 #include "graph_internal.hpp"
 
 #include <rocprim/rocprim.hpp>
@@ -127,18 +128,28 @@ __device__ __forceinline__ uint32_t tc_slot0(int32_t key, int shift) {
     return (static_cast<uint32_t>(key) * 2654435761u) >> shift;
 }
 
-__device__ __forceinline__ void tc_insert(int32_t* keys, uint32_t mask, int shift, int32_t key) {
-    uint32_t s = tc_slot0(key, shift);
-    while (atomicCAS(&keys[s], -1, key) != -1) s = (s + 1) & mask;
+// Tables are arrays of 4-slot buckets (one 128-bit LDS read each).  An
+// insert takes the first free slot of its home bucket, else of the next
+// bucket, so a bucket's slots fill in order: a lookup that meets a bucket
+// with a free last slot has seen every key that could be there.
+__device__ __forceinline__ void tc_insert(int32_t* keys, uint32_t bmask, int shift, int32_t key) {
+    uint32_t b = tc_slot0(key, shift);
+    for (;;) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (atomicCAS(&keys[4 * b + j], -1, key) == -1) return;
+        b = (b + 1) & bmask;
+    }
 }
 
-__device__ __forceinline__ int tc_find(const int32_t* keys, uint32_t mask, int shift, int32_t key) {
-    uint32_t s = tc_slot0(key, shift);
+__device__ __forceinline__ int tc_find(const int32_t* keys, uint32_t bmask, int shift, int32_t key) {
+    uint32_t b = tc_slot0(key, shift);
     for (;;) {
-        const int32_t k = keys[s];
-        if (k == key) return static_cast<int>(s);
-        if (k < 0) return -1;
-        s = (s + 1) & mask;
+        const int4 q = reinterpret_cast<const int4*>(keys)[b];
+        const int r = q.x == key ? 0 : q.y == key ? 1 : q.z == key ? 2 : q.w == key ? 3 : -1;
+        if (r >= 0) return static_cast<int>(4 * b) + r;
+        if (q.w < 0) return -1;
+        b = (b + 1) & bmask;
     }
 }
 
@@ -219,7 +230,7 @@ __global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ row
                                                  const int32_t* __restrict__ in_rowptr,
                                                  const int32_t* __restrict__ in_col, const TcTask* __restrict__ tasks,
                                                  int32_t num_tasks, int vpt, int* task_ctr, unsigned long long* cnt) {
-    __shared__ int32_t s_keys[kBT / 64][kWaveSlots];
+    __shared__ __align__(16) int32_t s_keys[kBT / 64][kWaveSlots];
     __shared__ uint32_t s_cnts[kBT / 64][kWaveSlots];
     __shared__ TcBatch s_batch[kBT / 64];
     const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -236,10 +247,10 @@ __global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ row
             const int32_t v = task.v, b = rowptr[v], d = rowptr[v + 1] - b;
             const int32_t j0 = in_rowptr[v] + task.first;
             const int32_t j1 = min(in_rowptr[v + 1], j0 + kInWave);
-            int tbits = 6;
-            while ((1 << tbits) < 2 * d) ++tbits;
-            const uint32_t T = 1u << tbits, mask = T - 1;
-            const int shift = 32 - tbits;
+            int tbits = 6;  // 4 d slots (2 d at most for the longest rows)
+            while ((1 << tbits) < 4 * d && (1 << tbits) < kWaveSlots) ++tbits;
+            const uint32_t T = 1u << tbits, mask = (T >> 2) - 1;
+            const int shift = 32 - (tbits - 2);
             for (uint32_t sl = lane; sl < T; sl += 64) {
                 keys[sl] = -1;
                 cnts[sl] = 0;
@@ -283,7 +294,7 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
                                                           const int32_t* __restrict__ in_col,
                                                           const TcTask* __restrict__ tasks, int32_t num_tasks,
                                                           unsigned long long* cnt) {
-    __shared__ int32_t keys[kBlockSlots];
+    __shared__ __align__(16) int32_t keys[kBlockSlots];
     __shared__ uint32_t cnts[kBlockSlots];
     __shared__ TcBatch s_batch[kBlockBT / 64];
     __shared__ unsigned long long s_red[kBlockBT / 64];
@@ -293,10 +304,10 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
         const int32_t v = task.v, b = rowptr[v], d = rowptr[v + 1] - b;
         const int32_t j0 = in_rowptr[v] + task.first;
         const int32_t j1 = min(in_rowptr[v + 1], j0 + kInBlock);
-        int tbits = 6;
-        while ((1 << tbits) < 2 * d && (1 << tbits) < kBlockSlots) ++tbits;
-        const uint32_t T = 1u << tbits, mask = T - 1;
-        const int shift = 32 - tbits;
+        int tbits = 6;  // 4 d slots, at most kBlockSlots, filled to half at most
+        while ((1 << tbits) < 4 * d && (1 << tbits) < kBlockSlots) ++tbits;
+        const uint32_t T = 1u << tbits, mask = (T >> 2) - 1;
+        const int shift = 32 - (tbits - 2);
         const int32_t cap = static_cast<int32_t>(T / 2);
         unsigned long long tot = 0;
         for (int32_t c0 = 0; c0 < d; c0 += cap) {
