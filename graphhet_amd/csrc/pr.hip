@@ -1067,7 +1067,7 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t rows_all = static_cast<size_t>(n) + S;
     const size_t bytes = al(sizeof(int4) * size_t(num_long_rows)) + 2 * al(4 * size_t(n)) + al(4 * (rows_all + 1)) +
-                         al(4 * std::max<size_t>(rows_all, 1)) + al(4 * size_t(m)) + 256;
+                         al(4 * std::max<size_t>(rows_all, 1)) + al(4 * std::max<size_t>(size_t(m), 1)) + 256;
     GHET_HIP(hipMalloc(&L.mem, bytes));
     char* p = static_cast<char*>(L.mem);
     L.long_rows = p;
@@ -1077,7 +1077,10 @@ int build_pr_layout(ghet_graph* g, hipStream_t s, int32_t split_code) {
     L.perm = take(size_t(n));
     L.rowptr = take(rows_all + 1);
     L.slot = take(std::max<size_t>(rows_all, 1));
-    L.col = take(size_t(m));
+    L.col = take(std::max<size_t>(size_t(m), 1));
+    // k_pr_iter's masked lanes gather contrib[col[0]]: col[0] must be a
+    // vertex id, also when there are no edges
+    if (m == 0) GHET_CHECK(fill<int32_t>(L.col, 0, 1, s));
     hip_ok(hipMemcpyAsync(L.outdeg, outdeg, 4 * size_t(n), hipMemcpyDeviceToDevice, s));
     hip_ok(hipMemcpyAsync(L.perm, perm, 4 * size_t(n), hipMemcpyDeviceToDevice, s));
     scan(hot_len, L.rowptr, n);  // hot parts' offsets; L.rowptr[n] = hot edges
