@@ -4,12 +4,12 @@ One "step" is one pass of the metric's algorithms over the resident graph:
 BFS and SSSP from the max-out-degree vertex and PageRank (20 iterations,
 damping 0.85).  The graph is built and uploaded before the timed region;
 inputs are resident in HBM when it starts and results stay on the device.
-BFS uses the trained mode network (graphhet_amd/sched_net.txt, trained on
-scales 18-22, so this graph is held out) when it loads, else the heuristic;
-SSSP uses the heuristic bucket width (the width network does not transfer to
-scale 24; the `learned` section reports both networks against the
-heuristic).  The heuristic BFS is timed beside the step on the same source
-and reported under `plain_heuristic`.
+BFS and SSSP use the trained schedule networks (graphhet_amd/sched_net.txt:
+BFS mode per level, SSSP bucket width per phase; trained on R-MAT scales
+18-23, so this graph is held out) when they load, else the heuristic
+(`--step-schedule heuristic` forces it).  The heuristic forms are timed
+beside the step on the same source and reported under `plain_heuristic`;
+the `learned` section compares the two over nine sources.
 
 Prints ONE JSON line.  `value` is BFS GTEPS (m_r / t, m_r = out-edges of the
 reached vertices), the headline of the metric; SSSP GTEPS and PageRank HBM
@@ -110,8 +110,9 @@ def main():
                     help="largest small graph is 2^this vertices (vertex counts log-uniform in [2^10, 2^this])")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-heuristic section")
-    ap.add_argument("--step-schedule", choices=("heuristic", "learned"), default="heuristic",
-                    help="BFS schedule of the timed step (SSSP always uses the heuristic width)")
+    ap.add_argument("--step-schedule", choices=("heuristic", "learned"), default="learned",
+                    help="schedule of the timed step's BFS and SSSP (learned falls back to the heuristic "
+                         "when no network loads)")
     ap.add_argument("--sched-file", default=os.path.join(ROOT, "graphhet_amd", "sched_net.txt"),
                     help="trained schedule networks (python -m graphhet_amd.learn)")
     ap.add_argument("--gen", choices=("device", "host"), default="device",
@@ -143,21 +144,20 @@ def main():
     log(f"R-MAT scale {args.scale}: n={n} m={m} on the GPU in {time.time() - t0:.1f}s "
         f"({args.gen} generation, {dg.device_bytes() / 1e9:.2f} GB)")
     src = int(np.argmax(o.degrees()))
-    # The step's BFS uses the heuristic mode rule unless --step-schedule
-    # learned: on this graph the trained mode network (R-MAT scales 18-22,
-    # this graph held out) is within a few percent of the heuristic, and the
-    # learned section below compares the two on the same sources.  SSSP keeps
-    # the heuristic bucket width (the width network does not transfer from
-    # scales 18-22 to 24).  Outputs are identical whichever schedule runs.
+    # The step's BFS and SSSP run the learned schedule when the networks
+    # load (trained on R-MAT scales 18-23; this graph is held out), and the
+    # heuristic otherwise or with --step-schedule heuristic.  Outputs are
+    # identical whichever schedule runs; the heuristic forms are timed beside
+    # the step (plain_heuristic) and over nine sources (learned section).
     have_net = not args.no_learned and G.sched_load(args.sched_file)
     sched = G.SCHED_LEARNED if (args.step_schedule == "learned" and have_net) else G.SCHED_HEURISTIC
-    sched_name = "learned BFS modes, heuristic SSSP width" if sched == G.SCHED_LEARNED else "heuristic"
+    sched_name = "learned BFS modes and SSSP widths" if sched == G.SCHED_LEARNED else "heuristic"
 
     def run(alg):
         if alg == "bfs":
             return dg.bfs(src, schedule=sched, out=False)[1]
         if alg == "sssp":
-            return dg.sssp(src, schedule=G.SCHED_HEURISTIC, out=False)[1]
+            return dg.sssp(src, schedule=sched, out=False)[1]
         return dg.pr(args.pr_iters, 0.85, out=False)[1]
 
     def step():
@@ -217,7 +217,7 @@ def main():
 
     plain = {}
     if sched != G.SCHED_HEURISTIC:
-        for alg, fn in (("bfs", dg.bfs),):
+        for alg, fn in (("bfs", dg.bfs), ("sssp", dg.sssp)):
             if alg in algos:
                 sts = [fn(src, schedule=G.SCHED_HEURISTIC, out=False)[1] for _ in range(3)]
                 ms = statistics.median(x.device_ms for x in sts)
@@ -360,7 +360,7 @@ def make_graph(G, args, scale, edge_factor, with_all):
 def learned_section(G, dg, o, src):
     """Learned schedule vs the heuristic (and GraphHET's n/100 rule) on this
     graph, which the networks never saw (they are trained on R-MAT graphs of
-    scales 18-22).  Outputs are compared bit for bit."""
+    scales 18-23).  Outputs are compared bit for bit."""
     from graphhet_amd import learn
     srcs = np.concatenate([[src], learn.pick_sources(o, 8, 24)]).astype(np.int32)
     learn.compare_bfs(dg, srcs[:2])  # warm-up

@@ -45,7 +45,8 @@ class CsrView(C.Structure):
 
 
 class LevelTrace(C.Structure):
-    _fields_ = [("features", C.c_float * TRACE_FEATURES), ("mode", C.c_int32), ("ms", C.c_float)]
+    _fields_ = [("features", C.c_float * TRACE_FEATURES), ("mode", C.c_int32), ("ms", C.c_float),
+                ("threshold", C.c_double)]
 
 
 class Opts(C.Structure):

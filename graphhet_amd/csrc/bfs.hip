@@ -679,6 +679,7 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         const bool tr = trace && trace_len < trace_cap;
         if (tr) {
             sched_features(li, trace[trace_len].features);
+            trace[trace_len].threshold = 0;
             GHET_HIP(hipEventRecord(g->ev_k0, s));
         }
         const int32_t next_level = level + 1;

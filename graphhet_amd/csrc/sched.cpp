@@ -91,12 +91,31 @@ void mlp_transpose(const float* w, float* wt) {
 
 static inline float lg(double x) { return static_cast<float>(std::log2(1.0 + std::max(0.0, x))); }
 
+// log2(1 + x) / log2(1 + total): where x sits between 1 and the graph's own
+// size, the same fraction at every scale.
+static inline float rel_lg(double x, double total) {
+    const double d = std::log2(1.0 + std::max(1.0, total));
+    return static_cast<float>(std::log2(1.0 + std::max(0.0, x)) / d);
+}
+
+int heuristic_bfs(const LevelInfo& li);
+
+// Sizes enter as fractions of n or m, or as logs relative to log n or log
+// m, so that a network trained on small graphs sees the same ranges on large
+// ones; slot 15 alone carries the scale.  The scale is needed: a bottom-up
+// sweep of the tail costs about 2.3 ps per vertex on MI355X and a top-down
+// tail level about 13 us, so which is cheaper flips near 2^22.4 vertices.
+// Slot 15 is tanh((log2 n - 22.4) / 2): centred on that crossover and
+// saturating, so graphs beyond the training scales fall on the side of the
+// largest ones instead of extrapolating a linear input.  For BFS, slot 13
+// carries the heuristic rule's own pick at this level (push 0, bitmap 0.5,
+// pull 1).
 void sched_features(const LevelInfo& li, float* f) {
     const double n = std::max<int64_t>(li.n, 1), m = std::max<int64_t>(li.m, 1);
-    f[0] = lg(li.frontier) / 32.f;
-    f[1] = lg(li.frontier_edges) / 32.f;
-    f[2] = lg(li.unvisited) / 32.f;
-    f[3] = lg(li.unvisited_in_edges) / 32.f;
+    f[0] = rel_lg(li.frontier, n);
+    f[1] = rel_lg(li.frontier_edges, m);
+    f[2] = rel_lg(li.unvisited, n);
+    f[3] = rel_lg(li.unvisited_in_edges, std::max<double>(li.m_in, 1));
     f[4] = static_cast<float>(li.frontier / n);
     f[5] = static_cast<float>(li.frontier_edges / m);
     f[6] = static_cast<float>(li.unvisited / n);
@@ -106,9 +125,23 @@ void sched_features(const LevelInfo& li, float* f) {
     f[10] = li.prev_mode == GHET_MODE_PULL ? 1.f : 0.f;
     f[11] = li.prev_mode == GHET_MODE_BITMAP ? 1.f : 0.f;
     f[12] = static_cast<float>(std::tanh(std::log2((1.0 + li.frontier) / (1.0 + li.prev_frontier)) / 4.0));
-    f[13] = lg(li.n) / 32.f;
     f[14] = static_cast<float>(std::log2(1.0 + m / n) / 8.0);
-    f[15] = li.delta > 0 ? static_cast<float>(std::tanh(li.far_size / n * 8.0)) : 1.f;
+    if (li.delta > 0) {
+        // SSSP decision: frontier = the bucket just finished (vertices and
+        // relaxed edges), unvisited = vertices not yet final, far_size = the
+        // far list; prev_mode = the previous width class
+        f[3] = rel_lg(li.far_size, n);
+        f[7] = static_cast<float>(li.far_size / n);
+        f[9] = li.prev_mode == 0 ? 1.f : 0.f;
+        f[10] = li.prev_mode == 1 ? 1.f : 0.f;
+        f[11] = li.prev_mode == 2 ? 1.f : 0.f;
+        f[12] = li.prev_mode == 3 ? 1.f : 0.f;
+        f[13] = static_cast<float>(std::log2(1.0 + li.threshold / li.delta) / 16.0);
+    } else {
+        const int h = heuristic_bfs(li);
+        f[13] = h == GHET_MODE_PULL ? 1.f : (h == GHET_MODE_BITMAP ? 0.5f : 0.f);
+    }
+    f[15] = static_cast<float>(std::tanh((std::log2(n) - 22.4) / 2.0));
 }
 
 int heuristic_bfs(const LevelInfo& li) {
@@ -152,28 +185,20 @@ int sched_pick_bfs(SchedState* s, const LevelInfo& li) {
     }
 }
 
-// The SSSP network sees only scale-free features: the absolute sizes (log
-// frontier, frontier edges, unvisited, unvisited in-edges, frontier / n, log
-// n) are zeroed, so that a network trained on small graphs transfers to large
-// ones; the base width's size term carries the scale.  The trainer applies
-// the same mask (graphhet_amd/learn.py, SSSP_MASKED).
-void sssp_mask_features(float* f) {
-    for (int k : {0, 1, 2, 3, 4, 13}) f[k] = 0.f;
-}
-
+// The SSSP network picks the width class of every bucket from the state at
+// the threshold advance (all features scale-free).
 int sched_pick_sssp(SchedState* s, const LevelInfo& li) {
     if (s->schedule == GHET_SCHED_LEARNED && s->use_net) {
-        if (s->sssp_cls < 0) {
-            float x[GHET_TRACE_FEATURES], y[GHET_MLP_OUT];
-            sched_features(li, x);
-            sssp_mask_features(x);
-            net_forward_host(GHET_NET_SSSP, x, y);
-            int best = 0;
-            for (int k = 1; k < GHET_MLP_OUT; ++k)
-                if (y[k] > y[best]) best = k;
-            s->sssp_cls = best;
-        }
-        return s->sssp_cls;
+        LevelInfo x = li;
+        x.prev_mode = s->sssp_prev;
+        float f[GHET_TRACE_FEATURES], y[GHET_MLP_OUT];
+        sched_features(x, f);
+        net_forward_host(GHET_NET_SSSP, f, y);
+        int best = 0;
+        for (int k = 1; k < GHET_MLP_OUT; ++k)
+            if (y[k] > y[best]) best = k;
+        s->sssp_prev = best;
+        return best;
     }
     return 1;  // the base width
 }

@@ -31,7 +31,7 @@ struct SchedState {
     bool use_net = false;   // learned network installed and requested
     bool fallback = false;  // learned requested but unavailable
     int64_t n = 0, m = 0, m_in = 0;
-    int sssp_cls = -1;      // learned SSSP class, picked once per run
+    int sssp_prev = 1;      // SSSP class picked at the previous decision
 };
 
 void sched_begin(SchedState* s, int schedule, int kind, int64_t n, int64_t m, int64_t m_in);

@@ -723,15 +723,30 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     int32_t far_count = 0;
     double threshold = 0.0;
     double delta = base_delta;
+    // Bucket-width decision: before the first bucket and at every threshold
+    // advance.  A trace record keeps the features, the class, the threshold
+    // and the device time so far (one event sync per decision, only when
+    // tracing).
+    auto decide = [&](LevelInfo& li) -> int {
+        li.prev_mode = sched.sssp_prev;
+        const int cls = sched_pick_sssp(&sched, li);
+        if (trace && trace_len < trace_cap) {
+            ghet_level_trace& t = trace[trace_len++];
+            sched_features(li, t.features);
+            t.mode = cls;
+            t.threshold = li.threshold;
+            float ms = 0;
+            if (hipEventRecord(g->ev_k1, s) == hipSuccess && hipEventSynchronize(g->ev_k1) == hipSuccess)
+                (void)hipEventElapsedTime(&ms, g->ev_begin, g->ev_k1);
+            t.ms = ms;
+        }
+        return cls;
+    };
     {
         LevelInfo li;
         li.n = n; li.m = g->out.m; li.m_in = g->out.m; li.level = 0; li.frontier = F; li.frontier_edges = E;
         li.unvisited = n - 1; li.delta = base_delta; li.threshold = 0; li.far_size = 0;
-        int cls = sched_pick_sssp(&sched, li);
-        if (trace && trace_len < trace_cap) {
-            sched_features(li, trace[trace_len].features);
-            trace[trace_len].mode = cls;
-        }
+        const int cls = decide(li);
         delta = base_delta * std::ldexp(1.0, cls - 1);
         threshold = delta;
     }
@@ -779,7 +794,6 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     int64_t rounds = 0, relaxed = 0;
     for (;;) {
         // relaxation rounds of the current bucket (light edges only)
-        if (trace && trace_len < trace_cap) GHET_HIP(hipEventRecord(g->ev_k0, s));
         while (F > 0) {
             const int32_t next_round = round + 1;
             if (E > 0) {
@@ -805,28 +819,16 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
             relaxed += static_cast<int64_t>(hm->v[3]);
             far_count = static_cast<int32_t>(hm->v[5 + sel]);
         }
-        if (trace && trace_len < trace_cap) {
-            GHET_HIP(hipEventRecord(g->ev_k1, s));
-            GHET_HIP(hipEventSynchronize(g->ev_k1));
-            float ms = 0;
-            GHET_HIP(hipEventElapsedTime(&ms, g->ev_k0, g->ev_k1));
-            trace[trace_len].ms = ms;
-            ++trace_len;
-        }
         if (far_count == 0) break;
         ++phases;
         // advance the threshold and split the far list
         double old_thr = threshold;
         {
             LevelInfo li;
-            li.n = n; li.m = g->out.m; li.m_in = g->out.m; li.level = phases; li.frontier = 0;
-            li.frontier_edges = 0; li.unvisited = 0; li.delta = base_delta; li.threshold = threshold;
-            li.far_size = far_count; li.prev_mode = 0;
-            int cls = sched_pick_sssp(&sched, li);
-            if (trace && trace_len < trace_cap) {
-                sched_features(li, trace[trace_len].features);
-                trace[trace_len].mode = cls;
-            }
+            li.n = n; li.m = g->out.m; li.m_in = g->out.m; li.level = phases; li.frontier = list_len;
+            li.frontier_edges = list_edges; li.unvisited = n - list_count; li.delta = base_delta;
+            li.threshold = threshold; li.far_size = far_count;
+            const int cls = decide(li);
             delta = base_delta * std::ldexp(1.0, cls - 1);
         }
         double new_thr = std::max(threshold + delta, std::nextafter(threshold, HUGE_VAL));

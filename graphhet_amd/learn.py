@@ -10,18 +10,24 @@ frontier representation).  The "previous mode" features of a sample are
 those of the label path (the mode the network would have picked one level
 earlier), level 0 starting from push as the library does.
 
-SSSP.  The bucket width changes which vertices settle in which phase, so
-runs under different widths are not level-aligned.  Each source is run at
-the four width classes (delta = base * 2^(class-1)); every phase record of
-the base-width run is labelled with the class whose whole run was fastest.
+SSSP.  The network picks the bucket width class (delta = base *
+2^(class-1)) at every threshold advance.  Runs under different widths are not
+phase-aligned, but they are threshold-aligned: after a bucket with threshold
+t every vertex closer than t is final, whatever widths came before.  Each
+source is run at the four classes; every decision of the base-width run is
+labelled with the class whose run crossed the distance window up to the
+next decision fastest.
 
 Outputs never depend on the networks: every schedule computes the same
 depths and distances, which the tests check bit for bit.
 
 Command line (runs on a GPU):
     python -m graphhet_amd.learn --out graphhet_amd/sched_net.txt
-trains both networks on R-MAT graphs of scale 18-22 with a in [0.45, 0.65]
-(b = c = (1-a)*0.19/0.43, d = the rest) and edge factor in [8, 32].
+trains both networks on R-MAT graphs of scale 18-23 with a in [0.45, 0.65]
+(b = c = (1-a)*0.19/0.43, d = the rest) and edge factor in [8, 32].  The
+range reaches past 2^22.4 vertices, where a bottom-up sweep of the tail
+starts to cost more than a top-down level on MI355X, so that both sides of
+that crossover are in the data; scale 24 stays held out.
 """
 from __future__ import annotations
 
@@ -54,6 +60,25 @@ def pick_sources(hg, k: int, seed: int) -> np.ndarray:
     return rng.choice(cand, size=min(k, len(cand)), replace=False).astype(np.int32)
 
 
+HEUR = 13  # BFS feature slot holding the heuristic rule's pick (push 0, bitmap 0.5, pull 1)
+
+
+def heuristic_pick(f: np.ndarray, prev: int) -> int:
+    """The heuristic rule (sched.cpp heuristic_bfs) evaluated on a level's
+    features with `prev` as the previous mode: frontier/n = f[4], frontier
+    out-edges/m = f[5], unvisited in-edges/m = f[7], growth sign = sign of
+    f[12], level > 0 when f[8] > 0 (graphs with an in-CSR, m_in = m)."""
+    alpha, beta = 14.0, 24.0
+    small = f[4] * 64.0 < 1.0
+    if prev == 1 and f[8] > 0:
+        if f[4] < 1.0 / beta and f[12] < 0:
+            return 0 if small else 2
+        return 1
+    if f[5] > f[7] / alpha:
+        return 1
+    return 0 if small else 2
+
+
 def _trace_arrays(trace):
     f = np.array([list(t.features) for t in trace], np.float32).reshape(-1, MLP_IN)
     ms = np.array([t.ms for t in trace], np.float64)
@@ -63,13 +88,13 @@ def _trace_arrays(trace):
 def switch_ms(n: int) -> dict:
     """Extra device time (ms) of changing frontier representation between
     levels on an n-vertex graph: a push level after a bottom-up or bitmap
-    level first turns the bitmap into a queue (a pass over n bits and two
-    launches); a bottom-up or bitmap level after a push level first scatters
-    the queue into a cleared bitmap.  Fitted to MI355X runs at scales 20-22
-    (0.031 ms at 2^20 vertices, 0.05 ms at 2^22 for bitmap -> queue)."""
+    level first turns the bitmap into a queue (`k_bitmap_to_queue`, a pass
+    over n bits: 9.9 us at 2^24 vertices on MI355X); a bottom-up or bitmap
+    level after a push level needs the frontier as a bitmap, which a large
+    push level writes as it goes and a small one scatters (a few us)."""
     s = n / float(1 << 20)
-    b2q = 0.025 + 0.006 * s
-    q2b = 0.005 + 0.004 * s
+    b2q = 0.0025 + 0.00046 * s
+    q2b = 0.004
     return {(0, 1): q2b, (0, 2): q2b, (1, 0): b2q, (2, 0): b2q}
 
 
@@ -101,7 +126,7 @@ def regret_weights(T: np.ndarray) -> np.ndarray:
     return np.log(T.max(axis=1) / np.maximum(T.min(axis=1), 1e-6)) + 0.05
 
 
-def bfs_samples(dg: DeviceGraph, sources, reps: int = 2):
+def bfs_samples(dg: DeviceGraph, sources, reps: int = 3):
     """Features [N,16] and fastest-mode labels [N] from forced-mode runs;
     also the per-level mode times [N,3]."""
     X, Y, T = [], [], []
@@ -124,40 +149,79 @@ def bfs_samples(dg: DeviceGraph, sources, reps: int = 2):
         prev = np.concatenate([[0], lab[:-1]])
         f[:, PREV_MODE] = 0.0
         f[np.arange(L), 9 + prev] = 1.0
+        for lv in range(L):  # the heuristic's pick after the label path's previous mode
+            h = heuristic_pick(f[lv], int(prev[lv]))
+            f[lv, HEUR] = (0.0, 1.0, 0.5)[h]
         X.append(f)
         Y.append(lab)
         T.append(ms)
     return np.concatenate(X), np.concatenate(Y).astype(np.int32), np.concatenate(T)
 
 
-# Features the SSSP network does not see (absolute sizes; sched.cpp
-# sssp_mask_features zeroes the same ones at inference).
-SSSP_MASKED = [0, 1, 2, 3, 4, 13]
+def _curve_at(th: np.ndarray, ms: np.ndarray, total: float, x: float) -> float:
+    """Device time at which a run's threshold reached x: linear between its
+    decision records (threshold, ms so far); the run's total past the last
+    (x = inf: the end of the run)."""
+    if len(th) == 0 or x > th[-1]:
+        return float(total)
+    return float(np.interp(x, th, ms))
 
 
-def sssp_samples(dg: DeviceGraph, sources, tie: float = 1.03):
-    """First-phase features of each run (the learned schedule picks the class
-    there and keeps it), labelled with the fastest whole-run class; the base
-    class wins when it is within `tie` of the fastest (timing noise).  The
-    class regret of the run is the sample's weight."""
+def sssp_samples(dg: DeviceGraph, sources, tie: float = 1.03, trace_cap: int = 4096):
+    """Per-decision samples: the features of every bucket-width decision of
+    the base-width run, labelled with the width class that crosses the next
+    distance window fastest.
+
+    After a bucket with threshold t, every vertex closer than t is final
+    whatever the widths used before, so the runs at the four fixed widths can
+    be aligned on the threshold axis: each run's (threshold, device time so
+    far) records give its time to cross [t_k, t_k + 4 base), the widest
+    class's bucket from the base run's decision k.  (A narrower window would
+    credit a wide class with a share of one large bucket it must process
+    whole.)  The base class wins ties within `tie` (timing noise).  The
+    previous-class features follow the label path, as the network's own
+    earlier picks would.  A sample's weight is the device time at stake in
+    its window (largest minus smallest class cost), not a ratio: picking the
+    width of a window that holds most of the run's work matters more than a
+    short one."""
     X, Y, W = [], [], []
     for src in sources:
         src = int(src)
-        _, st0, tr = dg.sssp(src, out=False, trace_cap=1)
+        _, st0 = dg.sssp(src, out=False)
         base = st0.extra[0]
-        totals = []
+        curves, feats = [], None
         for cls in range(MLP_OUT):
-            _, st = dg.sssp(src, out=False, delta=base * 2.0 ** (cls - 1))
-            totals.append(st.device_ms)
-        f, _ = _trace_arrays(tr)
-        f[:, SSSP_MASKED] = 0.0
-        best = int(np.argmin(totals))
-        if totals[1] <= tie * totals[best]:
-            best = 1
-        X.append(f[:1])
-        Y.append(np.array([best], np.int32))
-        W.append(np.array([np.log(max(totals) / max(min(totals), 1e-6)) + 0.05]))
-    return np.concatenate(X), np.concatenate(Y), np.concatenate(W)
+            best = None
+            for _ in range(2):
+                _, st, tr = dg.sssp(src, out=False, delta=base * 2.0 ** (cls - 1), trace_cap=trace_cap)
+                th = np.array([t.threshold for t in tr], np.float64)
+                ms = np.array([t.ms for t in tr], np.float64)
+                if best is None or st.device_ms < best[2]:
+                    best = (th, ms, float(st.device_ms), tr)
+            curves.append(best[:3])
+            if cls == 1:
+                feats, _ = _trace_arrays(best[3])
+        th_b = curves[1][0]
+        span = base * 2.0 ** (MLP_OUT - 2)  # the widest class's bucket
+        prev = 1
+        for k in range(len(th_b)):
+            a = th_b[k]
+            b = a + span
+            cost = np.array([_curve_at(*c, b) - _curve_at(*c, a) for c in curves])
+            cost = np.maximum(cost, 1e-4)
+            lab = int(np.argmin(cost))
+            if cost[1] <= tie * cost[lab]:
+                lab = 1
+            f = feats[k].copy()
+            f[9:13] = 0.0
+            f[9 + prev] = 1.0
+            X.append(f)
+            Y.append(lab)
+            # the device time at stake in this window: the few windows that
+            # hold most of a run's time decide its length
+            W.append(cost.max() - cost.min() + 0.01 * cost.mean())
+            prev = lab
+    return np.array(X, np.float32).reshape(-1, MLP_IN), np.array(Y, np.int32), np.array(W, np.float64)
 
 
 def train(kind: int, X: np.ndarray, y: np.ndarray, steps: int = 1500, lr: float = 2e-3, batch: int = 256,
@@ -239,7 +303,7 @@ def trace_set(scales, graphs_per_scale: int, sources: int, seed: int, log=print)
             sy.append(y)
             sw.append(w)
             log(f"traces: scale {sc} a={a:.3f} ef={ef:.1f}: {len(by[-1])} BFS levels "
-                f"(labels {np.bincount(by[-1], minlength=3).tolist()}), SSSP runs by class "
+                f"(labels {np.bincount(by[-1], minlength=3).tolist()}), SSSP decisions by class "
                 f"{np.bincount(sy[-1], minlength=MLP_OUT).tolist()} in {time.time() - t0:.1f}s")
             dg.close()
             hg.close()
@@ -250,7 +314,7 @@ def trace_set(scales, graphs_per_scale: int, sources: int, seed: int, log=print)
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", default=None, help="schedule file to write")
-    ap.add_argument("--scales", default="18,19,20,21,22")
+    ap.add_argument("--scales", default="18,19,20,21,22,23")
     ap.add_argument("--graphs", type=int, default=2, help="graphs per scale")
     ap.add_argument("--sources", type=int, default=8, help="sources per graph")
     ap.add_argument("--steps", type=int, default=1500)

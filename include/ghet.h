@@ -204,11 +204,16 @@ enum ghet_bfs_mode { GHET_MODE_PUSH = 0, GHET_MODE_PULL = 1, GHET_MODE_BITMAP = 
 
 #define GHET_TRACE_FEATURES 16
 
-/* One record per BFS level / SSSP phase when tracing is requested. */
+/* One record per BFS level / SSSP phase when tracing is requested.
+ * BFS: ms = device time of the level.  SSSP: one record per bucket-width
+ * decision (before the first bucket and at every threshold advance); ms =
+ * device time from the call's start to that decision, threshold = the
+ * threshold at that moment (every vertex with a smaller distance is final). */
 typedef struct ghet_level_trace {
     float features[GHET_TRACE_FEATURES]; /* inputs of the schedule network      */
     int32_t mode;                        /* mode or Delta class actually used   */
-    float ms;                            /* device time of the level            */
+    float ms;                            /* see above                           */
+    double threshold;                    /* SSSP: threshold at the decision     */
 } ghet_level_trace;
 
 typedef struct ghet_opts {

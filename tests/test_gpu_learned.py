@@ -128,22 +128,45 @@ def test_learned_schedule_results_are_bit_exact_under_any_network():
 
 
 def test_trained_network_gain_on_held_out_graph():
-    """Trains on scale 16-18 traces, evaluates on a held-out scale-19 graph
-    (other seed, a and edge factor).  Prints the gains; results must be
-    identical and the learned schedule must beat the heuristic."""
+    """Trains on scale 16-18 traces, evaluates on a held-out scale-20 graph
+    (larger than any training graph; other seed, a and edge factor).  Prints
+    the gains; results must be identical and the learned BFS must beat the
+    heuristic (median of three timings of the eight sources)."""
     (bx, by, bw), (sx, sy, sw) = learn.trace_set([16, 17, 18], 2, 6, seed=21, log=lambda s: None)
     learn.train(G.NET_BFS, bx, by, steps=800, seed=1, weights=bw)
     learn.train(G.NET_SSSP, sx, sy, steps=400, seed=2, weights=sw)
-    hg, dg = _graph(19, 12345, a=0.6, ef=20)
+    hg, dg = _graph(20, 12345, a=0.6, ef=20)
     srcs = learn.pick_sources(hg, 8, 9)
     learn.compare_bfs(dg, srcs[:2])  # warm-up
-    r = learn.compare_bfs(dg, srcs)
-    print(f"\nheld-out s19 BFS: heuristic {r['heuristic']:.3f} ms, learned {r['learned']:.3f} ms, "
-          f"reference rule {r['reference']:.3f} ms, gain vs heuristic {r['gain_vs_heuristic']:.3f}, "
+    runs = [learn.compare_bfs(dg, srcs) for _ in range(3)]
+    gain = float(np.median([r["gain_vs_heuristic"] for r in runs]))
+    r = runs[0]
+    print(f"\nheld-out s20 BFS: heuristic {r['heuristic']:.3f} ms, learned {r['learned']:.3f} ms, "
+          f"reference rule {r['reference']:.3f} ms, median gain vs heuristic {gain:.3f}, "
           f"vs reference rule {r['gain_vs_reference']:.3f}")
-    assert r["identical"]
-    assert r["gain_vs_heuristic"] > 1.0
+    assert all(x["identical"] for x in runs)
+    assert gain > 1.0
     s = learn.compare_sssp(dg, srcs[:4])
-    print(f"held-out s19 SSSP: heuristic {s['heuristic']:.3f} ms, learned {s['learned']:.3f} ms, "
+    print(f"held-out s20 SSSP: heuristic {s['heuristic']:.3f} ms, learned {s['learned']:.3f} ms, "
           f"gain {s['gain_vs_heuristic']:.3f}")
     assert s["identical"]
+
+
+def test_sssp_width_decision_records():
+    """One trace record per bucket-width decision: before the first bucket and
+    at every threshold advance, with the threshold and the device time so far
+    (both nondecreasing).  Distances are unchanged when a network picks the
+    width at every decision."""
+    hg, dg = _graph(14, 5)
+    src = int(learn.pick_sources(hg, 1, 2)[0])
+    d0, st0, tr = dg.sssp(src, trace_cap=4096)
+    assert st0.trace_len == len(tr) == int(st0.extra[1]) + 1
+    th = np.array([t.threshold for t in tr])
+    ms = np.array([t.ms for t in tr])
+    assert th[0] == 0.0 and np.all(np.diff(th) > 0) and np.all(np.diff(ms) >= 0)
+    assert all(t.mode == 1 for t in tr)  # the heuristic keeps the base width
+    G.sched_set_params(G.NET_SSSP, _rand_params(7, scale=2.0))
+    d1, st1, tr1 = dg.sssp(src, schedule=G.SCHED_LEARNED, trace_cap=4096)
+    assert np.array_equal(d0, d1)
+    assert st1.trace_len == int(st1.extra[1]) + 1
+    assert all(0 <= t.mode < G.MLP_OUT for t in tr1)
