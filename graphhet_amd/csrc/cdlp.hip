@@ -23,6 +23,8 @@
 // labels, has a closed form and skips the counting (k_cdlp_first_*).
 #include "graph_internal.hpp"
 
+#include <rocprim/rocprim.hpp>
+
 namespace ghet {
 namespace {
 
@@ -332,64 +334,162 @@ __global__ void __launch_bounds__(kBT) k_cdlp_first_block(const int32_t* __restr
     }
 }
 
-int build_cdlp_layout(ghet_graph* g) {
+unsigned grid_cap(int64_t blocks) {
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 65535)));
+}
+
+// Degree class of a vertex (0 lane, 1 wave, 2 LDS hash, 3 heavy; 4: no
+// neighbours, keeps its label) as the key of a stable sort by class.
+__global__ void k_cdlp_class(const int32_t* __restrict__ rp, int32_t n, uint32_t* __restrict__ key,
+                             int32_t* __restrict__ val) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t d = rp[v + 1] - rp[v];
+        key[v] = d == 0 ? 4u : d <= kSmall ? 0u : d <= kWave ? 1u : d <= kLds ? 2u : 3u;
+        val[v] = static_cast<int32_t>(v);
+    }
+}
+
+// off[c] = first position of class c in the sorted keys; off[5] = n.
+__global__ void k_cdlp_class_offsets(const uint32_t* __restrict__ key, int32_t n, int32_t* __restrict__ off) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e <= n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t prev = e > 0 ? static_cast<int64_t>(key[e - 1]) : -1;
+        const int64_t cur = e < n ? static_cast<int64_t>(key[e]) : 5;
+        for (int64_t k = prev + 1; k <= cur; ++k) off[k] = static_cast<int32_t>(e);
+    }
+}
+
+// Per heavy vertex: hash slots (the power of two >= 2d) and chunk count.
+__global__ void k_cdlp_heavy_sizes(const int32_t* __restrict__ rp, const int32_t* __restrict__ heavy, int32_t h,
+                                   int64_t* __restrict__ cap, int32_t* __restrict__ nch) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t v = heavy[i], d = rp[v + 1] - rp[v];
+        int64_t c = 1;
+        while (c < 2LL * d) c <<= 1;
+        cap[i] = c;
+        nch[i] = (d + kHeavyChunk - 1) / kHeavyChunk;
+    }
+}
+
+__global__ void k_cdlp_heavy_chunks(const int32_t* __restrict__ rp, const int32_t* __restrict__ heavy, int32_t h,
+                                    const int32_t* __restrict__ choff, int32_t* __restrict__ cv,
+                                    int32_t* __restrict__ cb) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t v = heavy[i];
+        int32_t o = choff[i];
+        for (int32_t e = rp[v]; e < rp[v + 1]; e += kHeavyChunk, ++o) {
+            cv[o] = static_cast<int32_t>(i);
+            cb[o] = e;
+        }
+    }
+}
+
+// The class lists (ids ascending within a class) from a stable radix sort
+// of the class keys, then the heavy vertices' hash-table offsets and
+// 4096-edge chunks from scans; all on the GPU.
+int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
     CDLPLayout& L = g->cdlp;
     if (L.built) return GHET_OK;
     const int32_t n = g->n;
-    const std::vector<int32_t>& rp = g->h_all_rowptr;
-    std::vector<int32_t> lists[4];
-    for (int32_t v = 0; v < n; ++v) {
-        const int32_t d = rp[v + 1] - rp[v];
-        if (d == 0) continue;
-        lists[d <= kSmall ? 0 : d <= kWave ? 1 : d <= kLds ? 2 : 3].push_back(v);
+    const int32_t* rp = g->all.rowptr;
+    uint32_t *ka = nullptr, *kb = nullptr;
+    int32_t *va = nullptr, *vb = nullptr, *off = nullptr, *nch = nullptr, *choff = nullptr, *aux32 = nullptr;
+    int64_t *cap = nullptr, *aux64 = nullptr, *hbase = nullptr;
+    void* tmp = nullptr;
+    size_t tb = 0;
+    hipError_t e = hipSuccess;
+    auto get = [&](void** q, size_t b) {
+        if (e == hipSuccess) e = hipMalloc(q, std::max<size_t>(b, 256));
+    };
+    get(reinterpret_cast<void**>(&ka), 4 * size_t(n));
+    get(reinterpret_cast<void**>(&kb), 4 * size_t(n));
+    get(reinterpret_cast<void**>(&va), 4 * size_t(n));
+    get(reinterpret_cast<void**>(&vb), 4 * size_t(n));
+    get(reinterpret_cast<void**>(&off), 4 * 8);
+    const unsigned gn = grid_cap(ceil_div(n, 256));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_cdlp_class, dim3(gn), dim3(256), 0, s, rp, n, ka, va);
+        e = hipGetLastError();
     }
-    std::vector<int64_t> hbase(lists[3].size() + 1, 0);
-    std::vector<int32_t> cv, cb;
-    for (size_t h = 0; h < lists[3].size(); ++h) {
-        const int32_t v = lists[3][h], d = rp[v + 1] - rp[v];
-        int64_t cap = 1;
-        while (cap < 2LL * d) cap <<= 1;
-        hbase[h + 1] = hbase[h] + cap;
-        for (int32_t e = rp[v]; e < rp[v + 1]; e += kHeavyChunk) {
-            cv.push_back(static_cast<int32_t>(h));
-            cb.push_back(e);
-        }
+    rocprim::double_buffer<uint32_t> kd(ka, kb);
+    rocprim::double_buffer<int32_t> vd(va, vb);
+    if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, tb, kd, vd, static_cast<unsigned>(n), 0, 3, s);
+    get(&tmp, tb);
+    if (e == hipSuccess) e = rocprim::radix_sort_pairs(tmp, tb, kd, vd, static_cast<unsigned>(n), 0, 3, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_cdlp_class_offsets, dim3(grid_cap(ceil_div(int64_t(n) + 1, 256))), dim3(256), 0, s,
+                           kd.current(), n, off);
+        e = hipGetLastError();
     }
+    int32_t h_off[6] = {};
+    if (e == hipSuccess) e = hipMemcpyAsync(h_off, off, sizeof(h_off), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const int32_t* sorted = vd.current();
+    const int32_t nh = h_off[4] - h_off[3];
+    const int32_t* heavy = sorted + h_off[3];
+    get(reinterpret_cast<void**>(&cap), 8 * size_t(nh));
+    get(reinterpret_cast<void**>(&hbase), 8 * (size_t(nh) + 1));
+    get(reinterpret_cast<void**>(&nch), 4 * size_t(nh));
+    get(reinterpret_cast<void**>(&choff), 4 * (size_t(nh) + 1));
+    get(reinterpret_cast<void**>(&aux32), 4 * size_t(scan_aux_ints(nh)));
+    get(reinterpret_cast<void**>(&aux64), 8 * size_t(scan_aux_ints(nh)));
+    if (e == hipSuccess && nh > 0) {
+        hipLaunchKernelGGL(k_cdlp_heavy_sizes, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rp, heavy, nh, cap, nch);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && exclusive_scan_i64(cap, hbase, nh, aux64, s) != GHET_OK) e = hipErrorUnknown;
+    if (e == hipSuccess && exclusive_scan_i32(nch, choff, nh, aux32, s) != GHET_OK) e = hipErrorUnknown;
+    int64_t slots = 0;
+    int32_t chunks = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&slots, hbase + nh, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&chunks, choff + nh, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     size_t bytes = 0;
-    for (auto& l : lists) bytes += ((4 * l.size() + 255) & ~size_t(255));
-    bytes += ((8 * hbase.size() + 255) & ~size_t(255)) + 2 * ((4 * cv.size() + 255) & ~size_t(255));
-    const int64_t slots = hbase.back();
+    for (int c = 0; c < 4; ++c) bytes += (4 * size_t(h_off[c + 1] - h_off[c]) + 255) & ~size_t(255);
+    bytes += ((8 * (size_t(nh) + 1) + 255) & ~size_t(255)) + 2 * ((4 * size_t(chunks) + 255) & ~size_t(255));
     bytes += 2 * ((4 * size_t(slots) + 255) & ~size_t(255));
-    GHET_HIP(hipMalloc(&L.mem, std::max<size_t>(bytes, 256)));
-    char* p = static_cast<char*>(L.mem);
-    auto take = [&](size_t b) { char* r = p; p += (b + 255) & ~size_t(255); return r; };
-    for (int c = 0; c < 4; ++c) {
-        L.count[c] = static_cast<int32_t>(lists[c].size());
-        L.list[c] = reinterpret_cast<int32_t*>(take(4 * lists[c].size()));
-        if (!lists[c].empty()) GHET_HIP(hipMemcpy(L.list[c], lists[c].data(), 4 * lists[c].size(), hipMemcpyHostToDevice));
+    if (e == hipSuccess) e = hipMalloc(&L.mem, std::max<size_t>(bytes, 256));
+    if (e == hipSuccess) {
+        char* p = static_cast<char*>(L.mem);
+        auto take = [&](size_t b) { char* r = p; p += (b + 255) & ~size_t(255); return r; };
+        for (int c = 0; c < 4 && e == hipSuccess; ++c) {
+            L.count[c] = h_off[c + 1] - h_off[c];
+            L.list[c] = reinterpret_cast<int32_t*>(take(4 * size_t(L.count[c])));
+            if (L.count[c] > 0)
+                e = hipMemcpyAsync(L.list[c], sorted + h_off[c], 4 * size_t(L.count[c]), hipMemcpyDeviceToDevice, s);
+        }
+        L.heavy_base = reinterpret_cast<int64_t*>(take(8 * (size_t(nh) + 1)));
+        if (e == hipSuccess) e = hipMemcpyAsync(L.heavy_base, hbase, 8 * (size_t(nh) + 1), hipMemcpyDeviceToDevice, s);
+        L.heavy_chunks = chunks;
+        L.heavy_chunk_vtx = reinterpret_cast<int32_t*>(take(4 * size_t(chunks)));
+        L.heavy_chunk_beg = reinterpret_cast<int32_t*>(take(4 * size_t(chunks)));
+        if (e == hipSuccess && nh > 0) {
+            hipLaunchKernelGGL(k_cdlp_heavy_chunks, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rp, heavy, nh,
+                               choff, L.heavy_chunk_vtx, L.heavy_chunk_beg);
+            e = hipGetLastError();
+        }
+        L.heavy_slots = slots;
+        L.heavy_keys = reinterpret_cast<int32_t*>(take(4 * size_t(slots)));
+        L.heavy_cnts = reinterpret_cast<uint32_t*>(take(4 * size_t(slots)));
+        if (e == hipSuccess && slots) e = hipMemsetAsync(L.heavy_keys, 0xFF, 4 * size_t(slots), s);
+        if (e == hipSuccess && slots) e = hipMemsetAsync(L.heavy_cnts, 0, 4 * size_t(slots), s);
     }
-    L.heavy_base = reinterpret_cast<int64_t*>(take(8 * hbase.size()));
-    GHET_HIP(hipMemcpy(L.heavy_base, hbase.data(), 8 * hbase.size(), hipMemcpyHostToDevice));
-    L.heavy_chunks = static_cast<int32_t>(cv.size());
-    L.heavy_chunk_vtx = reinterpret_cast<int32_t*>(take(4 * cv.size()));
-    L.heavy_chunk_beg = reinterpret_cast<int32_t*>(take(4 * cb.size()));
-    if (!cv.empty()) {
-        GHET_HIP(hipMemcpy(L.heavy_chunk_vtx, cv.data(), 4 * cv.size(), hipMemcpyHostToDevice));
-        GHET_HIP(hipMemcpy(L.heavy_chunk_beg, cb.data(), 4 * cb.size(), hipMemcpyHostToDevice));
-    }
-    L.heavy_slots = slots;
-    L.heavy_keys = reinterpret_cast<int32_t*>(take(4 * size_t(slots)));
-    L.heavy_cnts = reinterpret_cast<uint32_t*>(take(4 * size_t(slots)));
-    if (slots) {
-        GHET_HIP(hipMemset(L.heavy_keys, 0xFF, 4 * size_t(slots)));
-        GHET_HIP(hipMemset(L.heavy_cnts, 0, 4 * size_t(slots)));
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(va), static_cast<void*>(vb),
+                    static_cast<void*>(off), static_cast<void*>(cap), static_cast<void*>(hbase), static_cast<void*>(nch),
+                    static_cast<void*>(choff), static_cast<void*>(aux32), static_cast<void*>(aux64), tmp})
+        if (q) hipFree(q);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (L.mem) hipFree(L.mem);
+        L = CDLPLayout{};
+        return set_error(GHET_ERR_HIP, "CDLP layout: %s", hipGetErrorString(e));
     }
     L.built = true;
     return GHET_OK;
-}
-
-unsigned grid_cap(int64_t blocks) {
-    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 65535)));
 }
 
 }  // namespace
@@ -401,7 +501,7 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
         return set_error(GHET_ERR_INVALID, "CDLP needs the all-CSR (in-list ++ out-list)");
     if (n == 0) return GHET_OK;
     hipStream_t s = pick_stream(g, o);
-    GHET_CHECK(build_cdlp_layout(g));
+    GHET_CHECK(build_cdlp_layout(g, s));
     CDLPLayout& L = g->cdlp;
     void* p;
     int32_t *la, *lb;
