@@ -492,18 +492,29 @@ class DeviceGraph:
         return lab, st
 
     def pr(self, iters: int = 10, damping: float = 0.85, normalise: bool = True, out=True, split=None,
-           phases=False):
-        """split = (log2 hot sources, log2 window[, blocked]) forces the
-        source-window layout, with the source-blocked hot phase when blocked
-        is true and per-edge gathers otherwise (None: automatic).  phases:
-        run the hot and cold gathers as separate launches and report the hot
-        phase's ms in stats.extra[2]."""
+           phases=False, form=None):
+        """form: None (automatic), "gather" (pull gathers over the
+        degree-relabeled in-CSR) or "binned" (destination bins over source
+        windows; stats.extra[2] = its gather launches' ms, extra[3] = 2).
+        split = (log2 hot sources, log2 window[, blocked]) forces the gather
+        form's source-window layout, with the source-blocked hot phase when
+        blocked is true and per-edge gathers otherwise.  phases: run the
+        gather form's hot and cold gathers as separate launches and report
+        the hot phase's ms in stats.extra[2]."""
+        if form not in (None, "gather", "binned"):
+            raise ValueError(f"form {form!r}")
+        if form == "binned" and split is not None:
+            raise ValueError("split applies to the gather form")
         r = np.empty(self.n, np.float64) if out else None
         st = RunStats()
         o = _opts(flags=(0 if normalise else 1) | (2 if phases else 0))
         if split is not None:
             blocked = len(split) > 2 and bool(split[2])
             o.reserved[4] = int(split[0]) | (int(split[1]) << 8) | ((2 if blocked else 1) << 16)
+        elif form == "binned":
+            o.reserved[4] = 3 << 16
+        elif form == "gather":
+            o.reserved[4] = 1 << 18
         _check(lib().ghet_pr(self._h, iters, damping, _ptr(r, C.c_double), C.byref(o), C.byref(st)))
         return r, st
 

@@ -160,6 +160,7 @@ void destroy_graph(ghet_graph* g) {
     if (g->pr.mem) hipFree(g->pr.mem);
     if (g->pr.hot_mem) hipFree(g->pr.hot_mem);
     if (g->pr.items_mem) hipFree(g->pr.items_mem);
+    if (g->prb.mem) hipFree(g->prb.mem);
     if (g->cdlp.mem) hipFree(g->cdlp.mem);
     if (g->tc.mem) hipFree(g->tc.mem);
     if (g->tc.in_mem) hipFree(g->tc.in_mem);

@@ -83,6 +83,26 @@ struct PRLayout {
     void* hot_mem = nullptr;
 };
 
+// PageRank by destination bins (pr_bin.hip), built once per graph.  The
+// in-edges are grouped by (source window of 2^15 ids, destination bin): a
+// window's edges are contiguous, and inside a window each bin's edges form
+// one segment.
+struct PRBinLayout {
+    bool built = false;
+    int32_t num_win = 0;       // source windows
+    int32_t num_bin = 0;       // destination bins (contiguous ids, <= 2^14 each)
+    int32_t num_items = 0;     // gather items (a window's edge range each)
+    uint16_t* src = nullptr;   // [m] source - window start, window-major, bins grouped
+    uint16_t* dst = nullptr;   // [m] destination - bin start, same order
+    float* val = nullptr;      // [m] the gathered contributions of one iteration
+    int2* seg = nullptr;       // [num_bin * num_win] {first edge, edges} of segment (window, bin), bin-major
+    int32_t* bin_start = nullptr;  // [num_bin + 1] first destination of each bin
+    int32_t* bin_order = nullptr;  // [num_bin] bins by edges, descending (launch order)
+    int4* items = nullptr;     // [num_items] {window, first edge, end edge, 0}
+    int32_t* outdeg = nullptr; // [n]
+    void* mem = nullptr;
+};
+
 // Degree buckets over the all-CSR for CDLP (vertex lists per class).
 struct CDLPLayout {
     bool built = false;
@@ -158,6 +178,7 @@ struct ghet_graph {
     std::vector<int32_t> h_out_rowptr, h_in_rowptr, h_all_rowptr;  // host copies of offsets
     ghet::Workspace ws;
     ghet::PRLayout pr;
+    ghet::PRBinLayout prb;
     ghet::CDLPLayout cdlp;
     ghet::TCLayout tc;
     ghet::MSLayout ms;
@@ -215,6 +236,13 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
 int run_wcc(ghet_graph* g, int32_t* label_out, const ghet_opts* o, ghet_run_stats* st);
 int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const ghet_opts* o, ghet_run_stats* st);
 int run_pr_gather_ceiling(ghet_graph* g, int32_t reps, const ghet_opts* o, double* out4);
+// PageRank by destination bins (pr_bin.hip); run_pr dispatches to it.
+int run_pr_binned(ghet_graph* g, int32_t iters, double damping, double* pr_out, bool normalise, const ghet_opts* o,
+                  ghet_run_stats* st);
+// True when the binned form can hold this graph (source windows and bins fit its LDS tables).
+bool pr_binned_fits(int32_t n, int64_t m);
+// The automatic choice of form: destination bins for graphs past the L2-resident size.
+bool pr_binned_default(int32_t n, int64_t m);
 int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* o, ghet_run_stats* st);
 int run_tc(ghet_graph* g, int64_t* count_out, const ghet_opts* o, ghet_run_stats* st);
 int run_bfs_multi(ghet_graph* g, int32_t k, const int32_t* srcs, int32_t* depth_out, const ghet_opts* o,

@@ -41,6 +41,7 @@
 #include <cmath>
 
 #include "graph_internal.hpp"
+#include "pr_common.hpp"
 
 #include <rocprim/rocprim.hpp>
 
@@ -68,23 +69,6 @@ constexpr int kHotChunk = kHotBT * kHotK;
 constexpr double kHotFix = 1152921504606846976.0;  // 2^60
 static_assert(kHotT <= (1 << (32 - kHotShift)), "row in tile must fit beside the source in a word");
 
-struct PRScalars {
-    double sink;   // S of the current ranks
-    double total;  // sum of ranks (normalisation)
-};
-
-__device__ __forceinline__ double block_sum_f64(double x, double* s_red) {
-    x = wave_sum(x);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) s_red[w] = x;
-    __syncthreads();
-    double t = 0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kBT / 64; ++i) t += s_red[i];
-    __syncthreads();
-    return t;  // valid in thread 0
-}
-
 // Compensated wave reduction of per-lane Kahan accumulators (fixed tree order).
 __device__ __forceinline__ Kahan32 wave_kahan(Kahan32 k) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -97,40 +81,6 @@ __device__ __forceinline__ Kahan32 wave_kahan(Kahan32 k) {
         k = lo;
     }
     return k;
-}
-
-__global__ void __launch_bounds__(kBT) k_pr_init(const int32_t* __restrict__ outdeg, int32_t n, double pr0,
-                                                 float* __restrict__ contrib, double* __restrict__ part) {
-    __shared__ double s_red[kBT / 64];
-    double sink = 0;
-    for (int64_t v = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; v < n;
-         v += static_cast<int64_t>(gridDim.x) * kBT) {
-        const int32_t d = outdeg[v];
-        if (d > 0) contrib[v] = static_cast<float>(pr0 / d);
-        else {
-            contrib[v] = 0.f;
-            sink += pr0;
-        }
-    }
-    double t = block_sum_f64(sink, s_red);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-// Sums part[0..count) in index order (strided lanes, then fixed tree) and
-// stores it in *out; also zeroes *zero when given.
-__global__ void __launch_bounds__(1024) k_pr_reduce(const double* __restrict__ part, int32_t count, double* out,
-                                                    int* zero) {
-    __shared__ double s[1024];
-    if (zero && threadIdx.x == 0) *zero = 0;  // k_pr_hot's tile counter, for the next launch
-    double x = 0;
-    for (int i = threadIdx.x; i < count; i += 1024) x += part[i];
-    s[threadIdx.x] = x;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = s[0];
 }
 
 // A work item: a tile of whole rows (row_end > row) or one kTileNnz-edge
@@ -524,25 +474,6 @@ __global__ void __launch_bounds__(kBT) k_gather_probe(const int32_t* __restrict_
     if (lane_id() == 0) out[(static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6] = acc;
 }
 
-__global__ void __launch_bounds__(kBT) k_pr_sum(const double* __restrict__ rank, int32_t n, double* __restrict__ part) {
-    __shared__ double s_red[kBT / 64];
-    double x = 0;
-    for (int64_t v = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; v < n;
-         v += static_cast<int64_t>(gridDim.x) * kBT)
-        x += rank[v];
-    double t = block_sum_f64(x, s_red);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-// out[perm[r]] = rank[r] / total (total = 1 without normalisation).
-__global__ void k_pr_unpermute(const double* __restrict__ rank, const int32_t* __restrict__ perm, int32_t n,
-                               const PRScalars* scal, int normalise, double* __restrict__ out) {
-    const double inv = normalise ? 1.0 / scal->total : 1.0;
-    for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < n;
-         r += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        out[perm[r]] = normalise ? rank[r] * inv : rank[r];
-}
-
 // Relabeled in-CSR: row r is old vertex perm[r], its in-neighbours renamed by inv.
 __global__ void __launch_bounds__(kBT) k_permute_rows(const int32_t* __restrict__ perm, const int32_t* __restrict__ inv,
                                                       const int32_t* __restrict__ in_rowptr,
@@ -576,7 +507,8 @@ __global__ void __launch_bounds__(kBT) k_permute_rows(const int32_t* __restrict_
 // gathers (it trades L2 requests for LDS fills, FP64 runs and 64-bit LDS
 // atomics, and is instruction-bound), so it is available only on request.
 void pr_split(int32_t n, int32_t code, int32_t* hot, int32_t* window, bool* blocked) {
-    if (code > 0) {
+    const int form = (code >> 16) & 3;
+    if (form == 1 || form == 2) {
         *hot = static_cast<int32_t>(std::min<int64_t>(int64_t(1) << std::min(code & 0xFF, 30), n));
         *window = static_cast<int32_t>(int64_t(1) << std::min(std::max((code >> 8) & 0xFF, 1), 30));
         *blocked = ((code >> 16) & 3) == 2;
@@ -893,24 +825,6 @@ __global__ void __launch_bounds__(kBT) k_pr_iota(int32_t* __restrict__ out, int3
         out[i] = static_cast<int32_t>(i);
 }
 
-// Owns the temporary device buffers of one layout build.
-struct DevTemp {
-    std::vector<void*> ptrs;
-    hipError_t err = hipSuccess;
-    template <typename T>
-    T* get(size_t count) {
-        void* p = nullptr;
-        if (err == hipSuccess) err = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256));
-        if (err != hipSuccess) return nullptr;
-        ptrs.push_back(p);
-        return static_cast<T*>(p);
-    }
-    ~DevTemp() {
-        if (!ptrs.empty()) (void)hipDeviceSynchronize();  // no launch may still read them
-        for (void* p : ptrs) hipFree(p);
-    }
-};
-
 unsigned pr_grid(int64_t threads) {
     return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(threads, kBT), 65536)));
 }
@@ -1206,6 +1120,12 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     hipStream_t s = pick_stream(g, o);
     const bool normalise = !(o && o->size >= sizeof(ghet_opts) && (o->reserved[0] & 1));
     const int32_t split_code = (o && o->size >= sizeof(ghet_opts)) ? o->reserved[4] : 0;
+    // Form (code bits 16-17): 3 = destination bins (pr_bin.hip); 1, 2 = this
+    // file's gathers with the given split; 0 = automatic: bins above 2^21
+    // vertices unless bit 18 asks for the gathers.
+    const int form = (split_code >> 16) & 3;
+    if (form == 3 || (form == 0 && !((split_code >> 18) & 1) && pr_binned_default(n, g->in.m)))
+        return run_pr_binned(g, iters, damping, pr_out, normalise, o, st);
     GHET_CHECK(build_pr_layout(g, s, split_code));
     PRLayout& L = g->pr;
     // One persistent k_pr_iter launch per iteration walks all items (hot

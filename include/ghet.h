@@ -231,11 +231,13 @@ typedef struct ghet_opts {
      * reserved[2]     : batched BFS pushes while frontier out-edges < m / this
      *                   (0 = default 16)
      * reserved[3]     : batched SSSP sources per pass (0 = 32, at most 32)
-     * reserved[4]     : PR source windows: 0 = automatic, else
-     *                   log2(hot sources) | log2(window) << 8, plus 2 << 16
-     *                   for the source-blocked hot phase (LDS copies of
-     *                   the hot sources instead of per-edge gathers); the
-     *                   layout is rebuilt when this changes
+     * reserved[4]     : PR form and source windows: 0 = automatic;
+     *                   3 << 16 = destination bins; 1 << 18 = pull gathers
+     *                   with automatic windows; else pull gathers with
+     *                   log2(hot sources) | log2(window) << 8, plus 1 << 16,
+     *                   or 2 << 16 for the source-blocked hot phase (LDS
+     *                   copies of the hot sources instead of per-edge
+     *                   gathers); the layout is rebuilt when this changes
      * reserved[5]     : batched SSSP phase budget per pass (0 = automatic,
      *                   64 (n + 1024)); a pass that spends it, or two
      *                   minutes, stops and the call returns GHET_ERR_HIP

@@ -322,3 +322,61 @@ def test_pr_gather_ceiling_covers_every_edge(medium):
     assert c["hot_edges"] == i.m and c["cold_edges"] == 0 and c["cold_ms"] == 0
     with pytest.raises(G.GhetError):
         dg.pr_gather_ceiling(0)
+
+
+def test_golden_pr_binned(golden):
+    """The destination-bin form against GraphHET's own CPU outputs, in the
+    reference form and normalised."""
+    dg = dev_from_golden(golden)
+    for norm in (True, False):
+        r, st = dg.pr(10, 0.85, normalise=norm, form="binned")
+        err = np.abs(r - golden["pr"])
+        assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (norm, err.sum(), err.max())
+        assert st.extra[3] == 2
+
+
+def test_pr_binned_medium(medium):
+    """Binned and gather forms each within the bar of the FP64 oracle, and
+    the binned sums exact integers: two runs are bit-identical."""
+    o, i, dg = medium["o"], medium["i"], medium["dg"]
+    ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 20)
+    r, st = dg.pr(20, 0.85, form="binned")
+    err = np.abs(r - ref)
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (err.sum(), err.max())
+    np.testing.assert_array_equal(r, dg.pr(20, 0.85, form="binned")[0])
+    assert st.extra[3] == 2 and 0 < st.extra[2] < st.hot_kernel_ms
+    g, stg = dg.pr(20, 0.85, form="gather")
+    err = np.abs(g - ref)
+    assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS and stg.extra[3] != 2
+
+
+def test_pr_binned_windows_bins_and_items():
+    """Several source windows (2^15 ids) and destination bins; a vertex whose
+    in-edges exceed a bin's edge target (a bin of its own); a window with
+    more edges than one gather item (vertex 0's out-edges); R-MAT with its
+    hubs at low ids."""
+    n = 150000
+    src = list(range(1, n)) + [0] * (n - 1) + [1] * 3000 + list(range(2, 3002))
+    dst = [0] * (n - 1) + list(range(1, n)) + list(range(2, 3002)) + [1] * 3000
+    hg = G.HostGraph.from_edges(n, src, dst)
+    for h in (hg, G.HostGraph.rmat(17, 16, seed=3)):
+        o, i = h.view(G.VIEW_OUT), h.view(G.VIEW_IN)
+        dg = G.DeviceGraph(o, i)
+        ref = O.pr(i.rowptr, i.col, o.rowptr, 0.85, 10)
+        for norm in (True, False):
+            r, _ = dg.pr(10, 0.85, normalise=norm, form="binned")
+            want = ref / ref.sum() if norm else ref
+            err = np.abs(r - want)
+            assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (o.n, norm, err.sum(), err.max())
+
+
+def test_pr_binned_edge_cases():
+    dg = G.DeviceGraph(G.Csr(1, np.array([0, 0], np.int32), np.zeros(0, np.int32), np.zeros(0)))
+    np.testing.assert_allclose(dg.pr(5, form="binned")[0], [1.0])
+    n = 100
+    dg = G.DeviceGraph(G.Csr(n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0)))
+    np.testing.assert_allclose(dg.pr(3, form="binned")[0], np.full(n, 1.0 / n), rtol=1e-15)
+    r0, _ = dg.pr(0, form="binned")
+    np.testing.assert_allclose(r0, np.full(n, 1.0 / n), rtol=1e-15)
+    with pytest.raises(ValueError):
+        dg.pr(3, form="binned", split=(4, 6))
