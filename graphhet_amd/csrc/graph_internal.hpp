@@ -84,21 +84,24 @@ struct PRLayout {
 };
 
 // PageRank by destination bins (pr_bin.hip), built once per graph.  The
-// in-edges are grouped by (source window of 2^15 ids, destination bin): a
-// window's edges are contiguous, and inside a window each bin's edges form
-// one segment.
+// edges are grouped by (source window of 2^15 ids, destination bin): window-
+// major for the gather (a window's edges contiguous, one segment per bin
+// inside it, each ordered by source) and bin-major for the accumulation (a
+// bin's edges contiguous, its segments in window order).
 struct PRBinLayout {
     bool built = false;
     int32_t num_win = 0;       // source windows
     int32_t num_bin = 0;       // destination bins (contiguous ids, <= 2^14 each)
-    int32_t num_items = 0;     // gather items (a window's edge range each)
-    uint16_t* src = nullptr;   // [m] source - window start, window-major, bins grouped
-    uint16_t* dst = nullptr;   // [m] destination - bin start, same order
-    float* val = nullptr;      // [m] the gathered contributions of one iteration
-    int2* seg = nullptr;       // [num_bin * num_win] {first edge, edges} of segment (window, bin), bin-major
+    int32_t num_items = 0;     // gather items (an edge range of one window each)
+    uint16_t* src = nullptr;   // [m] source - window start, window-major
+    uint16_t* dst = nullptr;   // [m] destination - bin start, bin-major
+    uint32_t* val = nullptr;   // [m + 1] one iteration's contribution words, bin-major (val[m]: spare)
+    int2* segtab = nullptr;    // [num_win * num_bin + 1] {window-major start, bin-major start} of segment (w, b) at w * num_bin + b
+    int32_t* binoff = nullptr;     // [num_bin + 1] first bin-major edge of each bin
     int32_t* bin_start = nullptr;  // [num_bin + 1] first destination of each bin
     int32_t* bin_order = nullptr;  // [num_bin] bins by edges, descending (launch order)
-    int4* items = nullptr;     // [num_items] {window, first edge, end edge, 0}
+    int4* items = nullptr;     // [num_items] {window, first edge, end edge, first segment's bin}
+    int32_t* item_nseg = nullptr;  // [num_items] segments the item's edges touch
     int32_t* outdeg = nullptr; // [n]
     void* mem = nullptr;
 };

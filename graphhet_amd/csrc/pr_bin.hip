@@ -8,28 +8,34 @@
 // The pull gather (pr.hip) reads contrib[src] at random: one L2 request per
 // edge, and a 64-128 B line per 4 useful bytes for the sources that miss L2.
 // This form moves every random access into LDS and leaves global memory with
-// streams only.  The in-edges are grouped once per graph by (source window,
+// streams.  Once per graph the edges are grouped by (source window,
 // destination bin):
-//   * a source window is 2^15 consecutive ids: its FP32 contributions are
+//   * a source window is 2^15 consecutive ids: its contribution words are
 //     128 KB, one LDS buffer;
 //   * a destination bin is a run of consecutive ids, at most 2^14 of them
 //     (128 KB of 64-bit accumulators) and about kBinEdges in-edges.
-// One iteration is two launches:
+// The edges of window w going to bin b form segment (w, b).  Sources are
+// stored window-major (segments of a window in bin order, each ordered by
+// source); destinations and the per-iteration values bin-major (segments of
+// a bin in window order).  One iteration is two launches:
 //   1. k_prb_gather: per item (an edge range of one window) the window's
-//      contributions are copied to LDS, and each edge's contribution is read
-//      from there and written to val[e] in layout order.  2 B read + 4 B
-//      written per edge, both sequential.
-//   2. k_prb_accum: per bin, the bin's segment of every window is read (4 B
-//      value + 2 B destination per edge, runs of consecutive edges); each
-//      value is converted to fixed point with 62 fraction bits and added to
-//      its destination's LDS accumulator with an integer atomic.  Then every
-//      vertex of the bin is finished: rank, next contribution, sink mass and,
-//      on the last iteration, the FP64 rank.
-// Integer addition is exact and order-free, so a vertex's sum is the exact
-// sum of its FP32 contributions truncated below 2^-62 per term, whatever the
-// schedule: runs are bit-reproducible and need no compensation.  A row sum
-// is at most the total rank (1), far below the 2^64 / 2^62 = 4 the
-// accumulator holds.
+//      contribution words are copied to LDS with the item's segment table;
+//      each edge reads its source's word from LDS and stores it at its
+//      bin-major position (a segment is one contiguous run).  2 B read and
+//      4 B written per edge.
+//   2. k_prb_accum: per bin, the bin's values and destinations are read
+//      sequentially (6 B per edge) and each value is added to its
+//      destination's LDS accumulator with a 64-bit integer atomic; then every
+//      vertex of the bin is finished: rank, next contribution word, sink mass
+//      and, on the last iteration, the FP64 rank.
+// Contribution words.  contrib = (float)(pr / outdeg) as in the gather form,
+// stored as (s << 24) | m with value m * 2^(s - 62): the FP32 value in fixed
+// point with 62 fraction bits (m is the 24-bit significand, shifted right and
+// s = 0 for values below 2^-38; bits below 2^-62 are dropped).  Adding words
+// is integer addition, exact and order-free, so every vertex's sum is the
+// exact sum of its FP32 contributions (each truncated below 2^-62) whatever
+// the schedule, and runs are bit-reproducible.  A row sum is at most the
+// total rank (1), far below the 4 that 64 bits with 62 fraction bits hold.
 #include <algorithm>
 #include <numeric>
 
@@ -42,165 +48,174 @@ namespace ghet {
 namespace {
 
 constexpr int kWinShift = 15;
-constexpr int kWin = 1 << kWinShift;      // sources per window (128 KB of FP32 in LDS)
+constexpr int kWin = 1 << kWinShift;      // sources per window (128 KB of words in LDS)
 constexpr int kBinMax = 1 << 14;          // destinations per bin (128 KB of u64 in LDS)
-constexpr int kMaxWin = 2048;             // windows the accumulate kernel's LDS tables hold
+constexpr int kMaxWin = 2048;             // windows (n <= 2^26)
+constexpr int kMaxSeg = 3072;             // bins a gather item's LDS segment table holds
 constexpr int64_t kBinEdges = 1 << 17;    // in-edges a bin aims for
 constexpr int64_t kItemEdges = 1 << 17;   // edges per gather item
 constexpr int kPB = 1024;                 // threads per block, both kernels (one block per CU)
-constexpr int kQ = 8;                     // edges per thread per step
-constexpr int kChunk = 64 * kQ;           // edges per wave per step (accumulate)
-static_assert(kMaxWin <= 2 * kPB, "the segment scan gives each thread two windows");
+constexpr int kWaves = kPB / 64;
+constexpr int kQ = 8;                     // rows (gather) / elements (accumulate) in flight per thread
 
-// x >= 0 in FP32 -> x * 2^62 as an unsigned integer, truncated.  x =
-// mant * 2^(ex - 150) for a normal x, so x * 2^62 = mant * 2^(ex - 88).
-// Contributions are at most 1 (ex <= 127), so the result stays below 2^63.
-__device__ __forceinline__ unsigned long long to_fix62(float x) {
+// FP32 x in [0, 2) -> contribution word; x = m * 2^(ex - 150) for a normal x.
+__device__ __forceinline__ uint32_t fix_pack(float x) {
     const uint32_t bits = __float_as_uint(x);
     const int ex = static_cast<int>(bits >> 23);
-    if (ex == 0) return 0ull;  // zero or subnormal: below 2^-126, under the resolution
-    const unsigned long long mant = (bits & 0x7FFFFFu) | 0x800000u;
-    const int sh = ex - 88;
-    return sh >= 0 ? (mant << sh) : (sh > -24 ? (mant >> -sh) : 0ull);
+    if (ex == 0) return 0u;  // zero or subnormal: below 2^-126, under the resolution
+    uint32_t mant = (bits & 0x7FFFFFu) | 0x800000u;
+    int sh = ex - 88;        // x * 2^62 = mant * 2^(ex - 88)
+    if (sh < 0) {
+        mant = sh > -24 ? mant >> -sh : 0u;
+        sh = 0;
+    }
+    return (static_cast<uint32_t>(sh) << 24) | mant;
 }
 
-__global__ void __launch_bounds__(kPB) k_prb_gather(const int4* __restrict__ items, const uint16_t* __restrict__ src,
-                                                    const float* __restrict__ contrib, int32_t n,
-                                                    float* __restrict__ val) {
-    __shared__ float s_win[kWin];
-    const int4 it = items[blockIdx.x];
-    const int64_t w0 = static_cast<int64_t>(it.x) << kWinShift;
-    const int32_t wn = static_cast<int32_t>(n - w0 < kWin ? n - w0 : kWin);
-    // contrib + w0 is 16-byte aligned: w0 is a multiple of 2^15
-    const int32_t wn4 = wn >> 2;
-    const float4* __restrict__ c4 = reinterpret_cast<const float4*>(contrib + w0);
-    float4* s4 = reinterpret_cast<float4*>(s_win);
-    for (int32_t i = threadIdx.x; i < wn4; i += kPB) s4[i] = c4[i];
-    for (int32_t i = (wn4 << 2) + threadIdx.x; i < wn; i += kPB) s_win[i] = contrib[w0 + i];
-    __syncthreads();
-    const int32_t e1 = it.z;
-    for (int32_t b = it.y + static_cast<int32_t>(threadIdx.x); b < e1; b += kPB * kQ) {
-        uint32_t si[kQ];
-#pragma unroll
-        for (int j = 0; j < kQ; ++j) {
-            const int32_t e = b + j * kPB;
-            si[j] = e < e1 ? src[e] : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < kQ; ++j) {
-            const int32_t e = b + j * kPB;
-            if (e < e1) val[e] = s_win[si[j]];
+// Contribution word -> x * 2^62 as an integer (below 2^63 for x < 2).
+__device__ __forceinline__ unsigned long long fix_unpack(uint32_t w) {
+    return static_cast<unsigned long long>(w & 0xFFFFFFu) << (w >> 24);
+}
+
+__global__ void __launch_bounds__(256) k_prb_init(const int32_t* __restrict__ outdeg, int32_t n, double pr0,
+                                                  uint32_t* __restrict__ contrib, double* __restrict__ part) {
+    __shared__ double s_red[256 / 64];
+    double sink = 0;
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * 256) {
+        const int32_t d = outdeg[v];
+        if (d > 0) contrib[v] = fix_pack(static_cast<float>(pr0 / d));
+        else {
+            contrib[v] = 0u;
+            sink += pr0;
         }
     }
+    const double t = block_sum_f64<256>(sink, s_red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// Last segment w in [lo, hi) with s_pre[w] <= k, given s_pre[lo] <= k <
-// s_pre[hi] (uniform k: every lane reads the same LDS words).
-__device__ __forceinline__ int prb_seg_of(const int32_t* s_pre, int32_t k, int lo, int hi) {
+// Last c in [lo, hi) with s[c] <= k, given s[lo] <= k < s[hi]; k is the same
+// in every lane (the reads are broadcasts and the loop is uniform).
+__device__ __forceinline__ int prb_search(const int32_t* s, int32_t k, int lo, int hi) {
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (__builtin_amdgcn_readfirstlane(s_pre[mid]) <= k) lo = mid;
+        if (__builtin_amdgcn_readfirstlane(s[mid]) <= k) lo = mid;
         else hi = mid;
     }
     return lo;
 }
 
-// Loads one wave chunk of a bin's edges (k in [c0, c0 + kChunk), lane j*64 +
-// lane).  The chunk's first and last segments come from two wave-uniform
-// searches; each lane then finds its edge's segment between them in a fixed
-// number of steps, so no lane diverges, and the loads are issued together
-// with no branch between them (lanes past T load edge 0 and are masked
-// where the values are used).
-__device__ __forceinline__ void prb_load_chunk(int32_t c0, int32_t T, int32_t num_win, const int32_t* s_pre,
-                                               const int32_t* s_delta, const uint16_t* __restrict__ dst,
-                                               const float* __restrict__ val, int lane, uint32_t (&dd)[kQ],
-                                               float (&vv)[kQ]) {
-    c0 = __builtin_amdgcn_readfirstlane(c0);
-    const int32_t c1 = min(c0 + kChunk, T) - 1;
-    const int w0 = prb_seg_of(s_pre, c0, 0, num_win);
-    const int w1 = prb_seg_of(s_pre, c1, w0, num_win);
-    const int steps = __builtin_amdgcn_readfirstlane(32 - __clz(w1 - w0));  // ceil(log2(w1 - w0 + 1))
-    int32_t ei[kQ];
-#pragma unroll
-    for (int j = 0; j < kQ; ++j) {
-        const int32_t k = min(c0 + j * 64 + lane, c1);
-        int lo = w0, hi = w1 + 1;
-        for (int t = 0; t < steps; ++t) {
-            const int mid = (lo + hi) >> 1;
-            const bool le = s_pre[mid] <= k;
-            lo = le ? mid : lo;
-            hi = le ? hi : mid;
+// One block per item {w, e0, e1, b0}.  Each wave takes a contiguous run of
+// 64-edge rows of [e0, e1) and walks it with its current segment c (bin
+// b0 + c): a row inside the segment stores at e + D(c); a row that reaches
+// the next segment start steps c forward (a uniform loop; lanes at or past
+// each start take that segment's D).  kQ rows' positions are formed first,
+// then their source loads are issued together.
+__global__ void __launch_bounds__(kPB) k_prb_gather(const int4* __restrict__ items, const int32_t* __restrict__ item_nseg,
+                                                    const int2* __restrict__ segtab, int32_t num_bin,
+                                                    const uint16_t* __restrict__ src, const uint32_t* __restrict__ contrib,
+                                                    int32_t n, int32_t spare, uint32_t* __restrict__ val) {
+    __shared__ uint32_t s_win[kWin];
+    __shared__ int32_t s_S[kMaxSeg + 1];  // window-major start of segment b0 + c (s_S[nseg]: past the item)
+    __shared__ int32_t s_D[kMaxSeg];      // bin-major start minus window-major start
+    const int4 it = items[blockIdx.x];
+    const int32_t nseg = item_nseg[blockIdx.x];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t w0 = static_cast<int64_t>(it.x) << kWinShift;
+    const int32_t wn = static_cast<int32_t>(n - w0 < kWin ? n - w0 : kWin);
+    {
+        // contrib + w0 is 16-byte aligned: w0 is a multiple of 2^15
+        const int32_t wn4 = wn >> 2;
+        const uint4* __restrict__ c4 = reinterpret_cast<const uint4*>(contrib + w0);
+        uint4* s4 = reinterpret_cast<uint4*>(s_win);
+        for (int32_t i = tid; i < wn4; i += kPB) s4[i] = c4[i];
+        for (int32_t i = (wn4 << 2) + tid; i < wn; i += kPB) s_win[i] = contrib[w0 + i];
+        const int2* st = segtab + static_cast<int64_t>(it.x) * num_bin + it.w;
+        for (int32_t i = tid; i <= nseg; i += kPB) {
+            const int2 d = st[i];
+            s_S[i] = d.x;
+            if (i < nseg) s_D[i] = d.y - d.x;
         }
-        ei[j] = c0 + j * 64 + lane <= c1 ? k + s_delta[lo] : 0;
     }
+    __syncthreads();
+    const int32_t e0 = it.y, e1 = it.z;
+    const int32_t R = (e1 - e0 + 63) >> 6;
+    const int32_t r0 = static_cast<int32_t>(static_cast<int64_t>(R) * wave / kWaves);
+    const int32_t r1 = static_cast<int32_t>(static_cast<int64_t>(R) * (wave + 1) / kWaves);
+    const int32_t a = __builtin_amdgcn_readfirstlane(e0 + 64 * r0);
+    const int32_t z = __builtin_amdgcn_readfirstlane(min(e1, e0 + 64 * r1));
+    if (a >= z) return;
+    int c = prb_search(s_S, a, 0, nseg);  // s_S[0] <= e0 <= a < e1 <= s_S[nseg]
+    int32_t D = __builtin_amdgcn_readfirstlane(s_D[c]);
+    int32_t nb = __builtin_amdgcn_readfirstlane(s_S[c + 1]);
+    for (int32_t k0 = a; k0 < z; k0 += 64 * kQ) {
+        int32_t pos[kQ], ee[kQ];
 #pragma unroll
-    for (int j = 0; j < kQ; ++j) {
-        dd[j] = dst[ei[j]];
-        vv[j] = val[ei[j]];
+        for (int j = 0; j < kQ; ++j) {
+            const int32_t r = k0 + 64 * j;
+            const int32_t e = r + lane;
+            int32_t p = e + D;
+            if (r < z) {
+                const int32_t rl = min(r + 63, z - 1);
+                while (nb <= rl) {  // the row reaches segment c + 1 (c + 1 < nseg: nb <= rl < e1 <= s_S[nseg])
+                    ++c;
+                    const int32_t Dn = __builtin_amdgcn_readfirstlane(s_D[c]);
+                    p = e >= nb ? e + Dn : p;
+                    D = Dn;
+                    nb = __builtin_amdgcn_readfirstlane(s_S[c + 1]);
+                }
+            }
+            // lanes past the run load edge e0 and store to the spare slot
+            // val[m], so the loads and stores need no branches
+            pos[j] = e < z ? p : spare;
+            ee[j] = e < z ? e : e0;
+        }
+        uint32_t si[kQ];
+#pragma unroll
+        for (int j = 0; j < kQ; ++j) si[j] = src[ee[j]];
+#pragma unroll
+        for (int j = 0; j < kQ; ++j) val[pos[j]] = s_win[si[j]];
     }
 }
 
-// One block per bin (bins in descending edge order).  Each wave takes
-// 512-edge chunks of the bin's edges, concatenated over the windows, and
-// keeps two in flight: the next chunk's loads are issued before the current
-// chunk's values are added.
+// One block per bin (bins in descending edge order): the bin's edges are
+// [binoff[b], binoff[b+1]) in bin-major order.  Each thread keeps two steps
+// of kQ elements in flight: the next step's loads are issued before the
+// current step's values are added.
 __global__ void __launch_bounds__(kPB) k_prb_accum(
-    const int32_t* __restrict__ bin_order, const int32_t* __restrict__ bin_start, const int2* __restrict__ seg,
-    int32_t num_win, const uint16_t* __restrict__ dst, const float* __restrict__ val,
-    const int32_t* __restrict__ outdeg, const PRScalars* __restrict__ scal, double nd, double damping,
-    float* __restrict__ next, double* __restrict__ part, double* __restrict__ rank_out, int last) {
+    const int32_t* __restrict__ bin_order, const int32_t* __restrict__ bin_start, const int32_t* __restrict__ binoff,
+    const uint16_t* __restrict__ dst, const uint32_t* __restrict__ val, const int32_t* __restrict__ outdeg,
+    const PRScalars* __restrict__ scal, double nd, double damping, uint32_t* __restrict__ next,
+    double* __restrict__ part, double* __restrict__ rank_out, int last) {
     __shared__ unsigned long long s_acc[kBinMax];
-    __shared__ int32_t s_pre[kMaxWin + 1];
-    __shared__ int32_t s_delta[kMaxWin];  // first edge of segment w minus s_pre[w]
-    __shared__ int32_t s_wt[kPB / 64];
-    __shared__ double s_red[kPB / 64];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    __shared__ double s_red[kWaves];
+    const int tid = threadIdx.x;
     const int32_t b = bin_order[blockIdx.x];
     const int32_t v0 = bin_start[b], nv = bin_start[b + 1] - v0;
+    const int32_t base = binoff[b], T = binoff[b + 1] - base;
     for (int32_t i = tid; i < nv; i += kPB) s_acc[i] = 0ull;
-    // exclusive scan of the segment lengths, two windows per thread
-    {
-        const int2* sb = seg + static_cast<int64_t>(b) * num_win;
-        const int i0 = 2 * tid;
-        const int2 a = i0 < num_win ? sb[i0] : make_int2(0, 0);
-        const int2 c = i0 + 1 < num_win ? sb[i0 + 1] : make_int2(0, 0);
-        const int32_t mine = a.y + c.y;
-        int32_t incl = mine;
+    __syncthreads();
+    constexpr int32_t kStep = kPB * kQ;
+    uint32_t dd[kQ], vv[kQ];
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int32_t y = __shfl_up(incl, off);
-            if (lane >= off) incl += y;
-        }
-        if (lane == 63) s_wt[wave] = incl;
-        __syncthreads();
-        int32_t wbase = 0;
-        for (int w = 0; w < wave; ++w) wbase += s_wt[w];
-        const int32_t excl = wbase + incl - mine;
-        if (i0 < num_win) {
-            s_pre[i0] = excl;
-            s_delta[i0] = a.x - excl;
-        }
-        if (i0 + 1 < num_win) {
-            s_pre[i0 + 1] = excl + a.y;
-            s_delta[i0 + 1] = c.x - (excl + a.y);
-        }
-        if (tid == kPB - 1) s_pre[num_win] = wbase + incl;
-        __syncthreads();
+    for (int j = 0; j < kQ; ++j) {
+        const int32_t k = j * kPB + tid;
+        const int32_t e = k < T ? base + k : 0;  // lanes past the bin load element 0, never used
+        dd[j] = dst[e];
+        vv[j] = val[e];
     }
-    const int32_t T = s_pre[num_win];
-    constexpr int32_t kStride = (kPB / 64) * kChunk;
-    uint32_t dd[kQ];
-    float vv[kQ];
-    int32_t c0 = wave * kChunk;
-    if (c0 < T) prb_load_chunk(c0, T, num_win, s_pre, s_delta, dst, val, lane, dd, vv);
-    for (; c0 < T; c0 += kStride) {
-        uint32_t dn[kQ];
-        float vn[kQ];
-        // the last pass reloads the current chunk (in bounds, never used)
-        prb_load_chunk(c0 + kStride < T ? c0 + kStride : c0, T, num_win, s_pre, s_delta, dst, val, lane, dn, vn);
+    for (int32_t k0 = 0; k0 < T; k0 += kStep) {
+        uint32_t dn[kQ], vn[kQ];
+#pragma unroll
+        for (int j = 0; j < kQ; ++j) {
+            const int32_t k = k0 + kStep + j * kPB + tid;
+            const int32_t e = k < T ? base + k : 0;
+            dn[j] = dst[e];
+            vn[j] = val[e];
+        }
 #pragma unroll
         for (int j = 0; j < kQ; ++j)
-            if (c0 + j * 64 + lane < T) atomicAdd(&s_acc[dd[j] & (kBinMax - 1)], to_fix62(vv[j]));
+            if (k0 + j * kPB + tid < T) atomicAdd(&s_acc[dd[j] & (kBinMax - 1)], fix_unpack(vv[j]));
 #pragma unroll
         for (int j = 0; j < kQ; ++j) {
             dd[j] = dn[j];
@@ -208,12 +223,12 @@ __global__ void __launch_bounds__(kPB) k_prb_accum(
         }
     }
     __syncthreads();
-    const double base = scal->sink * damping / nd + (1.0 - damping) / nd;
+    const double base_pr = scal->sink * damping / nd + (1.0 - damping) / nd;
     double sink = 0;
     for (int32_t i = tid; i < nv; i += kPB) {
-        const double pr = base + damping * (static_cast<double>(s_acc[i]) * 0x1p-62);
+        const double pr = base_pr + damping * (static_cast<double>(s_acc[i]) * 0x1p-62);
         const int32_t d = outdeg[v0 + i];
-        if (d > 0) next[v0 + i] = static_cast<float>(pr / d);
+        if (d > 0) next[v0 + i] = fix_pack(static_cast<float>(pr / d));
         else sink += pr;
         if (last) rank_out[v0 + i] = pr;
     }
@@ -270,29 +285,75 @@ __global__ void k_prb_soff(const uint32_t* __restrict__ key, int64_t count, uint
     }
 }
 
-__global__ void k_prb_split(const uint32_t* __restrict__ word, int64_t m, uint16_t* __restrict__ src,
-                            uint16_t* __restrict__ dst) {
-    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
-         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const uint32_t x = word[e];
-        src[e] = static_cast<uint16_t>(x & 0xFFFFu);
-        dst[e] = static_cast<uint16_t>(x >> 16);
+// Segment lengths in bin-major order: len[b * num_win + w] = |segment (w, b)|.
+__global__ void k_prb_lens(const int32_t* __restrict__ soff, int32_t num_win, int32_t num_bin, int32_t* __restrict__ len) {
+    const int64_t total = static_cast<int64_t>(num_win) * num_bin;
+    for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < total;
+         q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t b = q / num_win, w = q - b * num_win;
+        const int64_t k = w * num_bin + b;
+        len[q] = soff[k + 1] - soff[k];
     }
 }
 
-// Bin-major segment table, and the first edge of every window (win[w] =
-// soff[w * num_bin], win[num_win] = m).
-__global__ void k_prb_seg(const int32_t* __restrict__ soff, int32_t num_win, int32_t num_bin, int2* __restrict__ seg,
-                          int32_t* __restrict__ win) {
+// segtab[w * num_bin + b] = {window-major start, bin-major start}, segtab[K] =
+// {m, m}; binoff[b] = bin-major start of bin b (binoff[num_bin] = m);
+// win[w] = first edge of window w (win[num_win] = m).
+__global__ void k_prb_segtab(const int32_t* __restrict__ soff, const int32_t* __restrict__ pos_bm, int32_t num_win,
+                             int32_t num_bin, int64_t m, int2* __restrict__ segtab, int32_t* __restrict__ binoff,
+                             int32_t* __restrict__ win) {
     const int64_t total = static_cast<int64_t>(num_win) * num_bin;
     for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q <= total;
          q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        if (q < total) {
-            const int64_t b = q / num_win, w = q - b * num_win;
-            const int64_t k = w * num_bin + b;
-            seg[q] = make_int2(soff[k], soff[k + 1] - soff[k]);
-        }
-        if (q % num_bin == 0 && q / num_bin <= num_win) win[q / num_bin] = soff[q];
+        const int64_t w = q / num_bin, b = q - w * num_bin;
+        if (q < total) segtab[q] = make_int2(soff[q], pos_bm[b * num_win + w]);
+        else segtab[q] = make_int2(static_cast<int32_t>(m), static_cast<int32_t>(m));
+        if (b == 0) win[w] = soff[q];
+        if (q < num_bin) binoff[q] = pos_bm[q * num_win];
+        if (q == num_bin) binoff[q] = static_cast<int32_t>(m);
+    }
+}
+
+__global__ void k_prb_src(const uint32_t* __restrict__ word, int64_t m, uint16_t* __restrict__ src) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        src[e] = static_cast<uint16_t>(word[e] & 0xFFFFu);
+}
+
+// Destinations to bin-major order: one wave per segment.
+__global__ void k_prb_dst(const uint32_t* __restrict__ word, const int2* __restrict__ segtab, int64_t num_seg,
+                          uint16_t* __restrict__ dst) {
+    const int lane = lane_id();
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int64_t waves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
+    for (int64_t q = wave; q < num_seg; q += waves) {
+        const int2 a = segtab[q];
+        const int32_t len = segtab[q + 1].x - a.x;  // window-major segments are contiguous
+        for (int32_t i = lane; i < len; i += 64) dst[a.y + i] = static_cast<uint16_t>(word[a.x + i] >> 16);
+    }
+}
+
+// Per gather item: the bin of its first edge's segment (it.w) and the number
+// of segments its edges touch.
+__global__ void k_prb_item_segs(int4* __restrict__ items, int32_t num_items, const int2* __restrict__ segtab,
+                                int32_t num_bin, int32_t* __restrict__ nseg) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < num_items;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int4 it = items[i];
+        const int2* st = segtab + static_cast<int64_t>(it.x) * num_bin;
+        auto last_le = [&](int32_t e) {  // last b in [0, num_bin) with st[b].x <= e
+            int lo = 0, hi = num_bin;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (st[mid].x <= e) lo = mid;
+                else hi = mid;
+            }
+            return lo;
+        };
+        const int b0 = last_le(it.y), b1 = last_le(it.z - 1);
+        it.w = b0;
+        items[i] = it;
+        nseg[i] = b1 - b0 + 1;
     }
 }
 
@@ -306,6 +367,28 @@ unsigned grid_for(int64_t threads, int bt = 256) {
     return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(threads, bt), 65536)));
 }
 
+// Bins: consecutive destinations, closed before a bin would pass kBinMax
+// vertices or kBinEdges in-edges (a vertex with more in-edges than that is a
+// bin of its own).
+std::vector<int32_t> prb_bins(const std::vector<int32_t>& rp, int32_t n, std::vector<int64_t>* edges) {
+    std::vector<int32_t> bs{0};
+    int64_t e = 0;
+    int32_t start = 0;
+    for (int32_t v = 0; v < n; ++v) {
+        const int64_t d = rp[v + 1] - rp[v];
+        if (v > start && (v - start == kBinMax || e + d > kBinEdges)) {
+            bs.push_back(v);
+            edges->push_back(e);
+            start = v;
+            e = 0;
+        }
+        e += d;
+    }
+    bs.push_back(n);
+    edges->push_back(e);
+    return bs;
+}
+
 int build_prb_layout(ghet_graph* g, hipStream_t s) {
     PRBinLayout& L = g->prb;
     if (L.built) return GHET_OK;
@@ -314,32 +397,13 @@ int build_prb_layout(ghet_graph* g, hipStream_t s) {
     if (!g->out.present() || g->out.col == nullptr || g->out.m != m)
         return set_error(GHET_ERR_INVALID, "PageRank bins need the out-CSR with the in-CSR's edges");
     if (!pr_binned_fits(n, m)) return set_error(GHET_ERR_UNSUPPORTED, "PageRank bins: graph too large for the bin tables");
-    // Bins: consecutive destinations, closed before a bin would pass
-    // kBinMax vertices or kBinEdges in-edges (a vertex with more in-edges
-    // than that is a bin of its own).
-    const std::vector<int32_t>& rp = g->h_in_rowptr;
-    std::vector<int32_t> bs{0};
     std::vector<int64_t> bedges;
-    {
-        int64_t e = 0;
-        int32_t start = 0;
-        for (int32_t v = 0; v < n; ++v) {
-            const int64_t d = rp[v + 1] - rp[v];
-            if (v > start && (v - start == kBinMax || e + d > kBinEdges)) {
-                bs.push_back(v);
-                bedges.push_back(e);
-                start = v;
-                e = 0;
-            }
-            e += d;
-        }
-        bs.push_back(n);
-        bedges.push_back(e);
-    }
+    const std::vector<int32_t> bs = prb_bins(g->h_in_rowptr, n, &bedges);
     const int32_t NB = static_cast<int32_t>(bs.size() - 1);
     const int32_t NW = static_cast<int32_t>(ceil_div(n, kWin));
     const int64_t K = int64_t(NW) * NB;
-    if (K >= INT32_MAX) return set_error(GHET_ERR_UNSUPPORTED, "PageRank bins: %lld segments", static_cast<long long>(K));
+    if (NB > kMaxSeg || K >= INT32_MAX)
+        return set_error(GHET_ERR_UNSUPPORTED, "PageRank bins: %d bins over %d windows", NB, NW);
     std::vector<int32_t> order(NB);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return bedges[a] > bedges[b]; });
@@ -347,19 +411,21 @@ int build_prb_layout(ghet_graph* g, hipStream_t s) {
 
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t me = std::max<size_t>(size_t(m), 1);
-    const size_t bytes = 2 * al(2 * me) + al(4 * me) + al(8 * size_t(K)) + al(4 * (size_t(NB) + 1)) + al(4 * size_t(NB)) +
-                         al(16 * size_t(max_items)) + al(4 * size_t(n));
+    const size_t bytes = 2 * al(2 * me) + al(4 * (me + 1)) + al(8 * size_t(K + 1)) + 2 * al(4 * (size_t(NB) + 1)) +
+                         al(4 * size_t(NB)) + al(16 * size_t(max_items)) + al(4 * size_t(max_items)) + al(4 * size_t(n));
     GHET_HIP(hipMalloc(&L.mem, bytes));
     {
         char* p = static_cast<char*>(L.mem);
         auto take = [&](size_t b) { char* r = p; p += al(b); return r; };
         L.src = reinterpret_cast<uint16_t*>(take(2 * me));
         L.dst = reinterpret_cast<uint16_t*>(take(2 * me));
-        L.val = reinterpret_cast<float*>(take(4 * me));
-        L.seg = reinterpret_cast<int2*>(take(8 * size_t(K)));
+        L.val = reinterpret_cast<uint32_t*>(take(4 * (me + 1)));  // val[m]: the gather's spare slot
+        L.segtab = reinterpret_cast<int2*>(take(8 * size_t(K + 1)));
+        L.binoff = reinterpret_cast<int32_t*>(take(4 * (size_t(NB) + 1)));
         L.bin_start = reinterpret_cast<int32_t*>(take(4 * (size_t(NB) + 1)));
         L.bin_order = reinterpret_cast<int32_t*>(take(4 * size_t(NB)));
         L.items = reinterpret_cast<int4*>(take(16 * size_t(max_items)));
+        L.item_nseg = reinterpret_cast<int32_t*>(take(4 * size_t(max_items)));
         L.outdeg = reinterpret_cast<int32_t*>(take(4 * size_t(n)));
     }
     auto fail = [&](const char* what, hipError_t e) {
@@ -382,6 +448,9 @@ int build_prb_layout(ghet_graph* g, hipStream_t s) {
         uint32_t* wa = t.get<uint32_t>(me);
         uint32_t* wb = t.get<uint32_t>(me);
         int32_t* soff = t.get<int32_t>(size_t(K) + 1);
+        int32_t* len = t.get<int32_t>(size_t(K) + 1);
+        int32_t* pos = t.get<int32_t>(size_t(K) + 1);
+        int32_t* aux = t.get<int32_t>(size_t(scan_aux_ints(K)));
         int32_t* d_win = t.get<int32_t>(size_t(NW) + 1);
         if (t.err != hipSuccess) return fail("alloc", t.err);
         ok(hipMemcpyAsync(L.bin_start, bs.data(), 4 * bs.size(), hipMemcpyHostToDevice, s));
@@ -402,33 +471,55 @@ int build_prb_layout(ghet_graph* g, hipStream_t s) {
         }
         hipLaunchKernelGGL(k_prb_soff, dim3(grid_for(m + 1)), dim3(256), 0, s, kd.current(), m, static_cast<uint32_t>(K),
                            soff);
-        hipLaunchKernelGGL(k_prb_split, dim3(grid_for(m)), dim3(256), 0, s, wd.current(), m, L.src, L.dst);
-        hipLaunchKernelGGL(k_prb_seg, dim3(grid_for(K + 1)), dim3(256), 0, s, soff, NW, NB, L.seg, d_win);
+        hipLaunchKernelGGL(k_prb_lens, dim3(grid_for(K)), dim3(256), 0, s, soff, NW, NB, len);
+        ok(hipGetLastError());
+        if (t.err == hipSuccess && exclusive_scan_i32(len, pos, K, aux, s) != GHET_OK) t.err = hipErrorUnknown;
+        hipLaunchKernelGGL(k_prb_segtab, dim3(grid_for(K + 1)), dim3(256), 0, s, soff, pos, NW, NB, m, L.segtab, L.binoff,
+                           d_win);
+        hipLaunchKernelGGL(k_prb_src, dim3(grid_for(m)), dim3(256), 0, s, wd.current(), m, L.src);
+        hipLaunchKernelGGL(k_prb_dst, dim3(grid_for(K * 64)), dim3(256), 0, s, wd.current(), L.segtab, K, L.dst);
         ok(hipGetLastError());
         ok(hipMemcpyAsync(win.data(), d_win, 4 * win.size(), hipMemcpyDeviceToHost, s));
         ok(hipStreamSynchronize(s));
         if (t.err != hipSuccess) return fail("build", t.err);
     }
     // Gather items: each window's edges cut into ranges of at most
-    // kItemEdges; every index is checked here once.
+    // kItemEdges; the window offsets are checked here once.
     std::vector<int4> items;
     bool good = win[0] == 0 && win[NW] == m;
     for (int32_t w = 0; good && w < NW; ++w) {
         good = win[w] <= win[w + 1];
         for (int64_t e0 = win[w]; good && e0 < win[w + 1]; e0 += kItemEdges)
-            items.push_back(make_int4(w, static_cast<int32_t>(e0), static_cast<int32_t>(std::min<int64_t>(win[w + 1], e0 + kItemEdges)), 0));
+            items.push_back(make_int4(w, static_cast<int32_t>(e0),
+                                      static_cast<int32_t>(std::min<int64_t>(win[w + 1], e0 + kItemEdges)), 0));
     }
     if (!good || static_cast<int64_t>(items.size()) > max_items) {
         hipFree(L.mem);
         L = PRBinLayout{};
         return set_error(GHET_ERR_INVALID, "PageRank bins: window offsets failed their check");
     }
+    std::vector<int32_t> nseg(items.size());
     if (!items.empty()) {
-        const hipError_t e = hipMemcpyAsync(L.items, items.data(), 16 * items.size(), hipMemcpyHostToDevice, s);
+        hipError_t e = hipMemcpyAsync(L.items, items.data(), 16 * items.size(), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return fail("items", e);
+        hipLaunchKernelGGL(k_prb_item_segs, dim3(grid_for(int64_t(items.size()))), dim3(256), 0, s, L.items,
+                           static_cast<int32_t>(items.size()), L.segtab, NB, L.item_nseg);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(items.data(), L.items, 16 * items.size(), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(nseg.data(), L.item_nseg, 4 * nseg.size(), hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return fail("items", e);
     }
     const hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail("items", e);
+    // each item's segments fit the gather kernel's LDS table and stay inside
+    // the window's part of the segment table
+    for (size_t i = 0; i < items.size() && good; ++i)
+        good = items[i].w >= 0 && nseg[i] >= 1 && items[i].w + nseg[i] <= NB && nseg[i] <= kMaxSeg;
+    if (!good) {
+        hipFree(L.mem);
+        L = PRBinLayout{};
+        return set_error(GHET_ERR_INVALID, "PageRank bins: item segments failed their check");
+    }
     L.num_win = NW;
     L.num_bin = NB;
     L.num_items = static_cast<int32_t>(items.size());
@@ -454,11 +545,11 @@ int run_pr_binned(ghet_graph* g, int32_t iters, double damping, double* pr_out, 
     const PRBinLayout& L = g->prb;
     const int32_t init_blocks = static_cast<int32_t>(std::min<int64_t>(ceil_div(n, 256), 4096));
     void* p;
-    float *ca, *cb;
+    uint32_t *ca, *cb;
     double *rank, *out, *part;
     PRScalars* scal;
-    GHET_CHECK(g->ws.get(WS_PR_CONTRIB, 4 * size_t(n), &p)); ca = static_cast<float*>(p);
-    GHET_CHECK(g->ws.get(WS_LABEL_A, 4 * size_t(n), &p)); cb = static_cast<float*>(p);
+    GHET_CHECK(g->ws.get(WS_PR_CONTRIB, 4 * size_t(n), &p)); ca = static_cast<uint32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_LABEL_A, 4 * size_t(n), &p)); cb = static_cast<uint32_t*>(p);
     GHET_CHECK(g->ws.get(WS_PR_RANK, 8 * size_t(n), &p)); rank = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_PR_OUT, 8 * size_t(n), &p)); out = static_cast<double*>(p);
     GHET_CHECK(g->ws.get(WS_PR_PART, 8 * size_t(std::max(L.num_bin, init_blocks) + 1), &p)); part = static_cast<double*>(p);
@@ -468,17 +559,18 @@ int run_pr_binned(ghet_graph* g, int32_t iters, double damping, double* pr_out, 
     if (st) GHET_CHECK(kernel_events(g, 3 * iters, &kev));
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
-    hipLaunchKernelGGL(k_pr_init, dim3(init_blocks), dim3(256), 0, s, L.outdeg, n, inv_n, ca, part);
+    hipLaunchKernelGGL(k_prb_init, dim3(init_blocks), dim3(256), 0, s, L.outdeg, n, inv_n, ca, part);
     hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, init_blocks, &scal->sink, static_cast<int*>(nullptr));
     if (iters == 0) GHET_CHECK(fill<double>(rank, inv_n, n, s));
     for (int32_t it = 0; it < iters; ++it) {
         const int last = (it == iters - 1);
         if (st) GHET_HIP(hipEventRecord(kev[3 * it], s));
         if (L.num_items > 0)
-            hipLaunchKernelGGL(k_prb_gather, dim3(L.num_items), dim3(kPB), 0, s, L.items, L.src, ca, n, L.val);
+            hipLaunchKernelGGL(k_prb_gather, dim3(L.num_items), dim3(kPB), 0, s, L.items, L.item_nseg, L.segtab, L.num_bin,
+                               L.src, ca, n, static_cast<int32_t>(g->in.m), L.val);
         if (st) GHET_HIP(hipEventRecord(kev[3 * it + 1], s));
-        hipLaunchKernelGGL(k_prb_accum, dim3(L.num_bin), dim3(kPB), 0, s, L.bin_order, L.bin_start, L.seg, L.num_win,
-                           L.dst, L.val, L.outdeg, scal, static_cast<double>(n), damping, cb, part, rank, last);
+        hipLaunchKernelGGL(k_prb_accum, dim3(L.num_bin), dim3(kPB), 0, s, L.bin_order, L.bin_start, L.binoff, L.dst, L.val,
+                           L.outdeg, scal, static_cast<double>(n), damping, cb, part, rank, last);
         if (st) GHET_HIP(hipEventRecord(kev[3 * it + 2], s));
         hipLaunchKernelGGL(k_pr_reduce, dim3(1), dim3(1024), 0, s, part, L.num_bin, &scal->sink, static_cast<int*>(nullptr));
         std::swap(ca, cb);
