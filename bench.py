@@ -53,18 +53,39 @@ def pr_iter_bytes(n: int, m: int) -> int:
     return 4 * (n + 1) + 4 * m + 4 * n + 4 * n + 4 * n
 
 
+PR_SOURCES = ("pr.hip", "pr_bin.hip", "pr_common.hpp")
+
+
 def pr_source_hash() -> str:
-    """First 16 hex digits of the SHA-256 of the PageRank kernels' source."""
+    """First 16 hex digits of the SHA-256 of the PageRank kernels' sources
+    (both forms and their shared kernels, concatenated in a fixed order)."""
     import hashlib
-    with open(os.path.join(ROOT, "graphhet_amd", "csrc", "pr.hip"), "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()[:16]
+    h = hashlib.sha256()
+    for name in PR_SOURCES:
+        with open(os.path.join(ROOT, "graphhet_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def pr_stream_bytes(out_rowptr, n: int, m: int) -> int:
+    """Bytes the binned form (pr_bin.hip) moves per iteration: 2 B source +
+    4 B value written + 4 B value read + 2 B destination per edge, a 128 KB
+    window of contribution words per gather item (each source window of 2^15
+    ids cut into items of at most 2^17 edges), and per vertex the out-degree
+    read and the next contribution written."""
+    import numpy as np
+    starts = np.minimum(np.arange(0, n + (1 << 15), 1 << 15), n)
+    per_win = np.diff(np.asarray(out_rowptr, np.int64)[starts])
+    items = int(((per_win + (1 << 17) - 1) >> 17).sum())
+    return 12 * m + items * 4 * (1 << 15) + 8 * n
 
 
 def measured_traffic(workload: str, seed: int):
-    """L2-to-fabric bytes per PR iteration (k_pr_iter + k_pr_long) from the
-    rocprofv3 PMC runs recorded under profiles/ for this workload, and a note.
-    None when not measured, or when pr.hip has changed since the runs (the
-    file records the hash of the source it was measured with)."""
+    """L2-to-fabric bytes per PR iteration (both kernels of the default form)
+    from the rocprofv3 PMC runs recorded under profiles/ for this workload,
+    and a note.  None when not measured, or when the PR sources have changed
+    since the runs (the file records the hash of the sources it was measured
+    with)."""
     path = os.path.join(ROOT, "profiles", f"pr_iteration__{workload}.traffic.txt")
     if not os.path.exists(path):
         return None, "not measured for this workload"
@@ -80,9 +101,9 @@ def measured_traffic(workload: str, seed: int):
     if kv.get("workload") != workload or kv.get("seed") != str(seed):
         return None, "not measured for this workload"
     if kv.get("pr_src_sha") != pr_source_hash():
-        return None, "stale: pr.hip changed since the PMC runs"
+        return None, "stale: the PR sources changed since the PMC runs"
     return int(kv["traffic_bytes"]), ("L2-to-fabric bytes (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, MALL hits included), "
-                                      f"from {os.path.relpath(path, ROOT)}, measured with this pr.hip")
+                                      f"from {os.path.relpath(path, ROOT)}, measured with these PR sources")
 
 
 def gteps(edges, ms):
@@ -198,22 +219,32 @@ def main():
         bytes_it = pr_iter_bytes(n, m)
         pr_gbs = bytes_it / (k_ms * 1e-3) / 1e9
         traffic, traffic_note = measured_traffic(f"rmat-s{args.scale}-ef{int(args.edge_factor)}", args.seed)
-        # after the timed region: the phases timed apart (two launches per
-        # iteration), and the per-edge gather pattern's own rate over the
-        # same layout
-        ph = dg.pr(args.pr_iters, 0.85, out=False, phases=True)[1]
-        hot_ms = ph.extra[2] / max(ph.hot_kernel_launches, 1)
-        split_ms = ph.hot_kernel_ms / max(ph.hot_kernel_launches, 1)
-        ceil = dg.pr_gather_ceiling(5)
+        binned = runs[0]["pr"].extra[3] == 2
         roofline = {"bound": "hbm", "achieved": round(pr_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(pr_gbs / PEAK_HBM_GBS, 4), "traffic": traffic, "traffic_note": traffic_note,
-                    "kernel": "k_pr_iter+k_pr_long", "kernel_avg_us": round(k_ms * 1e3, 2),
-                    "bytes_per_launch": bytes_it,
-                    "phases_us": {"hot": round(hot_ms * 1e3, 2), "cold_and_finish": round((split_ms - hot_ms) * 1e3, 2),
-                                  "note": "timed as two launches per iteration, after the timed region"},
-                    "gather_ceiling_us": {"hot": round(ceil["hot_ms"] * 1e3, 2), "cold": round(ceil["cold_ms"] * 1e3, 2),
-                                          "hot_edges": ceil["hot_edges"], "cold_edges": ceil["cold_edges"]},
-                    "frac_of_gather_ceiling": round((ceil["hot_ms"] + ceil["cold_ms"]) / k_ms, 4) if k_ms > 0 else None}
+                    "kernel": "k_prb_gather+k_prb_accum" if binned else "k_pr_iter+k_pr_long",
+                    "kernel_avg_us": round(k_ms * 1e3, 2), "bytes_per_launch": bytes_it}
+        if binned:
+            # the two launches of an iteration, and the bytes the form streams
+            g_ms = med("pr", lambda s: s.extra[2] / max(s.hot_kernel_launches, 1))
+            sb = pr_stream_bytes(o.rowptr, n, m)
+            roofline["phases_us"] = {"gather": round(g_ms * 1e3, 2), "accumulate": round((k_ms - g_ms) * 1e3, 2)}
+            roofline["stream_bytes"] = sb
+            roofline["stream_gbs"] = round(sb / (k_ms * 1e-3) / 1e9, 1)
+        # after the timed region: the pull-gather form on the same graph, its
+        # phases timed apart, and the per-edge gather pattern's own rate over
+        # its layout
+        gf = dg.pr(args.pr_iters, 0.85, out=False, form="gather", phases=True)[1]
+        hot_ms = gf.extra[2] / max(gf.hot_kernel_launches, 1)
+        split_ms = gf.hot_kernel_ms / max(gf.hot_kernel_launches, 1)
+        ceil = dg.pr_gather_ceiling(5)
+        roofline["gather_form"] = {
+            "iteration_us": round(split_ms * 1e3, 2),
+            "phases_us": {"hot": round(hot_ms * 1e3, 2), "cold_and_finish": round((split_ms - hot_ms) * 1e3, 2),
+                          "note": "timed as two launches per iteration"},
+            "gather_ceiling_us": {"hot": round(ceil["hot_ms"] * 1e3, 2), "cold": round(ceil["cold_ms"] * 1e3, 2),
+                                  "hot_edges": ceil["hot_edges"], "cold_edges": ceil["cold_edges"]},
+            "frac_of_gather_ceiling": round((ceil["hot_ms"] + ceil["cold_ms"]) / split_ms, 4) if split_ms > 0 else None}
 
     plain = {}
     if sched != G.SCHED_HEURISTIC:

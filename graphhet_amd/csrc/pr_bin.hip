@@ -533,8 +533,12 @@ bool pr_binned_fits(int32_t n, int64_t m) {
     return n > 0 && ceil_div(n, kWin) <= kMaxWin && m < INT32_MAX;
 }
 
+// From 2^19 vertices on the bins win.  Measured per iteration (ms, binned
+// against gathered), R-MAT ef 16: s19 0.059 / 0.069, s20 0.068 / 0.092,
+// s21 0.129 / 0.177, s22 0.259 / 0.377, s24 0.889 / 1.62; Erdos-Renyi with
+// the s24 sizes 0.861 / 4.91 (no hot sources for the gathers to keep).
 bool pr_binned_default(int32_t n, int64_t m) {
-    return false && n > (1 << 21) && pr_binned_fits(n, m);
+    return n >= (1 << 19) && pr_binned_fits(n, m);
 }
 
 int run_pr_binned(ghet_graph* g, int32_t iters, double damping, double* pr_out, bool normalise, const ghet_opts* o,
