@@ -530,10 +530,13 @@ class DeviceGraph:
         _check(lib().ghet_pr_gather_ceiling(self._h, reps, C.byref(o), _ptr(r, C.c_double)))
         return {"hot_ms": float(r[0]), "cold_ms": float(r[1]), "hot_edges": int(r[2]), "cold_edges": int(r[3])}
 
-    def cdlp(self, iters: int = 10, out=True):
+    def cdlp(self, iters: int = 10, out=True, small_merge_table=False):
+        """small_merge_table: cap the hub merge tables at 64 slots (a test
+        setting that sends the merge through its sub-pass path)."""
         lab = np.empty(self.n, np.int32) if out else None
         st = RunStats()
-        _check(lib().ghet_cdlp(self._h, iters, _ptr(lab, C.c_int32), C.byref(_opts()), C.byref(st)))
+        o = _opts(flags=4 if small_merge_table else 0)
+        _check(lib().ghet_cdlp(self._h, iters, _ptr(lab, C.c_int32), C.byref(o), C.byref(st)))
         return lab, st
 
     def tc(self, out=True):

@@ -13,11 +13,15 @@
 // counts labels where it fits best:
 //   d <= 32      one lane, labels staged in LDS, pairwise counting;
 //   d <= 256     one wave, labels in LDS, each lane counts its labels;
-//   d <= 2048    one workgroup, open-addressing hash table in LDS (4096 slots);
-//   d > 2048     one workgroup per 4096-edge chunk counts in an LDS hash table,
-//                then merges (label, count) pairs into the vertex's global
-//                hash table (2^ceil(log2(2d)) slots); one workgroup takes the
-//                argmax and clears the table for the next iteration.
+//   d <= 2048    one 256-thread block, open-addressing hash table in LDS
+//                (the power of two >= 2d slots of a 4096-slot table);
+//   d <= 8192    one 1024-thread block, the same with a 16384-slot table;
+//   d > 8192     hubs: every 8192-edge chunk counts its labels in an LDS
+//                table and writes the distinct (label, count) pairs to its
+//                own region; a plan gives each hub ceil(pairs / 4096) merge
+//                items, item (hub, p) adds up the pairs whose label hashes to
+//                partition p in an LDS table and offers its winner with one
+//                64-bit atomicMax.  No global hash table, no table scan.
 // Winner = max count, ties to the smallest label: the maximum of the key
 // (count << 32) | (0xFFFFFFFF - label).  The first iteration, from identity
 // labels, has a closed form and skips the counting (k_cdlp_first_*).
@@ -30,8 +34,10 @@ namespace {
 
 constexpr int kBT = 256;
 constexpr int kSmall = 32, kWave = 256, kLds = 2048, kLdsSlots = 4096;
-constexpr int kHeavyChunk = 4096;  // edges per heavy-vertex chunk
-constexpr int kHeavySlots = 8192;  // LDS hash slots of a chunk (load <= 1/2)
+constexpr int kBig = 8192, kBigSlots = 16384, kBigBT = 1024;  // class 3 and the hub kernels
+constexpr int kChunk = 8192;          // edges per hub chunk (at most 8192 distinct labels: half a table)
+constexpr int kMergeTarget = 4096;    // pairs per hub merge item
+constexpr int kMaxHubChunks = 4096;   // chunks of one hub the merge kernel's LDS prefix holds
 constexpr int32_t kEmpty = -1;
 
 __device__ __forceinline__ uint64_t pack(uint32_t count, int32_t label) {
@@ -103,126 +109,239 @@ __global__ void __launch_bounds__(kBT) k_cdlp_wave(const int32_t* __restrict__ l
     }
 }
 
-__global__ void __launch_bounds__(kBT) k_cdlp_lds(const int32_t* __restrict__ list, int32_t count,
-                                                  const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                                  const int32_t* __restrict__ lab, int32_t* __restrict__ out) {
-    __shared__ int32_t keys[kLdsSlots];
-    __shared__ uint32_t cnts[kLdsSlots];
-    __shared__ uint64_t s_red[kBT / 64];
-    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
-        const int32_t v = list[i];
-        const int32_t b = rowptr[v], d = rowptr[v + 1] - b;
-        for (int j = threadIdx.x; j < kLdsSlots; j += kBT) {
-            keys[j] = kEmpty;
-            cnts[j] = 0;
+// Inserts one label into an open-addressing table of `mask + 1` slots
+// (keys kEmpty when free) and adds `c` to its count.  Returns false when the
+// table has no slot left for it (only a capped merge table can fill up).
+__device__ __forceinline__ bool table_add(int32_t* keys, uint32_t* cnts, uint32_t mask, int32_t x, uint32_t c) {
+    uint32_t h = hash_slot(x, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const int32_t prev = atomicCAS(&keys[h], kEmpty, x);
+        if (prev == kEmpty || prev == x) {
+            atomicAdd(&cnts[h], c);
+            return true;
         }
-        __syncthreads();
-        for (int j = threadIdx.x; j < d; j += kBT) {
-            const int32_t x = lab[col[b + j]];
-            uint32_t h = hash_slot(x, kLdsSlots - 1);
-            for (;;) {
-                int32_t prev = atomicCAS(&keys[h], kEmpty, x);
-                if (prev == kEmpty || prev == x) {
-                    atomicAdd(&cnts[h], 1u);
-                    break;
-                }
-                h = (h + 1) & (kLdsSlots - 1);
-            }
-        }
-        __syncthreads();
-        uint64_t best = 0;
-        for (int j = threadIdx.x; j < kLdsSlots; j += kBT) {
-            if (keys[j] != kEmpty) {
-                const uint64_t key = pack(cnts[j], keys[j]);
-                best = key > best ? key : best;
-            }
-        }
-        best = wave_max(best);
-        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = best;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint64_t t = 0;
-            for (int w = 0; w < kBT / 64; ++w) t = s_red[w] > t ? s_red[w] : t;
-            out[v] = unpack_label(t);
-        }
-        __syncthreads();
+        h = (h + 1) & mask;
     }
+    return false;
 }
 
-// Heavy vertices: chunk c counts the labels of edges [beg, beg+kHeavyChunk)
-// of heavy vertex hv in an LDS hash table first, then adds one (label,
-// count) pair per distinct label to the vertex's global table.  Once labels
-// converge most of a hub's neighbours share a few labels; counting edge by
-// edge in global memory would pile every one of them onto one counter.
-__global__ void __launch_bounds__(kBT) k_cdlp_heavy_count(
-    const int32_t* __restrict__ list, const int64_t* __restrict__ base, const int32_t* __restrict__ chunk_vtx,
-    const int32_t* __restrict__ chunk_beg, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-    const int32_t* __restrict__ lab, int32_t* tkeys, uint32_t* tcnts) {
-    __shared__ int32_t s_keys[kHeavySlots];
-    __shared__ uint32_t s_cnts[kHeavySlots];
-    const int32_t hv = chunk_vtx[blockIdx.x];
-    const int32_t v = list[hv];
-    const int32_t beg = chunk_beg[blockIdx.x], end = min(beg + kHeavyChunk, rowptr[v + 1]);
-    for (int j = threadIdx.x; j < kHeavySlots; j += kBT) {
-        s_keys[j] = kEmpty;
-        s_cnts[j] = 0;
-    }
-    __syncthreads();
-    for (int32_t j = beg + threadIdx.x; j < end; j += kBT) {
-        const int32_t x = lab[col[j]];
-        uint32_t h = hash_slot(x, kHeavySlots - 1);
-        for (;;) {
-            const int32_t prev = atomicCAS(&s_keys[h], kEmpty, x);
-            if (prev == kEmpty || prev == x) {
-                atomicAdd(&s_cnts[h], 1u);
-                break;
-            }
-            h = (h + 1) & (kHeavySlots - 1);
-        }
-    }
-    __syncthreads();
-    const int64_t b0 = base[hv];
-    const uint32_t mask = static_cast<uint32_t>(base[hv + 1] - b0 - 1);
-    int32_t* keys = tkeys + b0;
-    uint32_t* cnts = tcnts + b0;
-    for (int j = threadIdx.x; j < kHeavySlots; j += kBT) {
-        const int32_t x = s_keys[j];
-        if (x == kEmpty) continue;
-        uint32_t h = hash_slot(x, mask);
-        for (;;) {
-            const int32_t prev = atomicCAS(&keys[h], kEmpty, x);
-            if (prev == kEmpty || prev == x) {
-                atomicAdd(&cnts[h], s_cnts[j]);
-                break;
-            }
-            h = (h + 1) & mask;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(kBT) k_cdlp_heavy_pick(const int32_t* __restrict__ list,
-                                                         const int64_t* __restrict__ base, int32_t* tkeys,
-                                                         uint32_t* tcnts, int32_t* __restrict__ out) {
-    __shared__ uint64_t s_red[kBT / 64];
-    const int32_t hv = blockIdx.x;
-    const int64_t b0 = base[hv], size = base[hv + 1] - b0;
+// Packed winner of the first `slots` table entries, reduced over the block
+// (valid in every thread).  BT = threads per block.
+template <int BT>
+__device__ __forceinline__ uint64_t table_best(const int32_t* keys, const uint32_t* cnts, uint32_t slots,
+                                               uint64_t* s_red) {
     uint64_t best = 0;
-    for (int64_t j = threadIdx.x; j < size; j += kBT) {
-        const int32_t k = tkeys[b0 + j];
+    for (uint32_t j = threadIdx.x; j < slots; j += BT) {
+        const int32_t k = keys[j];
         if (k != kEmpty) {
-            const uint64_t key = pack(tcnts[b0 + j], k);
+            const uint64_t key = pack(cnts[j], k);
             best = key > best ? key : best;
-            tkeys[b0 + j] = kEmpty;
-            tcnts[b0 + j] = 0;
         }
     }
     best = wave_max(best);
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = best;
     __syncthreads();
+    uint64_t t = 0;
+    for (int w = 0; w < BT / 64; ++w) t = s_red[w] > t ? s_red[w] : t;
+    __syncthreads();
+    return t;
+}
+
+__host__ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t x) {
+    uint32_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// One block per vertex (grid-stride over the class list); the table uses the
+// power of two >= 2d of its SLOTS slots, so only that much is cleared.
+template <int BT, int SLOTS>
+__global__ void __launch_bounds__(BT) k_cdlp_lds(const int32_t* __restrict__ list, int32_t count,
+                                                 const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                 const int32_t* __restrict__ lab, int32_t* __restrict__ out) {
+    __shared__ int32_t keys[SLOTS];
+    __shared__ uint32_t cnts[SLOTS];
+    __shared__ uint64_t s_red[BT / 64];
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const int32_t v = list[i];
+        const int32_t b = rowptr[v], d = rowptr[v + 1] - b;
+        const uint32_t slots = pow2_at_least(2u * static_cast<uint32_t>(d));  // <= SLOTS: d <= SLOTS / 2
+        for (uint32_t j = threadIdx.x; j < slots; j += BT) {
+            keys[j] = kEmpty;
+            cnts[j] = 0;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < d; j += BT) table_add(keys, cnts, slots - 1, lab[col[b + j]], 1u);
+        __syncthreads();
+        const uint64_t t = table_best<BT>(keys, cnts, slots, s_red);
+        if (threadIdx.x == 0) out[v] = unpack_label(t);
+    }
+}
+
+// Hub chunk c: count the labels of edges [beg, beg + kChunk) in an LDS table
+// and write the distinct (label, count) pairs to pairs[c * kChunk ...]; the
+// pair count goes to chunk_pairs[c] and is added to the hub's total.
+__global__ void __launch_bounds__(kBigBT) k_cdlp_hub_count(
+    const int32_t* __restrict__ hubs, const int32_t* __restrict__ chunk_hub, const int32_t* __restrict__ chunk_beg,
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col, const int32_t* __restrict__ lab,
+    uint2* __restrict__ pairs, int32_t* __restrict__ chunk_pairs, int32_t* __restrict__ hub_pairs) {
+    __shared__ int32_t keys[kBigSlots];
+    __shared__ uint32_t cnts[kBigSlots];
+    __shared__ int32_t s_n;
+    const int32_t c = blockIdx.x, h = chunk_hub[c], v = hubs[h];
+    const int32_t beg = chunk_beg[c], end = min(beg + kChunk, rowptr[v + 1]);
+    const uint32_t slots = pow2_at_least(2u * static_cast<uint32_t>(end - beg));
+    for (uint32_t j = threadIdx.x; j < slots; j += kBigBT) {
+        keys[j] = kEmpty;
+        cnts[j] = 0;
+    }
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int32_t j = beg + threadIdx.x; j < end; j += kBigBT) table_add(keys, cnts, slots - 1, lab[col[j]], 1u);
+    __syncthreads();
+    uint2* out = pairs + static_cast<int64_t>(c) * kChunk;
+    for (uint32_t j = threadIdx.x; j < slots; j += kBigBT) {
+        const int32_t k = keys[j];
+        const int slot = wave_append(&s_n, k != kEmpty);
+        if (k != kEmpty) out[slot] = make_uint2(static_cast<uint32_t>(k), cnts[j]);
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int w = 0; w < kBT / 64; ++w) t = s_red[w] > t ? s_red[w] : t;
-        out[list[hv]] = unpack_label(t);
+        chunk_pairs[c] = s_n;
+        atomicAdd(&hub_pairs[h], s_n);
+    }
+}
+
+// Merge items per hub: ceil(pairs / kMergeTarget) (at least one), as an
+// exclusive scan into item_off[0..hubs]; clears the pair totals.  One block.
+__global__ void __launch_bounds__(1024) k_cdlp_hub_plan(int32_t* __restrict__ hub_pairs, int32_t hubs,
+                                                        int32_t* __restrict__ item_off) {
+    __shared__ int32_t s_wt[16];
+    __shared__ int32_t s_base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_base = 0;
+    __syncthreads();
+    for (int32_t h0 = 0; h0 < hubs; h0 += 1024) {
+        const int32_t h = h0 + tid;
+        int32_t p = 0;
+        if (h < hubs) {
+            p = max(1, (hub_pairs[h] + kMergeTarget - 1) / kMergeTarget);
+            hub_pairs[h] = 0;
+        }
+        int32_t incl = p;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) s_wt[wave] = incl;
+        __syncthreads();
+        int32_t wb = s_base;
+        for (int w = 0; w < wave; ++w) wb += s_wt[w];
+        if (h < hubs) item_off[h] = wb + incl - p;
+        __syncthreads();
+        if (tid == 1023) s_base = wb + incl;
+        __syncthreads();
+    }
+    if (tid == 0) item_off[hubs] = s_base;
+}
+
+// Merge item i = (hub h, partition p of P): the pairs of h's chunks whose
+// label hashes to p are added up in an LDS table and the best goes to
+// hub_best[h] by atomicMax (max is order-free, so the result does not depend
+// on which item lands first).  If the table fills up, the partition is redone
+// in 2, 4, ... sub-passes (each taking the labels of one sub-hash value)
+// until every sub-pass fits.  Persistent blocks walk the items.  cap (a power
+// of two) bounds the table; a cap of 64 is a test setting that forces the
+// sub-passes.
+__device__ __forceinline__ uint32_t hash_part(int32_t label, uint32_t parts, uint32_t mult) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(static_cast<uint32_t>(label) * mult) * parts) >> 32);
+}
+
+__global__ void __launch_bounds__(kBigBT) k_cdlp_hub_merge(
+    const int32_t* __restrict__ item_off, int32_t hubs, const int32_t* __restrict__ hub_chunk0,
+    const uint2* __restrict__ pairs, const int32_t* __restrict__ chunk_pairs, uint32_t cap,
+    unsigned long long* __restrict__ hub_best) {
+    __shared__ int32_t keys[kBigSlots];
+    __shared__ uint32_t cnts[kBigSlots];
+    __shared__ int32_t s_pre[kMaxHubChunks + 1];  // pair prefix over the hub's chunks
+    __shared__ uint64_t s_red[kBigBT / 64];
+    __shared__ int s_full;
+    const int tid = threadIdx.x;
+    const int32_t items = item_off[hubs];
+    for (int32_t i = blockIdx.x; i < items; i += gridDim.x) {
+        int32_t lo = 0, hi = hubs;  // last h with item_off[h] <= i (item_off[hubs] = items > i)
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (item_off[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const int32_t h = lo, P = item_off[h + 1] - item_off[h], p = i - item_off[h];
+        const int32_t c0 = hub_chunk0[h], nc = hub_chunk0[h + 1] - c0;
+        // prefix of the chunks' pair counts (nc <= kMaxHubChunks, checked by
+        // the layout): loaded in parallel, summed by one thread in LDS
+        for (int32_t k = tid; k < nc; k += kBigBT) s_pre[k + 1] = chunk_pairs[c0 + k];
+        __syncthreads();
+        if (tid == 0) {
+            s_pre[0] = 0;
+            for (int32_t k = 1; k <= nc; ++k) s_pre[k] += s_pre[k - 1];
+        }
+        __syncthreads();
+        const int32_t total = s_pre[nc];
+        // a table of four slots per pair this partition should get (at
+        // least 64, at most cap); a partition that gets more distinct
+        // labels than that takes the sub-pass path below
+        const uint32_t want = pow2_at_least(4u * static_cast<uint32_t>((total + P - 1) / P));
+        const uint32_t slots = want < 64u ? 64u : (want > cap ? cap : want);
+        uint64_t best = 0;
+        for (uint32_t nsub = 1;; nsub <<= 1) {
+            bool full = false;
+            uint64_t cand = 0;
+            for (uint32_t sub = 0; sub < nsub && !full; ++sub) {
+                for (uint32_t j = tid; j < slots; j += kBigBT) {
+                    keys[j] = kEmpty;
+                    cnts[j] = 0;
+                }
+                if (tid == 0) s_full = 0;
+                __syncthreads();
+                for (int32_t k = tid; k < total; k += kBigBT) {
+                    int32_t a = 0, b = nc;  // chunk of pair k: last a with s_pre[a] <= k
+                    while (b - a > 1) {
+                        const int32_t mid = (a + b) >> 1;
+                        if (s_pre[mid] <= k) a = mid;
+                        else b = mid;
+                    }
+                    const uint2 pr = pairs[static_cast<int64_t>(c0 + a) * kChunk + (k - s_pre[a])];
+                    const int32_t x = static_cast<int32_t>(pr.x);
+                    if (hash_part(x, P, 0xC2B2AE35u) == static_cast<uint32_t>(p) &&
+                        hash_part(x, nsub, 0x27D4EB2Fu) == sub && !table_add(keys, cnts, slots - 1, x, pr.y))
+                        s_full = 1;
+                }
+                __syncthreads();
+                full = s_full != 0;
+                if (!full) {
+                    const uint64_t t = table_best<kBigBT>(keys, cnts, slots, s_red);
+                    cand = t > cand ? t : cand;
+                } else {
+                    __syncthreads();
+                }
+            }
+            if (!full) {
+                best = cand;
+                break;
+            }
+        }
+        if (tid == 0 && best) atomicMax(&hub_best[h], static_cast<unsigned long long>(best));
+        __syncthreads();
+    }
+}
+
+// out[hub] = the merged winner; the slot is cleared for the next iteration.
+__global__ void k_cdlp_hub_final(const int32_t* __restrict__ hubs, int32_t count,
+                                 unsigned long long* __restrict__ hub_best, int32_t* __restrict__ out) {
+    for (int64_t h = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; h < count;
+         h += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        out[hubs[h]] = unpack_label(hub_best[h]);
+        hub_best[h] = 0ull;
     }
 }
 
@@ -305,7 +424,8 @@ __global__ void __launch_bounds__(kBT) k_cdlp_first_wave(const int32_t* __restri
     }
 }
 
-// One workgroup per heavy vertex; the in-list in 256-element steps.
+// One workgroup per vertex of more than 2048 neighbours; the in-list in
+// 256-element steps.
 __global__ void __launch_bounds__(kBT) k_cdlp_first_block(const int32_t* __restrict__ list, int32_t count,
                                                           const int32_t* __restrict__ rowptr,
                                                           const int32_t* __restrict__ col,
@@ -338,49 +458,47 @@ unsigned grid_cap(int64_t blocks) {
     return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 65535)));
 }
 
-// Degree class of a vertex (0 lane, 1 wave, 2 LDS hash, 3 heavy; 4: no
-// neighbours, keeps its label) as the key of a stable sort by class.
+// Degree class of a vertex (0 lane, 1 wave, 2 4K-slot table, 3 16K-slot
+// table, 4 hub; 5: no neighbours, keeps its label) as the key of a stable
+// sort by class.
 __global__ void k_cdlp_class(const int32_t* __restrict__ rp, int32_t n, uint32_t* __restrict__ key,
                              int32_t* __restrict__ val) {
     for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
          v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int32_t d = rp[v + 1] - rp[v];
-        key[v] = d == 0 ? 4u : d <= kSmall ? 0u : d <= kWave ? 1u : d <= kLds ? 2u : 3u;
+        key[v] = d == 0 ? 5u : d <= kSmall ? 0u : d <= kWave ? 1u : d <= kLds ? 2u : d <= kBig ? 3u : 4u;
         val[v] = static_cast<int32_t>(v);
     }
 }
 
-// off[c] = first position of class c in the sorted keys; off[5] = n.
+// off[c] = first position of class c in the sorted keys; off[6] = n.
 __global__ void k_cdlp_class_offsets(const uint32_t* __restrict__ key, int32_t n, int32_t* __restrict__ off) {
     for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e <= n;
          e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int64_t prev = e > 0 ? static_cast<int64_t>(key[e - 1]) : -1;
-        const int64_t cur = e < n ? static_cast<int64_t>(key[e]) : 5;
+        const int64_t cur = e < n ? static_cast<int64_t>(key[e]) : 6;
         for (int64_t k = prev + 1; k <= cur; ++k) off[k] = static_cast<int32_t>(e);
     }
 }
 
-// Per heavy vertex: hash slots (the power of two >= 2d) and chunk count.
-__global__ void k_cdlp_heavy_sizes(const int32_t* __restrict__ rp, const int32_t* __restrict__ heavy, int32_t h,
-                                   int64_t* __restrict__ cap, int32_t* __restrict__ nch) {
+// Chunks per hub.
+__global__ void k_cdlp_hub_nch(const int32_t* __restrict__ rp, const int32_t* __restrict__ hubs, int32_t h,
+                               int32_t* __restrict__ nch) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int32_t v = heavy[i], d = rp[v + 1] - rp[v];
-        int64_t c = 1;
-        while (c < 2LL * d) c <<= 1;
-        cap[i] = c;
-        nch[i] = (d + kHeavyChunk - 1) / kHeavyChunk;
+        const int32_t v = hubs[i];
+        nch[i] = (rp[v + 1] - rp[v] + kChunk - 1) / kChunk;
     }
 }
 
-__global__ void k_cdlp_heavy_chunks(const int32_t* __restrict__ rp, const int32_t* __restrict__ heavy, int32_t h,
-                                    const int32_t* __restrict__ choff, int32_t* __restrict__ cv,
-                                    int32_t* __restrict__ cb) {
+__global__ void k_cdlp_hub_chunks(const int32_t* __restrict__ rp, const int32_t* __restrict__ hubs, int32_t h,
+                                  const int32_t* __restrict__ choff, int32_t* __restrict__ cv,
+                                  int32_t* __restrict__ cb) {
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int32_t v = heavy[i];
+        const int32_t v = hubs[i];
         int32_t o = choff[i];
-        for (int32_t e = rp[v]; e < rp[v + 1]; e += kHeavyChunk, ++o) {
+        for (int32_t e = rp[v]; e < rp[v + 1]; e += kChunk, ++o) {
             cv[o] = static_cast<int32_t>(i);
             cb[o] = e;
         }
@@ -388,8 +506,7 @@ __global__ void k_cdlp_heavy_chunks(const int32_t* __restrict__ rp, const int32_
 }
 
 // The class lists (ids ascending within a class) from a stable radix sort
-// of the class keys, then the heavy vertices' hash-table offsets and
-// 4096-edge chunks from scans; all on the GPU.
+// of the class keys, then the hubs' chunks from a scan; all on the GPU.
 int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
     CDLPLayout& L = g->cdlp;
     if (L.built) return GHET_OK;
@@ -397,7 +514,6 @@ int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
     const int32_t* rp = g->all.rowptr;
     uint32_t *ka = nullptr, *kb = nullptr;
     int32_t *va = nullptr, *vb = nullptr, *off = nullptr, *nch = nullptr, *choff = nullptr, *aux32 = nullptr;
-    int64_t *cap = nullptr, *aux64 = nullptr, *hbase = nullptr;
     void* tmp = nullptr;
     size_t tb = 0;
     hipError_t e = hipSuccess;
@@ -424,63 +540,75 @@ int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
                            kd.current(), n, off);
         e = hipGetLastError();
     }
-    int32_t h_off[6] = {};
+    int32_t h_off[7] = {};
     if (e == hipSuccess) e = hipMemcpyAsync(h_off, off, sizeof(h_off), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     const int32_t* sorted = vd.current();
-    const int32_t nh = h_off[4] - h_off[3];
-    const int32_t* heavy = sorted + h_off[3];
-    get(reinterpret_cast<void**>(&cap), 8 * size_t(nh));
-    get(reinterpret_cast<void**>(&hbase), 8 * (size_t(nh) + 1));
+    const int32_t nh = h_off[5] - h_off[4];
+    const int32_t* hubs = sorted + h_off[4];
     get(reinterpret_cast<void**>(&nch), 4 * size_t(nh));
     get(reinterpret_cast<void**>(&choff), 4 * (size_t(nh) + 1));
     get(reinterpret_cast<void**>(&aux32), 4 * size_t(scan_aux_ints(nh)));
-    get(reinterpret_cast<void**>(&aux64), 8 * size_t(scan_aux_ints(nh)));
     if (e == hipSuccess && nh > 0) {
-        hipLaunchKernelGGL(k_cdlp_heavy_sizes, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rp, heavy, nh, cap, nch);
+        hipLaunchKernelGGL(k_cdlp_hub_nch, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rp, hubs, nh, nch);
         e = hipGetLastError();
     }
-    if (e == hipSuccess && exclusive_scan_i64(cap, hbase, nh, aux64, s) != GHET_OK) e = hipErrorUnknown;
     if (e == hipSuccess && exclusive_scan_i32(nch, choff, nh, aux32, s) != GHET_OK) e = hipErrorUnknown;
-    int64_t slots = 0;
+    std::vector<int32_t> h_nch(static_cast<size_t>(nh));
     int32_t chunks = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&slots, hbase + nh, 8, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&chunks, choff + nh, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && nh > 0) e = hipMemcpyAsync(h_nch.data(), nch, 4 * size_t(nh), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const int32_t max_nch = nh > 0 ? *std::max_element(h_nch.begin(), h_nch.end()) : 0;
+    if (e == hipSuccess && max_nch > kMaxHubChunks) {
+        for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(va), static_cast<void*>(vb),
+                        static_cast<void*>(off), static_cast<void*>(nch), static_cast<void*>(choff),
+                        static_cast<void*>(aux32), tmp})
+            if (q) hipFree(q);
+        return set_error(GHET_ERR_UNSUPPORTED, "CDLP: a vertex of more than %d neighbours", kMaxHubChunks * kChunk);
+    }
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     size_t bytes = 0;
-    for (int c = 0; c < 4; ++c) bytes += (4 * size_t(h_off[c + 1] - h_off[c]) + 255) & ~size_t(255);
-    bytes += ((8 * (size_t(nh) + 1) + 255) & ~size_t(255)) + 2 * ((4 * size_t(chunks) + 255) & ~size_t(255));
-    bytes += 2 * ((4 * size_t(slots) + 255) & ~size_t(255));
+    for (int c = 0; c < 5; ++c) bytes += al(4 * size_t(h_off[c + 1] - h_off[c]));
+    bytes += 2 * al(4 * size_t(chunks)) + al(4 * (size_t(nh) + 1)) + al(8 * size_t(chunks) * kChunk) +
+             al(4 * size_t(chunks)) + al(4 * size_t(nh)) + al(4 * (size_t(nh) + 1)) + al(8 * size_t(nh));
     if (e == hipSuccess) e = hipMalloc(&L.mem, std::max<size_t>(bytes, 256));
     if (e == hipSuccess) {
         char* p = static_cast<char*>(L.mem);
-        auto take = [&](size_t b) { char* r = p; p += (b + 255) & ~size_t(255); return r; };
-        for (int c = 0; c < 4 && e == hipSuccess; ++c) {
+        auto take = [&](size_t b) { char* r = p; p += al(b); return r; };
+        for (int c = 0; c < 5 && e == hipSuccess; ++c) {
             L.count[c] = h_off[c + 1] - h_off[c];
             L.list[c] = reinterpret_cast<int32_t*>(take(4 * size_t(L.count[c])));
             if (L.count[c] > 0)
                 e = hipMemcpyAsync(L.list[c], sorted + h_off[c], 4 * size_t(L.count[c]), hipMemcpyDeviceToDevice, s);
         }
-        L.heavy_base = reinterpret_cast<int64_t*>(take(8 * (size_t(nh) + 1)));
-        if (e == hipSuccess) e = hipMemcpyAsync(L.heavy_base, hbase, 8 * (size_t(nh) + 1), hipMemcpyDeviceToDevice, s);
-        L.heavy_chunks = chunks;
-        L.heavy_chunk_vtx = reinterpret_cast<int32_t*>(take(4 * size_t(chunks)));
-        L.heavy_chunk_beg = reinterpret_cast<int32_t*>(take(4 * size_t(chunks)));
+        L.count[5] = h_off[6] - h_off[5];
+        L.hub_chunks = chunks;
+        L.max_hub_chunks = max_nch;
+        L.chunk_hub = reinterpret_cast<int32_t*>(take(4 * size_t(chunks)));
+        L.chunk_beg = reinterpret_cast<int32_t*>(take(4 * size_t(chunks)));
+        L.hub_chunk0 = reinterpret_cast<int32_t*>(take(4 * (size_t(nh) + 1)));
+        L.pairs = reinterpret_cast<uint2*>(take(8 * size_t(chunks) * kChunk));
+        L.chunk_pairs = reinterpret_cast<int32_t*>(take(4 * size_t(chunks)));
+        L.hub_pairs = reinterpret_cast<int32_t*>(take(4 * size_t(nh)));
+        L.item_off = reinterpret_cast<int32_t*>(take(4 * (size_t(nh) + 1)));
+        L.hub_best = reinterpret_cast<unsigned long long*>(take(8 * size_t(nh)));
+        if (e == hipSuccess) e = hipMemcpyAsync(L.hub_chunk0, choff, 4 * (size_t(nh) + 1), hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess && nh > 0) {
-            hipLaunchKernelGGL(k_cdlp_heavy_chunks, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rp, heavy, nh,
-                               choff, L.heavy_chunk_vtx, L.heavy_chunk_beg);
+            hipLaunchKernelGGL(k_cdlp_hub_chunks, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rp, hubs, nh, choff,
+                               L.chunk_hub, L.chunk_beg);
             e = hipGetLastError();
         }
-        L.heavy_slots = slots;
-        L.heavy_keys = reinterpret_cast<int32_t*>(take(4 * size_t(slots)));
-        L.heavy_cnts = reinterpret_cast<uint32_t*>(take(4 * size_t(slots)));
-        if (e == hipSuccess && slots) e = hipMemsetAsync(L.heavy_keys, 0xFF, 4 * size_t(slots), s);
-        if (e == hipSuccess && slots) e = hipMemsetAsync(L.heavy_cnts, 0, 4 * size_t(slots), s);
+        // the plan clears hub_pairs after reading them and the final clears
+        // hub_best, so they start at zero once here (fill kernels, no memsets)
+        if (e == hipSuccess && nh > 0 && fill<int32_t>(L.hub_pairs, 0, nh, s) != GHET_OK) e = hipErrorUnknown;
+        if (e == hipSuccess && nh > 0 && fill<uint64_t>(reinterpret_cast<uint64_t*>(L.hub_best), 0, nh, s) != GHET_OK)
+            e = hipErrorUnknown;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(va), static_cast<void*>(vb),
-                    static_cast<void*>(off), static_cast<void*>(cap), static_cast<void*>(hbase), static_cast<void*>(nch),
-                    static_cast<void*>(choff), static_cast<void*>(aux32), static_cast<void*>(aux64), tmp})
+                    static_cast<void*>(off), static_cast<void*>(nch), static_cast<void*>(choff), static_cast<void*>(aux32),
+                    tmp})
         if (q) hipFree(q);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -510,6 +638,14 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
     const bool timing = st != nullptr;
     hipEvent_t* kev = nullptr;
     if (timing) GHET_CHECK(kernel_events(g, 2 * std::max(iters, 1), &kev));
+    // hub merge: persistent blocks (one per CU: the table is 128 KB of LDS);
+    // opts reserved[0] bit 2 caps its table at 64 slots, a test setting that
+    // makes the merge take its sub-pass path
+    int cus = 0;
+    GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    const int32_t merge_blocks = std::max(1, cus);
+    const bool small_cap = o && o->size >= sizeof(ghet_opts) && (o->reserved[0] & 4);
+    const uint32_t merge_cap = small_cap ? 64u : static_cast<uint32_t>(kBigSlots);
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_label_init, dim3(grid_cap(ceil_div(n, 256))), dim3(256), 0, s, la, lb, n);
@@ -530,9 +666,10 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
                 if (L.count[c])
                     hipLaunchKernelGGL(k_cdlp_first_wave, dim3(grid_cap(ceil_div(L.count[c], kBT / 64))), dim3(kBT), 0,
                                        s, L.list[c], L.count[c], A, C, in_rp, lb);
-            if (L.count[3])
-                hipLaunchKernelGGL(k_cdlp_first_block, dim3(grid_cap(L.count[3])), dim3(kBT), 0, s, L.list[3],
-                                   L.count[3], A, C, in_rp, lb);
+            for (int c = 3; c <= 4; ++c)
+                if (L.count[c])
+                    hipLaunchKernelGGL(k_cdlp_first_block, dim3(grid_cap(L.count[c])), dim3(kBT), 0, s, L.list[c],
+                                       L.count[c], A, C, in_rp, lb);
             if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
             std::swap(la, lb);
             continue;
@@ -544,13 +681,19 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
             hipLaunchKernelGGL(k_cdlp_wave, dim3(grid_cap(ceil_div(L.count[1], kBT / 64))), dim3(kBT), 0, s,
                                L.list[1], L.count[1], A, C, la, lb);
         if (L.count[2])
-            hipLaunchKernelGGL(k_cdlp_lds, dim3(grid_cap(L.count[2])), dim3(kBT), 0, s, L.list[2], L.count[2], A, C,
-                               la, lb);
-        if (L.count[3]) {
-            hipLaunchKernelGGL(k_cdlp_heavy_count, dim3(L.heavy_chunks), dim3(kBT), 0, s, L.list[3], L.heavy_base,
-                               L.heavy_chunk_vtx, L.heavy_chunk_beg, A, C, la, L.heavy_keys, L.heavy_cnts);
-            hipLaunchKernelGGL(k_cdlp_heavy_pick, dim3(L.count[3]), dim3(kBT), 0, s, L.list[3], L.heavy_base,
-                               L.heavy_keys, L.heavy_cnts, lb);
+            hipLaunchKernelGGL((k_cdlp_lds<kBT, kLdsSlots>), dim3(grid_cap(L.count[2])), dim3(kBT), 0, s, L.list[2],
+                               L.count[2], A, C, la, lb);
+        if (L.count[3])
+            hipLaunchKernelGGL((k_cdlp_lds<kBigBT, kBigSlots>), dim3(grid_cap(L.count[3])), dim3(kBigBT), 0, s, L.list[3],
+                               L.count[3], A, C, la, lb);
+        if (L.count[4]) {
+            hipLaunchKernelGGL(k_cdlp_hub_count, dim3(L.hub_chunks), dim3(kBigBT), 0, s, L.list[4], L.chunk_hub,
+                               L.chunk_beg, A, C, la, L.pairs, L.chunk_pairs, L.hub_pairs);
+            hipLaunchKernelGGL(k_cdlp_hub_plan, dim3(1), dim3(1024), 0, s, L.hub_pairs, L.count[4], L.item_off);
+            hipLaunchKernelGGL(k_cdlp_hub_merge, dim3(merge_blocks), dim3(kBigBT), 0, s, L.item_off, L.count[4],
+                               L.hub_chunk0, L.pairs, L.chunk_pairs, merge_cap, L.hub_best);
+            hipLaunchKernelGGL(k_cdlp_hub_final, dim3(grid_cap(ceil_div(L.count[4], 256))), dim3(256), 0, s, L.list[4],
+                               L.count[4], L.hub_best, lb);
         }
         if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
         std::swap(la, lb);
@@ -572,7 +715,8 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
         st->hot_kernel_ms = hot;
         st->hot_kernel_launches = iters;
         st->levels = iters;
-        for (int c = 0; c < 4; ++c) st->extra[c] = L.count[c];
+        for (int c = 0; c < 3; ++c) st->extra[c] = L.count[c];
+        st->extra[3] = L.count[3] + L.count[4];  // vertices counted by blocks of 1024 threads (16K tables, hubs)
     }
     return GHET_OK;
 }

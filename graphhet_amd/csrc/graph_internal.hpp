@@ -106,18 +106,25 @@ struct PRBinLayout {
     void* mem = nullptr;
 };
 
-// Degree buckets over the all-CSR for CDLP (vertex lists per class).
+// Degree buckets over the all-CSR for CDLP (vertex lists per class: 0 lane,
+// 1 wave, 2 block with a 4K-slot LDS table, 3 block with a 16K-slot table,
+// 4 hub, 5 no neighbours).  A hub's edges are cut into 8192-edge chunks;
+// each chunk writes its distinct (label, count) pairs to its own region of
+// `pairs`, and merge items (hub, hash partition) add them up.
 struct CDLPLayout {
     bool built = false;
-    int32_t count[5] = {};  // per class: 0 small, 1 wave, 2 lds, 3 heavy
-    int32_t* list[5] = {};
-    int64_t heavy_slots = 0;       // total hash slots for heavy vertices
-    int64_t* heavy_base = nullptr; // [count[3]+1] slot offset per heavy vertex
-    int32_t heavy_chunks = 0;
-    int32_t* heavy_chunk_vtx = nullptr;  // [heavy_chunks] index into heavy list
-    int32_t* heavy_chunk_beg = nullptr;  // [heavy_chunks] first edge of the chunk
-    int32_t* heavy_keys = nullptr;       // [heavy_slots] labels (-1 = empty)
-    uint32_t* heavy_cnts = nullptr;      // [heavy_slots] counts
+    int32_t count[6] = {};
+    int32_t* list[6] = {};
+    int32_t hub_chunks = 0;
+    int32_t* chunk_hub = nullptr;    // [hub_chunks] index into list[4]
+    int32_t* chunk_beg = nullptr;    // [hub_chunks] first edge of the chunk
+    int32_t* hub_chunk0 = nullptr;   // [count[4] + 1] first chunk of each hub
+    int32_t max_hub_chunks = 0;      // most chunks of one hub
+    uint2* pairs = nullptr;          // [hub_chunks * 8192] {label, count}
+    int32_t* chunk_pairs = nullptr;  // [hub_chunks] pairs a chunk wrote
+    int32_t* hub_pairs = nullptr;    // [count[4]] pairs of each hub (zeroed by the plan)
+    int32_t* item_off = nullptr;     // [count[4] + 1] first merge item of each hub; [count[4]] = items
+    unsigned long long* hub_best = nullptr;  // [count[4]] packed (count, label) winner (zeroed by the final)
     void* mem = nullptr;
 };
 

@@ -227,6 +227,8 @@ typedef struct ghet_opts {
     /* reserved[0] bit0: PR without the FP64 normalisation (reference form)
      *             bit1: PR runs its hot and cold gathers as two launches and
      *                   reports the hot phase's ms in stats extra[2]
+     *             bit2: CDLP caps its hub merge tables at 64 slots (a test
+     *                   setting: the merge then takes its sub-pass path)
      * reserved[1] bit0: many-graph calls skip the autotuner
      * reserved[2]     : batched BFS pushes while frontier out-edges < m / this
      *                   (0 = default 16)

@@ -380,3 +380,30 @@ def test_pr_binned_edge_cases():
     np.testing.assert_allclose(r0, np.full(n, 1.0 / n), rtol=1e-15)
     with pytest.raises(ValueError):
         dg.pr(3, form="binned", split=(4, 6))
+
+
+def test_cdlp_hub_merge_partitions():
+    """A hub whose neighbours keep many distinct labels: hub 0 points to N
+    leaves, each leaf is tied both ways to two private partners (so a leaf's
+    label is its smaller partner), and the hub sees N distinct labels -- its
+    pairs need several merge partitions.  Also with the merge tables capped
+    at 64 slots (the sub-pass path), and a second hub of 9000 neighbours
+    (two chunks)."""
+    N = 20000
+    src, dst = [], []
+    for i in range(1, N + 1):
+        a, b = N + 2 * i - 1, N + 2 * i
+        src += [0, i, a, i, b, a, b]
+        dst += [i, a, i, b, i, b, a]
+    n = 3 * N + 1 + 9001
+    h2 = 3 * N + 1
+    for j in range(1, 9001):
+        src += [h2 + j, h2]
+        dst += [h2, h2 + j]
+    hg = G.HostGraph.from_edges(n, src, dst)
+    o, i, a = hg.view(G.VIEW_OUT), hg.view(G.VIEW_IN), hg.view(G.VIEW_ALL)
+    dg = G.DeviceGraph(o, i, a)
+    for iters in (2, 3, 5):
+        ref = O.cdlp(i.rowptr, i.col, o.rowptr, o.col, iters)
+        np.testing.assert_array_equal(dg.cdlp(iters)[0], ref, err_msg=f"{iters} iterations")
+        np.testing.assert_array_equal(dg.cdlp(iters, small_merge_table=True)[0], ref, err_msg=f"capped, {iters}")
