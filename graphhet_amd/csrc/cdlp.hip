@@ -11,11 +11,13 @@
 //
 // No sort: vertices are bucketed by all-degree d once per graph and each class
 // counts labels where it fits best:
-//   d <= 32      one lane, labels staged in LDS, pairwise counting;
-//   d <= 256     one wave, labels in LDS, each lane counts its labels;
-//   d <= 2048    one 256-thread block, open-addressing hash table in LDS
-//                (the power of two >= 2d slots of a 4096-slot table);
-//   d <= 8192    one 1024-thread block, the same with a 16384-slot table;
+//   d <= 32      one lane; a wave gathers the labels of 64 such vertices
+//                into LDS together, then each lane counts pairwise;
+//   d <= 256     one wave, labels in LDS, counted in the wave's 512-slot
+//                open-addressing hash table;
+//   d <= 8192    one 256-thread block, open-addressing hash table in LDS
+//                (the power of two >= 2d slots of a 4096-slot table; above
+//                2048 neighbours, sub-passes if the labels overflow it);
 //   d > 8192     hubs: every 8192-edge chunk counts its labels in an LDS
 //                table and writes the distinct (label, count) pairs to its
 //                own region; a plan gives each hub ceil(pairs / 4096) merge
@@ -58,17 +60,84 @@ __global__ void k_label_init(int32_t* a, int32_t* b, int32_t n) {
     }
 }
 
+// Per-wave tiles for the lane class.  A wave takes a tile of 64 consecutive
+// list entries (at most 2048 labels), scans their degrees and gathers all
+// their labels into its LDS buffer in edge order: lane l takes the tile's
+// edges l, l + 64, ..., finds each edge's vertex by binary search over the
+// tile's offsets and issues kTileLoads edges' loads together.  The counting
+// then reads LDS only.
+constexpr int kTileLabels = 2048;
+constexpr int kTileLoads = 8;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+}
+
+// The tile's vertex of lane `lane` < nv is (b, d); lanes past the tile pass
+// d = 0.  TILE (a power of two) entries of s_off/s_b are used.  Returns the
+// lane's first label position; *total = the tile's labels.
+template <int TILE>
+__device__ __forceinline__ int32_t tile_gather(int32_t b, int32_t d, int32_t* s_off, int32_t* s_b, int32_t* s_lab,
+                                               const int32_t* __restrict__ col, const int32_t* __restrict__ lab,
+                                               int32_t* total) {
+    const int lane = lane_id();
+    int32_t incl = d;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const int32_t off = incl - d;
+    const int32_t E = __shfl(incl, 63);
+    if (lane < TILE) {
+        s_off[lane] = off;
+        s_b[lane] = b;
+    }
+    wave_lds_sync();
+    for (int32_t k0 = 0; k0 < E; k0 += 64 * kTileLoads) {
+        int32_t ce[kTileLoads];
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) {
+            const int32_t k = k0 + u * 64 + lane;
+            int j = 0;  // last tile vertex with s_off[j] <= k (degrees are >= 1)
+#pragma unroll
+            for (int step = TILE / 2; step >= 1; step >>= 1) j += s_off[j + step] <= k ? step : 0;
+            ce[u] = k < E ? s_b[j] + (k - s_off[j]) : 0;  // lanes past the tile load edge 0
+        }
+        int32_t cv[kTileLoads];
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) cv[u] = col[ce[u]];
+#pragma unroll
+        for (int u = 0; u < kTileLoads; ++u) {
+            const int32_t k = k0 + u * 64 + lane;
+            const int32_t x = lab[cv[u]];
+            if (k < E) s_lab[k] = x;
+        }
+    }
+    wave_lds_sync();
+    *total = E;
+    return off;
+}
+
+// d <= 32: lane per vertex, tiles of 64 list entries; pairwise counting.
 __global__ void __launch_bounds__(kBT) k_cdlp_small(const int32_t* __restrict__ list, int32_t count,
                                                     const int32_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ col, const int32_t* __restrict__ lab,
                                                     int32_t* __restrict__ out) {
-    __shared__ int32_t s[kBT * kSmall];
-    int32_t* mine = s + threadIdx.x * kSmall;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x; i < count;
-         i += static_cast<int64_t>(gridDim.x) * kBT) {
-        const int32_t v = list[i];
-        const int32_t b = rowptr[v], d = rowptr[v + 1] - b;
-        for (int j = 0; j < d; ++j) mine[j] = lab[col[b + j]];
+    __shared__ int32_t s_lab[kBT / 64][kTileLabels];
+    __shared__ int32_t s_off[kBT / 64][64];
+    __shared__ int32_t s_b[kBT / 64][64];
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int64_t tiles = ceil_div(count, 64);
+    for (int64_t t = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6; t < tiles;
+         t += (static_cast<int64_t>(gridDim.x) * kBT) >> 6) {
+        const int64_t i = t * 64 + lane;
+        const int32_t v = i < count ? list[i] : 0;
+        const int32_t b = i < count ? rowptr[v] : 0;
+        const int32_t d = i < count ? rowptr[v + 1] - b : 0;
+        int32_t E;
+        const int32_t off = tile_gather<64>(b, d, s_off[w], s_b[w], s_lab[w], col, lab, &E);
+        const int32_t* mine = s_lab[w] + off;
         uint64_t best = 0;
         for (int j = 0; j < d; ++j) {
             const int32_t x = mine[j];
@@ -77,35 +146,40 @@ __global__ void __launch_bounds__(kBT) k_cdlp_small(const int32_t* __restrict__ 
             const uint64_t key = pack(c, x);
             best = key > best ? key : best;
         }
-        out[v] = unpack_label(best);
+        if (i < count) out[v] = unpack_label(best);
+        wave_lds_sync();  // the next tile's gather overwrites s_lab
     }
 }
 
+// Counts the labels seg[0, d) (d <= 256) in the wave's 512-slot LDS table
+// and returns the packed winner in every lane.
+__device__ __forceinline__ uint64_t wave_hash_best(const int32_t* seg, int32_t d, int32_t* keys, uint32_t* cnts);
+
+// 32 < d <= 256: one wave per vertex; its labels are loaded together into
+// LDS and counted in the wave's hash table.
 __global__ void __launch_bounds__(kBT) k_cdlp_wave(const int32_t* __restrict__ list, int32_t count,
                                                    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                    const int32_t* __restrict__ lab, int32_t* __restrict__ out) {
-    __shared__ int32_t s[(kBT / 64) * kWave];
-    const int lane = lane_id();
-    int32_t* mine = s + (threadIdx.x >> 6) * kWave;
-    const int64_t wave_global = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6;
-    const int64_t wave_stride = (static_cast<int64_t>(gridDim.x) * kBT) >> 6;
-    for (int64_t i = wave_global; i < count; i += wave_stride) {
+    __shared__ int32_t s_lab[kBT / 64][kWave];
+    __shared__ int32_t s_keys[kBT / 64][2 * kWave];
+    __shared__ uint32_t s_cnts[kBT / 64][2 * kWave];
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    for (int64_t i = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6; i < count;
+         i += (static_cast<int64_t>(gridDim.x) * kBT) >> 6) {
         const int32_t v = list[i];
         const int32_t b = rowptr[v], d = rowptr[v + 1] - b;
-        for (int j = lane; j < d; j += 64) mine[j] = lab[col[b + j]];
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        uint64_t best = 0;
-        for (int j = lane; j < d; j += 64) {
-            const int32_t x = mine[j];
-            uint32_t c = 0;
-            for (int k = 0; k < d; ++k) c += (mine[k] == x);
-            const uint64_t key = pack(c, x);
-            best = key > best ? key : best;
+        int32_t cv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cv[u] = col[b + min(u * 64 + lane, d - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int32_t x = lab[cv[u]];
+            if (u * 64 + lane < d) s_lab[w][u * 64 + lane] = x;
         }
-        best = wave_max(best);
+        wave_lds_sync();
+        const uint64_t best = wave_hash_best(s_lab[w], d, s_keys[w], s_cnts[w]);
         if (lane == 0) out[v] = unpack_label(best);
-        __builtin_amdgcn_wave_barrier();
+        wave_lds_sync();
     }
 }
 
@@ -123,6 +197,30 @@ __device__ __forceinline__ bool table_add(int32_t* keys, uint32_t* cnts, uint32_
         h = (h + 1) & mask;
     }
     return false;
+}
+
+__device__ __forceinline__ uint64_t wave_hash_best(const int32_t* seg, int32_t d, int32_t* keys, uint32_t* cnts) {
+    const int lane = lane_id();
+    uint32_t slots = 64;
+    while (slots < 2u * static_cast<uint32_t>(d)) slots <<= 1;  // <= 512 for d <= 256
+    for (uint32_t j = lane; j < slots; j += 64) {
+        keys[j] = kEmpty;
+        cnts[j] = 0;
+    }
+    wave_lds_sync();
+    for (int32_t j = lane; j < d; j += 64) table_add(keys, cnts, slots - 1, seg[j], 1u);
+    wave_lds_sync();
+    uint64_t best = 0;
+    for (uint32_t j = lane; j < slots; j += 64) {
+        const int32_t k = keys[j];
+        if (k != kEmpty) {
+            const uint64_t key = pack(cnts[j], k);
+            best = key > best ? key : best;
+        }
+    }
+    best = wave_max(best);
+    wave_lds_sync();  // the next call clears the table
+    return best;
 }
 
 // Packed winner of the first `slots` table entries, reduced over the block
@@ -147,6 +245,11 @@ __device__ __forceinline__ uint64_t table_best(const int32_t* keys, const uint32
     return t;
 }
 
+// Partition of `label` among `parts` (a multiply-shift hash; `mult` odd).
+__device__ __forceinline__ uint32_t hash_part(int32_t label, uint32_t parts, uint32_t mult) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(static_cast<uint32_t>(label) * mult) * parts) >> 32);
+}
+
 __host__ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t x) {
     uint32_t p = 1;
     while (p < x) p <<= 1;
@@ -154,7 +257,10 @@ __host__ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t x) {
 }
 
 // One block per vertex (grid-stride over the class list); the table uses the
-// power of two >= 2d of its SLOTS slots, so only that much is cleared.
+// power of two >= 2d of its SLOTS slots, so only that much is cleared.  A
+// vertex with more distinct labels than the table holds (only possible when
+// 2d > SLOTS) is counted in 2, 4, ... sub-passes, each taking the labels of
+// one sub-hash value and gathering its labels again.
 template <int BT, int SLOTS>
 __global__ void __launch_bounds__(BT) k_cdlp_lds(const int32_t* __restrict__ list, int32_t count,
                                                  const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
@@ -162,19 +268,43 @@ __global__ void __launch_bounds__(BT) k_cdlp_lds(const int32_t* __restrict__ lis
     __shared__ int32_t keys[SLOTS];
     __shared__ uint32_t cnts[SLOTS];
     __shared__ uint64_t s_red[BT / 64];
+    __shared__ int s_full;
     for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
         const int32_t v = list[i];
         const int32_t b = rowptr[v], d = rowptr[v + 1] - b;
-        const uint32_t slots = pow2_at_least(2u * static_cast<uint32_t>(d));  // <= SLOTS: d <= SLOTS / 2
-        for (uint32_t j = threadIdx.x; j < slots; j += BT) {
-            keys[j] = kEmpty;
-            cnts[j] = 0;
+        const uint32_t want = pow2_at_least(2u * static_cast<uint32_t>(d));
+        const uint32_t slots = want < static_cast<uint32_t>(SLOTS) ? want : static_cast<uint32_t>(SLOTS);
+        uint64_t best = 0;
+        for (uint32_t nsub = 1;; nsub <<= 1) {
+            bool full = false;
+            uint64_t cand = 0;
+            for (uint32_t sub = 0; sub < nsub && !full; ++sub) {
+                for (uint32_t j = threadIdx.x; j < slots; j += BT) {
+                    keys[j] = kEmpty;
+                    cnts[j] = 0;
+                }
+                if (threadIdx.x == 0) s_full = 0;
+                __syncthreads();
+                for (int j = threadIdx.x; j < d; j += BT) {
+                    const int32_t x = lab[col[b + j]];
+                    if ((nsub == 1 || hash_part(x, nsub, 0x27D4EB2Fu) == sub) && !table_add(keys, cnts, slots - 1, x, 1u))
+                        s_full = 1;
+                }
+                __syncthreads();
+                full = s_full != 0;
+                if (!full) {
+                    const uint64_t t = table_best<BT>(keys, cnts, slots, s_red);
+                    cand = t > cand ? t : cand;
+                } else {
+                    __syncthreads();  // every thread has read s_full before it is cleared
+                }
+            }
+            if (!full) {
+                best = cand;
+                break;
+            }
         }
-        __syncthreads();
-        for (int j = threadIdx.x; j < d; j += BT) table_add(keys, cnts, slots - 1, lab[col[b + j]], 1u);
-        __syncthreads();
-        const uint64_t t = table_best<BT>(keys, cnts, slots, s_red);
-        if (threadIdx.x == 0) out[v] = unpack_label(t);
+        if (threadIdx.x == 0) out[v] = unpack_label(best);
     }
 }
 
@@ -253,9 +383,6 @@ __global__ void __launch_bounds__(1024) k_cdlp_hub_plan(int32_t* __restrict__ hu
 // until every sub-pass fits.  Persistent blocks walk the items.  cap (a power
 // of two) bounds the table; a cap of 64 is a test setting that forces the
 // sub-passes.
-__device__ __forceinline__ uint32_t hash_part(int32_t label, uint32_t parts, uint32_t mult) {
-    return static_cast<uint32_t>((static_cast<uint64_t>(static_cast<uint32_t>(label) * mult) * parts) >> 32);
-}
 
 __global__ void __launch_bounds__(kBigBT) k_cdlp_hub_merge(
     const int32_t* __restrict__ item_off, int32_t hubs, const int32_t* __restrict__ hub_chunk0,
@@ -684,7 +811,7 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
             hipLaunchKernelGGL((k_cdlp_lds<kBT, kLdsSlots>), dim3(grid_cap(L.count[2])), dim3(kBT), 0, s, L.list[2],
                                L.count[2], A, C, la, lb);
         if (L.count[3])
-            hipLaunchKernelGGL((k_cdlp_lds<kBigBT, kBigSlots>), dim3(grid_cap(L.count[3])), dim3(kBigBT), 0, s, L.list[3],
+            hipLaunchKernelGGL((k_cdlp_lds<kBT, kLdsSlots>), dim3(grid_cap(L.count[3])), dim3(kBT), 0, s, L.list[3],
                                L.count[3], A, C, la, lb);
         if (L.count[4]) {
             hipLaunchKernelGGL(k_cdlp_hub_count, dim3(L.hub_chunks), dim3(kBigBT), 0, s, L.list[4], L.chunk_hub,

@@ -387,19 +387,23 @@ def test_cdlp_hub_merge_partitions():
     leaves, each leaf is tied both ways to two private partners (so a leaf's
     label is its smaller partner), and the hub sees N distinct labels -- its
     pairs need several merge partitions.  Also with the merge tables capped
-    at 64 slots (the sub-pass path), and a second hub of 9000 neighbours
-    (two chunks)."""
+    at 64 slots (the sub-pass path), a second hub of 9000 neighbours (two
+    chunks), and a vertex of 6000 neighbours with distinct labels (more than
+    its 4096-slot table holds: the block's sub-passes)."""
     N = 20000
     src, dst = [], []
     for i in range(1, N + 1):
         a, b = N + 2 * i - 1, N + 2 * i
         src += [0, i, a, i, b, a, b]
         dst += [i, a, i, b, i, b, a]
-    n = 3 * N + 1 + 9001
+    n = 3 * N + 1 + 9001 + 1
     h2 = 3 * N + 1
     for j in range(1, 9001):
         src += [h2 + j, h2]
         dst += [h2, h2 + j]
+    h3 = n - 1
+    src += [h3] * 6000
+    dst += list(range(1, 6001))
     hg = G.HostGraph.from_edges(n, src, dst)
     o, i, a = hg.view(G.VIEW_OUT), hg.view(G.VIEW_IN), hg.view(G.VIEW_ALL)
     dg = G.DeviceGraph(o, i, a)
