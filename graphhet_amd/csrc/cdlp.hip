@@ -11,8 +11,9 @@
 //
 // No sort: vertices are bucketed by all-degree d once per graph and each class
 // counts labels where it fits best:
-//   d <= 32      one lane; a wave gathers the labels of 64 such vertices
-//                into LDS together, then each lane counts pairwise;
+//   d <= 32      a wave gathers the labels of 64 such vertices into LDS
+//                together, then each edge's lane counts its label over its
+//                vertex's labels and offers the result to the vertex's slot;
 //   d <= 256     one wave, labels in LDS, counted in the wave's 512-slot
 //                open-addressing hash table;
 //   d <= 8192    one 256-thread block, open-addressing hash table in LDS
@@ -119,7 +120,8 @@ __device__ __forceinline__ int32_t tile_gather(int32_t b, int32_t d, int32_t* s_
     return off;
 }
 
-// d <= 32: lane per vertex, tiles of 64 list entries; pairwise counting.
+// d <= 32: tiles of 64 list entries (a lane per vertex), labels gathered
+// together, then counted edge-parallel.
 __global__ void __launch_bounds__(kBT) k_cdlp_small(const int32_t* __restrict__ list, int32_t count,
                                                     const int32_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ col, const int32_t* __restrict__ lab,
@@ -127,6 +129,7 @@ __global__ void __launch_bounds__(kBT) k_cdlp_small(const int32_t* __restrict__ 
     __shared__ int32_t s_lab[kBT / 64][kTileLabels];
     __shared__ int32_t s_off[kBT / 64][64];
     __shared__ int32_t s_b[kBT / 64][64];
+    __shared__ unsigned long long s_best[kBT / 64][64];
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const int64_t tiles = ceil_div(count, 64);
     for (int64_t t = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) >> 6; t < tiles;
@@ -136,18 +139,27 @@ __global__ void __launch_bounds__(kBT) k_cdlp_small(const int32_t* __restrict__ 
         const int32_t b = i < count ? rowptr[v] : 0;
         const int32_t d = i < count ? rowptr[v + 1] - b : 0;
         int32_t E;
+        s_best[w][lane] = 0ull;
         const int32_t off = tile_gather<64>(b, d, s_off[w], s_b[w], s_lab[w], col, lab, &E);
-        const int32_t* mine = s_lab[w] + off;
-        uint64_t best = 0;
-        for (int j = 0; j < d; ++j) {
-            const int32_t x = mine[j];
+        // edge-parallel counting: the lane of edge k counts its label over
+        // k's vertex's labels and offers (count, label) to that vertex's
+        // slot; the work per lane is the sum of its edges' degrees, not the
+        // square of the tile's largest degree
+        for (int32_t k = lane; k < E; k += 64) {
+            int j = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) j += s_off[w][j + step] <= k ? step : 0;
+            const int32_t so = s_off[w][j];
+            const int32_t se = j < 63 ? s_off[w][j + 1] : E;
+            const int32_t x = s_lab[w][k];
             uint32_t c = 0;
-            for (int k = 0; k < d; ++k) c += (mine[k] == x);
-            const uint64_t key = pack(c, x);
-            best = key > best ? key : best;
+            for (int32_t q = so; q < se; ++q) c += (s_lab[w][q] == x);
+            atomicMax(&s_best[w][j], static_cast<unsigned long long>(pack(c, x)));
         }
-        if (i < count) out[v] = unpack_label(best);
-        wave_lds_sync();  // the next tile's gather overwrites s_lab
+        wave_lds_sync();
+        if (i < count) out[v] = unpack_label(s_best[w][lane]);
+        (void)off;
+        wave_lds_sync();  // the next tile's gather overwrites s_lab and s_best
     }
 }
 
@@ -199,6 +211,29 @@ __device__ __forceinline__ bool table_add(int32_t* keys, uint32_t* cnts, uint32_
     return false;
 }
 
+// table_add for a whole wave (every lane calls it, `valid` marks the lanes
+// with a label): the two labels held by the first pending lanes are added
+// once each with the number of lanes holding them, the rest lane by lane.
+// Once labels converge most of a wave shares one or two of them, and their
+// LDS atomics would otherwise serialise on one slot.  Returns false in the
+// lanes whose add found the table full.
+__device__ __forceinline__ bool wave_table_add(int32_t* keys, uint32_t* cnts, uint32_t mask, int32_t x, bool valid) {
+    const int lane = lane_id();
+    uint64_t pending = __ballot(valid);
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        if (!pending) break;
+        const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+        const int32_t lx = __shfl(x, leader);
+        const uint64_t same = __ballot(valid && x == lx) & pending;
+        if (lane == leader) ok = table_add(keys, cnts, mask, lx, static_cast<uint32_t>(__popcll(same)));
+        pending &= ~same;
+    }
+    if ((pending >> lane) & 1) ok = table_add(keys, cnts, mask, x, 1u) && ok;
+    return ok;
+}
+
 __device__ __forceinline__ uint64_t wave_hash_best(const int32_t* seg, int32_t d, int32_t* keys, uint32_t* cnts) {
     const int lane = lane_id();
     uint32_t slots = 64;
@@ -208,7 +243,10 @@ __device__ __forceinline__ uint64_t wave_hash_best(const int32_t* seg, int32_t d
         cnts[j] = 0;
     }
     wave_lds_sync();
-    for (int32_t j = lane; j < d; j += 64) table_add(keys, cnts, slots - 1, seg[j], 1u);
+    for (int32_t j0 = 0; j0 < d; j0 += 64) {
+        const int32_t j = j0 + lane;
+        wave_table_add(keys, cnts, slots - 1, j < d ? seg[j] : 0, j < d);
+    }
     wave_lds_sync();
     uint64_t best = 0;
     for (uint32_t j = lane; j < slots; j += 64) {
@@ -285,10 +323,11 @@ __global__ void __launch_bounds__(BT) k_cdlp_lds(const int32_t* __restrict__ lis
                 }
                 if (threadIdx.x == 0) s_full = 0;
                 __syncthreads();
-                for (int j = threadIdx.x; j < d; j += BT) {
-                    const int32_t x = lab[col[b + j]];
-                    if ((nsub == 1 || hash_part(x, nsub, 0x27D4EB2Fu) == sub) && !table_add(keys, cnts, slots - 1, x, 1u))
-                        s_full = 1;
+                for (int j0 = threadIdx.x & ~63; j0 < d; j0 += BT) {  // whole waves: wave_table_add
+                    const int j = j0 + lane_id();
+                    const int32_t x = j < d ? lab[col[b + j]] : 0;
+                    const bool mine = j < d && (nsub == 1 || hash_part(x, nsub, 0x27D4EB2Fu) == sub);
+                    if (!wave_table_add(keys, cnts, slots - 1, x, mine)) s_full = 1;
                 }
                 __syncthreads();
                 full = s_full != 0;
@@ -327,7 +366,10 @@ __global__ void __launch_bounds__(kBigBT) k_cdlp_hub_count(
     }
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (int32_t j = beg + threadIdx.x; j < end; j += kBigBT) table_add(keys, cnts, slots - 1, lab[col[j]], 1u);
+    for (int32_t j0 = beg + (threadIdx.x & ~63); j0 < end; j0 += kBigBT) {  // whole waves: wave_table_add
+        const int32_t j = j0 + lane_id();
+        wave_table_add(keys, cnts, slots - 1, j < end ? lab[col[j]] : 0, j < end);
+    }
     __syncthreads();
     uint2* out = pairs + static_cast<int64_t>(c) * kChunk;
     for (uint32_t j = threadIdx.x; j < slots; j += kBigBT) {
