@@ -90,6 +90,7 @@ struct PRLayout {
 // bin's edges contiguous, its segments in window order).
 struct PRBinLayout {
     bool built = false;
+    bool unfit = false;        // the graph needs more bins than the tables hold (automatic form: gathers)
     int32_t num_win = 0;       // source windows
     int32_t num_bin = 0;       // destination bins (contiguous ids, <= 2^14 each)
     int32_t num_items = 0;     // gather items (an edge range of one window each)

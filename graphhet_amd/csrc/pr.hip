@@ -1124,8 +1124,15 @@ int run_pr(ghet_graph* g, int32_t iters, double damping, double* pr_out, const g
     // file's gathers with the given split; 0 = automatic: bins above 2^21
     // vertices unless bit 18 asks for the gathers.
     const int form = (split_code >> 16) & 3;
-    if (form == 3 || (form == 0 && !((split_code >> 18) & 1) && pr_binned_default(n, g->in.m)))
-        return run_pr_binned(g, iters, damping, pr_out, normalise, o, st);
+    if (form == 3) return run_pr_binned(g, iters, damping, pr_out, normalise, o, st);
+    if (form == 0 && !((split_code >> 18) & 1) && !g->prb.unfit && pr_binned_default(n, g->in.m)) {
+        const int rc = run_pr_binned(g, iters, damping, pr_out, normalise, o, st);
+        if (rc != GHET_ERR_UNSUPPORTED) return rc;
+        // more bins than the binned form's tables hold: this graph takes the
+        // gathers, now and on later calls
+        g->prb.unfit = true;
+        clear_error();
+    }
     GHET_CHECK(build_pr_layout(g, s, split_code));
     PRLayout& L = g->pr;
     // One persistent k_pr_iter launch per iteration walks all items (hot
