@@ -343,6 +343,13 @@ def _opts(schedule=SCHED_HEURISTIC, sssp_delta=0.0, trace=None, flags=0, stream=
     return o
 
 
+def _heavy_flags(heavy) -> int:
+    """opts reserved[0] bits of the SSSP heavy-pass form."""
+    if heavy not in (None, "push", "pull"):
+        raise ValueError(f"heavy {heavy!r}")
+    return {None: 0, "push": 8, "pull": 16}[heavy]
+
+
 class DeviceGraph:
     """A graph resident on the GPU (ghet_graph handle)."""
 
@@ -467,22 +474,27 @@ class DeviceGraph:
         return depth, pre, st
 
     def sssp(self, src: int, inf: float = DBL_MAX, delta: float = 0.0, schedule=SCHED_HEURISTIC, out=True,
-             trace_cap=0, stream=None):
+             trace_cap=0, stream=None, heavy=None):
+        """heavy: None (automatic), "push" or "pull" -- how a bucket's heavy
+        pass runs: over the bucket's out-edges, or over the in-edges of the
+        unsettled vertices (needs the in-CSR; "pull" takes it for every
+        heavy pass).  stats.extra[3] = heavy passes run as pulls."""
         dist = np.empty(self.n, np.float64) if out else None
         st = RunStats()
         tr = (LevelTrace * trace_cap)() if trace_cap else None
         _check(lib().ghet_sssp(self._h, src, inf, _ptr(dist, C.c_double),
-                               C.byref(_opts(schedule, sssp_delta=delta, trace=tr, stream=stream)), C.byref(st)))
+                               C.byref(_opts(schedule, sssp_delta=delta, trace=tr, stream=stream,
+                                             flags=_heavy_flags(heavy))), C.byref(st)))
         if trace_cap:
             return dist, st, [tr[i] for i in range(st.trace_len)]
         return dist, st
 
-    def sssp_pre(self, src: int, inf: float = DBL_MAX, delta: float = 0.0):
+    def sssp_pre(self, src: int, inf: float = DBL_MAX, delta: float = 0.0, heavy=None):
         dist = np.empty(self.n, np.float64)
         pre = np.empty(self.n, np.int32)
         st = RunStats()
         _check(lib().ghet_sssp_pre(self._h, src, inf, _ptr(dist, C.c_double), _ptr(pre, C.c_int32),
-                                   C.byref(_opts(sssp_delta=delta)), C.byref(st)))
+                                   C.byref(_opts(sssp_delta=delta, flags=_heavy_flags(heavy))), C.byref(st)))
         return dist, pre, st
 
     def wcc(self, out=True):

@@ -312,9 +312,20 @@ __global__ void __launch_bounds__(kSmallBT) k_row_split_small(const int32_t* __r
 // Round (kHeavy false): relaxes the edges with d[u] + w < thr; an improved v
 // joins next_q (stamp_near != next_round) and, on its first entry into this
 // bucket (old stamp < r0, the bucket's first round), the bucket list.
-// Heavy (kHeavy true): q is the drained bucket's list; relaxes the edges with
+// Heavy (kRelaxHeavy): q is the drained bucket's list; relaxes the edges with
 // d[u] + w >= thr and sends improved vertices to far list far_sel.
-template <bool kHeavy>
+// Pull (kRelaxPull): the heavy pass from the other side.  q holds the
+// unsettled vertices v (d[v] >= thr) with in-edges, rowptr/col/wts are the
+// in-CSR, and an edge (x, v) relaxes v from x when x is settled (d[x] < thr).
+// Edges from earlier buckets' vertices change nothing (their heavy passes
+// already relaxed them), and no settled x has d[x] + w < thr into an
+// unsettled v (the rounds relaxed those), so v ends at the same minimum as
+// after the push pass and joins the far list under the same stamp rule.
+// Targets stay at or above thr, sources below: no edge reads a distance
+// another edge writes.
+enum RelaxMode { kRelaxRound, kRelaxHeavy, kRelaxPull };
+
+template <int kMode>
 __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, int32_t F,
                                                const int32_t* __restrict__ scan,
                                                const int32_t* __restrict__ start,
@@ -330,6 +341,7 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
     __shared__ int32_t s_qa[kOut], s_qb[kOut];  // round: near, list; heavy: far
     __shared__ int32_t s_i0, s_span, s_na, s_nb, s_base;
     __shared__ unsigned long long s_red[kBT / 64];
+    constexpr bool kHeavy = kMode != kRelaxRound;  // heavy and pull feed the far list
     const int tid = threadIdx.x;
     if (tid == 0) {
         s_na = 0;
@@ -388,17 +400,37 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
         }
         double nd[kItems];
         int32_t v[kItems];
-#pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            nd[k] = dist[u[k]] + wts[pos[k]];
-            v[k] = col[pos[k]];
-        }
         bool cand[kItems];
         double dv[kItems];
+        if (kMode == kRelaxPull) {
+            // u[k] is the target; its in-neighbour, the weight and the
+            // target's distance load together, then the neighbour's distance
+            int32_t x[kItems];
+            double wv[kItems];
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            cand[k] = ok[k] && (kHeavy ? nd[k] >= threshold : nd[k] < threshold);
-            dv[k] = cand[k] ? dist[v[k]] : 0.0;
+            for (int k = 0; k < kItems; ++k) {
+                x[k] = col[pos[k]];
+                wv[k] = wts[pos[k]];
+                dv[k] = dist[u[k]];
+                v[k] = u[k];
+            }
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                const double dx = dist[x[k]];
+                nd[k] = dx + wv[k];
+                cand[k] = ok[k] && dx < threshold;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                nd[k] = dist[u[k]] + wts[pos[k]];
+                v[k] = col[pos[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                cand[k] = ok[k] && (kHeavy ? nd[k] >= threshold : nd[k] < threshold);
+                dv[k] = cand[k] ? dist[v[k]] : 0.0;
+            }
         }
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
@@ -438,6 +470,116 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
         }
     }
     post_pass(ctr, mail, seq, token);
+}
+
+// Preparation of a pull pass: the unsettled vertices (d >= thr) with
+// in-edges, as entries in the format k_relax reads (scan[] first edge,
+// start[] first in-CSR position, vert[] the vertex).  A block takes
+// kPullItems * kPullBT consecutive vertices (item-major, so every load is
+// coalesced), scans their in-degrees in that order and reserves its entry
+// and edge ranges with one 64-bit atomic on alloc ((entries << 32) | edges),
+// as k_row_prep does; a block without entries skips it.  The last block to
+// arrive writes scan[F] = E, posts E (v[0]), F (v[4]) and the far and list
+// counts, and clears the counters for the relax launch that follows.  The
+// counters must be zero before the launch; next_alloc, the other reservation
+// word, is cleared for the next pass.
+constexpr int kPullBT = 1024;
+constexpr int kPullItems = 4;
+
+__global__ void __launch_bounds__(kPullBT) k_pull_prep(int32_t n, const int32_t* __restrict__ in_rowptr,
+                                                      const double* __restrict__ dist, double threshold,
+                                                      int32_t* __restrict__ scan, int32_t* __restrict__ start,
+                                                      int32_t* __restrict__ vert, unsigned long long* alloc,
+                                                      unsigned long long* next_alloc, SsspCtr* ctr, HostMail* mail,
+                                                      unsigned long long seq) {
+    constexpr int W = kPullBT / 64;
+    __shared__ int32_t s_ent[kPullItems][W], s_edg[kPullItems][W];
+    __shared__ unsigned long long s_base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (blockIdx.x == 0 && tid == 0) *next_alloc = 0;
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kPullBT * kPullItems;
+    double d[kPullItems];
+    int32_t b[kPullItems], e[kPullItems];
+#pragma unroll
+    for (int k = 0; k < kPullItems; ++k) {  // n > 0: lanes past the end read vertex 0
+        const int64_t v = base + k * kPullBT + tid;
+        const int64_t vv = v < n ? v : 0;
+        d[k] = dist[vv];
+        b[k] = in_rowptr[vv];
+        e[k] = in_rowptr[vv + 1];
+        if (v >= n) e[k] = b[k];
+    }
+    int32_t c[kPullItems], incl[kPullItems], rank[kPullItems];
+    const unsigned long long below = (1ULL << lane) - 1;
+#pragma unroll
+    for (int k = 0; k < kPullItems; ++k) {
+        c[k] = d[k] >= threshold ? e[k] - b[k] : 0;
+        incl[k] = c[k];
+        for (int off = 1; off < 64; off <<= 1) {
+            const int32_t y = __shfl_up(incl[k], off);
+            if (lane >= off) incl[k] += y;
+        }
+        const unsigned long long mask = __ballot(c[k] > 0);
+        rank[k] = __popcll(mask & below);
+        if (lane == 63) s_edg[k][wave] = incl[k];
+        if (lane == 0) s_ent[k][wave] = __popcll(mask);
+    }
+    __syncthreads();
+    int32_t ent_off[kPullItems] = {}, edg_off[kPullItems] = {};
+    int32_t ent_tot = 0, edg_tot = 0;  // block-uniform
+#pragma unroll
+    for (int k = 0; k < kPullItems; ++k)
+        for (int w = 0; w < W; ++w) {
+            if (w == wave) {
+                ent_off[k] = ent_tot;
+                edg_off[k] = edg_tot;
+            }
+            ent_tot += s_ent[k][w];
+            edg_tot += s_edg[k][w];
+        }
+    if (tid == 0 && ent_tot > 0)
+        s_base = atomicAdd(alloc, (static_cast<unsigned long long>(ent_tot) << 32) | static_cast<uint32_t>(edg_tot));
+    __syncthreads();
+    unsigned long long token = 0;
+    if (ent_tot > 0) {
+        const unsigned long long sb = s_base;
+        const int32_t ebase = static_cast<int32_t>(sb >> 32), xbase = static_cast<int32_t>(sb & 0xFFFFFFFFu);
+#pragma unroll
+        for (int k = 0; k < kPullItems; ++k)
+            if (c[k] > 0) {
+                const int32_t slot = ebase + ent_off[k] + rank[k];
+                scan[slot] = xbase + edg_off[k] + incl[k] - c[k];
+                start[slot] = b[k];
+                vert[slot] = static_cast<int32_t>(base + k * kPullBT + tid);
+            }
+        if (tid == 0) token = sb;
+    }
+    if (arrive_last(ctr->slots, &ctr->done, token)) {
+        if (tid == 0) {
+            auto ld32 = [](const int32_t* p) {
+                return static_cast<unsigned long long>(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            };
+            const unsigned long long a = load_agent(alloc);
+            const int32_t F = static_cast<int32_t>(a >> 32), E = static_cast<int32_t>(a & 0xFFFFFFFFu);
+            scan[F] = E;
+            mail->v[0] = static_cast<unsigned long long>(E);
+            mail->v[1] = kInfBits;
+            mail->v[2] = 0;
+            mail->v[3] = 0;
+            mail->v[4] = static_cast<unsigned long long>(F);
+            mail->v[5] = ld32(&ctr->far_count[0]);
+            mail->v[6] = ld32(&ctr->far_count[1]);
+            mail->v[7] = ld32(&ctr->list_count);
+            mail_post(mail, seq);
+        }
+        // every other block has arrived: nothing reads the counters again
+        unsigned long long* words = reinterpret_cast<unsigned long long*>(ctr->slots);
+        for (int i = tid; i < kSlots * 8; i += kPullBT) words[i] = (i % 8 == 1) ? kInfBits : 0ULL;
+        if (tid == 0) {
+            ctr->near_count = 0;
+            ctr->done = 0;
+        }
+    }
 }
 
 // Splits far list `sel` at new_thr: entries below old_thr are stale (already
@@ -780,18 +922,66 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
         const unsigned long long seq = mail_next(g);
         if (heavy)
-            hipLaunchKernelGGL(k_relax<true>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
+            hipLaunchKernelGGL(k_relax<kRelaxHeavy>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
                                L.w, dist, threshold, stamp_near, next_round, r0, nxt, list, stamp_far, far_gen,
                                far[sel], sel, ctr, g->mail_dev, seq);
         else
-            hipLaunchKernelGGL(k_relax<false>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
+            hipLaunchKernelGGL(k_relax<kRelaxRound>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
                                L.w, dist, threshold, stamp_near, next_round, r0, nxt, list, stamp_far, far_gen,
                                far[sel], sel, ctr, g->mail_dev, seq);
         GHET_HIP(hipGetLastError());
         GHET_CHECK(mail_wait(g, s, seq));
         return GHET_OK;
     };
+    // Pull form of the heavy pass (k_pull_prep + k_relax<kRelaxPull>, over
+    // the weighted in-CSR): once most of the graph is settled, the in-edges
+    // of the unsettled vertices are far fewer than the heavy edges of a big
+    // bucket.  Their count is estimated first as the out-edges of the
+    // vertices not yet settled (exact when the graph is undirected); if that
+    // is under half the bucket's out-edges, the preparation counts them
+    // exactly, and the pull runs when they are fewer than the bucket's
+    // out-edges.  Below n/4 bucket edges the preparation's scan of all
+    // vertices costs more than it could save.  opts reserved[0] bit 3 keeps
+    // every heavy pass a push, bit 4 pulls every heavy pass (tests).
+    const int32_t heavy_flags = have_opts ? o->reserved[0] : 0;
+    const bool can_pull = g->in.present() && g->in.w != nullptr && !(heavy_flags & 8);
+    const bool force_pull = can_pull && (heavy_flags & 16);
+    int64_t settled_out = 0;  // out-edges of the vertices of finished buckets
+    int32_t pulls = 0;
     int64_t rounds = 0, relaxed = 0;
+    // *pulled: the heavy pass ran as a pull.
+    auto try_pull = [&](bool* pulled) -> int {
+        *pulled = false;
+        const int64_t est = g->out.m - settled_out - list_edges;
+        if (!force_pull && !(list_edges >= n / 4 && est < list_edges / 2)) return GHET_OK;
+        hipLaunchKernelGGL(k_ctr_reset, dim3(1), dim3(512), 0, s, ctr, -1);
+        unsigned long long* al = ctr->alloc + (passes & 1);
+        unsigned long long* next_al = ctr->alloc + ((passes + 1) & 1);
+        ++passes;
+        const unsigned gp = static_cast<unsigned>(ceil_div(n, kPullBT * kPullItems));
+        unsigned long long seq = mail_next(g);
+        hipLaunchKernelGGL(k_pull_prep, dim3(gp), dim3(kPullBT), 0, s, n, g->in.rowptr, dist, threshold, scan, qstart,
+                           qvert, al, next_al, ctr, g->mail_dev, seq);
+        GHET_HIP(hipGetLastError());
+        GHET_CHECK(mail_wait(g, s, seq));
+        const int32_t Fp = static_cast<int32_t>(hm->v[4]);
+        const int64_t Ep = static_cast<int64_t>(hm->v[0]);
+        if (!force_pull && Ep >= list_edges) return GHET_OK;
+        *pulled = true;
+        ++pulls;
+        if (Ep > 0) {
+            const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(Ep, kTile), kMaxBlocks));
+            seq = mail_next(g);
+            hipLaunchKernelGGL(k_relax<kRelaxPull>, dim3(grid), dim3(kBT), 0, s, qvert, Fp, scan, qstart,
+                               g->in.rowptr, g->in.col, g->in.w, dist, threshold, stamp_near, round, r0, nxt, list,
+                               stamp_far, far_gen, far[sel], sel, ctr, g->mail_dev, seq);
+            GHET_HIP(hipGetLastError());
+            GHET_CHECK(mail_wait(g, s, seq));
+            relaxed += static_cast<int64_t>(hm->v[3]);
+        }
+        far_count = static_cast<int32_t>(hm->v[5 + sel]);
+        return GHET_OK;
+    };
     for (;;) {
         // relaxation rounds of the current bucket (light edges only)
         while (F > 0) {
@@ -815,10 +1005,15 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         // heavy pass: the bucket's distances are final; its remaining edges
         // all land at or above the threshold
         if (list_edges > 0) {
-            GHET_CHECK(launch_pass(true, list + list_base, list_len, list_edges, round));
-            relaxed += static_cast<int64_t>(hm->v[3]);
-            far_count = static_cast<int32_t>(hm->v[5 + sel]);
+            bool pulled = false;
+            if (can_pull) GHET_CHECK(try_pull(&pulled));
+            if (!pulled) {
+                GHET_CHECK(launch_pass(true, list + list_base, list_len, list_edges, round));
+                relaxed += static_cast<int64_t>(hm->v[3]);
+                far_count = static_cast<int32_t>(hm->v[5 + sel]);
+            }
         }
+        settled_out += list_edges;
         if (far_count == 0) break;
         ++phases;
         // advance the threshold and split the far list
@@ -898,6 +1093,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         st->extra[0] = base_delta;
         st->extra[1] = phases;
         st->extra[2] = static_cast<double>(relaxed);  // edges scanned over all passes
+        st->extra[3] = pulls;                         // heavy passes run as pulls
         st->status_flags = sched.fallback ? 1 : 0;
     }
     return sched.fallback ? GHET_WARN_FALLBACK : GHET_OK;

@@ -229,6 +229,11 @@ typedef struct ghet_opts {
      *                   reports the hot phase's ms in stats extra[2]
      *             bit2: CDLP caps its hub merge tables at 64 slots (a test
      *                   setting: the merge then takes its sub-pass path)
+     *             bit3: SSSP runs every heavy pass as a push over the
+     *                   bucket's out-edges (no pull form)
+     *             bit4: SSSP runs every heavy pass as a pull over the
+     *                   in-edges of the unsettled vertices (a test setting;
+     *                   needs the in-CSR); stats extra[3] counts the pulls
      * reserved[1] bit0: many-graph calls skip the autotuner
      * reserved[2]     : batched BFS pushes while frontier out-edges < m / this
      *                   (0 = default 16)

@@ -100,6 +100,12 @@ def test_sssp_s24_exact_by_invariants(graphs, kind):
     src = int(np.argmax(o.degrees()))
     d, st = dg.sssp(src)
     assert d[src] == 0.0
+    if kind == "rmat":
+        assert st.extra[3] >= 1, "the big bucket's heavy pass did not pull"
+    # the other heavy-pass form reaches the same fixpoint, bit for bit
+    d2, st2 = dg.sssp(src, heavy="push" if st.extra[3] else "pull")
+    np.testing.assert_array_equal(d2, d)
+    del d2
     eu = _edge_sources(o)
     du = d[eu]
     fin = du != DBL_MAX

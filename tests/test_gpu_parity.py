@@ -39,9 +39,12 @@ def test_golden_bfs_all_schedules(golden):
 def test_golden_sssp(golden):
     dg = dev_from_golden(golden)
     for delta in (0.0, 0.01, 0.3, 5.0):
-        d, st = dg.sssp(int(golden["src"]), delta=delta)
-        np.testing.assert_allclose(d, golden["sssp"], rtol=1e-12, atol=0)
-        np.testing.assert_array_equal(d, golden["sssp"])  # fixpoint: bit-identical
+        for heavy in (None, "push", "pull"):
+            d, st = dg.sssp(int(golden["src"]), delta=delta, heavy=heavy)
+            np.testing.assert_allclose(d, golden["sssp"], rtol=1e-12, atol=0)
+            np.testing.assert_array_equal(d, golden["sssp"])  # fixpoint: bit-identical
+            if heavy == "push":
+                assert st.extra[3] == 0
 
 
 def test_golden_wcc(golden):
@@ -82,9 +85,10 @@ def test_golden_parents(golden):
     d, pre, _ = dg.bfs_pre(src)
     np.testing.assert_array_equal(d, g["bfs"])
     np.testing.assert_array_equal(pre, O.bfs_parent(g["in_rowptr"], g["in_col"], g["bfs"]))
-    dist, pre, _ = dg.sssp_pre(src)
-    np.testing.assert_array_equal(dist, g["sssp"])
-    np.testing.assert_array_equal(pre, O.sssp_parent(g["in_rowptr"], g["in_col"], g["in_w"], g["sssp"], src))
+    for heavy in (None, "pull"):
+        dist, pre, _ = dg.sssp_pre(src, heavy=heavy)
+        np.testing.assert_array_equal(dist, g["sssp"])
+        np.testing.assert_array_equal(pre, O.sssp_parent(g["in_rowptr"], g["in_col"], g["in_w"], g["sssp"], src))
 
 
 @pytest.fixture(scope="module", params=[("rmat", 16, 1), ("rmat", 14, 9), ("er", 14, 4)])
@@ -120,12 +124,60 @@ def test_medium_sssp(medium):
     o, dg, src = medium["o"], medium["dg"], medium["src"]
     ref = O.sssp(o.rowptr, o.col, o.w, src)
     for delta in (0.0, 0.05, 1.0):
-        d, st = dg.sssp(src, delta=delta)
-        np.testing.assert_array_equal(d, ref)
+        for heavy in (None, "push", "pull"):
+            d, st = dg.sssp(src, delta=delta, heavy=heavy)
+            np.testing.assert_array_equal(d, ref, err_msg=f"delta {delta} heavy {heavy}")
+            if heavy == "pull":
+                assert st.extra[3] >= 1
     # a finite `inf` like the reference's default leaves far vertices at inf
-    d, _ = dg.sssp(src, inf=0.5)
     ref_cap = np.where(ref < 0.5, ref, 0.5)
-    np.testing.assert_array_equal(d, ref_cap)
+    for heavy in (None, "pull"):
+        d, _ = dg.sssp(src, inf=0.5, heavy=heavy)
+        np.testing.assert_array_equal(d, ref_cap)
+
+
+def test_sssp_pull_forms():
+    """The pull form of the heavy pass over the in-CSR: taken by the
+    automatic form on an R-MAT graph whose big bucket settles most of it,
+    forced for every heavy pass, on an undirected graph (the in-CSR is the
+    out-CSR) and with zero-weight edges and multi-edges; every form is
+    bit-identical to the oracle."""
+    hg = G.HostGraph.rmat(17, 16, seed=5)
+    o, i, a = hg.view(G.VIEW_OUT), hg.view(G.VIEW_IN), hg.view(G.VIEW_ALL)
+    dg = G.DeviceGraph(o, i, a)
+    src = int(np.argmax(o.degrees()))
+    ref = O.sssp(o.rowptr, o.col, o.w, src)
+    d, st = dg.sssp(src)
+    np.testing.assert_array_equal(d, ref)
+    assert st.extra[3] >= 1, "the automatic form did not pull on R-MAT s17"
+    for heavy in ("push", "pull"):
+        d, st = dg.sssp(src, heavy=heavy)
+        np.testing.assert_array_equal(d, ref, err_msg=heavy)
+    # no in-CSR: always the push form
+    dn = G.DeviceGraph(o, with_in=False)
+    d, st = dn.sssp(src, heavy="pull")
+    np.testing.assert_array_equal(d, ref)
+    assert st.extra[3] == 0
+    # undirected: both directions in the out-CSR, which is also the in-CSR
+    rng = np.random.default_rng(3)
+    n = 3000
+    u = rng.integers(0, n, 20000)
+    v = rng.integers(0, n, 20000)
+    w = np.round(rng.random(20000), 2)  # ties and zero weights
+    w[:500] = 0.0
+    uu, vv, ww = np.concatenate([u, v, u[:50]]), np.concatenate([v, u, v[:50]]), np.concatenate([w, w, w[:50] + 0.5])
+    order = np.lexsort((vv, uu))
+    rowptr = np.zeros(n + 1, np.int32)
+    np.add.at(rowptr, uu + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    col, wt = vv[order].astype(np.int32), ww[order].astype(np.float64)
+    ug = G.DeviceGraph(G.Csr(n, rowptr, col, wt), undirected=True)
+    for s0 in (0, 17, n - 1):
+        ref = O.sssp(rowptr, col, wt, s0)
+        for heavy in (None, "push", "pull"):
+            for delta in (0.0, 0.02, 0.5):
+                d, _ = ug.sssp(s0, delta=delta, heavy=heavy)
+                np.testing.assert_array_equal(d, ref, err_msg=f"src {s0} heavy {heavy} delta {delta}")
 
 
 def test_medium_wcc_pr_cdlp(medium):

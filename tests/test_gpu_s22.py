@@ -50,8 +50,9 @@ def test_s22_bfs_and_parents(s22):
 def test_s22_sssp_and_parents(s22):
     dg, o, i, src = s22
     ref = O.sssp(o.rowptr, o.col, o.w, src)
-    d, _ = dg.sssp(src)
-    np.testing.assert_array_equal(d, ref)
+    for heavy in (None, "push", "pull"):
+        d, _ = dg.sssp(src, heavy=heavy)
+        np.testing.assert_array_equal(d, ref, err_msg=str(heavy))
     d, pre, _ = dg.sssp_pre(src)
     np.testing.assert_array_equal(d, ref)
     np.testing.assert_array_equal(pre, O.sssp_parent(i.rowptr, i.col, i.w, ref, src))
