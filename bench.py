@@ -131,6 +131,9 @@ def main():
                     help="largest small graph is 2^this vertices (vertex counts log-uniform in [2^10, 2^this])")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-heuristic section")
+    ap.add_argument("--no-learned-section", action="store_true",
+                    help="keep the learned schedule in the step but skip the learned-vs-heuristic section "
+                         "(profiles of the step alone)")
     ap.add_argument("--step-schedule", choices=("heuristic", "learned"), default="learned",
                     help="schedule of the timed step's BFS and SSSP (learned falls back to the heuristic "
                          "when no network loads)")
@@ -254,7 +257,7 @@ def main():
                 ms = statistics.median(x.device_ms for x in sts)
                 plain[alg] = {"ms": round(ms, 3), "gteps": round(gteps(sts[0].edges_traversed, ms), 3)}
     learned = None
-    if have_net:
+    if have_net and not args.no_learned_section:
         learned = learned_section(G, dg, o, src)
 
     other = {}
