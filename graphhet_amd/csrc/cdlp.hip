@@ -627,20 +627,39 @@ unsigned grid_cap(int64_t blocks) {
     return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 65535)));
 }
 
-// Degree class of a vertex (0 lane, 1 wave, 2 4K-slot table, 3 16K-slot
-// table, 4 hub; 5: no neighbours, keeps its label) as the key of a stable
-// sort by class.
-__global__ void k_cdlp_class(const int32_t* __restrict__ rp, int32_t n, uint32_t* __restrict__ key,
-                             int32_t* __restrict__ val) {
+// Rank order: all-degree descending, ties by vertex id (a stable sort of
+// these keys).
+__global__ void k_cdlp_rank_keys(const int32_t* __restrict__ rp, int32_t n, uint32_t* __restrict__ key,
+                                 int32_t* __restrict__ val) {
     for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
          v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int32_t d = rp[v + 1] - rp[v];
-        key[v] = d == 0 ? 5u : d <= kSmall ? 0u : d <= kWave ? 1u : d <= kLds ? 2u : d <= kBig ? 3u : 4u;
+        key[v] = 0xFFFFFFFFu - static_cast<uint32_t>(rp[v + 1] - rp[v]);
         val[v] = static_cast<int32_t>(v);
     }
 }
 
-// off[c] = first position of class c in the sorted keys; off[6] = n.
+// Position of each class in rank order: hubs first, then classes 3, 2, 1, 0
+// and the vertices without neighbours (degrees descend along the ranks).
+__host__ __device__ constexpr int class_order(int c) { return c == 5 ? 5 : 4 - c; }
+
+// rank[perm[r]] = r; deg_r[r] = all-degree of rank r; okey[r] = the class
+// position of rank r (non-decreasing along the ranks); iota[r] = r.
+__global__ void k_cdlp_rank_rows(const int32_t* __restrict__ perm, const int32_t* __restrict__ rp, int32_t n,
+                                 int32_t* __restrict__ rank, int32_t* __restrict__ deg_r, uint32_t* __restrict__ okey,
+                                 int32_t* __restrict__ iota) {
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < n;
+         r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t v = perm[r];
+        const int32_t d = rp[v + 1] - rp[v];
+        rank[v] = static_cast<int32_t>(r);
+        deg_r[r] = d;
+        const int c = d == 0 ? 5 : d <= kSmall ? 0 : d <= kWave ? 1 : d <= kLds ? 2 : d <= kBig ? 3 : 4;
+        okey[r] = static_cast<uint32_t>(class_order(c));
+        iota[r] = static_cast<int32_t>(r);
+    }
+}
+
+// off[k] = first rank of class position k in the non-decreasing keys; off[6] = n.
 __global__ void k_cdlp_class_offsets(const uint32_t* __restrict__ key, int32_t n, int32_t* __restrict__ off) {
     for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e <= n;
          e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -648,6 +667,92 @@ __global__ void k_cdlp_class_offsets(const uint32_t* __restrict__ key, int32_t n
         const int64_t cur = e < n ? static_cast<int64_t>(key[e]) : 6;
         for (int64_t k = prev + 1; k <= cur; ++k) off[k] = static_cast<int32_t>(e);
     }
+}
+
+// The all-CSR in rank space, edge-balanced: a block takes kColTile
+// consecutive positions of col_r, finds their rows (ranks) by binary search
+// in an LDS copy of the tile's row starts, and writes rank[col[...]] of the
+// same edge of the vertex's own row.  Rows keep their edge order here; a
+// segmented sort orders each afterwards.  Rows without edges come last in
+// rank order, so every row a position falls in has edges.
+constexpr int kColBT = 256, kColItems = 8, kColTile = kColBT * kColItems;
+
+__global__ void __launch_bounds__(kColBT) k_cdlp_rank_cols(const int32_t* __restrict__ perm,
+                                                           const int32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ col,
+                                                           const int32_t* __restrict__ rank,
+                                                           const int32_t* __restrict__ rowptr_r, int32_t n, int64_t m,
+                                                           int32_t* __restrict__ col_r) {
+    __shared__ int32_t s_start[kColTile], s_src[kColTile];
+    __shared__ int32_t s_r0, s_span;
+    for (int64_t tile = blockIdx.x; tile * kColTile < m; tile += gridDim.x) {
+        const int64_t t0 = tile * kColTile;
+        const int64_t t1 = std::min<int64_t>(t0 + kColTile, m);
+        if (threadIdx.x == 0) {
+            auto row_of = [&](int64_t pos) {  // last r with rowptr_r[r] <= pos
+                int32_t lo = 0, hi = n;
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi + 1) >> 1;
+                    if (rowptr_r[mid] <= pos) lo = mid;
+                    else hi = mid - 1;
+                }
+                return lo;
+            };
+            const int32_t r0 = row_of(t0), r1 = row_of(t1 - 1);
+            s_r0 = r0;
+            s_span = r1 - r0 + 1;  // <= kColTile: each row in it has an edge in the tile
+        }
+        __syncthreads();
+        const int32_t r0 = s_r0, span = s_span;
+        for (int j = threadIdx.x; j < span; j += kColBT) {
+            s_start[j] = rowptr_r[r0 + j];
+            s_src[j] = rp[perm[r0 + j]];
+        }
+        __syncthreads();
+        int32_t e[kColItems];
+#pragma unroll
+        for (int k = 0; k < kColItems; ++k) {
+            const int64_t pos = t0 + k * kColBT + threadIdx.x;
+            e[k] = 0;
+            if (pos < t1) {
+                int32_t lo = 0, hi = span - 1;  // last j with s_start[j] <= pos
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi + 1) >> 1;
+                    if (s_start[mid] <= pos) lo = mid;
+                    else hi = mid - 1;
+                }
+                e[k] = s_src[lo] + static_cast<int32_t>(pos - s_start[lo]);
+            }
+        }
+        int32_t u[kColItems];
+#pragma unroll
+        for (int k = 0; k < kColItems; ++k) u[k] = col[e[k]];  // m > 0: e = 0 is an edge
+#pragma unroll
+        for (int k = 0; k < kColItems; ++k) {
+            const int64_t pos = t0 + k * kColBT + threadIdx.x;
+            if (pos < t1) col_r[pos] = rank[u[k]];
+        }
+        __syncthreads();  // the next tile rewrites the LDS copies
+    }
+}
+
+// Labels by vertex id -> rank space (a and b: the pair of label buffers).
+__global__ void k_cdlp_to_rank(const int32_t* __restrict__ perm, const int32_t* __restrict__ lo, int32_t n,
+                               int32_t* __restrict__ a, int32_t* __restrict__ b) {
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < n;
+         r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t x = lo[perm[r]];
+        a[r] = x;
+        b[r] = x;
+    }
+}
+
+// Rank space -> labels by vertex id.
+__global__ void k_cdlp_from_rank(const int32_t* __restrict__ perm, const int32_t* __restrict__ a, int32_t n,
+                                 int32_t* __restrict__ lo) {
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < n;
+         r += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        lo[perm[r]] = a[r];
 }
 
 // Chunks per hub.
@@ -674,52 +779,101 @@ __global__ void k_cdlp_hub_chunks(const int32_t* __restrict__ rp, const int32_t*
     }
 }
 
-// The class lists (ids ascending within a class) from a stable radix sort
-// of the class keys, then the hubs' chunks from a scan; all on the GPU.
+// The rank order from a stable radix sort of the degree keys, the all-CSR in
+// rank space (edge-balanced gather, then a segmented radix sort of each
+// row), the class ranges, and the hubs' chunks from a scan; all on the GPU.
 int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
     CDLPLayout& L = g->cdlp;
     if (L.built) return GHET_OK;
     const int32_t n = g->n;
     const int32_t* rp = g->all.rowptr;
-    uint32_t *ka = nullptr, *kb = nullptr;
-    int32_t *va = nullptr, *vb = nullptr, *off = nullptr, *nch = nullptr, *choff = nullptr, *aux32 = nullptr;
-    void* tmp = nullptr;
-    size_t tb = 0;
-    hipError_t e = hipSuccess;
-    auto get = [&](void** q, size_t b) {
-        if (e == hipSuccess) e = hipMalloc(q, std::max<size_t>(b, 256));
-    };
-    get(reinterpret_cast<void**>(&ka), 4 * size_t(n));
-    get(reinterpret_cast<void**>(&kb), 4 * size_t(n));
-    get(reinterpret_cast<void**>(&va), 4 * size_t(n));
-    get(reinterpret_cast<void**>(&vb), 4 * size_t(n));
-    get(reinterpret_cast<void**>(&off), 4 * 8);
+    const int64_t m = g->all.m;
+    if (m >= (int64_t(1) << 31)) return set_error(GHET_ERR_UNSUPPORTED, "CDLP: %lld all-edges", static_cast<long long>(m));
+    DevTemp tmp;
+    uint32_t* ka = tmp.get<uint32_t>(size_t(n));
+    uint32_t* kb = tmp.get<uint32_t>(size_t(n));
+    int32_t* va = tmp.get<int32_t>(size_t(n));
+    int32_t* vb = tmp.get<int32_t>(size_t(n));
+    int32_t* rank = tmp.get<int32_t>(size_t(n));
+    int32_t* deg_r = tmp.get<int32_t>(size_t(n));
+    int32_t* off = tmp.get<int32_t>(8);
+    int32_t* aux = tmp.get<int32_t>(size_t(scan_aux_ints(n)));
+    int32_t* col_tmp = tmp.get<int32_t>(size_t(std::max<int64_t>(m, 1)));
+    hipError_t e = tmp.err;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    // the rank-space arrays live as long as the graph
+    const size_t rbytes = 2 * al(4 * size_t(n)) + al(4 * (size_t(n) + 1)) + al(4 * size_t(std::max<int64_t>(m, 1)));
+    void* rmem = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&rmem, rbytes);
+    int32_t *perm = nullptr, *iota = nullptr, *rowptr_r = nullptr, *col_r = nullptr;
+    if (e == hipSuccess) {
+        char* q = static_cast<char*>(rmem);
+        perm = reinterpret_cast<int32_t*>(q); q += al(4 * size_t(n));
+        iota = reinterpret_cast<int32_t*>(q); q += al(4 * size_t(n));
+        rowptr_r = reinterpret_cast<int32_t*>(q); q += al(4 * (size_t(n) + 1));
+        col_r = reinterpret_cast<int32_t*>(q);
+    }
     const unsigned gn = grid_cap(ceil_div(n, 256));
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_cdlp_class, dim3(gn), dim3(256), 0, s, rp, n, ka, va);
+        hipLaunchKernelGGL(k_cdlp_rank_keys, dim3(gn), dim3(256), 0, s, rp, n, ka, va);
         e = hipGetLastError();
     }
     rocprim::double_buffer<uint32_t> kd(ka, kb);
     rocprim::double_buffer<int32_t> vd(va, vb);
-    if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, tb, kd, vd, static_cast<unsigned>(n), 0, 3, s);
-    get(&tmp, tb);
-    if (e == hipSuccess) e = rocprim::radix_sort_pairs(tmp, tb, kd, vd, static_cast<unsigned>(n), 0, 3, s);
+    size_t sb = 0;
+    void* sort_tmp = nullptr;
+    if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, sb, kd, vd, static_cast<unsigned>(n), 0, 32, s);
+    if (e == hipSuccess) sort_tmp = tmp.get<char>(sb), e = tmp.err;
+    if (e == hipSuccess) e = rocprim::radix_sort_pairs(sort_tmp, sb, kd, vd, static_cast<unsigned>(n), 0, 32, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(perm, vd.current(), 4 * size_t(n), hipMemcpyDeviceToDevice, s);
+    // kb's keys are no longer needed: it takes the class positions
+    uint32_t* okey = kd.current() == ka ? kb : ka;
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_cdlp_class_offsets, dim3(grid_cap(ceil_div(int64_t(n) + 1, 256))), dim3(256), 0, s,
-                           kd.current(), n, off);
+        hipLaunchKernelGGL(k_cdlp_rank_rows, dim3(gn), dim3(256), 0, s, perm, rp, n, rank, deg_r, okey, iota);
         e = hipGetLastError();
+    }
+    if (e == hipSuccess && exclusive_scan_i32(deg_r, rowptr_r, n, aux, s) != GHET_OK) e = hipErrorUnknown;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_cdlp_class_offsets, dim3(grid_cap(ceil_div(int64_t(n) + 1, 256))), dim3(256), 0, s, okey,
+                           n, off);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && m > 0) {
+        hipLaunchKernelGGL(k_cdlp_rank_cols, dim3(grid_cap(ceil_div(m, kColTile))), dim3(kColBT), 0, s, perm, rp,
+                           g->all.col, rank, rowptr_r, n, m, col_tmp);
+        e = hipGetLastError();
+    }
+    // each row's ranks ascending: consecutive lanes of a counting kernel
+    // then read neighbouring labels, the hubs' first
+    int end_bit = 1;
+    while (end_bit < 31 && (int64_t(1) << end_bit) < n) ++end_bit;
+    if (e == hipSuccess && m > 0) {
+        size_t seg_b = 0;
+        e = rocprim::segmented_radix_sort_keys(nullptr, seg_b, col_tmp, col_r, static_cast<unsigned>(m),
+                                               static_cast<unsigned>(n), rowptr_r, rowptr_r + 1, 0, end_bit, s);
+        void* seg_tmp = nullptr;
+        if (e == hipSuccess) seg_tmp = tmp.get<char>(seg_b), e = tmp.err;
+        if (e == hipSuccess)
+            e = rocprim::segmented_radix_sort_keys(seg_tmp, seg_b, col_tmp, col_r, static_cast<unsigned>(m),
+                                                   static_cast<unsigned>(n), rowptr_r, rowptr_r + 1, 0, end_bit, s);
     }
     int32_t h_off[7] = {};
     if (e == hipSuccess) e = hipMemcpyAsync(h_off, off, sizeof(h_off), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    const int32_t* sorted = vd.current();
-    const int32_t nh = h_off[5] - h_off[4];
-    const int32_t* hubs = sorted + h_off[4];
-    get(reinterpret_cast<void**>(&nch), 4 * size_t(nh));
-    get(reinterpret_cast<void**>(&choff), 4 * (size_t(nh) + 1));
-    get(reinterpret_cast<void**>(&aux32), 4 * size_t(scan_aux_ints(nh)));
+    int32_t cbeg[6] = {}, ccnt[6] = {};
+    for (int c = 0; c < 6; ++c) {
+        const int k = class_order(c);
+        cbeg[c] = h_off[k];
+        ccnt[c] = (k + 1 <= 5 ? h_off[k + 1] : n) - h_off[k];
+    }
+    const int32_t nh = ccnt[4];
+    const int32_t* hubs = iota + cbeg[4];
+    int32_t* nch = tmp.get<int32_t>(size_t(nh));
+    int32_t* choff = tmp.get<int32_t>(size_t(nh) + 1);
+    int32_t* aux32 = tmp.get<int32_t>(size_t(scan_aux_ints(nh)));
+    if (e == hipSuccess) e = tmp.err;
     if (e == hipSuccess && nh > 0) {
-        hipLaunchKernelGGL(k_cdlp_hub_nch, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rp, hubs, nh, nch);
+        hipLaunchKernelGGL(k_cdlp_hub_nch, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rowptr_r, hubs, nh, nch);
         e = hipGetLastError();
     }
     if (e == hipSuccess && exclusive_scan_i32(nch, choff, nh, aux32, s) != GHET_OK) e = hipErrorUnknown;
@@ -730,28 +884,25 @@ int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     const int32_t max_nch = nh > 0 ? *std::max_element(h_nch.begin(), h_nch.end()) : 0;
     if (e == hipSuccess && max_nch > kMaxHubChunks) {
-        for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(va), static_cast<void*>(vb),
-                        static_cast<void*>(off), static_cast<void*>(nch), static_cast<void*>(choff),
-                        static_cast<void*>(aux32), tmp})
-            if (q) hipFree(q);
+        hipFree(rmem);
         return set_error(GHET_ERR_UNSUPPORTED, "CDLP: a vertex of more than %d neighbours", kMaxHubChunks * kChunk);
     }
-    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    size_t bytes = 0;
-    for (int c = 0; c < 5; ++c) bytes += al(4 * size_t(h_off[c + 1] - h_off[c]));
-    bytes += 2 * al(4 * size_t(chunks)) + al(4 * (size_t(nh) + 1)) + al(8 * size_t(chunks) * kChunk) +
-             al(4 * size_t(chunks)) + al(4 * size_t(nh)) + al(4 * (size_t(nh) + 1)) + al(8 * size_t(nh));
+    const size_t bytes = 2 * al(4 * size_t(chunks)) + al(4 * (size_t(nh) + 1)) + al(8 * size_t(chunks) * kChunk) +
+                         al(4 * size_t(chunks)) + al(4 * size_t(nh)) + al(4 * (size_t(nh) + 1)) + al(8 * size_t(nh));
     if (e == hipSuccess) e = hipMalloc(&L.mem, std::max<size_t>(bytes, 256));
     if (e == hipSuccess) {
-        char* p = static_cast<char*>(L.mem);
-        auto take = [&](size_t b) { char* r = p; p += al(b); return r; };
-        for (int c = 0; c < 5 && e == hipSuccess; ++c) {
-            L.count[c] = h_off[c + 1] - h_off[c];
-            L.list[c] = reinterpret_cast<int32_t*>(take(4 * size_t(L.count[c])));
-            if (L.count[c] > 0)
-                e = hipMemcpyAsync(L.list[c], sorted + h_off[c], 4 * size_t(L.count[c]), hipMemcpyDeviceToDevice, s);
+        char* q = static_cast<char*>(L.mem);
+        auto take = [&](size_t b) { char* r = q; q += al(b); return r; };
+        for (int c = 0; c < 6; ++c) {
+            L.count[c] = ccnt[c];
+            L.list[c] = iota + cbeg[c];
+            L.first[c] = perm + cbeg[c];
         }
-        L.count[5] = h_off[6] - h_off[5];
+        L.perm = perm;
+        L.iota = iota;
+        L.rowptr_r = rowptr_r;
+        L.col_r = col_r;
+        L.rmem = rmem;
         L.hub_chunks = chunks;
         L.max_hub_chunks = max_nch;
         L.chunk_hub = reinterpret_cast<int32_t*>(take(4 * size_t(chunks)));
@@ -762,10 +913,10 @@ int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
         L.hub_pairs = reinterpret_cast<int32_t*>(take(4 * size_t(nh)));
         L.item_off = reinterpret_cast<int32_t*>(take(4 * (size_t(nh) + 1)));
         L.hub_best = reinterpret_cast<unsigned long long*>(take(8 * size_t(nh)));
-        if (e == hipSuccess) e = hipMemcpyAsync(L.hub_chunk0, choff, 4 * (size_t(nh) + 1), hipMemcpyDeviceToDevice, s);
+        e = hipMemcpyAsync(L.hub_chunk0, choff, 4 * (size_t(nh) + 1), hipMemcpyDeviceToDevice, s);
         if (e == hipSuccess && nh > 0) {
-            hipLaunchKernelGGL(k_cdlp_hub_chunks, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rp, hubs, nh, choff,
-                               L.chunk_hub, L.chunk_beg);
+            hipLaunchKernelGGL(k_cdlp_hub_chunks, dim3(grid_cap(ceil_div(nh, 256))), dim3(256), 0, s, rowptr_r, hubs, nh,
+                               choff, L.chunk_hub, L.chunk_beg);
             e = hipGetLastError();
         }
         // the plan clears hub_pairs after reading them and the final clears
@@ -775,13 +926,10 @@ int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
             e = hipErrorUnknown;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(va), static_cast<void*>(vb),
-                    static_cast<void*>(off), static_cast<void*>(nch), static_cast<void*>(choff), static_cast<void*>(aux32),
-                    tmp})
-        if (q) hipFree(q);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         if (L.mem) hipFree(L.mem);
+        hipFree(rmem);
         L = CDLPLayout{};
         return set_error(GHET_ERR_HIP, "CDLP layout: %s", hipGetErrorString(e));
     }
@@ -801,9 +949,10 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
     GHET_CHECK(build_cdlp_layout(g, s));
     CDLPLayout& L = g->cdlp;
     void* p;
-    int32_t *la, *lb;
+    int32_t *la, *lb, *lo;  // la, lb: labels in rank space; lo: by vertex id
     GHET_CHECK(g->ws.get(WS_LABEL_A, 4 * size_t(n), &p)); la = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_LABEL_B, 4 * size_t(n), &p)); lb = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_HASH, 4 * size_t(n), &p)); lo = static_cast<int32_t*>(p);
     const bool timing = st != nullptr;
     hipEvent_t* kev = nullptr;
     if (timing) GHET_CHECK(kernel_events(g, 2 * std::max(iters, 1), &kev));
@@ -815,34 +964,42 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
     const int32_t merge_blocks = std::max(1, cus);
     const bool small_cap = o && o->size >= sizeof(ghet_opts) && (o->reserved[0] & 4);
     const uint32_t merge_cap = small_cap ? 64u : static_cast<uint32_t>(kBigSlots);
+    const unsigned gn = grid_cap(ceil_div(n, 256));
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
-    hipLaunchKernelGGL(k_label_init, dim3(grid_cap(ceil_div(n, 256))), dim3(256), 0, s, la, lb, n);
+    hipLaunchKernelGGL(k_label_init, dim3(gn), dim3(256), 0, s, lo, lo, n);
     // the identity-label first iteration has a closed form (see k_cdlp_first_*);
     // it needs the in-degrees to split each all-list, or an undirected graph
     const bool first_closed = !g->directed || (g->in.present() && g->in.rowptr != g->all.rowptr);
     const int32_t* in_rp = g->directed ? g->in.rowptr : nullptr;
+    bool ranked = false;  // the current labels are in la (rank space), else in lo
     for (int32_t it = 0; it < iters; ++it) {
         if (timing) GHET_HIP(hipEventRecord(kev[2 * it], s));
-        const int32_t* A = g->all.rowptr;
-        const int32_t* C = g->all.col;
         if (it == 0 && first_closed) {
-            // vertices with no neighbours are in no list and keep their id (lb holds ids)
+            // on the all-CSR itself, by vertex id; vertices with no neighbours
+            // are in no list and keep their id (lo holds ids)
+            const int32_t* A = g->all.rowptr;
+            const int32_t* C = g->all.col;
             if (L.count[0])
                 hipLaunchKernelGGL(k_cdlp_first_small, dim3(grid_cap(ceil_div(L.count[0], kBT))), dim3(kBT), 0, s,
-                                   L.list[0], L.count[0], A, C, in_rp, lb);
+                                   L.first[0], L.count[0], A, C, in_rp, lo);
             for (int c = 1; c <= 2; ++c)
                 if (L.count[c])
                     hipLaunchKernelGGL(k_cdlp_first_wave, dim3(grid_cap(ceil_div(L.count[c], kBT / 64))), dim3(kBT), 0,
-                                       s, L.list[c], L.count[c], A, C, in_rp, lb);
+                                       s, L.first[c], L.count[c], A, C, in_rp, lo);
             for (int c = 3; c <= 4; ++c)
                 if (L.count[c])
-                    hipLaunchKernelGGL(k_cdlp_first_block, dim3(grid_cap(L.count[c])), dim3(kBT), 0, s, L.list[c],
-                                       L.count[c], A, C, in_rp, lb);
+                    hipLaunchKernelGGL(k_cdlp_first_block, dim3(grid_cap(L.count[c])), dim3(kBT), 0, s, L.first[c],
+                                       L.count[c], A, C, in_rp, lo);
             if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
-            std::swap(la, lb);
             continue;
         }
+        if (!ranked) {
+            hipLaunchKernelGGL(k_cdlp_to_rank, dim3(gn), dim3(256), 0, s, L.perm, lo, n, la, lb);
+            ranked = true;
+        }
+        const int32_t* A = L.rowptr_r;
+        const int32_t* C = L.col_r;
         if (L.count[0])
             hipLaunchKernelGGL(k_cdlp_small, dim3(grid_cap(ceil_div(L.count[0], kBT))), dim3(kBT), 0, s, L.list[0],
                                L.count[0], A, C, la, lb);
@@ -867,9 +1024,10 @@ int run_cdlp(ghet_graph* g, int32_t iters, int32_t* label_out, const ghet_opts* 
         if (timing) GHET_HIP(hipEventRecord(kev[2 * it + 1], s));
         std::swap(la, lb);
     }
+    if (ranked) hipLaunchKernelGGL(k_cdlp_from_rank, dim3(gn), dim3(256), 0, s, L.perm, la, n, lo);
     GHET_HIP(hipEventRecord(g->ev_end, s));
     GHET_HIP(hipGetLastError());
-    if (label_out) GHET_CHECK(copy_out(g, label_out, la, 4 * size_t(n), s));
+    if (label_out) GHET_CHECK(copy_out(g, label_out, lo, 4 * size_t(n), s));
     GHET_HIP(hipStreamSynchronize(s));
     if (st) {
         double hot = 0;

@@ -162,6 +162,7 @@ void destroy_graph(ghet_graph* g) {
     if (g->pr.items_mem) hipFree(g->pr.items_mem);
     if (g->prb.mem) hipFree(g->prb.mem);
     if (g->cdlp.mem) hipFree(g->cdlp.mem);
+    if (g->cdlp.rmem) hipFree(g->cdlp.rmem);
     if (g->tc.mem) hipFree(g->tc.mem);
     if (g->tc.in_mem) hipFree(g->tc.in_mem);
     if (g->tc.task_mem) hipFree(g->tc.task_mem);

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -47,6 +48,24 @@ struct Workspace {
         int64_t t = 0;
         for (size_t b : bytes) t += static_cast<int64_t>(b);
         return t;
+    }
+};
+
+// Owns the temporary device buffers of one layout build.
+struct DevTemp {
+    std::vector<void*> ptrs;
+    hipError_t err = hipSuccess;
+    template <typename T>
+    T* get(size_t count) {
+        void* p = nullptr;
+        if (err == hipSuccess) err = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256));
+        if (err != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    ~DevTemp() {
+        if (!ptrs.empty()) (void)hipDeviceSynchronize();  // no launch may still read them
+        for (void* p : ptrs) hipFree(p);
     }
 };
 
@@ -107,15 +126,28 @@ struct PRBinLayout {
     void* mem = nullptr;
 };
 
-// Degree buckets over the all-CSR for CDLP (vertex lists per class: 0 lane,
+// CDLP runs its counting iterations in rank space: vertices ordered by
+// all-degree, descending (ties by id), so the labels of the hubs, which most
+// edges point at, sit together at the front of the label array.  rowptr_r /
+// col_r are the all-CSR in rank space (each row's neighbours as ranks,
+// ascending).  Label values stay vertex ids.  The degree classes (0 lane,
 // 1 wave, 2 block with a 4K-slot LDS table, 3 block with a 16K-slot table,
-// 4 hub, 5 no neighbours).  A hub's edges are cut into 8192-edge chunks;
-// each chunk writes its distinct (label, count) pairs to its own region of
-// `pairs`, and merge items (hub, hash partition) add them up.
+// 4 hub, 5 no neighbours) are then contiguous rank ranges: list[c] is the
+// range in rank space (a slice of iota), first[c] the same vertices by id
+// (a slice of perm) for the closed-form first iteration, which runs on the
+// all-CSR itself.  A hub's edges are cut into 8192-edge chunks; each chunk
+// writes its distinct (label, count) pairs to its own region of `pairs`,
+// and merge items (hub, hash partition) add them up.
 struct CDLPLayout {
     bool built = false;
     int32_t count[6] = {};
     int32_t* list[6] = {};
+    int32_t* first[6] = {};
+    int32_t* perm = nullptr;      // [n] rank -> vertex
+    int32_t* iota = nullptr;      // [n] r -> r
+    int32_t* rowptr_r = nullptr;  // [n + 1]
+    int32_t* col_r = nullptr;     // [m_all]
+    void* rmem = nullptr;         // perm, iota, rowptr_r, col_r
     int32_t hub_chunks = 0;
     int32_t* chunk_hub = nullptr;    // [hub_chunks] index into list[4]
     int32_t* chunk_beg = nullptr;    // [hub_chunks] first edge of the chunk

@@ -90,23 +90,5 @@ __global__ void k_pr_unpermute(const double* __restrict__ rank, const int32_t* _
         out[perm ? perm[r] : r] = normalise ? rank[r] * inv : rank[r];
 }
 
-// Owns the temporary device buffers of one layout build.
-struct DevTemp {
-    std::vector<void*> ptrs;
-    hipError_t err = hipSuccess;
-    template <typename T>
-    T* get(size_t count) {
-        void* p = nullptr;
-        if (err == hipSuccess) err = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256));
-        if (err != hipSuccess) return nullptr;
-        ptrs.push_back(p);
-        return static_cast<T*>(p);
-    }
-    ~DevTemp() {
-        if (!ptrs.empty()) (void)hipDeviceSynchronize();  // no launch may still read them
-        for (void* p : ptrs) hipFree(p);
-    }
-};
-
 }  // namespace
 }  // namespace ghet

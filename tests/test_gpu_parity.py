@@ -188,6 +188,23 @@ def test_medium_wcc_pr_cdlp(medium):
     err = np.abs(r - ref)
     assert err.sum() <= PR_L1 and err.max() <= PR_MAXABS, (err.sum(), err.max())
     np.testing.assert_array_equal(dg.cdlp(10)[0], O.cdlp(i.rowptr, i.col, o.rowptr, o.col, 10))
+    # 0 and 1 iterations stay by vertex id (identity, closed form); from 2 on
+    # the counting runs in rank space and is mapped back
+    for it in (0, 1, 2, 3):
+        np.testing.assert_array_equal(dg.cdlp(it)[0], O.cdlp(i.rowptr, i.col, o.rowptr, o.col, it),
+                                      err_msg=f"{it} iterations")
+
+
+def test_cdlp_undirected_rank_space(medium):
+    """Undirected input (the all-list is the out-list of a symmetric CSR):
+    the closed-form first iteration takes the smallest neighbour, then the
+    counting runs in rank space.  The oracle with in = out counts every
+    neighbour twice, which leaves the mode and its ties unchanged."""
+    s = medium["hg"].view(G.VIEW_SYM)
+    sg = G.DeviceGraph(s, undirected=True, weights=False)
+    for it in (1, 2, 5):
+        np.testing.assert_array_equal(sg.cdlp(it)[0], O.cdlp(s.rowptr, s.col, s.rowptr, s.col, it),
+                                      err_msg=f"{it} iterations")
 
 
 def test_medium_tc(medium):
