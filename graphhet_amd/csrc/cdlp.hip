@@ -642,6 +642,20 @@ __global__ void k_cdlp_rank_keys(const int32_t* __restrict__ rp, int32_t n, uint
 // and the vertices without neighbours (degrees descend along the ranks).
 __host__ __device__ constexpr int class_order(int c) { return c == 5 ? 5 : 4 - c; }
 
+__device__ __forceinline__ int degree_class(int32_t d) {
+    return d == 0 ? 5 : d <= kSmall ? 0 : d <= kWave ? 1 : d <= kLds ? 2 : d <= kBig ? 3 : 4;
+}
+
+// Keys of a stable sort of the vertex ids by class position.
+__global__ void k_cdlp_first_keys(const int32_t* __restrict__ rp, int32_t n, uint32_t* __restrict__ key,
+                                  int32_t* __restrict__ val) {
+    for (int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+         v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        key[v] = static_cast<uint32_t>(class_order(degree_class(rp[v + 1] - rp[v])));
+        val[v] = static_cast<int32_t>(v);
+    }
+}
+
 // rank[perm[r]] = r; deg_r[r] = all-degree of rank r; okey[r] = the class
 // position of rank r (non-decreasing along the ranks); iota[r] = r.
 __global__ void k_cdlp_rank_rows(const int32_t* __restrict__ perm, const int32_t* __restrict__ rp, int32_t n,
@@ -653,8 +667,7 @@ __global__ void k_cdlp_rank_rows(const int32_t* __restrict__ perm, const int32_t
         const int32_t d = rp[v + 1] - rp[v];
         rank[v] = static_cast<int32_t>(r);
         deg_r[r] = d;
-        const int c = d == 0 ? 5 : d <= kSmall ? 0 : d <= kWave ? 1 : d <= kLds ? 2 : d <= kBig ? 3 : 4;
-        okey[r] = static_cast<uint32_t>(class_order(c));
+        okey[r] = static_cast<uint32_t>(class_order(degree_class(d)));
         iota[r] = static_cast<int32_t>(r);
     }
 }
@@ -802,13 +815,14 @@ int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
     hipError_t e = tmp.err;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     // the rank-space arrays live as long as the graph
-    const size_t rbytes = 2 * al(4 * size_t(n)) + al(4 * (size_t(n) + 1)) + al(4 * size_t(std::max<int64_t>(m, 1)));
+    const size_t rbytes = 3 * al(4 * size_t(n)) + al(4 * (size_t(n) + 1)) + al(4 * size_t(std::max<int64_t>(m, 1)));
     void* rmem = nullptr;
     if (e == hipSuccess) e = hipMalloc(&rmem, rbytes);
-    int32_t *perm = nullptr, *iota = nullptr, *rowptr_r = nullptr, *col_r = nullptr;
+    int32_t *perm = nullptr, *iota = nullptr, *rowptr_r = nullptr, *col_r = nullptr, *first_ids = nullptr;
     if (e == hipSuccess) {
         char* q = static_cast<char*>(rmem);
         perm = reinterpret_cast<int32_t*>(q); q += al(4 * size_t(n));
+        first_ids = reinterpret_cast<int32_t*>(q); q += al(4 * size_t(n));
         iota = reinterpret_cast<int32_t*>(q); q += al(4 * size_t(n));
         rowptr_r = reinterpret_cast<int32_t*>(q); q += al(4 * (size_t(n) + 1));
         col_r = reinterpret_cast<int32_t*>(q);
@@ -847,6 +861,28 @@ int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
     // then read neighbouring labels, the hubs' first
     int end_bit = 1;
     while (end_bit < 31 && (int64_t(1) << end_bit) < n) ++end_bit;
+    // the closed-form first iteration reads the all-CSR by vertex id: each
+    // class's vertices in id order (a stable sort of the ids by class
+    // position), so consecutive waves read neighbouring rows
+    if (e == hipSuccess) {
+        uint32_t* fk0 = tmp.get<uint32_t>(size_t(n));
+        uint32_t* fk1 = tmp.get<uint32_t>(size_t(n));
+        int32_t* fv0 = tmp.get<int32_t>(size_t(n));
+        e = tmp.err;
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_cdlp_first_keys, dim3(gn), dim3(256), 0, s, rp, n, fk0, fv0);
+            e = hipGetLastError();
+        }
+        rocprim::double_buffer<uint32_t> fkd(fk0, fk1);
+        rocprim::double_buffer<int32_t> fvd(fv0, first_ids);
+        size_t fb = 0;
+        if (e == hipSuccess) e = rocprim::radix_sort_pairs(nullptr, fb, fkd, fvd, static_cast<unsigned>(n), 0, 3, s);
+        void* ftmp = nullptr;
+        if (e == hipSuccess) ftmp = tmp.get<char>(fb), e = tmp.err;
+        if (e == hipSuccess) e = rocprim::radix_sort_pairs(ftmp, fb, fkd, fvd, static_cast<unsigned>(n), 0, 3, s);
+        if (e == hipSuccess && fvd.current() != first_ids)
+            e = hipMemcpyAsync(first_ids, fvd.current(), 4 * size_t(n), hipMemcpyDeviceToDevice, s);
+    }
     if (e == hipSuccess && m > 0) {
         size_t seg_b = 0;
         e = rocprim::segmented_radix_sort_keys(nullptr, seg_b, col_tmp, col_r, static_cast<unsigned>(m),
@@ -896,7 +932,7 @@ int build_cdlp_layout(ghet_graph* g, hipStream_t s) {
         for (int c = 0; c < 6; ++c) {
             L.count[c] = ccnt[c];
             L.list[c] = iota + cbeg[c];
-            L.first[c] = perm + cbeg[c];
+            L.first[c] = first_ids + cbeg[c];
         }
         L.perm = perm;
         L.iota = iota;

@@ -133,9 +133,9 @@ struct PRBinLayout {
 // ascending).  Label values stay vertex ids.  The degree classes (0 lane,
 // 1 wave, 2 block with a 4K-slot LDS table, 3 block with a 16K-slot table,
 // 4 hub, 5 no neighbours) are then contiguous rank ranges: list[c] is the
-// range in rank space (a slice of iota), first[c] the same vertices by id
-// (a slice of perm) for the closed-form first iteration, which runs on the
-// all-CSR itself.  A hub's edges are cut into 8192-edge chunks; each chunk
+// range in rank space (a slice of iota), first[c] the same vertices by id,
+// ascending, for the closed-form first iteration, which runs on the all-CSR
+// itself.  A hub's edges are cut into 8192-edge chunks; each chunk
 // writes its distinct (label, count) pairs to its own region of `pairs`,
 // and merge items (hub, hash partition) add them up.
 struct CDLPLayout {
@@ -147,7 +147,7 @@ struct CDLPLayout {
     int32_t* iota = nullptr;      // [n] r -> r
     int32_t* rowptr_r = nullptr;  // [n + 1]
     int32_t* col_r = nullptr;     // [m_all]
-    void* rmem = nullptr;         // perm, iota, rowptr_r, col_r
+    void* rmem = nullptr;         // perm, the first[] ids, iota, rowptr_r, col_r
     int32_t hub_chunks = 0;
     int32_t* chunk_hub = nullptr;    // [hub_chunks] index into list[4]
     int32_t* chunk_beg = nullptr;    // [hub_chunks] first edge of the chunk
