@@ -42,6 +42,9 @@ constexpr int kChunk = 8192;          // edges per hub chunk (at most 8192 disti
 constexpr int kMergeTarget = 4096;    // pairs per hub merge item
 constexpr int kMaxHubChunks = 4096;   // chunks of one hub the merge kernel's LDS prefix holds
 constexpr int32_t kEmpty = -1;
+constexpr int kLdsLoads = 4;                // edges per lane loaded together in k_cdlp_lds
+constexpr int kHubLoads = kChunk / kBigBT;  // a hub chunk's edges per lane, loaded together
+static_assert(kHubLoads * kBigBT == kChunk, "a hub chunk is whole steps of the block");
 
 __device__ __forceinline__ uint64_t pack(uint32_t count, int32_t label) {
     return (static_cast<uint64_t>(count) << 32) | (0xFFFFFFFFu - static_cast<uint32_t>(label));
@@ -323,11 +326,23 @@ __global__ void __launch_bounds__(BT) k_cdlp_lds(const int32_t* __restrict__ lis
                 }
                 if (threadIdx.x == 0) s_full = 0;
                 __syncthreads();
-                for (int j0 = threadIdx.x & ~63; j0 < d; j0 += BT) {  // whole waves: wave_table_add
-                    const int j = j0 + lane_id();
-                    const int32_t x = j < d ? lab[col[b + j]] : 0;
-                    const bool mine = j < d && (nsub == 1 || hash_part(x, nsub, 0x27D4EB2Fu) == sub);
-                    if (!wave_table_add(keys, cnts, slots - 1, x, mine)) s_full = 1;
+                // whole waves (wave_table_add); each lane's kLdsLoads edges
+                // load together (d >= 1: lanes past the end read the last edge)
+                for (int j0 = threadIdx.x & ~63; j0 < d; j0 += BT * kLdsLoads) {
+                    int32_t u[kLdsLoads], x[kLdsLoads];
+#pragma unroll
+                    for (int k = 0; k < kLdsLoads; ++k) {
+                        const int j = j0 + k * BT + lane_id();
+                        u[k] = col[b + (j < d ? j : d - 1)];
+                    }
+#pragma unroll
+                    for (int k = 0; k < kLdsLoads; ++k) x[k] = lab[u[k]];
+#pragma unroll
+                    for (int k = 0; k < kLdsLoads; ++k) {
+                        const int j = j0 + k * BT + lane_id();
+                        const bool mine = j < d && (nsub == 1 || hash_part(x[k], nsub, 0x27D4EB2Fu) == sub);
+                        if (!wave_table_add(keys, cnts, slots - 1, x[k], mine)) s_full = 1;
+                    }
                 }
                 __syncthreads();
                 full = s_full != 0;
@@ -366,9 +381,23 @@ __global__ void __launch_bounds__(kBigBT) k_cdlp_hub_count(
     }
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (int32_t j0 = beg + (threadIdx.x & ~63); j0 < end; j0 += kBigBT) {  // whole waves: wave_table_add
-        const int32_t j = j0 + lane_id();
-        wave_table_add(keys, cnts, slots - 1, j < end ? lab[col[j]] : 0, j < end);
+    // whole waves (wave_table_add); the chunk's kHubLoads edges per lane load
+    // together (lanes past the end read the chunk's last edge)
+    {
+        int32_t u[kHubLoads], x[kHubLoads];
+        const int32_t j0 = beg + (threadIdx.x & ~63);
+#pragma unroll
+        for (int k = 0; k < kHubLoads; ++k) {
+            const int32_t j = j0 + k * kBigBT + lane_id();
+            u[k] = col[j < end ? j : end - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kHubLoads; ++k) x[k] = lab[u[k]];
+#pragma unroll
+        for (int k = 0; k < kHubLoads; ++k) {
+            const int32_t j = j0 + k * kBigBT + lane_id();
+            wave_table_add(keys, cnts, slots - 1, x[k], j < end);
+        }
     }
     __syncthreads();
     uint2* out = pairs + static_cast<int64_t>(c) * kChunk;
