@@ -19,6 +19,8 @@ properties that determine them completely, vectorised over every edge:
   PR    every one of 20 iterations against the FP64 oracle stepping from
         the same start: L1 <= 1e-6 and max-abs reported; the reference form
         (no normalisation) and the normalised form.
+  CDLP  3 iterations against the oracle itself (16 threads), label for
+        label.
 """
 import os
 
@@ -168,3 +170,20 @@ def test_pr_s24_every_iteration_vs_fp64_oracle(graphs, kind):
     print(f"\nPR {kind} s24 per iteration: (iter, L1, max-abs, L1 normalised)")
     for r in rows:
         print("  %2d  %.3e  %.3e  %.3e" % r)
+
+
+@pytest.mark.parametrize("kind", ["rmat", "er"])
+def test_cdlp_s24_three_iterations(graphs, kind):
+    """CDLP at full size against the oracle, label for label: 3 iterations,
+    the closed form by vertex id and then two counted in degree-rank space
+    (every class, hubs' chunked merge included, at its real size)."""
+    hg, o, i, dg = graphs(kind)
+    if kind == "rmat" and not os.environ.get("GHET_GRAPH_CACHE"):
+        da = G.DeviceGraph.rmat(24, 16, seed=1)  # the same graph, with its all-CSR
+    else:
+        da = G.DeviceGraph(o, i, hg.view(G.VIEW_ALL, copy=False))
+    try:
+        lab, _ = da.cdlp(3)
+    finally:
+        da.close()
+    np.testing.assert_array_equal(lab, O.cdlp(i.rowptr, i.col, o.rowptr, o.col, 3, threads=16))
