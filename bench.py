@@ -106,6 +106,79 @@ def measured_traffic(workload: str, seed: int):
                                       f"from {os.path.relpath(path, ROOT)}, measured with these PR sources")
 
 
+def live_pr_traffic(args, kernels):
+    """L2-to-fabric bytes per PR iteration, measured in this run: two
+    rocprofv3 PMC passes (FETCH_SIZE, then WRITE_SIZE, one counter each)
+    over a child run of this script that runs the PR step alone on the same
+    graph (the step's iterations, whose last also writes the FP64 ranks, and
+    one 1 GiB device copy).  Each counter is
+    scaled by what it reads for that copy (FETCH_SIZE about half the true
+    bytes, WRITE_SIZE all of them), and the per-dispatch averages of the
+    iteration's kernels are summed.  Returns (bytes, details) or (None,
+    reason).  A pass that exceeds its time limit is killed with its process
+    group.  (rocprofv3 may exit non-zero after the program has finished and
+    its files are whole: the counters are used if they are there.)"""
+    import csv
+    import re
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    child = [sys.executable, os.path.join(ROOT, "bench.py"), "--algos", "pr", "--steps", "1", "--warmup", "0",
+             "--pr-iters", str(args.pr_iters), "--scale", str(args.scale), "--edge-factor", str(args.edge_factor),
+             "--seed", str(args.seed), "--gen", args.gen, "--no-extras", "--no-batched", "--no-dynamic",
+             "--no-learned", "--no-cpu-baseline", "--no-pmc", "--copy-probe-mb", "1024"]
+    if args.graph_cache:
+        child += ["--graph-cache", args.graph_cache]
+    copy_kib = 1024 * 1024  # 1 GiB
+    per = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(td, counter)
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--"] + child
+            t0 = time.time()
+            log(f"PR traffic: {counter} pass (rocprofv3 over a child run of the PR step)")
+            try:
+                proc = subprocess.Popen(cmd, cwd="/tmp", env={**os.environ, "TMPDIR": "/tmp"},
+                                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, start_new_session=True)
+            except OSError as e:
+                return None, f"rocprofv3 did not start: {e}"
+            try:
+                proc.wait(timeout=150)
+            except subprocess.TimeoutExpired:
+                os.killpg(proc.pid, signal.SIGKILL)
+                proc.wait()
+                return None, f"the {counter} pass exceeded 150 s"
+            path = os.path.join(d, "run_counter_collection.csv")
+            if not os.path.exists(path):
+                return None, f"the {counter} pass wrote no counters (exit {proc.returncode})"
+            vals, names = {}, {}
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    dsp = int(r["Dispatch_Id"])
+                    vals[dsp] = vals.get(dsp, 0.0) + float(r["Counter_Value"])
+                    mt = re.search(r"(k_\w+)", r["Kernel_Name"])
+                    names[dsp] = mt.group(1) if mt else r["Kernel_Name"]
+            copies = [v for dsp, v in vals.items() if "copyBuffer" in names[dsp]]
+            if not copies or max(copies) <= 0:
+                return None, f"no calibration copy in the {counter} pass"
+            factor = copy_kib / max(copies)
+            avg = {}
+            for k in kernels:
+                xs = [v for dsp, v in vals.items() if names[dsp] == k]
+                if not xs:
+                    return None, f"no {k} dispatch in the {counter} pass"
+                avg[k] = statistics.mean(xs)
+            per[counter] = {"factor": round(factor, 3), "kib_per_dispatch": {k: round(v, 1) for k, v in avg.items()},
+                            "pass_s": round(time.time() - t0, 1)}
+            log(f"PR traffic: {counter} pass done in {time.time() - t0:.0f}s")
+    total = sum(per[c]["factor"] * v * 1024 for c in per for v in per[c]["kib_per_dispatch"].values())
+    return int(total), per
+
+
 def gteps(edges, ms):
     return edges / (ms * 1e-3) / 1e9 if ms > 0 else None
 
@@ -144,6 +217,9 @@ def main():
     ap.add_argument("--graph-cache", default=os.environ.get("GHET_GRAPH_CACHE", ""),
                     help="directory of binary CSR caches for the R-MAT graphs (default: $GHET_GRAPH_CACHE, "
                          "empty = always generate)")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="do not measure PR's L2-to-fabric traffic in this run (two rocprofv3 PMC passes over a "
+                         "child run of the PR step); use the committed profiles/ figure instead")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -310,6 +386,22 @@ def main():
     dynamic = None
     if not args.no_dynamic:
         dynamic = dynamic_section(G, args)
+
+    if roofline is not None and not args.no_pmc:
+        # after every timed and reported section: PR's traffic measured now
+        kernels = roofline["kernel"].split("+")
+        live, detail = live_pr_traffic(args, kernels)
+        if live is not None:
+            roofline["traffic_committed"] = roofline["traffic"]
+            roofline["traffic"] = live
+            roofline["traffic_note"] = ("L2-to-fabric bytes per iteration measured in this run (MALL hits "
+                                        "included): rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over a child run "
+                                        "of the PR step, each scaled by what it reads for a 1 GiB device copy in "
+                                        "the same pass; the committed profiles/ figure is traffic_committed")
+            roofline["traffic_pmc"] = detail
+        else:
+            roofline["traffic_note"] = f"not measured in this run ({detail}); " + roofline["traffic_note"]
+        log("PR traffic measured")
 
     out = {
         "metric": "GTEPS (BFS/SSSP) + PR HBM GB/s, scale-24 R-MAT; learned/batched vs plain",
