@@ -46,7 +46,7 @@ void gpu_usage(size_t free_before) {
                 used * 100.0 / static_cast<double>(total));
 }
 
-// One warm-up call, then `reps` timed calls; the checker judges the output.
+// One warm-up call, then `reps` timed calls; the checker validates the output.
 template <typename Run, typename Check>
 void checked(Row& row, const std::string& name, const char* label, int reps, Run&& run, Check&& check) {
     try {
