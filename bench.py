@@ -179,6 +179,41 @@ def live_pr_traffic(args, kernels):
     return int(total), per
 
 
+def algorithmic_bandwidth(n, m, runs, per_alg_ms, other, pr_roofline):
+    """Each algorithm's algorithmic bytes (the per-unit figures the roofline
+    counts), its device time in this run, and the rate against the 8 TB/s
+    HBM peak.  BFS: 4 m_r + 8 n_r + 4 n + n/8; SSSP: 12 m_r + 8 n_r + 8 n;
+    PR: per iteration, as the roofline counts it; WCC: one edge pass,
+    4m + 4(n+1) + 8n, over the whole run; CDLP: per iteration, 4(2m) + 4(2m)
+    + 4(n+1) + 4n over the mean iteration; TC: the compulsory 4 m_dag +
+    4(n+1) (the intersection traffic is reported beside it).  A direction-
+    optimising BFS reads fewer edges than 4 m_r counts, so its figure is an
+    effective rate."""
+    out = {}
+
+    def put(name, nbytes, ms, unit):
+        if ms:
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            out[name] = {"bytes": int(nbytes), "ms": round(ms, 4), "gbs": round(gbs, 1),
+                         "frac": round(gbs / PEAK_HBM_GBS, 4), "unit": unit}
+
+    for alg, per_edge in (("bfs", 4), ("sssp", 12)):
+        st = runs[-1].get(alg)
+        if st is not None:
+            m_r, n_r = st.edges_traversed, st.vertices_reached
+            extra = n / 8 if alg == "bfs" else 4 * n
+            put(alg, per_edge * m_r + 8 * n_r + 4 * n + extra, per_alg_ms.get(alg), "one call")
+    if pr_roofline is not None:
+        put("pr", pr_roofline["bytes_per_launch"], pr_roofline["kernel_avg_us"] * 1e-3, "one iteration")
+    if "wcc_ms" in other:
+        put("wcc", 4 * m + 4 * (n + 1) + 8 * n, other["wcc_ms"], "one edge pass over the whole call")
+    if "cdlp_iter_ms" in other:
+        put("cdlp", 16 * m + 4 * (n + 1) + 4 * n, other["cdlp_iter_ms"], "one iteration (mean of the call's)")
+    if "tc_ms" in other:
+        put("tc", 4 * other["tc_dag_edges"] + 4 * (n + 1), other["tc_ms"], "compulsory bytes of one call")
+    return out
+
+
 def gteps(edges, ms):
     return edges / (ms * 1e-3) / 1e9 if ms > 0 else None
 
@@ -432,6 +467,7 @@ def main():
         "batched": batched,
         "dynamic": dynamic,
         "roofline": roofline,
+        "algorithmic_bandwidth": algorithmic_bandwidth(n, m, runs, per_alg_ms, other, roofline),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
