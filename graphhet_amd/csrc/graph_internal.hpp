@@ -32,7 +32,7 @@ enum WsSlot {
     WS_COUNTERS, WS_DIST, WS_STAMP_NEAR, WS_STAMP_FAR, WS_FAR_A, WS_FAR_B, WS_PARENT,
     WS_PR_CONTRIB, WS_PR_RANK, WS_PR_PART, WS_PR_LONGPART, WS_LABEL_A, WS_LABEL_B, WS_HASH,
     WS_TC_COUNT, WS_PRE, WS_PR_OUT, WS_MULTI_A, WS_MULTI_B, WS_MULTI_C, WS_MULTI_D, WS_MISC,
-    WS_SSSP_LIST, WS_SSSP_START, WS_BFS_START, WS_SSSP_VERT, WS_BAR, WS_NUM_SLOTS
+    WS_SSSP_LIST, WS_SSSP_START, WS_BFS_START, WS_SSSP_VERT, WS_BAR, WS_SSSP_TILE, WS_NUM_SLOTS
 };
 
 struct Workspace {

@@ -102,6 +102,21 @@ __device__ __forceinline__ int32_t light_end(const double* __restrict__ w, int32
     return lo;
 }
 
+// The relax launches find each tile's entries through tile_first: entry
+// tile_first[k] holds edge k * kTile of the pass (the preps write it: an
+// entry with edges [a, a + c) marks every k with a <= k kTile < a + c), and
+// tile_first[ceil(E / kTile)] = F - 1 closes the last tile.  A tile's
+// entries are then tile_first[t] .. tile_first[t + 1] (the last may start
+// past the tile: its edges are never looked up), with no binary search of
+// the pass's scan per tile.
+__device__ __forceinline__ void mark_tiles(int32_t* tile_first, int32_t slot, int32_t a, int32_t c) {
+    for (int64_t k = (static_cast<int64_t>(a) + kTile - 1) / kTile; k * kTile < static_cast<int64_t>(a) + c; ++k)
+        tile_first[k] = slot;
+}
+__device__ __forceinline__ void close_tiles(int32_t* tile_first, int32_t F, int32_t E) {
+    tile_first[(static_cast<int64_t>(E) + kTile - 1) / kTile] = F - 1;
+}
+
 __global__ void k_sssp_seed(double* dist, int32_t* list, int32_t* stamp_near, int32_t src, SsspCtr* ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         dist[src] = 0.0;
@@ -176,8 +191,9 @@ __global__ void __launch_bounds__(kPrepBT) k_row_prep(const int32_t* __restrict_
                                                   const int32_t* __restrict__ rowptr, const double* __restrict__ w,
                                                   const double* __restrict__ dist, double threshold,
                                                   int32_t* __restrict__ scan, int32_t* __restrict__ start,
-                                                  int32_t* __restrict__ vert, unsigned long long* alloc,
-                                                  unsigned long long* next_alloc, SsspCtr* ctr) {
+                                                  int32_t* __restrict__ vert, int32_t* __restrict__ tile_first,
+                                                  unsigned long long* alloc, unsigned long long* next_alloc,
+                                                  SsspCtr* ctr) {
     __shared__ int32_t s_wave[kPrepBT / 64];
     __shared__ unsigned long long s_base;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -236,10 +252,14 @@ __global__ void __launch_bounds__(kPrepBT) k_row_prep(const int32_t* __restrict_
             scan[slot] = xbase + run;
             start[slot] = kHeavy ? lo[k] : b[k];
             vert[slot] = u[k];
+            mark_tiles(tile_first, slot, xbase + run, c[k]);
             run += c[k];
         }
     }
-    if (tid == 0 && ebase + cnt == F) scan[F] = xbase + total;
+    if (tid == 0 && ebase + cnt == F) {
+        scan[F] = xbase + total;
+        close_tiles(tile_first, F, xbase + total);
+    }
 }
 
 // Small queues (at most kSmallMax entries): one 1024-thread block resets the
@@ -255,7 +275,7 @@ __global__ void __launch_bounds__(kSmallBT) k_row_split_small(const int32_t* __r
                                                               const double* __restrict__ w,
                                                               const double* __restrict__ dist, double threshold,
                                                               int32_t* __restrict__ scan, int32_t* __restrict__ start,
-                                                              SsspCtr* ctr) {
+                                                              int32_t* __restrict__ tile_first, SsspCtr* ctr) {
     __shared__ int32_t s_wave[kSmallBT / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     unsigned long long* words = reinterpret_cast<unsigned long long*>(ctr->slots);
@@ -300,10 +320,14 @@ __global__ void __launch_bounds__(kSmallBT) k_row_split_small(const int32_t* __r
         if (i < F) {
             scan[i] = run;
             start[i] = st[k];
+            mark_tiles(tile_first, i, run, c[k]);
             run += c[k];
         }
     }
-    if (tid == 0) scan[F] = total;
+    if (tid == 0) {
+        scan[F] = total;
+        close_tiles(tile_first, F, total);
+    }
 }
 
 // One pass, persistent blocks over kTile-edge tiles of queue q; entry j
@@ -329,6 +353,7 @@ template <int kMode>
 __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, int32_t F,
                                                const int32_t* __restrict__ scan,
                                                const int32_t* __restrict__ start,
+                                               const int32_t* __restrict__ tile_first,
                                                const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                const double* __restrict__ wts, double* dist, double threshold,
                                                int32_t* stamp_near, int32_t next_round, int32_t r0,
@@ -359,8 +384,8 @@ __global__ void __launch_bounds__(kBT) k_relax(const int32_t* __restrict__ q, in
         const int32_t t0 = static_cast<int32_t>(tile * kTile);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
         if (tid == 0) {
-            int i0 = upper_bound_i32(scan, F + 1, t0) - 1;
-            int i1 = upper_bound_i32(scan, F + 1, t1 - 1) - 1;
+            const int i0 = tile_first[tile];      // holds edge t0
+            const int i1 = tile_first[tile + 1];  // holds edge t1, or the pass's last entry
             s_i0 = i0;
             s_span = i1 - i0 + 1;
         }
@@ -513,9 +538,9 @@ constexpr int kPullChunk = kPullBT * kPullItems;
 __global__ void __launch_bounds__(kPullBT) k_pull_prep(int32_t n, const int32_t* __restrict__ in_rowptr,
                                                       const double* __restrict__ dist, double threshold,
                                                       int32_t* __restrict__ scan, int32_t* __restrict__ start,
-                                                      int32_t* __restrict__ vert, unsigned long long* alloc,
-                                                      unsigned long long* next_alloc, SsspCtr* ctr, HostMail* mail,
-                                                      unsigned long long seq) {
+                                                      int32_t* __restrict__ vert, int32_t* __restrict__ tile_first,
+                                                      unsigned long long* alloc, unsigned long long* next_alloc,
+                                                      SsspCtr* ctr, HostMail* mail, unsigned long long seq) {
     constexpr int W = kPullBT / 64;
     __shared__ int32_t s_ent[kPullItems][W], s_edg[kPullItems][W];
     __shared__ unsigned long long s_red[W];
@@ -594,9 +619,11 @@ __global__ void __launch_bounds__(kPullBT) k_pull_prep(int32_t n, const int32_t*
             for (int k = 0; k < kPullItems; ++k)
                 if (cnt[k] > 0) {
                     const int32_t slot = ebase + ent_off[k] + rank[k];
-                    scan[slot] = xbase + edg_off[k] + incl[k] - cnt[k];
+                    const int32_t first_edge = xbase + edg_off[k] + incl[k] - cnt[k];
+                    scan[slot] = first_edge;
                     start[slot] = b[k];
                     vert[slot] = static_cast<int32_t>(c * kPullChunk + k * kPullBT + tid);
+                    mark_tiles(tile_first, slot, first_edge, cnt[k]);
                 }
             ebase += ent_tot;
             xbase += edg_tot;
@@ -610,6 +637,7 @@ __global__ void __launch_bounds__(kPullBT) k_pull_prep(int32_t n, const int32_t*
             const unsigned long long a = load_agent(alloc);
             const int32_t F = static_cast<int32_t>(a >> 32), E = static_cast<int32_t>(a & 0xFFFFFFFFu);
             scan[F] = E;
+            close_tiles(tile_first, F, E);
             mail->v[0] = static_cast<unsigned long long>(E);
             mail->v[1] = kInfBits;
             mail->v[2] = 0;
@@ -886,6 +914,10 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
     GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_SSSP_VERT, 4 * size_t(n), &p)); qvert = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_COUNTERS, sizeof(SsspCtr), &p)); ctr = static_cast<SsspCtr*>(p);
+    // a pass's edges are distinct out-edges (or in-edges): at most m of them
+    const int64_t m_pass = std::max<int64_t>(g->out.m, g->in.present() ? g->in.m : 0);
+    GHET_CHECK(g->ws.get(WS_SSSP_TILE, 4 * (size_t(ceil_div(m_pass, kTile)) + 2), &p));
+    int32_t* tile_first = static_cast<int32_t*>(p);
     GHET_CHECK(mail_init(g));
     const HostMail* hm = g->mail;  // totals of the last pass (mail_wait)
 
@@ -953,31 +985,33 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         if (Fq <= kSmallMax) {
             if (heavy)
                 hipLaunchKernelGGL(k_row_split_small<true>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, g->out.rowptr, L.w,
-                                   dist, threshold, scan, qstart, ctr);
+                                   dist, threshold, scan, qstart, tile_first, ctr);
             else
                 hipLaunchKernelGGL(k_row_split_small<false>, dim3(1), dim3(kSmallBT), 0, s, q, Fq, g->out.rowptr, L.w,
-                                   dist, threshold, scan, qstart, ctr);
+                                   dist, threshold, scan, qstart, tile_first, ctr);
         } else {
             const unsigned gq = static_cast<unsigned>(ceil_div(Fq, kPrepBT * kPrepItems));
             unsigned long long* al = ctr->alloc + (passes & 1);
             unsigned long long* next_al = ctr->alloc + ((passes + 1) & 1);
             if (heavy)
                 hipLaunchKernelGGL(k_row_prep<true>, dim3(gq), dim3(kPrepBT), 0, s, q, Fq, g->out.rowptr, L.w, dist,
-                                   threshold, scan, qstart, qvert, al, next_al, ctr);
+                                   threshold, scan, qstart, qvert, tile_first, al, next_al, ctr);
             else
                 hipLaunchKernelGGL(k_row_prep<false>, dim3(gq), dim3(kPrepBT), 0, s, q, Fq, g->out.rowptr, L.w, dist,
-                                   threshold, scan, qstart, qvert, al, next_al, ctr);
+                                   threshold, scan, qstart, qvert, tile_first, al, next_al, ctr);
             q = qvert;  // the relax reads the entries in their reserved order
         }
         ++passes;
         const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E_max, kTile), kMaxBlocks));
         const unsigned long long seq = mail_next(g);
         if (heavy)
-            hipLaunchKernelGGL(k_relax<kRelaxHeavy>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
+            hipLaunchKernelGGL(k_relax<kRelaxHeavy>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, tile_first,
+                               g->out.rowptr, L.col,
                                L.w, dist, threshold, stamp_near, next_round, r0, nxt, list, stamp_far, far_gen,
                                far[sel], sel, ctr, g->mail_dev, seq);
         else
-            hipLaunchKernelGGL(k_relax<kRelaxRound>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, g->out.rowptr, L.col,
+            hipLaunchKernelGGL(k_relax<kRelaxRound>, dim3(grid), dim3(kBT), 0, s, q, Fq, scan, qstart, tile_first,
+                               g->out.rowptr, L.col,
                                L.w, dist, threshold, stamp_near, next_round, r0, nxt, list, stamp_far, far_gen,
                                far[sel], sel, ctr, g->mail_dev, seq);
         GHET_HIP(hipGetLastError());
@@ -1012,7 +1046,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         const unsigned gp = static_cast<unsigned>(std::min<int64_t>(ceil_div(n, kPullChunk), 2 * cus));
         unsigned long long seq = mail_next(g);
         hipLaunchKernelGGL(k_pull_prep, dim3(gp), dim3(kPullBT), 0, s, n, g->in.rowptr, dist, threshold, scan, qstart,
-                           qvert, al, next_al, ctr, g->mail_dev, seq);
+                           qvert, tile_first, al, next_al, ctr, g->mail_dev, seq);
         GHET_HIP(hipGetLastError());
         GHET_CHECK(mail_wait(g, s, seq));
         const int32_t Fp = static_cast<int32_t>(hm->v[4]);
@@ -1023,7 +1057,7 @@ int run_sssp(ghet_graph* g, int32_t src, double inf, double* dist_out, int32_t* 
         if (Ep > 0) {
             const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(Ep, kTile), kMaxBlocks));
             seq = mail_next(g);
-            hipLaunchKernelGGL(k_relax<kRelaxPull>, dim3(grid), dim3(kBT), 0, s, qvert, Fp, scan, qstart,
+            hipLaunchKernelGGL(k_relax<kRelaxPull>, dim3(grid), dim3(kBT), 0, s, qvert, Fp, scan, qstart, tile_first,
                                g->in.rowptr, g->in.col, g->in.w, dist, threshold, stamp_near, round, r0, nxt, list,
                                stamp_far, far_gen, far[sel], sel, ctr, g->mail_dev, seq);
             GHET_HIP(hipGetLastError());
