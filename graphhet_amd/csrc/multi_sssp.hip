@@ -92,6 +92,7 @@ struct MssArgs {
     uint32_t *fmask, *bmask;           // far masks, bucket masks
     int32_t *qa, *qb, *far0, *far1, *list;
     int32_t *scan, *start, *vert;      // a pass's entries: first edge, first position, vertex
+    int32_t* tile_first;               // [ceil(m / kTile) + 2] entry holding each tile's first edge
     uint32_t* emask;                   // and source mask
     int32_t* btot;                     // [gridDim.x] edges of each block's entry range
     unsigned long long* bar;           // grid-barrier counters (kBarWords, zeroed before the launch)
@@ -194,6 +195,16 @@ __device__ __forceinline__ int32_t row_segment(const MssArgs& a, int32_t u, uint
     return kHeavy ? re - lo : lo - rb;
 }
 
+// Tile index of a pass: tile_first[k] = the entry holding edge k * kTile (an
+// entry with edges [o, o + c) marks every k with o <= k kTile < o + c), and
+// tile_first[ceil(E / kTile)] = F - 1 closes the last tile; a relax tile's
+// entries are then tile_first[t] .. tile_first[t + 1], with no binary search
+// of the scan.
+__device__ __forceinline__ void mark_tiles(int32_t* tile_first, int32_t slot, int32_t o, int32_t c) {
+    for (int64_t k = (static_cast<int64_t>(o) + kTile - 1) / kTile; k * kTile < static_cast<int64_t>(o) + c; ++k)
+        tile_first[k] = slot;
+}
+
 // Exclusive scan of one value per thread over the block; *total gets the sum.
 __device__ __forceinline__ int32_t block_excl_scan(int32_t x, int32_t* total, MssShared& sh) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -235,8 +246,12 @@ __device__ bool mss_count(const MssArgs& a, const int32_t* __restrict__ q, int32
                 a.start[tid] = st;
                 a.vert[tid] = u;
                 a.emask[tid] = m;
+                mark_tiles(a.tile_first, tid, off, cnt);
             }
-            if (tid == 0) a.scan[F] = total;
+            if (tid == 0) {
+                a.scan[F] = total;
+                a.tile_first[(static_cast<int64_t>(total) + kTile - 1) / kTile] = F - 1;
+            }
         }
         return true;
     }
@@ -274,10 +289,16 @@ __device__ void mss_scan(const MssArgs& a, int32_t F, MssShared& sh) {
         const int32_t c = i < b1 ? a.scan[i] : 0;
         int32_t t;
         const int32_t off = block_excl_scan(c, &t, sh);
-        if (i < b1) a.scan[i] = carry + off;
+        if (i < b1) {
+            a.scan[i] = carry + off;
+            mark_tiles(a.tile_first, static_cast<int32_t>(i), carry + off, c);
+        }
         carry += t;
     }
-    if (blockIdx.x == gridDim.x - 1 && tid == 0) a.scan[F] = carry;
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) {
+        a.scan[F] = carry;
+        a.tile_first[(static_cast<int64_t>(carry) + kTile - 1) / kTile] = F - 1;
+    }
 }
 
 // One pass over the prepared entries, in kTile-edge tiles; entry j covers
@@ -309,8 +330,8 @@ __device__ void mss_relax(const MssArgs& a, int32_t F, double thr, uint32_t* mas
         const int32_t t0 = static_cast<int32_t>(tile * kTile);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + kTile, E));
         if (tid == 0) {
-            const int i0 = upper_bound_i32(a.scan, F + 1, t0) - 1;
-            const int i1 = upper_bound_i32(a.scan, F + 1, t1 - 1) - 1;
+            const int i0 = a.tile_first[tile];      // holds edge t0
+            const int i1 = a.tile_first[tile + 1];  // holds edge t1, or the pass's last entry
             sh.i0 = i0;
             sh.span = i1 - i0 + 1;
             sh.next_tile = static_cast<int64_t>(gridDim.x) + atomicAdd(&c->tiles, 1);
@@ -709,6 +730,9 @@ int run_sssp_multi(ghet_graph* g, int32_t k, const int32_t* srcs, double inf, do
     a.vert = a.start + n;
     a.emask = reinterpret_cast<uint32_t*>(a.vert + n);
     GHET_CHECK(g->ws.get(WS_MULTI_D, 4 * (size_t(n) + 1), &p)); a.scan = static_cast<int32_t*>(p);
+    // a pass's entries are distinct vertices: at most m edges
+    GHET_CHECK(g->ws.get(WS_SSSP_TILE, 4 * (size_t(ceil_div(g->out.m, kTile)) + 2), &p));
+    a.tile_first = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_COUNTERS, 3 * sizeof(PhaseCtr) + 64, &p));
     a.ctr = static_cast<PhaseCtr*>(p);
     a.stats = reinterpret_cast<long long*>(a.ctr + 3);
