@@ -185,9 +185,10 @@ struct MSLayout {
 struct TCLayout {
     bool built = false;
     bool symmetric = false;   // out-CSR symmetric and loop-free
-    DevCSR dag;               // oriented by (degree, id)
+    DevCSR dag;               // oriented by (degree, id); each row in that order
     int32_t* in_rowptr = nullptr;  // transpose of the counted CSR (DAG or out-CSR): in-lists
     int32_t* in_col = nullptr;
+    int32_t* in_pos = nullptr;     // DAG: position of each in-entry's edge u -> v in u's row
     int* task_ctr = nullptr;       // k_tc_wave's task counter (zeroed before each count)
     void* in_mem = nullptr;
     void* wave_tasks = nullptr;    // {vertex, first in-list entry} per task (tc.hip)

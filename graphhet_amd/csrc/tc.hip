@@ -21,7 +21,11 @@
 // edge u -> v costs sum over u of |N(u)|^2 entries, against sum over v of
 // |in(v)| |N(v)| for streaming N(v) past u's table: on the degree-ordered
 // DAG of R-MAT the first is about 1.8 times smaller (2.5e9 against 4.4e9
-// entries at scale 20).  When v's task is done, the slot counters go to the
+// entries at scale 20).  In the DAG form each row is kept in the DAG's own
+// order and each in-entry knows where v sits in N(u), so only the part of
+// N(u) after v streams: N(v) holds only vertices after v, so nothing before
+// it can hit, and the entries fall to sum over u of |N(u)| (|N(u)| - 1) / 2.
+// When v's task is done, the slot counters go to the
 // global counts with one 64-bit atomic per neighbour that was hit, v's total
 // with one, and each u's with one: device-scope atomics per vertex and edge
 // instead of per triangle corner (they go to the fabric: the XCDs' L2s are
@@ -223,12 +227,25 @@ struct alignas(8) TcTask {
     int32_t v, first;
 };
 
+// The part of N(u) an in-entry j of v streams: all of it, or (in_pos given,
+// DAG form) the entries after v, which sits at in_pos[j] in u's row.
+__device__ __forceinline__ void in_entry_list(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ in_col,
+                                              const int32_t* __restrict__ in_pos, int32_t j, int32_t* u, int32_t* ub,
+                                              int32_t* len) {
+    *u = in_col[j];
+    const int32_t end = rowptr[*u + 1];
+    *ub = in_pos ? in_pos[j] + 1 : rowptr[*u];
+    *len = end - *ub;
+}
+
 // Tasks of vertices with |N(v)| <= kWaveMax: one wave each.  The wave
 // builds v's table, then every u of the task's in-list run streams N(u)
-// past it: c hits add c to u (one atomic per u with hits) and to v.
+// (or its part after v) past it: c hits add c to u (one atomic per u with
+// hits) and to v.
 __global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                  const int32_t* __restrict__ in_rowptr,
-                                                 const int32_t* __restrict__ in_col, const TcTask* __restrict__ tasks,
+                                                 const int32_t* __restrict__ in_col, const int32_t* __restrict__ in_pos,
+                                                 const TcTask* __restrict__ tasks,
                                                  int32_t num_tasks, int vpt, int* task_ctr, unsigned long long* cnt) {
     __shared__ __align__(16) int32_t s_keys[kBT / 64][kWaveSlots];
     __shared__ uint32_t s_cnts[kBT / 64][kWaveSlots];
@@ -262,11 +279,7 @@ __global__ void __launch_bounds__(kBT) k_tc_wave(const int32_t* __restrict__ row
             for (int32_t jb = j0; jb < j1; jb += 64) {
                 const int32_t j = jb + lane;
                 int32_t u = -1, ub = 0, du = 0;
-                if (j < j1) {
-                    u = in_col[j];
-                    ub = rowptr[u];
-                    du = rowptr[u + 1] - ub;
-                }
+                if (j < j1) in_entry_list(rowptr, in_col, in_pos, j, &u, &ub, &du);
                 const uint32_t c = tc_stream(ub, du, col, keys, cnts, mask, shift, s_batch[wave]);
                 if (c) {
                     atomicAdd(&cnt[u], static_cast<unsigned long long>(c));
@@ -292,6 +305,7 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
                                                           const int32_t* __restrict__ col,
                                                           const int32_t* __restrict__ in_rowptr,
                                                           const int32_t* __restrict__ in_col,
+                                                          const int32_t* __restrict__ in_pos,
                                                           const TcTask* __restrict__ tasks, int32_t num_tasks,
                                                           unsigned long long* cnt) {
     __shared__ __align__(16) int32_t keys[kBlockSlots];
@@ -322,11 +336,7 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
             for (int32_t jb = j0 + wave * 64; jb < j1; jb += kBlockBT) {
                 const int32_t j = jb + lane;
                 int32_t u = -1, ub = 0, du = 0;
-                if (j < j1) {
-                    u = in_col[j];
-                    ub = rowptr[u];
-                    du = rowptr[u + 1] - ub;
-                }
+                if (j < j1) in_entry_list(rowptr, in_col, in_pos, j, &u, &ub, &du);
                 const uint32_t c = tc_stream(ub, du, col, keys, cnts, mask, shift, s_batch[wave]);
                 if (c) {
                     atomicAdd(&cnt[u], static_cast<unsigned long long>(c));
@@ -346,6 +356,41 @@ __global__ void __launch_bounds__(kBlockBT, 8) k_tc_block(const int32_t* __restr
 }
 
 // Transpose of the counted CSR: the source of every edge, in row order.
+// Sort keys putting a DAG row in the DAG's order: (degree, id) of each
+// entry, the degree from the symmetric CSR the DAG was oriented by.
+__global__ void k_dag_order_keys(const int32_t* __restrict__ col, int64_t m, const int32_t* __restrict__ rowptr,
+                                 unsigned long long* __restrict__ key) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t w = col[e];
+        key[e] = (static_cast<unsigned long long>(rowptr[w + 1] - rowptr[w]) << 32) | static_cast<uint32_t>(w);
+    }
+}
+__global__ void k_dag_order_ids(const unsigned long long* __restrict__ key, int64_t m, int32_t* __restrict__ col) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        col[e] = static_cast<int32_t>(key[e] & 0xFFFFFFFFull);
+}
+
+// Edge indices 0 .. m-1 (the values of the transpose's sort).
+__global__ void k_tc_iota(int32_t* __restrict__ v, int64_t m) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        v[e] = static_cast<int32_t>(e);
+}
+
+// In-entries from the sorted edge indices: the source of each edge and its
+// position in the source's row.
+__global__ void k_tc_in_fill(const int32_t* __restrict__ sorted_e, const int32_t* __restrict__ src, int64_t m,
+                             int32_t* __restrict__ in_col, int32_t* __restrict__ in_pos) {
+    for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < m;
+         j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t e = sorted_e[j];
+        in_col[j] = src[e];
+        in_pos[j] = e;
+    }
+}
+
 __global__ void __launch_bounds__(kBT) k_tc_sources(const int32_t* __restrict__ rowptr, int32_t n,
                                                     int32_t* __restrict__ src) {
     const int lane = lane_id();
@@ -444,6 +489,25 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
         GHET_CHECK(exclusive_scan_i32(deg, drp, n, aux, s));
         hipLaunchKernelGGL(k_dag, dim3(gw), dim3(kBT), 0, s, g->out.rowptr, g->out.col, n, deg, drp, dcol, 1);
         GHET_HIP(hipGetLastError());
+        if (m_dag > 0) {
+            // each row in the DAG's order, (degree, id) ascending: a
+            // segmented radix sort of 64-bit keys, one segment per row
+            DevTemp t;
+            unsigned long long* k0 = t.get<unsigned long long>(size_t(m_dag));
+            unsigned long long* k1 = t.get<unsigned long long>(size_t(m_dag));
+            GHET_HIP(t.err);
+            const unsigned ge = grid_cap(std::min<int64_t>(ceil_div(m_dag, 256), 65535));
+            hipLaunchKernelGGL(k_dag_order_keys, dim3(ge), dim3(256), 0, s, dcol, m_dag, g->out.rowptr, k0);
+            size_t sb = 0;
+            GHET_HIP(rocprim::segmented_radix_sort_keys(nullptr, sb, k0, k1, static_cast<unsigned>(m_dag),
+                                                        static_cast<unsigned>(n), drp, drp + 1, 0, 64, s));
+            void* st = t.get<char>(sb);
+            GHET_HIP(t.err);
+            GHET_HIP(rocprim::segmented_radix_sort_keys(st, sb, k0, k1, static_cast<unsigned>(m_dag),
+                                                        static_cast<unsigned>(n), drp, drp + 1, 0, 64, s));
+            hipLaunchKernelGGL(k_dag_order_ids, dim3(ge), dim3(256), 0, s, k1, m_dag, dcol);
+            GHET_HIP(hipGetLastError());
+        }
         GHET_HIP(hipStreamSynchronize(s));
         L.dag = {n, m_dag, drp, dcol, nullptr};
     }
@@ -454,7 +518,7 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
         const int64_t mx = X.m;
         if (mx >= INT32_MAX) return set_error(GHET_ERR_UNSUPPORTED, "TC: %lld edges", static_cast<long long>(mx));
         int32_t *ka = nullptr, *kb = nullptr, *va = nullptr, *vb = nullptr, *nw = nullptr, *nbk = nullptr, *ow = nullptr,
-                *ob = nullptr, *aux = nullptr;
+                *ob = nullptr, *aux = nullptr, *srcs = nullptr;
         void* tmp = nullptr;
         size_t tb = 0;
         hipError_t e = hipSuccess;
@@ -465,19 +529,26 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
         get(&kb, size_t(mx));
         get(&va, size_t(mx));
         get(&vb, size_t(mx));
+        get(&srcs, size_t(mx));
         get(&nw, size_t(n));
         get(&nbk, size_t(n));
         get(&ow, size_t(n) + 1);
         get(&ob, size_t(n) + 1);
         get(&aux, size_t(scan_aux_ints(n)));
-        const size_t in_b = ((4 * (size_t(n) + 1) + 255) & ~size_t(255)) + ((4 * size_t(mx) + 255) & ~size_t(255));
+        const size_t col_b = (4 * size_t(mx) + 255) & ~size_t(255);
+        const size_t in_b = ((4 * (size_t(n) + 1) + 255) & ~size_t(255)) + 2 * col_b;
         if (e == hipSuccess) e = hipMalloc(&L.in_mem, in_b + 256);
         if (e == hipSuccess) {
             L.in_rowptr = static_cast<int32_t*>(L.in_mem);
             L.in_col = reinterpret_cast<int32_t*>(static_cast<char*>(L.in_mem) + ((4 * (size_t(n) + 1) + 255) & ~size_t(255)));
+            L.in_pos = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(L.in_col) + col_b);
             L.task_ctr = reinterpret_cast<int*>(static_cast<char*>(L.in_mem) + in_b);
             if (mx > 0) {
-                hipLaunchKernelGGL(k_tc_sources, dim3(grid_cap(ceil_div(n, kBT / 64))), dim3(kBT), 0, s, X.rowptr, n, va);
+                // sort the edge indices by target (stable: each in-list in
+                // source order), then read each edge's source and position
+                hipLaunchKernelGGL(k_tc_sources, dim3(grid_cap(ceil_div(n, kBT / 64))), dim3(kBT), 0, s, X.rowptr, n, srcs);
+                hipLaunchKernelGGL(k_tc_iota, dim3(grid_cap(std::min<int64_t>(ceil_div(mx, 256), 65535))), dim3(256), 0, s,
+                                   va, mx);
                 e = hipMemcpyAsync(ka, X.col, 4 * size_t(mx), hipMemcpyDeviceToDevice, s);
             }
         }
@@ -490,7 +561,11 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_tc_key_offsets, dim3(grid_cap(std::min<int64_t>(ceil_div(mx + 1, 256), 65535))), dim3(256), 0,
                                s, kd.current(), mx, n, L.in_rowptr);
-            if (mx > 0) e = hipMemcpyAsync(L.in_col, vd.current(), 4 * size_t(mx), hipMemcpyDeviceToDevice, s);
+            if (mx > 0) {
+                hipLaunchKernelGGL(k_tc_in_fill, dim3(grid_cap(std::min<int64_t>(ceil_div(mx, 256), 65535))), dim3(256), 0,
+                                   s, vd.current(), srcs, mx, L.in_col, L.in_pos);
+                e = hipGetLastError();
+            }
         }
         if (e == hipSuccess) {
             const unsigned gn = grid_cap(std::min<int64_t>(ceil_div(n, 256), 4096));
@@ -519,7 +594,7 @@ int build_tc_layout(ghet_graph* g, hipStream_t s) {
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         for (void* q : {static_cast<void*>(ka), static_cast<void*>(kb), static_cast<void*>(va), static_cast<void*>(vb),
                         static_cast<void*>(nw), static_cast<void*>(nbk), static_cast<void*>(ow), static_cast<void*>(ob),
-                        static_cast<void*>(aux), tmp})
+                        static_cast<void*>(aux), static_cast<void*>(srcs), tmp})
             if (q) hipFree(q);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -555,21 +630,23 @@ int enqueue_tc(ghet_graph* g, unsigned long long* cnt, int vpt, int waves_per_cu
     const int32_t n = g->n;
     const TCLayout& L = g->tc;
     const DevCSR& X = L.symmetric ? L.dag : g->out;
+    const int32_t* in_pos = L.symmetric ? L.in_pos : nullptr;  // the DAG streams only past v
     GHET_CHECK(fill<uint64_t>(reinterpret_cast<uint64_t*>(cnt), 0ULL, n, s));
     if (X.m > 0) {
         int cus = 0;
         GHET_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
         if (L.num_block_tasks > 0)
             hipLaunchKernelGGL(k_tc_block, dim3(static_cast<unsigned>(std::min(L.num_block_tasks, 65535))), dim3(kBlockBT),
-                               0, s, X.rowptr, X.col, L.in_rowptr, L.in_col, static_cast<const TcTask*>(L.block_tasks),
-                               L.num_block_tasks, cnt);
+                               0, s, X.rowptr, X.col, L.in_rowptr, L.in_col, in_pos,
+                               static_cast<const TcTask*>(L.block_tasks), L.num_block_tasks, cnt);
         if (L.num_wave_tasks > 0) {
             GHET_CHECK(fill<int32_t>(L.task_ctr, 0, 1, s));
             const int64_t grabs = ceil_div(L.num_wave_tasks, std::max(vpt, 1));
             const int64_t cap = int64_t(std::max(cus, 1)) * waves_per_cu / (kBT / 64);
             hipLaunchKernelGGL(k_tc_wave, dim3(grid_cap(std::min<int64_t>(ceil_div(grabs, kBT / 64), cap))), dim3(kBT), 0,
-                               s, X.rowptr, X.col, L.in_rowptr, L.in_col, static_cast<const TcTask*>(L.wave_tasks),
-                               L.num_wave_tasks, std::max(vpt, 1), L.task_ctr, cnt);
+                               s, X.rowptr, X.col, L.in_rowptr, L.in_col, in_pos,
+                               static_cast<const TcTask*>(L.wave_tasks), L.num_wave_tasks, std::max(vpt, 1), L.task_ctr,
+                               cnt);
         }
     }
     if (L.symmetric)
