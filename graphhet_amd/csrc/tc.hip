@@ -56,6 +56,7 @@ constexpr int kBlockBT = 1024;               // k_tc_block threads
 constexpr int kBlockSlots = 8192;            // k_tc_block table slots (keys + counters: 64 KB, two blocks per CU)
 constexpr int kInWave = 256;                 // in-list entries per wave task
 constexpr int kInBlock = 4096;               // in-list entries per block task
+constexpr int kStreamQ = 8;                  // list entries in flight per lane (4: 3 % slower at s24)
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int64_t len, int32_t key) {
     int64_t lo = 0, hi = len;
@@ -166,9 +167,9 @@ struct TcBatch {
 };
 
 // One list per lane (len 0: none), concatenated: the wave walks entries
-// k = 0 .. total-1 of the concatenation four per lane at a time, so
+// k = 0 .. total-1 of the concatenation kStreamQ per lane at a time, so
 // neighbouring lanes load neighbouring entries of a list (coalesced) and
-// four loads are in flight per lane.  Each entry found in the table adds 1
+// kStreamQ loads are in flight per lane.  Each entry found in the table adds 1
 // to its slot's counter and to its list's hits; returns the lane's own
 // list's hits.
 __device__ __forceinline__ uint32_t tc_stream(int32_t lb, int32_t len, const int32_t* __restrict__ col,
@@ -189,11 +190,11 @@ __device__ __forceinline__ uint32_t tc_stream(int32_t lb, int32_t len, const int
     if (lane == 0) bt.pre[64] = total;
     wave_lds_sync();
     int owner = 0;  // list of entry k: the last i with pre[i] <= k (k only grows)
-    for (int32_t k0 = 0; k0 < total; k0 += 4 * 64) {
-        int32_t w[4];
-        int own[4];
+    for (int32_t k0 = 0; k0 < total; k0 += kStreamQ * 64) {
+        int32_t w[kStreamQ];
+        int own[kStreamQ];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < kStreamQ; ++j) {
             const int32_t k = k0 + j * 64 + lane;
             w[j] = -1;
             own[j] = 0;
@@ -204,7 +205,7 @@ __device__ __forceinline__ uint32_t tc_stream(int32_t lb, int32_t len, const int
             }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < kStreamQ; ++j) {
             if (w[j] >= 0) {
                 const int sl = tc_find(keys, mask, shift, w[j]);
                 if (sl >= 0) {
