@@ -8,8 +8,10 @@
 // output; the mode only changes speed.
 //
 // Modes per level:
-//   PUSH    queue frontier; edge-balanced expansion (global degree scan, 2048-edge
-//           tiles, LDS-staged owner search and LDS-staged next queue).
+//   PUSH    queue frontier; edge-balanced expansion (2048-edge tiles, LDS-staged
+//           owner search and LDS-staged next queue).  Every queue is written
+//           with its push entries {first edge, row offset} by its producer,
+//           so a push level needs no degree-scan launch.
 //   PULL    bitmap frontier; each unvisited vertex scans its in-list and stops at
 //           the first frontier parent; one wave owns 64 vertices = one 64-bit
 //           word of the bitmaps, so the next frontier is written without atomics.
@@ -23,19 +25,19 @@
 namespace ghet {
 namespace {
 
-// Per-level counters.  qcount/hubs/alloc/conv hand out queue positions (one
-// atomic per LDS flush / hub / block); the sums are spread over kSlots slots:
+// Per-level counters.  hubs/alloc/conv hand out queue positions (one atomic
+// per hub / LDS flush / block); the sums are spread over kSlots slots:
 // v[0] vertices discovered, v[1] their out-edges, v[2] their in-edges
 // (v[7] and done count block arrivals for the mailbox post).
 struct BfsCtr {
-    int32_t qcount;
     int32_t hubs;
-    unsigned long long alloc;  // push prep: (entries << 32) | edges reserved so far
-    int32_t conv;              // bitmap -> queue conversion
     unsigned int done;         // blocks finished (mailbox ticket)
+    unsigned long long alloc;  // next queue: (entries << 32) | edges reserved so far
+    unsigned long long conv;   // this level's queue from a bitmap: the same packing
     int32_t pad[10];
     CtrSlot slots[kSlots];
 };
+static_assert(offsetof(BfsCtr, slots) == 64, "counter header of 64 bytes");
 constexpr int kCtrWords = sizeof(BfsCtr) / 8;
 
 constexpr int kBT = 256;
@@ -45,7 +47,6 @@ constexpr int kOut = 2 * kTile;      // LDS output queue capacity
 static_assert(kOut >= kTile, "a push tile can stage kTile discoveries");
 constexpr int kHubDeg = 4096;
 constexpr int kMaxBlocks = 2048;     // persistent grid cap (8 blocks per CU)
-constexpr int kPrepItems = 4;        // queue entries per thread of k_push_prep
 constexpr int kPullSets = 4;         // vertices a k_pull thread scans side by side (2: 5 % slower at s24, 8: 9 %)
 
 __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_t key) {
@@ -58,13 +59,27 @@ __device__ __forceinline__ int upper_bound_i32(const int32_t* a, int len, int32_
     return lo;
 }
 
+// Entries of a push queue: e.x = the entry's first edge (ascending over the
+// queue), e.y = its row offset.  Number of entries with e.x <= key.
+__device__ __forceinline__ int upper_bound_ent(const int2* a, int len, int32_t key) {
+    int lo = 0, hi = len;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid].x <= key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Level-0 state in one pass: depth INT_MAX except the source (0), visited =
 // the no-in-edge mask (none without an in-CSR) plus the source, queue =
-// {src}, both counter buffers zero, and clear_bm (if given) zero.
+// {src} with its push entry, both counter buffers zero, and clear_bm (if
+// given) zero.
 __global__ void __launch_bounds__(kBT) k_bfs_init(int32_t n, int64_t words, const uint32_t* __restrict__ skip,
-                                                  int32_t src, int32_t* __restrict__ depth,
-                                                  uint32_t* __restrict__ visited, int32_t* __restrict__ queue,
-                                                  BfsCtr* ctrs, uint32_t* __restrict__ clear_bm) {
+                                                  int32_t src, const int32_t* __restrict__ rowptr,
+                                                  int32_t* __restrict__ depth, uint32_t* __restrict__ visited,
+                                                  int32_t* __restrict__ queue, int2* __restrict__ ent, BfsCtr* ctrs,
+                                                  uint32_t* __restrict__ clear_bm) {
     if (blockIdx.x == 0) {  // both counter buffers
         unsigned long long* w = reinterpret_cast<unsigned long long*>(ctrs);
         for (int i = threadIdx.x; i < 2 * kCtrWords; i += kBT) w[i] = 0;
@@ -80,7 +95,10 @@ __global__ void __launch_bounds__(kBT) k_bfs_init(int32_t n, int64_t words, cons
             if (clear_bm) clear_bm[i] = 0u;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) queue[0] = src;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        queue[0] = src;
+        ent[0] = make_int2(0, rowptr[src]);
+    }
 }
 
 __global__ void k_zero_ctr(BfsCtr* c) {
@@ -96,68 +114,6 @@ __device__ __forceinline__ void zero_next(BfsCtr* c) {
         unsigned long long* w = reinterpret_cast<unsigned long long*>(c);
         for (int i = threadIdx.x; i < kCtrWords; i += blockDim.x) w[i] = 0;
     }
-}
-
-// Push-level prep: the frontier's out-degree scan in one launch.  Each block
-// takes kPrepItems * kBT queue entries, scans their degrees and reserves a
-// range of entries AND of edges with one 64-bit atomic on ctr->alloc
-// ((entries << 32) | edges), then writes, at its entry range, scan[] (first
-// edge of the entry) and start[] (its row offset).  Blocks land in any order,
-// but entry and edge ranges are reserved together, so scan[] ascends; the
-// block that ends at F writes scan[F] = E.  The expansion reads start[]
-// instead of the queue.
-__global__ void __launch_bounds__(kBT) k_push_prep(const int32_t* __restrict__ q, int32_t F,
-                                                   const int32_t* __restrict__ rowptr, int32_t* __restrict__ scan,
-                                                   int32_t* __restrict__ start, BfsCtr* ctr) {
-    __shared__ int32_t s_wave[kBT / 64];
-    __shared__ unsigned long long s_base;
-    const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kBT * kPrepItems;
-    int32_t b[kPrepItems], d[kPrepItems];
-    int32_t sum = 0;
-    // thread tid owns entries i0 + tid*kPrepItems + k: the valid entries of
-    // the block are its first cnt, in scan order
-    int32_t u[kPrepItems];
-#pragma unroll
-    for (int k = 0; k < kPrepItems; ++k) {
-        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
-        u[k] = i < F ? q[i] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < kPrepItems; ++k) {
-        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
-        b[k] = rowptr[u[k]];
-        d[k] = i < F ? rowptr[u[k] + 1] - b[k] : 0;
-        sum += d[k];
-    }
-    int32_t incl = sum;
-    for (int off = 1; off < 64; off <<= 1) {
-        const int32_t y = __shfl_up(incl, off);
-        if (lane >= off) incl += y;
-    }
-    if (lane == 63) s_wave[wave] = incl;
-    __syncthreads();
-    int32_t run = incl - sum, total = 0;
-    for (int w = 0; w < kBT / 64; ++w) {
-        if (w < wave) run += s_wave[w];
-        total += s_wave[w];
-    }
-    const int32_t cnt = F - i0 < kBT * kPrepItems ? static_cast<int32_t>(F - i0) : kBT * kPrepItems;
-    if (tid == 0)
-        s_base = atomicAdd(&ctr->alloc, (static_cast<unsigned long long>(cnt) << 32) | static_cast<uint32_t>(total));
-    __syncthreads();
-    const int32_t ebase = static_cast<int32_t>(s_base >> 32), xbase = static_cast<int32_t>(s_base & 0xFFFFFFFFu);
-#pragma unroll
-    for (int k = 0; k < kPrepItems; ++k) {
-        const int64_t i = i0 + static_cast<int64_t>(tid) * kPrepItems + k;
-        if (i < F) {
-            const int32_t slot = ebase + tid * kPrepItems + k;
-            scan[slot] = xbase + run;
-            start[slot] = b[k];
-            run += d[k];
-        }
-    }
-    if (tid == 0 && ebase + cnt == F) scan[F] = xbase + total;
 }
 
 // Block totals of (count, out-edges, in-edges) into the block's slot.
@@ -196,41 +152,121 @@ __device__ __forceinline__ void post_level(BfsCtr* ctr, HostMail* mail, unsigned
     }
 }
 
+// Moves the c staged discoveries to the next queue together with their push
+// entries.  Thread t takes the consecutive items [t P, t P + P), P =
+// ceil(c / kBT); the block scans their out-degrees and reserves its entry
+// and edge ranges with one 64-bit atomic on *alloc ((entries << 32) |
+// edges), so the next level's entries ascend in both, whatever order the
+// blocks flush in.  Adds the items and their out- and in-degrees to the
+// block's counts.  Block-uniform; flushes only above limit staged items.
+__device__ __forceinline__ void flush_entries(int32_t* buf, int32_t* cnt, int limit, unsigned long long* alloc,
+                                              int32_t* __restrict__ q, int2* __restrict__ ent,
+                                              const int32_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ in_rowptr, int32_t* s_wave,
+                                              unsigned long long* s_base, unsigned long long& items,
+                                              unsigned long long& od, unsigned long long& id) {
+    __syncthreads();
+    const int c = *cnt;
+    if (c == 0 || c <= limit) {
+        __syncthreads();
+        return;
+    }
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int per = (c + kBT - 1) / kBT;
+    const int j0 = threadIdx.x * per, j1 = min(j0 + per, c);
+    int32_t sum = 0;
+    for (int j = j0; j < j1; j += 4) {  // four rows' offsets in flight
+        int32_t rb[4], re[4], ib[4], ie[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int32_t v = buf[min(j + k, j1 - 1)];
+            rb[k] = rowptr[v];
+            re[k] = rowptr[v + 1];
+            ib[k] = in_rowptr ? in_rowptr[v] : 0;
+            ie[k] = in_rowptr ? in_rowptr[v + 1] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (j + k < j1) {
+                sum += re[k] - rb[k];
+                id += ie[k] - ib[k];
+                ++items;
+            }
+    }
+    od += sum;
+    int32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int32_t run = incl - sum, total = 0;
+    for (int w = 0; w < kBT / 64; ++w) {
+        if (w < wave) run += s_wave[w];
+        total += s_wave[w];
+    }
+    if (threadIdx.x == 0)
+        *s_base = atomicAdd(alloc, (static_cast<unsigned long long>(c) << 32) | static_cast<uint32_t>(total));
+    __syncthreads();
+    int32_t slot = static_cast<int32_t>(*s_base >> 32) + j0;
+    int32_t first = static_cast<int32_t>(*s_base & 0xFFFFFFFFu) + run;
+    for (int j = j0; j < j1; j += 4) {
+        int32_t v[4], rb[4], re[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = buf[min(j + k, j1 - 1)];
+            rb[k] = rowptr[v[k]];
+            re[k] = rowptr[v[k] + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (j + k < j1) {
+                q[slot] = v[k];
+                ent[slot] = make_int2(first, rb[k]);
+                ++slot;
+                first += re[k] - rb[k];
+            }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *cnt = 0;
+    __syncthreads();
+}
+
 // PUSH: persistent blocks over tiles of tile_edges (<= kTile) consecutive
-// frontier edges; entry j of the level covers edges [scan[j], scan[j+1]) of
-// the row starting at start[j].  Discoveries go to the next queue and, when
-// next_bm is given (a cleared bitmap), to the next frontier bitmap too, so a
-// pull level after a large push level needs no queue-to-bitmap pass.
-// Discoveries are staged in LDS and flushed to the next queue in bulk.
-__global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__ start, int32_t F,
-                                                     const int32_t* __restrict__ scan, int32_t E, int32_t tile_edges,
-                                                     const int32_t* __restrict__ rowptr,
+// frontier edges; entry j of the level covers edges [ent[j].x, ent[j+1].x)
+// (E for the last) of the row starting at ent[j].y.  Discoveries go to the
+// next queue with their entries and, when next_bm is given (a cleared
+// bitmap), to the next frontier bitmap too, so a pull level after a large
+// push level needs no queue-to-bitmap pass.  Discoveries are staged in LDS
+// and flushed to the next queue in bulk.
+__global__ void __launch_bounds__(kBT) k_push_expand(const int2* __restrict__ ent, int32_t F, int32_t E,
+                                                     int32_t tile_edges, const int32_t* __restrict__ rowptr,
                                                      const int32_t* __restrict__ col, uint32_t* visited,
                                                      int32_t* __restrict__ depth, int32_t next_level,
-                                                     int32_t* __restrict__ next_q, BfsCtr* ctr,
-                                                     const int32_t* __restrict__ in_rowptr, BfsCtr* next_ctr,
-                                                     uint32_t* next_bm, HostMail* mail, unsigned long long seq) {
+                                                     int32_t* __restrict__ next_q, int2* __restrict__ next_ent,
+                                                     BfsCtr* ctr, const int32_t* __restrict__ in_rowptr,
+                                                     BfsCtr* next_ctr, uint32_t* next_bm, HostMail* mail,
+                                                     unsigned long long seq) {
     __shared__ int32_t s_scan[kTile];
     __shared__ int32_t s_start[kTile];
     __shared__ int32_t s_out[kOut];
-    __shared__ int32_t s_i0, s_span, s_cnt, s_base;
+    __shared__ int32_t s_i0, s_span, s_cnt;
+    __shared__ int32_t s_wave[kBT / 64];
+    __shared__ unsigned long long s_base;
     __shared__ unsigned long long s_red[kBT / 64];
     const int tid = threadIdx.x;
     if (tid == 0) s_cnt = 0;
     zero_next(next_ctr);
     __syncthreads();  // counter zeroed before any wave appends
     unsigned long long cnt = 0, od = 0, id = 0;
-    auto on_item = [&](int32_t v) {
-        ++cnt;
-        od += rowptr[v + 1] - rowptr[v];
-        if (in_rowptr) id += in_rowptr[v + 1] - in_rowptr[v];
-    };
     for (int64_t tile = blockIdx.x; tile * tile_edges < E; tile += gridDim.x) {
         const int32_t t0 = static_cast<int32_t>(tile * tile_edges);
         const int32_t t1 = static_cast<int32_t>(std::min<int64_t>(static_cast<int64_t>(t0) + tile_edges, E));
         if (tid == 0) {
-            int i0 = upper_bound_i32(scan, F + 1, t0) - 1;
-            int i1 = upper_bound_i32(scan, F + 1, t1 - 1) - 1;
+            // ent[0].x = 0 <= t0 < t1 <= E, so both searches land in [0, F)
+            int i0 = upper_bound_ent(ent, F, t0) - 1;
+            int i1 = upper_bound_ent(ent, F, t1 - 1) - 1;
             s_i0 = i0;
             s_span = i1 - i0 + 1;
         }
@@ -239,8 +275,9 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
         const bool use_lds = span <= kTile;
         if (use_lds) {
             for (int j = tid; j < span; j += kBT) {
-                s_scan[j] = scan[i0 + j];
-                s_start[j] = start[i0 + j];
+                const int2 en = ent[i0 + j];
+                s_scan[j] = en.x;
+                s_start[j] = en.y;
             }
         }
         __syncthreads();
@@ -258,8 +295,9 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
                     const int j = upper_bound_i32(s_scan, span, e) - 1;
                     pos[k] = s_start[j] + (e - s_scan[j]);
                 } else {
-                    const int j = upper_bound_i32(scan + i0, span, e) - 1;
-                    pos[k] = start[i0 + j] + (e - scan[i0 + j]);
+                    const int j = upper_bound_ent(ent + i0, span, e) - 1;
+                    const int2 en = ent[i0 + j];
+                    pos[k] = en.y + (e - en.x);
                 }
             }
         }
@@ -282,9 +320,10 @@ __global__ void __launch_bounds__(kBT) k_push_expand(const int32_t* __restrict__
             }
             lq_push(s_out, &s_cnt, disc, v[k]);
         }
-        lq_flush<kBT>(s_out, &s_cnt, &s_base, &ctr->qcount, next_q, kOut - kTile, on_item);
+        flush_entries(s_out, &s_cnt, kOut - kTile, &ctr->alloc, next_q, next_ent, rowptr, in_rowptr, s_wave, &s_base,
+                      cnt, od, id);
     }
-    lq_flush<kBT>(s_out, &s_cnt, &s_base, &ctr->qcount, next_q, 0, on_item);
+    flush_entries(s_out, &s_cnt, 0, &ctr->alloc, next_q, next_ent, rowptr, in_rowptr, s_wave, &s_base, cnt, od, id);
     post_level(ctr, mail, seq, block_report(cnt, od, id, ctr, s_red));
 }
 
@@ -501,14 +540,17 @@ __global__ void k_queue_to_bitmap(const int32_t* __restrict__ q, int32_t F, uint
     }
 }
 
-// Frontier bitmap -> queue: each thread takes 4 bitmap words (128 vertices),
-// the block scans the popcounts and reserves its range of the queue with one
-// global atomic.  One pass, no loop: words is a multiple of 8, so a thread's
-// 4 words are either all present or all past the end.
+// Frontier bitmap -> queue with its push entries: each thread takes 4 bitmap
+// words (128 vertices), the block scans their popcounts and out-degrees and
+// reserves its entry and edge ranges with one 64-bit atomic ((entries << 32)
+// | edges), so the entries ascend in both.  One pass, no loop: words is a
+// multiple of 8, so a thread's 4 words are either all present or all past
+// the end.
 __global__ void __launch_bounds__(kBT) k_bitmap_to_queue(int64_t words, int32_t n, const uint32_t* __restrict__ bm,
-                                                         int32_t* __restrict__ q, int32_t* counter) {
-    __shared__ int32_t s_wave[kBT / 64];
-    __shared__ int32_t s_base;
+                                                         const int32_t* __restrict__ rowptr, int32_t* __restrict__ q,
+                                                         int2* __restrict__ ent, unsigned long long* counter) {
+    __shared__ int32_t s_wave[kBT / 64], s_wdeg[kBT / 64];
+    __shared__ unsigned long long s_base;
     const int lane = lane_id(), wave = threadIdx.x >> 6;
     const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kBT + threadIdx.x) * 4;
     uint4 x = make_uint4(0u, 0u, 0u, 0u);
@@ -523,26 +565,51 @@ __global__ void __launch_bounds__(kBT) k_bitmap_to_queue(int64_t words, int32_t 
     x.z &= valid(w0 + 2);
     x.w &= valid(w0 + 3);
     const int c = __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
-    int incl = c;
+    const uint32_t ws[4] = {x.x, x.y, x.z, x.w};
+    int32_t dsum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        for (uint32_t b = ws[k]; b; b &= b - 1) {
+            const int64_t v = (w0 + k) * 32 + (__ffs(b) - 1);
+            dsum += rowptr[v + 1] - rowptr[v];
+        }
+    int incl = c, dincl = dsum;
     for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(incl, off);
-        if (lane >= off) incl += y;
+        const int y = __shfl_up(incl, off), yd = __shfl_up(dincl, off);
+        if (lane >= off) {
+            incl += y;
+            dincl += yd;
+        }
     }
-    if (lane == 63) s_wave[wave] = incl;
+    if (lane == 63) {
+        s_wave[wave] = incl;
+        s_wdeg[wave] = dincl;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < kBT / 64; ++w) t += s_wave[w];
-        s_base = t > 0 ? atomicAdd(counter, t) : 0;
+        int t = 0, td = 0;
+        for (int w = 0; w < kBT / 64; ++w) {
+            t += s_wave[w];
+            td += s_wdeg[w];
+        }
+        s_base = t > 0 ? atomicAdd(counter, (static_cast<unsigned long long>(t) << 32) | static_cast<uint32_t>(td)) : 0;
     }
     __syncthreads();
-    int off = s_base + incl - c;
-    for (int w = 0; w < wave; ++w) off += s_wave[w];
-    const uint32_t ws[4] = {x.x, x.y, x.z, x.w};
+    int off = static_cast<int32_t>(s_base >> 32) + incl - c;
+    int32_t first = static_cast<int32_t>(s_base & 0xFFFFFFFFu) + dincl - dsum;
+    for (int w = 0; w < wave; ++w) {
+        off += s_wave[w];
+        first += s_wdeg[w];
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         for (uint32_t b = ws[k]; b; b &= b - 1) {
-            q[off++] = static_cast<int32_t>((w0 + k) * 32 + (__ffs(b) - 1));
+            const int32_t v = static_cast<int32_t>((w0 + k) * 32 + (__ffs(b) - 1));
+            const int32_t rb = rowptr[v];
+            q[off] = v;
+            ent[off] = make_int2(first, rb);
+            ++off;
+            first += rowptr[v + 1] - rb;
         }
     }
 }
@@ -608,7 +675,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     // 8 words per 256 vertices, padded to whole k_pull block iterations
     // (kPullSets * 256 vertices), so no iteration indexes past the end
     const int64_t words = 8 * kPullSets * ceil_div(std::max(n, 1), 256 * kPullSets);
-    int32_t *depth, *qa, *qb, *scan, *start, *hubq;
+    int32_t *depth, *qa, *qb, *hubq;
+    int2 *enta, *entb;  // push entries of qa / qb
     uint32_t *visited, *bma, *bmb;
     BfsCtr *ctrs, *ctr;
     void* p;
@@ -618,8 +686,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
     GHET_CHECK(g->ws.get(WS_FRONT_B, 4 * size_t(words), &p)); bmb = static_cast<uint32_t*>(p);
     GHET_CHECK(g->ws.get(WS_QUEUE_A, 4 * size_t(n), &p)); qa = static_cast<int32_t*>(p);
     GHET_CHECK(g->ws.get(WS_QUEUE_B, 4 * size_t(n), &p)); qb = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_SCAN, 4 * (size_t(n) + 1), &p)); scan = static_cast<int32_t*>(p);
-    GHET_CHECK(g->ws.get(WS_BFS_START, 4 * size_t(n), &p)); start = static_cast<int32_t*>(p);
+    GHET_CHECK(g->ws.get(WS_SCAN, 8 * size_t(std::max(n, 1)), &p)); enta = static_cast<int2*>(p);
+    GHET_CHECK(g->ws.get(WS_BFS_START, 8 * size_t(std::max(n, 1)), &p)); entb = static_cast<int2*>(p);
     GHET_CHECK(g->ws.get(WS_MISC, 4 * size_t(n) + 256, &p)); hubq = static_cast<int32_t*>(p);
     // two counter buffers: level L counts into ctrs[L & 1], and its main
     // kernel clears the other one for level L + 1
@@ -648,8 +716,8 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
 
     GHET_HIP(hipEventRecord(g->ev_begin, s));
     hipLaunchKernelGGL(k_bfs_init, dim3(static_cast<unsigned>(std::min<int64_t>(ceil_div(std::max<int64_t>(n, words), kBT), 8192))),
-                       dim3(kBT), 0, s, n, words, have_in ? g->bfs_skip : nullptr, src, depth, visited, qa, ctrs,
-                       have_in ? bma : nullptr);
+                       dim3(kBT), 0, s, n, words, have_in ? g->bfs_skip : nullptr, src, g->out.rowptr, depth, visited,
+                       qa, enta, ctrs, have_in ? bma : nullptr);
     bool bma_clear = have_in;  // bma is all zero (until a level writes it)
     bool zeroed[2] = {true, true};  // ctrs[i] known to be all zero
 
@@ -693,15 +761,13 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
         };
         if (mode == GHET_MODE_PUSH && !have_q) {
             clear();
-            // counts in ctr->conv: qcount, alloc and the slots stay zero
+            // counts in ctr->conv: alloc and the slots stay zero
             hipLaunchKernelGGL(k_bitmap_to_queue, dim3(static_cast<unsigned>(ceil_div(words, 4 * kBT))), dim3(kBT), 0, s,
-                               words, n, bma, qa, &ctr->conv);
+                               words, n, bma, g->out.rowptr, qa, enta, &ctr->conv);
             have_q = true;
         }
         if (mode == GHET_MODE_PUSH) {
             clear();
-            hipLaunchKernelGGL(k_push_prep, dim3(static_cast<unsigned>(ceil_div(F, kBT * kPrepItems))), dim3(kBT), 0, s,
-                               qa, F, g->out.rowptr, scan, start, ctr);
             const int32_t E = static_cast<int32_t>(m_f);
             // a level this large is usually followed by a pull level: write
             // its frontier bitmap on the way
@@ -714,12 +780,13 @@ int run_bfs(ghet_graph* g, int32_t src, int32_t* depth_out, int32_t* pre_out, co
                     std::min<int64_t>(kTile, std::max<int64_t>(64, ceil_div(ceil_div(E, 512), 64) * 64)));
                 const unsigned grid = static_cast<unsigned>(std::min<int64_t>(ceil_div(E, tile), kMaxBlocks));
                 seq = mail_next(g);
-                hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, start, F, scan, E, tile, g->out.rowptr,
-                                   g->out.col, visited, depth, next_level, qb, ctr, in_rp, next_ctr,
+                hipLaunchKernelGGL(k_push_expand, dim3(grid), dim3(kBT), 0, s, enta, F, E, tile, g->out.rowptr,
+                                   g->out.col, visited, depth, next_level, qb, entb, ctr, in_rp, next_ctr,
                                    want_bm ? bma : nullptr, g->mail_dev, seq);
                 next_zeroed = true;
             }
             std::swap(qa, qb);
+            std::swap(enta, entb);
             have_q = true;
             have_bm = want_bm;  // bma then holds the same frontier
         } else {
