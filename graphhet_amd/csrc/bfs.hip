@@ -385,15 +385,22 @@ __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restri
             v[p] = base + p * kBT + threadIdx.x;
             vis[p] = visited[2 * (v[p] >> 6) + (lane >> 5)];
         }
-        int32_t fst[kPullSets];
+        int32_t fst[kPullSets], ob[kPullSets], oe[kPullSets];
 #pragma unroll
         for (int p = 0; p < kPullSets; ++p) {
             const bool need = v[p] < n && !((vis[p] >> (lane & 31)) & 1u);
             i[p] = need ? in_rowptr[v[p]] : 0;
             e[p] = need ? in_rowptr[v[p] + 1] : 0;
             fst[p] = need ? first_in[v[p]] : -1;
+            // the out-offsets load now, beside the in-offsets, instead of
+            // after the scan: a found vertex's degrees add no round trip
+            ob[p] = need ? out_rowptr[v[p]] : 0;
+            oe[p] = need ? out_rowptr[v[p] + 1] : 0;
             found[p] = false;
         }
+        int32_t indeg[kPullSets];
+#pragma unroll
+        for (int p = 0; p < kPullSets; ++p) indeg[p] = e[p] - i[p];
         {  // first probe: the coalesced first in-neighbour
             uint32_t wd[kPullSets];
 #pragma unroll
@@ -437,8 +444,8 @@ __global__ void __launch_bounds__(kBT) k_pull(int32_t n, const int32_t* __restri
             if (found[p]) {
                 depth[v[p]] = next_level;
                 ++cnt;
-                od += out_rowptr[v[p] + 1] - out_rowptr[v[p]];
-                id += in_rowptr[v[p] + 1] - in_rowptr[v[p]];
+                od += oe[p] - ob[p];
+                id += indeg[p];
             }
         }
     }
